@@ -1,0 +1,164 @@
+/* shapes_gpu.h - C ABI of the MI355X (gfx950) contact-physics step.
+ *
+ * This library replaces the per-step path of the Haskell engine `shapes`
+ * that Physics.Engine.Main drives (shapes/src/Physics/Engine/Main.hs:32-98):
+ * AABB broadphase, SAT polygon contact generation, and the warm-started
+ * sequential-impulse solve of NonPenetration and Friction constraints,
+ * followed by integration and shape transforms.  Each entry point below
+ * names the reference definition it stands for.
+ *
+ * Conventions
+ *   - Plain C types only; all arrays are caller-owned host memory that is
+ *     copied in (create) or filled (read_*).  Nothing is retained.
+ *   - Every call returns shp_status; on error shp_last_error() gives a
+ *     thread-local message.  No C++ exception crosses this boundary.
+ *   - A handle is not thread-safe; distinct handles may be used from
+ *     distinct threads.  Each handle owns one HIP stream on the current
+ *     device at creation time.
+ *   - Body indices are world slot indices, i.e. the reference's fromList
+ *     order (shapes/src/Physics/World.hs:144-151).
+ */
+#ifndef SHAPES_GPU_H
+#define SHAPES_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t shp_status;
+#define SHP_OK 0
+#define SHP_E_INVAL 1       /* bad argument */
+#define SHP_E_OOM 2         /* device allocation failed */
+#define SHP_E_HIP 3         /* HIP runtime error */
+#define SHP_E_UNSUPPORTED 4 /* circles, hulls with > 8 vertices, ... */
+#define SHP_E_OVERFLOW 5    /* pair/contact capacity exceeded */
+#define SHP_E_NONFINITE 6   /* a body reached a non-finite state */
+
+/* Solver modes */
+#define SHP_SOLVER_ORDERED_F64 0 /* reference order, FP64, bitwise drop-in */
+#define SHP_SOLVER_COLORED_F32 1 /* graph-coloured Gauss-Seidel, FP32 math,
+                                    FP64 impulse accumulators */
+
+/* Warm-start modes for contacts absent from last frame's cache */
+#define SHP_WARM_ZERO 0    /* the reference: start at 0
+                              (shapes/src/Physics/Solvers/Contact.hs:95) */
+#define SHP_WARM_LEARNED 1 /* MLP-predicted initial impulses */
+
+/* One world, structure of arrays (n bodies).  Mirrors makePhysicalObj vel
+ * rotvel pos rotpos (mass, inertia) (shapes/src/Physics/Engine.hs:125-137),
+ * makeWorldObj phys mu bounce shape (World.hs:169-176) and makeHull
+ * (Engine.hs:145-146).  mass/inertia 0 = infinite (toInvMass2,
+ * Constraint.hs:79-83).  Hull vertices are local-space, counter-clockwise,
+ * 3..8 per body, CSR: body b owns hull_x/y[hull_off[b] .. hull_off[b+1]). */
+typedef struct shp_world_desc {
+  int32_t n_bodies;
+  const double* px;
+  const double* py;
+  const double* theta;
+  const double* vx;
+  const double* vy;
+  const double* omega;
+  const double* mass;
+  const double* inertia;
+  const double* mu;
+  const double* bounce;
+  const int32_t* hull_off; /* n_bodies + 1 entries */
+  const double* hull_x;
+  const double* hull_y;
+} shp_world_desc;
+
+/* EngineConfig + ContactBehavior (Engine/Main.hs:34-37, Contact/Types.hs:
+ * 65-70), the constant-acceleration External (World/External.hs:249-254)
+ * and the sweep count, which the reference fixes at 2 (Main.hs:81-82). */
+typedef struct shp_config {
+  double dt;
+  double baumgarte;
+  double slop;
+  double gravity_x;
+  double gravity_y;
+  int32_t iterations;
+  int32_t solver;             /* SHP_SOLVER_* */
+  int32_t warm_start;         /* SHP_WARM_* */
+  int32_t max_pairs_per_body; /* capacity; 0 = default (16) */
+} shp_config;
+
+typedef struct shp_batch shp_batch;
+
+/* Per-step counters of the last step (summed over worlds). */
+typedef struct shp_step_stats {
+  int64_t pairs;
+  int64_t contacts;
+  int64_t warm_hits;
+  int64_t learned_starts;
+  int32_t max_levels;   /* deepest per-world schedule (levels or colours) */
+  int32_t flags;        /* bit 0 pair overflow, 1 contact overflow,
+                           2 non-finite */
+} shp_step_stats;
+
+/* ---- engine lifecycle (initEngine / runEngine, Engine/Main.hs:43-98) ---- */
+
+/* Create n_worlds independent worlds stepped together.  n_worlds = 1 is the
+ * reference engine; each world keeps its own warm-start cache
+ * (EngineCache, Main.hs:32).  The cache starts empty (initEngine, :43-46). */
+shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds,
+                            const shp_config* cfg, shp_batch** out);
+shp_status shp_batch_destroy(shp_batch* b);
+
+/* updateWorld repeated n_steps times (stepWorld, Main.hs:71-89). */
+shp_status shp_batch_step(shp_batch* b, int32_t n_steps);
+
+/* Enqueue n_steps on the handle's stream without waiting; shp_batch_sync
+ * waits and reports deferred errors.  With use_graph != 0 each step is
+ * replayed from a captured HIP graph. */
+shp_status shp_batch_step_async(shp_batch* b, int32_t n_steps,
+                                int32_t use_graph);
+shp_status shp_batch_sync(shp_batch* b);
+
+/* ---- readback (readPhysObj, World.hs:120-121; debug as IOWorld.hs:74-81) */
+
+shp_status shp_batch_num_bodies(const shp_batch* b, int32_t world,
+                                int32_t* n);
+/* px, py, theta, vx, vy, omega of every body of `world` (NULL skips). */
+shp_status shp_batch_read_bodies(shp_batch* b, int32_t world, double* px,
+                                 double* py, double* theta, double* vx,
+                                 double* vy, double* omega);
+/* Last step's broadphase pairs (G.culledKeys, Broadphase/Grid.hs:74-78),
+ * descending.  Call with ij = NULL to get the count only. */
+shp_status shp_batch_read_pairs(shp_batch* b, int32_t world, int32_t* count,
+                                int32_t* ij);
+/* Last step's contacts in descending ObjectFeatureKey order (prepareFrame,
+ * Solvers/Contact.hs:40-52), with the lagrangians accumulated by the last
+ * sweep (the next step's cache).  keys: 4 ints (i, j, fi, fj); geom: nx, ny,
+ * px, py, depth; lam: normal, friction; warm: 1 if reused from the cache,
+ * 2 if started by the learned model.  Any output may be NULL. */
+shp_status shp_batch_read_contacts(shp_batch* b, int32_t world,
+                                   int32_t* count, int32_t* keys,
+                                   int32_t* flip, double* geom, double* lam,
+                                   int32_t* warm);
+/* World-space vertices and unit edge normals (4 doubles per vertex, CSR
+ * order) as left by moveShapes (World.hs:163-167). */
+shp_status shp_batch_read_hulls(shp_batch* b, int32_t world, double* out);
+shp_status shp_batch_last_stats(shp_batch* b, shp_step_stats* out);
+
+/* Replace `world`'s warm-start cache (keys descending, 4 ints each;
+ * lam: 2 doubles each). */
+shp_status shp_batch_set_cache(shp_batch* b, int32_t world, int32_t count,
+                               const int32_t* keys, const double* lam);
+
+/* ---- learned warm start (no reference counterpart: the reference always
+ * zero-starts new contacts, Solvers/Contact.hs:95) ------------------------ */
+
+/* Parameter blob layout: see shp_mlp_param_count.  Weights are FP32 on the
+ * host; the device keeps BF16 weights for inference. */
+int32_t shp_mlp_param_count(void);
+shp_status shp_batch_set_mlp(shp_batch* b, const float* params);
+
+const char* shp_last_error(void);
+const char* shp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,100 @@
+// shp_engine.h - device buffer layout of a batch of worlds (internal).
+//
+// Structure of arrays in HBM.  Bodies of all worlds are concatenated
+// (world w owns [body_off[w], body_off[w+1])); hull vertices are CSR
+// (body g owns [vert_off[g], vert_off[g+1])).  Pairs and contacts of a
+// step are stored world after world, each world's list in the reference's
+// descending order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/shapes_gpu.h"
+
+enum { CNT_FLAGS = 0, CNT_WARM = 1, CNT_LEARNED = 2, CNT_LEVELS = 3 };
+enum {
+  SHP_FLAG_PAIR_OVERFLOW = 1,
+  SHP_FLAG_CONTACT_OVERFLOW = 2,
+  SHP_FLAG_NONFINITE = 4,
+  SHP_FLAG_SCHEDULE_OVERFLOW = 8
+};
+
+struct Ctx {
+  int W, NB, NV;
+  int pair_cap, contact_cap;
+  double dt, beta, slop, gx, gy;
+  int iterations, solver, warm_mode;
+
+  // bodies (NB)
+  double *px, *py, *th, *vx, *vy, *w, *iml, *imr, *mu, *e;
+  uint8_t* is_static;
+  int32_t* body_world;
+  int32_t* body_off;  // W + 1
+  // hulls (NV; vert_off NB + 1)
+  int32_t* vert_off;
+  double *lx, *ly;
+  uint8_t *ext_min, *ext_max;
+  double *wx, *wy, *wnx, *wny;
+  double *bminx, *bmaxx, *bminy, *bmaxy;
+  // broadphase
+  int32_t *cnt_rank, *off_rank;  // NB + 1, indexed by descending-body rank
+  int32_t *pa, *pb;              // pair_cap, global body indices, pa > pb
+  // narrowphase slots (2 per pair) and contact counts/offsets (pair_cap+1)
+  int32_t *ccount, *coff;
+  uint16_t* s_feat;
+  uint8_t* s_flip;
+  double *s_nx, *s_ny, *s_px, *s_py, *s_depth;
+  int32_t *wpair_off, *wcont_off;  // W + 1
+  // contacts (contact_cap)
+  uint64_t* key;
+  int32_t *ga, *gb;
+  uint8_t* flip;
+  double *cnx, *cny, *cpx, *cpy, *depth;
+  double *jn, *jt;  // 6 * contact_cap, row-major by component
+  double *bn, *rax, *ray, *rbx, *rby, *mcn, *mct;
+  double *ln, *lt;
+  uint8_t* warm;
+  // warm-start cache (last frame)
+  uint64_t* cache_key;
+  double *cache_ln, *cache_lt;
+  int32_t* cache_woff;  // W + 1
+  // scheduling (ordered: levels; coloured: colours)
+  int32_t* sched_level;   // contact_cap
+  int32_t* sched_order;   // contact_cap: contacts sorted by level in world
+  int32_t* sched_woff;    // W + 1: level-bucket offsets base per world
+  int32_t* sched_bucket;  // per world: level offsets (max_levels + 1 each)
+  int max_levels;
+  // coloured solver state (FP32 velocities, FP64 accumulators)
+  float *fvx, *fvy, *fw;
+  // learned warm start (BF16 weights as uint16)
+  const uint16_t* mlp_w;
+  const float* mlp_b;
+  int mlp_ready;
+  int32_t* counters;  // 8 ints: flags, warm hits, learned starts, levels
+};
+
+struct shp_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  shp_config cfg{};
+  Ctx ctx{};
+  std::vector<int32_t> h_body_off, h_vert_off;
+  void* scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+  hipGraphExec_t graph_exec{};
+  bool graph_ready = false;
+  int32_t last_flags = 0;
+  std::vector<void*> allocs;
+};
+
+shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds, long NB, long NV,
+                       long pair_cap);
+shp_status solver_alloc(shp_batch* b);
+void launch_solver(shp_batch* b);
+void launch_serial_solver(shp_batch* b);
+void launch_ordered_solver(shp_batch* b);
+void launch_colored_solver(shp_batch* b);

@@ -1,0 +1,987 @@
+// shp_engine.hip - gfx950 kernels and C ABI for the shapes contact step.
+//
+// One step (the reference's updateWorld, shapes/src/Physics/Engine/
+// Main.hs:71-86) is this sequence of launches on the batch's stream:
+//
+//   k_reset          clear per-step counters
+//   k_broadphase(0)  one wavefront per body: gravity (applyExternal) and the
+//                    count of its pairs (i, j<i)
+//   scan             exclusive sum of the counts, in descending body order
+//   k_broadphase(1)  same sweep, ballot/popcount compaction -> pairs in
+//                    descending (i, j) order (G.culledKeys)
+//   k_sat            one lane per pair: SAT + clipping -> <= 2 contacts
+//   scan             exclusive sum of contacts per pair
+//   k_world_ranges   per-world pair and contact ranges
+//   k_contacts       compaction, constraintGen, warm-start join
+//   k_solve_*        warm-start application + Gauss-Seidel sweeps
+//   k_cache_store    this frame's (key, lambda) become next frame's cache
+//   k_integrate      advance + moveShapes + AABBs
+//
+// Worlds are independent: every kernel handles all worlds of the batch at
+// once, and the ordered solver runs one workgroup per world.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/shapes_gpu.h"
+#include "shp_geom.h"
+#include "shp_trig.h"
+#include "shp_engine.h"
+
+using namespace shp;
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ bool body_static(const Ctx& c, int g) { return c.is_static[g] != 0; }
+
+// Key = ObjectFeatureKey ((i, j), (fi, fj)) (Constraints/Contact.hs:194-197)
+// packed so that unsigned order equals the derived lexicographic order.
+__device__ __forceinline__ uint64_t pack_key(int i, int j, int fi, int fj) {
+  return ((uint64_t)(uint32_t)i << 40) | ((uint64_t)(uint32_t)j << 16) |
+         ((uint64_t)(uint32_t)fi << 8) | (uint64_t)(uint32_t)fj;
+}
+
+__device__ __forceinline__ HullRef hull_of(const Ctx& c, int g) {
+  int o = c.vert_off[g];
+  HullRef h;
+  h.wx = c.wx + o;
+  h.wy = c.wy + o;
+  h.nx = c.wnx + o;
+  h.ny = c.wny + o;
+  h.emin = c.ext_min + o;
+  h.emax = c.ext_max + o;
+  h.n = c.vert_off[g + 1] - o;
+  return h;
+}
+
+// ---------------------------------------------------------------- kernels
+
+__global__ void k_reset(Ctx c) {
+  if (threadIdx.x < 8) c.counters[threadIdx.x] = 0;
+}
+
+// listToHull in local space (ConvexHull.hs:151-167): unit edge normals and
+// the per-edge (min, max) extent vertex indices, first index on ties.
+__global__ void k_init_hulls(Ctx c) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= c.NB) return;
+  int o = c.vert_off[g], n = c.vert_off[g + 1] - o;
+  for (int e = 0; e < n; ++e) {
+    int e1 = e < n - 1 ? e + 1 : 0;
+    d2 dir = normalize(clockwise(sub(mk(c.lx[o + e1], c.ly[o + e1]), mk(c.lx[o + e], c.ly[o + e]))));
+    double dmin = dot(mk(c.lx[o], c.ly[o]), dir), dmax = dmin;
+    int imin = 0, imax = 0;
+    for (int k = 1; k < n; ++k) {
+      double d = dot(mk(c.lx[o + k], c.ly[o + k]), dir);
+      if (d < dmin) {
+        dmin = d;
+        imin = k;
+      }
+      if (d > dmax) {
+        dmax = d;
+        imax = k;
+      }
+    }
+    c.ext_min[o + e] = (uint8_t)imin;
+    c.ext_max[o + e] = (uint8_t)imax;
+  }
+}
+
+// advance (World.hs:194-196, Constraint.hs:225-229) when `advance`, then
+// moveShapes (World.hs:163-167): T(pos) R(theta) applied to the local
+// vertices (Transform.hs:34-38, Linear.hs:217-220), unit edge normals
+// (ConvexHull.hs:184-195) and the AABB fold (Broadphase/Aabb.hs:81-110).
+__global__ void k_integrate(Ctx c, int advance) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= c.NB) return;
+  double px = c.px[g], py = c.py[g], th = c.th[g];
+  if (advance) {
+    double dt = c.dt;
+    px = (dt * c.vx[g]) + px;
+    py = (dt * c.vy[g]) + py;
+    th = (dt * c.w[g]) + th;
+    c.px[g] = px;
+    c.py[g] = py;
+    c.th[g] = th;
+  }
+  double s, co;
+  shp_sincos(th, &s, &co);
+  const double one = 1.0, zer = 0.0;
+  double m00 = (one * co + zer * s) + px * zer;
+  double m01 = (one * (-s) + zer * co) + px * zer;
+  double m02 = (one * zer + zer * zer) + px * one;
+  double m10 = (zer * co + one * s) + py * zer;
+  double m11 = (zer * (-s) + one * co) + py * zer;
+  double m12 = (zer * zer + one * zer) + py * one;
+  int o = c.vert_off[g], n = c.vert_off[g + 1] - o;
+  double vx[SHP_MAXV], vy[SHP_MAXV];
+#pragma unroll
+  for (int k = 0; k < SHP_MAXV; ++k) {
+    if (k < n) {
+      double lx = c.lx[o + k], ly = c.ly[o + k];
+      vx[k] = (m00 * lx + m01 * ly) + m02 * one;
+      vy[k] = (m10 * lx + m11 * ly) + m12 * one;
+      c.wx[o + k] = vx[k];
+      c.wy[o + k] = vy[k];
+    }
+  }
+  double bminx = vx[0], bmaxx = vx[0], bminy = vy[0], bmaxy = vy[0];
+#pragma unroll
+  for (int k = 0; k < SHP_MAXV; ++k) {
+    if (k < n) {
+      int k1 = (k < n - 1) ? k + 1 : 0;
+      double ex = 0.0, ey = 0.0;
+#pragma unroll
+      for (int q = 0; q < SHP_MAXV; ++q)
+        if (q == k1) {
+          ex = vx[q];
+          ey = vy[q];
+        }
+      d2 nn = normalize(clockwise(sub(mk(ex, ey), mk(vx[k], vy[k]))));
+      c.wnx[o + k] = nn.x;
+      c.wny[o + k] = nn.y;
+      if (k > 0) {
+        bminx = bminx < vx[k] ? bminx : vx[k];
+        bmaxx = bmaxx > vx[k] ? bmaxx : vx[k];
+        bminy = bminy < vy[k] ? bminy : vy[k];
+        bmaxy = bmaxy > vy[k] ? bmaxy : vy[k];
+      }
+    }
+  }
+  c.bminx[g] = bminx;
+  c.bmaxx[g] = bmaxx;
+  c.bminy[g] = bminy;
+  c.bmaxy[g] = bmaxy;
+  double chk = px + py + th + c.vx[g] + c.vy[g] + c.w[g];
+  if (!(chk - chk == 0.0)) atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_NONFINITE);
+}
+
+// Broadphase: one wavefront per body i scans j = i-1 .. 0 of its world in
+// chunks of 64 lanes (culledKeys, Broadphase/Aabb.hs:168-183, which is the
+// pair set the grid of Grid.hs:67-86 returns).  Pass 0 counts and applies
+// the constant acceleration (applyExternal, World.hs:183-185 with
+// constantAccel, World/External.hs:249-254); pass 1 writes the pairs, in
+// descending j within descending i, using ballot + prefix popcount.
+__global__ void k_broadphase(Ctx c, int fill) {
+  int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  int lane = threadIdx.x & 63;
+  if (wave >= c.NB) return;
+  int g = wave;
+  int w = c.body_world[g];
+  int b0 = c.body_off[w];
+  int li = g - b0;
+  int nw = c.body_off[w + 1] - b0;
+  int rank = b0 + (nw - 1 - li);
+  if (!fill && lane == 0 && c.iml[g] != 0.0) {
+    c.vx[g] = c.vx[g] + c.gx * c.dt;
+    c.vy[g] = c.vy[g] + c.gy * c.dt;
+  }
+  bool si = body_static(c, g);
+  double aminx = c.bminx[g], amaxx = c.bmaxx[g], aminy = c.bminy[g], amaxy = c.bmaxy[g];
+  int count = 0;
+  int base = fill ? c.off_rank[rank] : 0;
+  for (int j0 = li - 1; j0 >= 0; j0 -= 64) {
+    int lj = j0 - lane;
+    bool hit = false;
+    if (lj >= 0) {
+      int gj = b0 + lj;
+      if (!(si && body_static(c, gj))) {
+        // aabbCheck (Aabb.hs:69-78), inclusive
+        double bminx = c.bminx[gj], bmaxx = c.bmaxx[gj];
+        double bminy = c.bminy[gj], bmaxy = c.bmaxy[gj];
+        hit = !(bminx > amaxx || bmaxx < aminx) && !(bminy > amaxy || bmaxy < aminy);
+      }
+    }
+    unsigned long long m = __ballot(hit);
+    if (fill && hit) {
+      int pos = base + count + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < c.pair_cap) {
+        c.pa[pos] = g;
+        c.pb[pos] = b0 + lj;
+      }
+    }
+    count += __popcll(m);
+  }
+  if (!fill && lane == 0) c.cnt_rank[rank] = count;
+  if (fill && lane == 0 && base + count > c.pair_cap)
+    atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_PAIR_OVERFLOW);
+}
+
+__device__ __forceinline__ int total_pairs(const Ctx& c) {
+  int t = c.off_rank[c.NB];
+  return t < c.pair_cap ? t : c.pair_cap;
+}
+
+// SAT narrowphase, one lane per pair (generateContacts, Contact.hs:22-40,
+// for two hulls).  Writes up to two contacts into the pair's slots.
+__global__ void k_sat(Ctx c) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > c.pair_cap) return;
+  int tp = total_pairs(c);
+  if (p >= tp) {
+    c.ccount[p] = 0;
+    return;
+  }
+  HullRef A = hull_of(c, c.pa[p]);
+  HullRef B = hull_of(c, c.pb[p]);
+  Cgeom gm[2];
+  int flip = 0;
+  int n = hull_contacts(A, B, gm, &flip);
+  for (int k = 0; k < n; ++k) {
+    int s = 2 * p + k;
+    c.s_feat[s] = (uint16_t)((gm[k].fa << 8) | gm[k].fb);
+    c.s_flip[s] = (uint8_t)flip;
+    c.s_nx[s] = gm[k].nx;
+    c.s_ny[s] = gm[k].ny;
+    c.s_px[s] = gm[k].px;
+    c.s_py[s] = gm[k].py;
+    c.s_depth[s] = gm[k].depth;
+  }
+  c.ccount[p] = n;
+}
+
+// Per-world pair and contact ranges after the scans.
+__global__ void k_world_ranges(Ctx c) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w > c.W) return;
+  int ps = c.off_rank[c.body_off[w]];
+  if (ps > c.pair_cap) ps = c.pair_cap;
+  int cs = c.coff[ps];
+  if (cs > c.contact_cap) cs = c.contact_cap;
+  c.wpair_off[w] = ps;
+  c.wcont_off[w] = cs;
+  if (w == c.W && c.coff[ps] > c.contact_cap) atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_CONTACT_OVERFLOW);
+}
+
+// Compaction + constraintGen (Constraints/Contact.hs:218-229) + the warm
+// start join of applyCachedSlns (Solvers/Contact.hs:74-121): a binary search
+// of the key in last frame's cache of the same world, which is strictly
+// descending, finds exactly the element descZipVector (Utils/Descending.hs:
+// 47-71) would pair it with.
+__global__ void k_contacts(Ctx c) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int tp = total_pairs(c);
+  if (p >= tp) return;
+  int n = c.ccount[p];
+  if (n == 0) return;
+  int ga = c.pa[p], gb = c.pb[p];
+  int w = c.body_world[ga];
+  int b0 = c.body_off[w];
+  d2 xa = mk(c.px[ga], c.py[ga]), xb = mk(c.px[gb], c.py[gb]);
+  double im[6] = {c.iml[ga], c.iml[ga], c.imr[ga], c.iml[gb], c.iml[gb], c.imr[gb]};
+  int cache_lo = c.cache_woff[w], cache_hi = c.cache_woff[w + 1];
+  for (int k = 0; k < n; ++k) {
+    int dst = c.coff[p] + k;
+    if (dst >= c.contact_cap) continue;
+    int s = 2 * p + k;
+    int fa = c.s_feat[s] >> 8, fb = c.s_feat[s] & 255;
+    int flip = c.s_flip[s];
+    uint64_t key = pack_key(ga - b0, gb - b0, fa, fb);
+    d2 nn = mk(c.s_nx[s], c.s_ny[s]);
+    d2 pp = mk(c.s_px[s], c.s_py[s]);
+    double depth = c.s_depth[s];
+    c.key[dst] = key;
+    c.ga[dst] = ga;
+    c.gb[dst] = gb;
+    c.flip[dst] = (uint8_t)flip;
+    c.cnx[dst] = nn.x;
+    c.cny[dst] = nn.y;
+    c.cpx[dst] = pp.x;
+    c.cpy[dst] = pp.y;
+    c.depth[dst] = depth;
+    double jn[6], jt[6];
+    if (!flip) {
+      np_jacobian(nn, pp, xa, xb, jn);
+      f_jacobian(nn, pp, xa, xb, jt);
+    } else {  // flipMap + flipExtract: swapped pair, halves swapped back
+      double t[6];
+      np_jacobian(nn, pp, xb, xa, t);
+      for (int q = 0; q < 3; ++q) {
+        jn[q] = t[q + 3];
+        jn[q + 3] = t[q];
+      }
+      f_jacobian(nn, pp, xb, xa, t);
+      for (int q = 0; q < 3; ++q) {
+        jt[q] = t[q + 3];
+        jt[q + 3] = t[q];
+      }
+    }
+    for (int q = 0; q < 6; ++q) {
+      c.jn[q * c.contact_cap + dst] = jn[q];
+      c.jt[q * c.contact_cap + dst] = jt[q];
+    }
+    // baumgarte (NonPenetration.hs:48-56)
+    c.bn[dst] = depth > c.slop ? (c.beta / c.dt) * (c.slop - depth) : 0.0;
+    // Restitution.constraintGen (Restitution.hs:21-31)
+    d2 ra = sub(pp, xa), rb = sub(pp, xb);
+    c.rax[dst] = ra.x;
+    c.ray[dst] = ra.y;
+    c.rbx[dst] = rb.x;
+    c.rby[dst] = rb.y;
+    c.mcn[dst] = eff_mass(jn, im);
+    c.mct[dst] = eff_mass(jt, im);
+    // warm start lookup in the descending cache segment
+    int lo = cache_lo, hi = cache_hi;  // search [lo, hi)
+    int found = -1;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      uint64_t km = c.cache_key[mid];
+      if (km == key) {
+        found = mid;
+        break;
+      }
+      if (km > key) lo = mid + 1;
+      else hi = mid;
+    }
+    if (found >= 0) {
+      c.ln[dst] = c.cache_ln[found];
+      c.lt[dst] = c.cache_lt[found];
+      c.warm[dst] = 1;
+      atomicAdd(&c.counters[CNT_WARM], 1);
+    } else {
+      c.ln[dst] = 0.0;
+      c.lt[dst] = 0.0;
+      c.warm[dst] = 0;
+    }
+  }
+}
+
+// ---- ordered FP64 solver ---------------------------------------------------
+
+struct Vel6 {
+  double v[6];
+};
+
+__device__ __forceinline__ void load_vel(const Ctx& c, int ga, int gb, double* v) {
+  v[0] = c.vx[ga];
+  v[1] = c.vy[ga];
+  v[2] = c.w[ga];
+  v[3] = c.vx[gb];
+  v[4] = c.vy[gb];
+  v[5] = c.w[gb];
+}
+
+// Static bodies (both inverse masses 0) receive v + (+-0) in the reference,
+// which leaves a finite velocity unchanged (-0 is normalised at load), so
+// their velocities are never written.
+__device__ __forceinline__ void store_vel(const Ctx& c, int ga, int gb, const double* v) {
+  if (!body_static(c, ga)) {
+    c.vx[ga] = v[0];
+    c.vy[ga] = v[1];
+    c.w[ga] = v[2];
+  }
+  if (!body_static(c, gb)) {
+    c.vx[gb] = v[3];
+    c.vy[gb] = v[4];
+    c.w[gb] = v[5];
+  }
+}
+
+// applyLagrangian (Constraint.hs:182-221): v_k + ((j_k * l) * im_k)
+__device__ __forceinline__ void apply_lag(double l, const double* j, const double* im, double* v) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = v[k] + ((j[k] * l) * im[k]);
+}
+
+__device__ __forceinline__ void load_rows(const Ctx& c, int k, double* jn, double* jt) {
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    jn[q] = c.jn[q * c.contact_cap + k];
+    jt[q] = c.jt[q * c.contact_cap + k];
+  }
+}
+
+// useCache branch of applyCachedSlns (Solvers/Contact.hs:100-114):
+// applySln = applyFriction . applyNonPen (:55-65)
+__device__ __forceinline__ void warm_apply(const Ctx& c, int k) {
+  int ga = c.ga[k], gb = c.gb[k];
+  double v[6], jn[6], jt[6];
+  double im[6] = {c.iml[ga], c.iml[ga], c.imr[ga], c.iml[gb], c.iml[gb], c.imr[gb]};
+  load_vel(c, ga, gb, v);
+  load_rows(c, k, jn, jt);
+  apply_lag(c.ln[k], jn, im, v);
+  apply_lag(c.lt[k], jt, im, v);
+  store_vel(c, ga, gb, v);
+}
+
+// improveContactSln (Solvers/Contact.hs:124-143): contactLagrangian with
+// bounceB (Constraints/Contact.hs:232-253, Restitution.hs:34-47), then
+// solutionProcessor = positive / clampAbs (Constraints/Contact.hs:259-268,
+// SolutionProcessors.hs:427-447), then applySln.
+__device__ __forceinline__ void improve(const Ctx& c, int k) {
+  int ga = c.ga[k], gb = c.gb[k];
+  double v[6], jn[6], jt[6];
+  double im[6] = {c.iml[ga], c.iml[ga], c.imr[ga], c.iml[gb], c.iml[gb], c.imr[gb]};
+  load_vel(c, ga, gb, v);
+  load_rows(c, k, jn, jt);
+  d2 ra = mk(c.rax[k], c.ray[k]), rb = mk(c.rbx[k], c.rby[k]);
+  d2 nr = c.flip[k] ? mk(-c.cnx[k], -c.cny[k]) : mk(c.cnx[k], c.cny[k]);
+  d2 closing = add(add(add(neg(mk(v[0], v[1])), zcross(-v[2], ra)), mk(v[3], v[4])), zcross(v[5], rb));
+  double e = hs_min(c.e[ga], c.e[gb]);
+  double bb = hs_min(0.0, e * dot(closing, nr));
+  double bn = c.bn[k] + bb;
+  double new_n = (-(dot6(jn, v) + bn)) / c.mcn[k];
+  double new_t = (-(dot6(jt, v) + 0.0)) / c.mct[k];
+  double ln = c.ln[k], lt = c.lt[k];
+  double app_n = hs_max(new_n, -ln);
+  double acc_n = ln + app_n;
+  double mu = (c.mu[ga] + c.mu[gb]) / 2.0;
+  double thr = acc_n * mu;
+  double acc_t = lt + new_t;
+  double accp = acc_t > thr ? thr : (acc_t < -thr ? -thr : acc_t);
+  double app_t = accp - lt;
+  apply_lag(app_n, jn, im, v);
+  apply_lag(app_t, jt, im, v);
+  store_vel(c, ga, gb, v);
+  c.ln[k] = acc_n;
+  c.lt[k] = accp;
+}
+
+// Reference order, one lane per world: the warm-start pass then
+// `iterations` improveWorld sweeps (Solvers/Contact.hs:146-157), each over
+// the world's contacts in descending key order.
+__global__ void k_solve_serial(Ctx c) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= c.W) return;
+  int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
+  for (int k = c0; k < c1; ++k)
+    if (c.warm[k]) warm_apply(c, k);
+  for (int it = 0; it < c.iterations; ++it)
+    for (int k = c0; k < c1; ++k) improve(c, k);
+}
+
+// This frame's lagrangians become the next frame's EngineCache
+// (Engine/Main.hs:32, wrapInitializer :61-69).
+__global__ void k_cache_store(Ctx c) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= c.W) c.cache_woff[k] = c.wcont_off[k];
+  int tc = c.wcont_off[c.W];
+  if (k >= tc) return;
+  c.cache_key[k] = c.key[k];
+  c.cache_ln[k] = c.ln[k];
+  c.cache_lt[k] = c.lt[k];
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int n) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+// ---------------------------------------------------------------- host side
+
+static thread_local std::string g_err;
+
+static shp_status fail(shp_status s, const std::string& msg) {
+  g_err = msg;
+  return s;
+}
+
+#define HIPCHK(expr)                                                            \
+  do {                                                                          \
+    hipError_t _e = (expr);                                                     \
+    if (_e != hipSuccess)                                                       \
+      return fail(SHP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+template <typename T>
+static shp_status dalloc(shp_batch* b, T** p, size_t n) {
+  void* q = nullptr;
+  size_t bytes = n * sizeof(T);
+  if (bytes == 0) bytes = sizeof(T);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(SHP_E_OOM, "hipMalloc failed");
+  b->allocs.push_back(q);
+  *p = (T*)q;
+  return SHP_OK;
+}
+
+#define TRY(expr)                  \
+  do {                             \
+    shp_status _s = (expr);        \
+    if (_s != SHP_OK) return _s;   \
+  } while (0)
+
+static inline double nz(double x) { return x == 0.0 ? 0.0 : x; }
+
+static int blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+static shp_status enqueue_step(shp_batch* b) {
+  Ctx& c = b->ctx;
+  hipStream_t s = b->stream;
+  k_reset<<<1, 64, 0, s>>>(c);
+  k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 0);
+  size_t tb = b->scan_tmp_bytes;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, c.cnt_rank, c.off_rank, c.NB + 1, s));
+  k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 1);
+  k_sat<<<blocks(c.pair_cap + 1, 256), 256, 0, s>>>(c);
+  tb = b->scan_tmp_bytes;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, c.ccount, c.coff, c.pair_cap + 1, s));
+  k_world_ranges<<<blocks(c.W + 1, 256), 256, 0, s>>>(c);
+  k_contacts<<<blocks(c.pair_cap, 256), 256, 0, s>>>(c);
+  launch_solver(b);
+  k_cache_store<<<blocks(std::max(c.contact_cap, c.W + 1), 256), 256, 0, s>>>(c);
+  k_integrate<<<blocks(c.NB, 256), 256, 0, s>>>(c, 1);
+  HIPCHK(hipGetLastError());
+  return SHP_OK;
+}
+
+void launch_solver(shp_batch* b) {
+  Ctx& c = b->ctx;
+  if (c.solver == SHP_SOLVER_ORDERED_F64) {
+    launch_ordered_solver(b);
+  } else {
+    launch_colored_solver(b);
+  }
+}
+
+void launch_serial_solver(shp_batch* b) {
+  k_solve_serial<<<blocks(b->ctx.W, 64), 64, 0, b->stream>>>(b->ctx);
+}
+
+extern "C" {
+
+const char* shp_last_error(void) { return g_err.c_str(); }
+const char* shp_version(void) { return "shapes-mi355x 0.1 (gfx950)"; }
+
+shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, const shp_config* cfg,
+                            shp_batch** out) {
+  if (!worlds || n_worlds <= 0 || !cfg || !out) return fail(SHP_E_INVAL, "null argument");
+  if (!(cfg->dt > 0.0)) return fail(SHP_E_INVAL, "dt must be positive");
+  if (cfg->iterations < 0) return fail(SHP_E_INVAL, "iterations must be >= 0");
+  if (cfg->solver != SHP_SOLVER_ORDERED_F64 && cfg->solver != SHP_SOLVER_COLORED_F32)
+    return fail(SHP_E_INVAL, "unknown solver mode");
+  if (cfg->warm_start != SHP_WARM_ZERO && cfg->warm_start != SHP_WARM_LEARNED)
+    return fail(SHP_E_INVAL, "unknown warm-start mode");
+  long NB = 0, NV = 0;
+  for (int w = 0; w < n_worlds; ++w) {
+    const shp_world_desc& d = worlds[w];
+    if (d.n_bodies <= 0) return fail(SHP_E_INVAL, "world with no bodies");
+    if (d.n_bodies >= (1 << 24)) return fail(SHP_E_UNSUPPORTED, "world exceeds 2^24 bodies");
+    if (!d.px || !d.py || !d.theta || !d.vx || !d.vy || !d.omega || !d.mass || !d.inertia ||
+        !d.mu || !d.bounce || !d.hull_off || !d.hull_x || !d.hull_y)
+      return fail(SHP_E_INVAL, "null array in world description");
+    if (d.hull_off[0] != 0) return fail(SHP_E_INVAL, "hull_off[0] must be 0");
+    for (int i = 0; i < d.n_bodies; ++i) {
+      int nv = d.hull_off[i + 1] - d.hull_off[i];
+      if (nv < 3) return fail(SHP_E_INVAL, "hull with fewer than 3 vertices");
+      if (nv > SHP_MAXV) return fail(SHP_E_UNSUPPORTED, "hull with more than 8 vertices");
+    }
+    NB += d.n_bodies;
+    NV += d.hull_off[d.n_bodies];
+  }
+  if (NB >= (1L << 30)) return fail(SHP_E_UNSUPPORTED, "too many bodies");
+  int ppb = cfg->max_pairs_per_body > 0 ? cfg->max_pairs_per_body : 16;
+  long pair_cap = NB * (long)ppb + 1024;
+  if (pair_cap >= (1L << 30)) return fail(SHP_E_UNSUPPORTED, "pair capacity too large");
+
+  shp_batch* b = new shp_batch();
+  b->cfg = *cfg;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    delete b;
+    return fail(SHP_E_HIP, "no HIP device");
+  }
+  b->device = dev;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return fail(SHP_E_HIP, "hipStreamCreate failed");
+  }
+  shp_status st = batch_alloc(b, worlds, n_worlds, NB, NV, pair_cap);
+  if (st != SHP_OK) {
+    std::string keep = g_err;
+    shp_batch_destroy(b);
+    g_err = keep;
+    return st;
+  }
+  *out = b;
+  return SHP_OK;
+}
+
+}  // extern "C"
+
+shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds, long NB, long NV,
+                       long pair_cap) {
+  Ctx& c = b->ctx;
+  memset(&c, 0, sizeof(c));
+  c.W = n_worlds;
+  c.NB = (int)NB;
+  c.NV = (int)NV;
+  c.pair_cap = (int)pair_cap;
+  c.contact_cap = (int)(2 * pair_cap);
+  c.dt = b->cfg.dt;
+  c.beta = b->cfg.baumgarte;
+  c.slop = b->cfg.slop;
+  c.gx = b->cfg.gravity_x;
+  c.gy = b->cfg.gravity_y;
+  c.iterations = b->cfg.iterations;
+  c.solver = b->cfg.solver;
+  c.warm_mode = b->cfg.warm_start;
+
+  // host staging in SoA, -0 normalised to +0 (see store_vel)
+  std::vector<double> h[10];
+  for (auto& v : h) v.resize(NB);
+  std::vector<int32_t> body_world(NB), vert_off(NB + 1), body_off(n_worlds + 1);
+  std::vector<uint8_t> is_static(NB);
+  std::vector<double> lx(NV), ly(NV);
+  long g = 0, vo = 0;
+  for (int w = 0; w < n_worlds; ++w) {
+    const shp_world_desc& d = worlds[w];
+    body_off[w] = (int32_t)g;
+    for (int i = 0; i < d.n_bodies; ++i, ++g) {
+      h[0][g] = nz(d.px[i]);
+      h[1][g] = nz(d.py[i]);
+      h[2][g] = nz(d.theta[i]);
+      h[3][g] = nz(d.vx[i]);
+      h[4][g] = nz(d.vy[i]);
+      h[5][g] = nz(d.omega[i]);
+      // toInvMass2 (Constraint.hs:79-83)
+      h[6][g] = d.mass[i] == 0.0 ? 0.0 : 1.0 / d.mass[i];
+      h[7][g] = d.inertia[i] == 0.0 ? 0.0 : 1.0 / d.inertia[i];
+      h[8][g] = d.mu[i];
+      h[9][g] = d.bounce[i];
+      is_static[g] = (h[6][g] == 0.0 && h[7][g] == 0.0) ? 1 : 0;
+      body_world[g] = w;
+      vert_off[g] = (int32_t)vo;
+      for (int k = d.hull_off[i]; k < d.hull_off[i + 1]; ++k, ++vo) {
+        lx[vo] = d.hull_x[k];
+        ly[vo] = d.hull_y[k];
+      }
+    }
+  }
+  body_off[n_worlds] = (int32_t)g;
+  vert_off[NB] = (int32_t)vo;
+  b->h_body_off = body_off;
+  b->h_vert_off = vert_off;
+
+  double** dptr[10] = {&c.px, &c.py, &c.th, &c.vx, &c.vy, &c.w, &c.iml, &c.imr, &c.mu, &c.e};
+  for (int k = 0; k < 10; ++k) {
+    TRY(dalloc(b, dptr[k], NB));
+    HIPCHK(hipMemcpy(*dptr[k], h[k].data(), NB * sizeof(double), hipMemcpyHostToDevice));
+  }
+  TRY(dalloc(b, &c.is_static, NB));
+  HIPCHK(hipMemcpy(c.is_static, is_static.data(), NB, hipMemcpyHostToDevice));
+  TRY(dalloc(b, &c.body_world, NB));
+  HIPCHK(hipMemcpy(c.body_world, body_world.data(), NB * 4, hipMemcpyHostToDevice));
+  TRY(dalloc(b, &c.vert_off, NB + 1));
+  HIPCHK(hipMemcpy(c.vert_off, vert_off.data(), (NB + 1) * 4, hipMemcpyHostToDevice));
+  TRY(dalloc(b, &c.body_off, n_worlds + 1));
+  HIPCHK(hipMemcpy(c.body_off, body_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
+  TRY(dalloc(b, &c.lx, NV));
+  TRY(dalloc(b, &c.ly, NV));
+  HIPCHK(hipMemcpy(c.lx, lx.data(), NV * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.ly, ly.data(), NV * 8, hipMemcpyHostToDevice));
+  TRY(dalloc(b, &c.ext_min, NV));
+  TRY(dalloc(b, &c.ext_max, NV));
+  TRY(dalloc(b, &c.wx, NV));
+  TRY(dalloc(b, &c.wy, NV));
+  TRY(dalloc(b, &c.wnx, NV));
+  TRY(dalloc(b, &c.wny, NV));
+  TRY(dalloc(b, &c.bminx, NB));
+  TRY(dalloc(b, &c.bmaxx, NB));
+  TRY(dalloc(b, &c.bminy, NB));
+  TRY(dalloc(b, &c.bmaxy, NB));
+
+  long PC = pair_cap, CC = 2 * pair_cap;
+  TRY(dalloc(b, &c.cnt_rank, NB + 1));
+  TRY(dalloc(b, &c.off_rank, NB + 1));
+  TRY(dalloc(b, &c.pa, PC));
+  TRY(dalloc(b, &c.pb, PC));
+  TRY(dalloc(b, &c.ccount, PC + 1));
+  TRY(dalloc(b, &c.coff, PC + 1));
+  TRY(dalloc(b, &c.s_feat, 2 * PC));
+  TRY(dalloc(b, &c.s_flip, 2 * PC));
+  TRY(dalloc(b, &c.s_nx, 2 * PC));
+  TRY(dalloc(b, &c.s_ny, 2 * PC));
+  TRY(dalloc(b, &c.s_px, 2 * PC));
+  TRY(dalloc(b, &c.s_py, 2 * PC));
+  TRY(dalloc(b, &c.s_depth, 2 * PC));
+  TRY(dalloc(b, &c.wpair_off, n_worlds + 1));
+  TRY(dalloc(b, &c.wcont_off, n_worlds + 1));
+  TRY(dalloc(b, &c.key, CC));
+  TRY(dalloc(b, &c.ga, CC));
+  TRY(dalloc(b, &c.gb, CC));
+  TRY(dalloc(b, &c.flip, CC));
+  TRY(dalloc(b, &c.cnx, CC));
+  TRY(dalloc(b, &c.cny, CC));
+  TRY(dalloc(b, &c.cpx, CC));
+  TRY(dalloc(b, &c.cpy, CC));
+  TRY(dalloc(b, &c.depth, CC));
+  TRY(dalloc(b, &c.jn, 6 * CC));
+  TRY(dalloc(b, &c.jt, 6 * CC));
+  TRY(dalloc(b, &c.bn, CC));
+  TRY(dalloc(b, &c.rax, CC));
+  TRY(dalloc(b, &c.ray, CC));
+  TRY(dalloc(b, &c.rbx, CC));
+  TRY(dalloc(b, &c.rby, CC));
+  TRY(dalloc(b, &c.mcn, CC));
+  TRY(dalloc(b, &c.mct, CC));
+  TRY(dalloc(b, &c.ln, CC));
+  TRY(dalloc(b, &c.lt, CC));
+  TRY(dalloc(b, &c.warm, CC));
+  TRY(dalloc(b, &c.cache_key, CC));
+  TRY(dalloc(b, &c.cache_ln, CC));
+  TRY(dalloc(b, &c.cache_lt, CC));
+  TRY(dalloc(b, &c.cache_woff, n_worlds + 1));
+  TRY(dalloc(b, &c.counters, 8));
+  TRY(solver_alloc(b));
+
+  size_t t1 = 0, t2 = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, c.cnt_rank, c.off_rank, c.NB + 1, b->stream));
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, c.ccount, c.coff, c.pair_cap + 1, b->stream));
+  b->scan_tmp_bytes = std::max(t1, t2);
+  TRY(dalloc(b, (uint8_t**)&b->scan_tmp, b->scan_tmp_bytes));
+
+  hipStream_t s = b->stream;
+  k_fill_i32<<<1, 1, 0, s>>>(c.cnt_rank + NB, 0, 1);
+  k_fill_i32<<<blocks(n_worlds + 1, 256), 256, 0, s>>>(c.cache_woff, 0, n_worlds + 1);
+  k_fill_i32<<<blocks(n_worlds + 1, 256), 256, 0, s>>>(c.wcont_off, 0, n_worlds + 1);
+  k_fill_i32<<<blocks(n_worlds + 1, 256), 256, 0, s>>>(c.wpair_off, 0, n_worlds + 1);
+  k_fill_i32<<<1, 64, 0, s>>>(c.counters, 0, 8);
+  // makeWorldObj places each shape with moveShape (World.hs:169-176)
+  k_init_hulls<<<blocks(NB, 256), 256, 0, s>>>(c);
+  k_integrate<<<blocks(NB, 256), 256, 0, s>>>(c, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  return SHP_OK;
+}
+
+static shp_status check_flags(shp_batch* b) {
+  int32_t cnt[8];
+  HIPCHK(hipMemcpyAsync(cnt, b->ctx.counters, sizeof(cnt), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->last_flags |= cnt[CNT_FLAGS];
+  if (b->last_flags & (SHP_FLAG_PAIR_OVERFLOW | SHP_FLAG_CONTACT_OVERFLOW))
+    return fail(SHP_E_OVERFLOW, "pair/contact capacity exceeded; raise max_pairs_per_body");
+  if (b->last_flags & SHP_FLAG_NONFINITE) return fail(SHP_E_NONFINITE, "non-finite body state");
+  return SHP_OK;
+}
+
+static shp_status capture_graph(shp_batch* b) {
+  if (b->graph_ready) return SHP_OK;
+  hipGraph_t graph;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  shp_status st = enqueue_step(b);
+  hipError_t e = hipStreamEndCapture(b->stream, &graph);
+  if (st != SHP_OK) return st;
+  if (e != hipSuccess) return fail(SHP_E_HIP, "graph capture failed");
+  HIPCHK(hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0));
+  HIPCHK(hipGraphDestroy(graph));
+  b->graph_ready = true;
+  return SHP_OK;
+}
+
+extern "C" {
+
+shp_status shp_batch_destroy(shp_batch* b) {
+  if (!b) return SHP_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->graph_ready) (void)hipGraphExecDestroy(b->graph_exec);
+  for (void* p : b->allocs) (void)hipFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return SHP_OK;
+}
+
+shp_status shp_batch_step_async(shp_batch* b, int32_t n_steps, int32_t use_graph) {
+  if (!b || n_steps < 0) return fail(SHP_E_INVAL, "bad argument");
+  if (use_graph) TRY(capture_graph(b));
+  for (int s = 0; s < n_steps; ++s) {
+    if (use_graph) {
+      HIPCHK(hipGraphLaunch(b->graph_exec, b->stream));
+    } else {
+      TRY(enqueue_step(b));
+    }
+  }
+  return SHP_OK;
+}
+
+shp_status shp_batch_sync(shp_batch* b) {
+  if (!b) return fail(SHP_E_INVAL, "null batch");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return check_flags(b);
+}
+
+shp_status shp_batch_step(shp_batch* b, int32_t n_steps) {
+  TRY(shp_batch_step_async(b, n_steps, 0));
+  return shp_batch_sync(b);
+}
+
+shp_status shp_batch_num_bodies(const shp_batch* b, int32_t world, int32_t* n) {
+  if (!b || !n || world < 0 || world >= b->ctx.W) return fail(SHP_E_INVAL, "bad argument");
+  *n = b->h_body_off[world + 1] - b->h_body_off[world];
+  return SHP_OK;
+}
+
+shp_status shp_batch_read_bodies(shp_batch* b, int32_t world, double* px, double* py, double* theta,
+                                 double* vx, double* vy, double* omega) {
+  if (!b || world < 0 || world >= b->ctx.W) return fail(SHP_E_INVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  int o = b->h_body_off[world], n = b->h_body_off[world + 1] - o;
+  double* outs[6] = {px, py, theta, vx, vy, omega};
+  double* srcs[6] = {b->ctx.px, b->ctx.py, b->ctx.th, b->ctx.vx, b->ctx.vy, b->ctx.w};
+  for (int k = 0; k < 6; ++k)
+    if (outs[k]) HIPCHK(hipMemcpy(outs[k], srcs[k] + o, n * sizeof(double), hipMemcpyDeviceToHost));
+  return SHP_OK;
+}
+
+shp_status shp_batch_read_hulls(shp_batch* b, int32_t world, double* out) {
+  if (!b || !out || world < 0 || world >= b->ctx.W) return fail(SHP_E_INVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  int v0 = b->h_vert_off[b->h_body_off[world]], v1 = b->h_vert_off[b->h_body_off[world + 1]];
+  int n = v1 - v0;
+  std::vector<double> t(4 * (size_t)n);
+  double* src[4] = {b->ctx.wx, b->ctx.wy, b->ctx.wnx, b->ctx.wny};
+  for (int k = 0; k < 4; ++k)
+    HIPCHK(hipMemcpy(t.data() + (size_t)k * n, src[k] + v0, n * 8, hipMemcpyDeviceToHost));
+  for (int q = 0; q < n; ++q)
+    for (int k = 0; k < 4; ++k) out[4 * q + k] = t[(size_t)k * n + q];
+  return SHP_OK;
+}
+
+shp_status shp_batch_read_pairs(shp_batch* b, int32_t world, int32_t* count, int32_t* ij) {
+  if (!b || !count || world < 0 || world >= b->ctx.W) return fail(SHP_E_INVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  int32_t r[2];
+  HIPCHK(hipMemcpy(r, b->ctx.wpair_off + world, 8, hipMemcpyDeviceToHost));
+  int n = r[1] - r[0];
+  *count = n;
+  if (!ij || n == 0) return SHP_OK;
+  std::vector<int32_t> a(n), bb(n);
+  HIPCHK(hipMemcpy(a.data(), b->ctx.pa + r[0], n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(bb.data(), b->ctx.pb + r[0], n * 4, hipMemcpyDeviceToHost));
+  int o = b->h_body_off[world];
+  for (int k = 0; k < n; ++k) {
+    ij[2 * k] = a[k] - o;
+    ij[2 * k + 1] = bb[k] - o;
+  }
+  return SHP_OK;
+}
+
+shp_status shp_batch_read_contacts(shp_batch* b, int32_t world, int32_t* count, int32_t* keys,
+                                   int32_t* flip, double* geom, double* lam, int32_t* warm) {
+  if (!b || !count || world < 0 || world >= b->ctx.W) return fail(SHP_E_INVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const Ctx& c = b->ctx;
+  int32_t r[2];
+  HIPCHK(hipMemcpy(r, c.wcont_off + world, 8, hipMemcpyDeviceToHost));
+  int n = r[1] - r[0];
+  *count = n;
+  if (n == 0) return SHP_OK;
+  if (keys) {
+    std::vector<uint64_t> k(n);
+    HIPCHK(hipMemcpy(k.data(), c.key + r[0], n * 8, hipMemcpyDeviceToHost));
+    for (int q = 0; q < n; ++q) {
+      keys[4 * q + 0] = (int32_t)(k[q] >> 40);
+      keys[4 * q + 1] = (int32_t)((k[q] >> 16) & 0xFFFFFF);
+      keys[4 * q + 2] = (int32_t)((k[q] >> 8) & 0xFF);
+      keys[4 * q + 3] = (int32_t)(k[q] & 0xFF);
+    }
+  }
+  if (flip || warm) {
+    std::vector<uint8_t> f(n), wm(n);
+    HIPCHK(hipMemcpy(f.data(), c.flip + r[0], n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(wm.data(), c.warm + r[0], n, hipMemcpyDeviceToHost));
+    for (int q = 0; q < n; ++q) {
+      if (flip) flip[q] = f[q];
+      if (warm) warm[q] = wm[q];
+    }
+  }
+  if (geom) {
+    double* src[5] = {c.cnx, c.cny, c.cpx, c.cpy, c.depth};
+    std::vector<double> t(n);
+    for (int k = 0; k < 5; ++k) {
+      HIPCHK(hipMemcpy(t.data(), src[k] + r[0], n * 8, hipMemcpyDeviceToHost));
+      for (int q = 0; q < n; ++q) geom[5 * q + k] = t[q];
+    }
+  }
+  if (lam) {
+    std::vector<double> t(n);
+    HIPCHK(hipMemcpy(t.data(), c.ln + r[0], n * 8, hipMemcpyDeviceToHost));
+    for (int q = 0; q < n; ++q) lam[2 * q] = t[q];
+    HIPCHK(hipMemcpy(t.data(), c.lt + r[0], n * 8, hipMemcpyDeviceToHost));
+    for (int q = 0; q < n; ++q) lam[2 * q + 1] = t[q];
+  }
+  return SHP_OK;
+}
+
+shp_status shp_batch_last_stats(shp_batch* b, shp_step_stats* out) {
+  if (!b || !out) return fail(SHP_E_INVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const Ctx& c = b->ctx;
+  int32_t cnt[8];
+  HIPCHK(hipMemcpy(cnt, c.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  int32_t pr[1], cr[1];
+  HIPCHK(hipMemcpy(pr, c.wpair_off + c.W, 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(cr, c.wcont_off + c.W, 4, hipMemcpyDeviceToHost));
+  out->pairs = pr[0];
+  out->contacts = cr[0];
+  out->warm_hits = cnt[CNT_WARM];
+  out->learned_starts = cnt[CNT_LEARNED];
+  out->max_levels = cnt[CNT_LEVELS];
+  out->flags = cnt[CNT_FLAGS] | b->last_flags;
+  return SHP_OK;
+}
+
+shp_status shp_batch_set_cache(shp_batch* b, int32_t world, int32_t count, const int32_t* keys,
+                               const double* lam) {
+  if (!b || world < 0 || world >= b->ctx.W || count < 0 || (count > 0 && (!keys || !lam)))
+    return fail(SHP_E_INVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  Ctx& c = b->ctx;
+  int W = c.W;
+  std::vector<int32_t> off(W + 1);
+  HIPCHK(hipMemcpy(off.data(), c.cache_woff, (W + 1) * 4, hipMemcpyDeviceToHost));
+  int total = off[W];
+  std::vector<uint64_t> k(total);
+  std::vector<double> ln(total), lt(total);
+  if (total) {
+    HIPCHK(hipMemcpy(k.data(), c.cache_key, total * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ln.data(), c.cache_ln, total * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lt.data(), c.cache_lt, total * 8, hipMemcpyDeviceToHost));
+  }
+  std::vector<uint64_t> nk;
+  std::vector<double> nln, nlt;
+  std::vector<int32_t> noff(W + 1);
+  for (int w = 0; w < W; ++w) {
+    noff[w] = (int32_t)nk.size();
+    if (w == world) {
+      for (int q = 0; q < count; ++q) {
+        const int32_t* kk = keys + 4 * q;
+        if (kk[0] < 0 || kk[1] < 0 || kk[2] < 0 || kk[3] < 0 || kk[2] > 255 || kk[3] > 255)
+          return fail(SHP_E_INVAL, "bad cache key");
+        uint64_t pk = ((uint64_t)kk[0] << 40) | ((uint64_t)kk[1] << 16) | ((uint64_t)kk[2] << 8) |
+                      (uint64_t)kk[3];
+        if (q > 0 && !(pk < nk.back())) return fail(SHP_E_INVAL, "cache keys must be strictly descending");
+        nk.push_back(pk);
+        nln.push_back(lam[2 * q]);
+        nlt.push_back(lam[2 * q + 1]);
+      }
+    } else {
+      for (int q = off[w]; q < off[w + 1]; ++q) {
+        nk.push_back(k[q]);
+        nln.push_back(ln[q]);
+        nlt.push_back(lt[q]);
+      }
+    }
+  }
+  noff[W] = (int32_t)nk.size();
+  if ((long)nk.size() > c.contact_cap) return fail(SHP_E_OVERFLOW, "cache larger than contact capacity");
+  if (!nk.empty()) {
+    HIPCHK(hipMemcpy(c.cache_key, nk.data(), nk.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.cache_ln, nln.data(), nln.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.cache_lt, nlt.data(), nlt.size() * 8, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(c.cache_woff, noff.data(), (W + 1) * 4, hipMemcpyHostToDevice));
+  return SHP_OK;
+}
+
+}  // extern "C"

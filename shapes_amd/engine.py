@@ -1,0 +1,293 @@
+"""Host-side mirror of the reference engine API over the gfx950 C ABI.
+
+The reference drives a world with Physics.Engine.Main
+(shapes/src/Physics/Engine/Main.hs:32-98): `initEngine` builds the engine
+state from a `Scene`, `updateWorld` advances it one step, `stepWorld n`
+repeats it and `runWorld dt scene n` does all of it.  This module keeps those
+names and meanings; the state lives on the GPU behind an opaque handle of
+libshapes_gpu.so (include/shapes_gpu.h).  There is no CPU fallback: if the
+library is missing or no GPU is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+
+from .scenes import Scene, SceneConfig, WorldDesc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libshapes_gpu.so")
+
+SOLVER_ORDERED_F64 = 0
+SOLVER_COLORED_F32 = 1
+WARM_ZERO = 0
+WARM_LEARNED = 1
+
+STATUS = {0: "OK", 1: "E_INVAL", 2: "E_OOM", 3: "E_HIP", 4: "E_UNSUPPORTED",
+          5: "E_OVERFLOW", 6: "E_NONFINITE"}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_fp = C.POINTER(C.c_float)
+
+
+class ShapesError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class WorldDescC(C.Structure):
+    _fields_ = [("n_bodies", C.c_int32)] + [
+        (n, _dp) for n in ("px", "py", "theta", "vx", "vy", "omega", "mass",
+                           "inertia", "mu", "bounce")
+    ] + [("hull_off", _ip), ("hull_x", _dp), ("hull_y", _dp)]
+
+
+class ConfigC(C.Structure):
+    _fields_ = [("dt", C.c_double), ("baumgarte", C.c_double),
+                ("slop", C.c_double), ("gravity_x", C.c_double),
+                ("gravity_y", C.c_double), ("iterations", C.c_int32),
+                ("solver", C.c_int32), ("warm_start", C.c_int32),
+                ("max_pairs_per_body", C.c_int32)]
+
+
+class StepStatsC(C.Structure):
+    _fields_ = [("pairs", C.c_int64), ("contacts", C.c_int64),
+                ("warm_hits", C.c_int64), ("learned_starts", C.c_int64),
+                ("max_levels", C.c_int32), ("flags", C.c_int32)]
+
+
+EXPORTS = [
+    "shp_batch_create", "shp_batch_destroy", "shp_batch_step",
+    "shp_batch_step_async", "shp_batch_sync", "shp_batch_num_bodies",
+    "shp_batch_read_bodies", "shp_batch_read_pairs", "shp_batch_read_contacts",
+    "shp_batch_read_hulls", "shp_batch_last_stats", "shp_batch_set_cache",
+    "shp_mlp_param_count", "shp_batch_set_mlp", "shp_last_error",
+    "shp_version",
+]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libshapes_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ShapesError(4, f"{path} is missing; run shapes_amd/build.py")
+    L = C.CDLL(path)
+    L.shp_batch_create.argtypes = [C.POINTER(WorldDescC), C.c_int32,
+                                   C.POINTER(ConfigC), C.POINTER(C.c_void_p)]
+    L.shp_batch_destroy.argtypes = [C.c_void_p]
+    L.shp_batch_step.argtypes = [C.c_void_p, C.c_int32]
+    L.shp_batch_step_async.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    L.shp_batch_sync.argtypes = [C.c_void_p]
+    L.shp_batch_num_bodies.argtypes = [C.c_void_p, C.c_int32, _ip]
+    L.shp_batch_read_bodies.argtypes = [C.c_void_p, C.c_int32] + [_dp] * 6
+    L.shp_batch_read_pairs.argtypes = [C.c_void_p, C.c_int32, _ip, _ip]
+    L.shp_batch_read_contacts.argtypes = [C.c_void_p, C.c_int32, _ip, _ip, _ip,
+                                          _dp, _dp, _ip]
+    L.shp_batch_read_hulls.argtypes = [C.c_void_p, C.c_int32, _dp]
+    L.shp_batch_last_stats.argtypes = [C.c_void_p, C.POINTER(StepStatsC)]
+    L.shp_batch_set_cache.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _ip, _dp]
+    L.shp_mlp_param_count.restype = C.c_int32
+    L.shp_batch_set_mlp.argtypes = [C.c_void_p, _fp]
+    L.shp_last_error.restype = C.c_char_p
+    L.shp_version.restype = C.c_char_p
+    for name in EXPORTS:
+        getattr(L, name)
+    _lib = L
+    return L
+
+
+def _check(status: int):
+    if status != 0:
+        msg = _lib.shp_last_error().decode() if _lib else ""
+        raise ShapesError(status, msg)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Batch:
+    """Independent worlds stepped together on one GPU (n = 1 is the
+    reference engine).  Each world keeps its own warm-start cache."""
+
+    def __init__(self, worlds: Sequence[WorldDesc], config: SceneConfig,
+                 solver: int = SOLVER_ORDERED_F64, warm_start: int = WARM_ZERO,
+                 max_pairs_per_body: int = 0):
+        L = load_library()
+        self._L = L
+        self._keep = []
+        descs = (WorldDescC * len(worlds))()
+        for k, w in enumerate(worlds):
+            d = descs[k]
+            d.n_bodies = w.n
+            for name in ("px", "py", "theta", "vx", "vy", "omega", "mass",
+                         "inertia", "mu", "bounce"):
+                a = np.ascontiguousarray(getattr(w, name), dtype=np.float64)
+                self._keep.append(a)
+                setattr(d, name, _dptr(a))
+            ho = np.ascontiguousarray(w.hull_off, dtype=np.int32)
+            hx = np.ascontiguousarray(w.hull_x, dtype=np.float64)
+            hy = np.ascontiguousarray(w.hull_y, dtype=np.float64)
+            self._keep += [ho, hx, hy]
+            d.hull_off = ho.ctypes.data_as(_ip)
+            d.hull_x = _dptr(hx)
+            d.hull_y = _dptr(hy)
+        cfg = ConfigC(config.dt, config.baumgarte, config.slop, config.gx,
+                      config.gy, config.iterations, solver, warm_start,
+                      max_pairs_per_body)
+        h = C.c_void_p()
+        _check(L.shp_batch_create(descs, len(worlds), C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._keep = []
+        self.n_worlds = len(worlds)
+        self.sizes = [w.n for w in worlds]
+        self._vert_counts = [int(w.hull_off[-1]) for w in worlds]
+        self.config = config
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.shp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- stepping (updateWorld / stepWorld)
+    def step(self, steps: int = 1):
+        _check(self._L.shp_batch_step(self._h, steps))
+
+    def step_async(self, steps: int = 1, use_graph: bool = False):
+        _check(self._L.shp_batch_step_async(self._h, steps, int(use_graph)))
+
+    def sync(self):
+        _check(self._L.shp_batch_sync(self._h))
+
+    # ---- readback
+    def bodies(self, world: int = 0) -> np.ndarray:
+        """(n, 6) array of px, py, theta, vx, vy, omega."""
+        n = self.sizes[world]
+        cols = [np.empty(n, dtype=np.float64) for _ in range(6)]
+        _check(self._L.shp_batch_read_bodies(self._h, world,
+                                             *[_dptr(c) for c in cols]))
+        return np.stack(cols, axis=1)
+
+    def pairs(self, world: int = 0) -> np.ndarray:
+        cnt = C.c_int32()
+        _check(self._L.shp_batch_read_pairs(self._h, world, C.byref(cnt), None))
+        out = np.empty((cnt.value, 2), dtype=np.int32)
+        if cnt.value:
+            _check(self._L.shp_batch_read_pairs(self._h, world, C.byref(cnt),
+                                                out.ctypes.data_as(_ip)))
+        return out
+
+    def contacts(self, world: int = 0) -> dict:
+        cnt = C.c_int32()
+        _check(self._L.shp_batch_read_contacts(self._h, world, C.byref(cnt),
+                                               None, None, None, None, None))
+        n = cnt.value
+        keys = np.empty((n, 4), dtype=np.int32)
+        flip = np.empty(n, dtype=np.int32)
+        geom = np.empty((n, 5), dtype=np.float64)
+        lam = np.empty((n, 2), dtype=np.float64)
+        warm = np.empty(n, dtype=np.int32)
+        if n:
+            _check(self._L.shp_batch_read_contacts(
+                self._h, world, C.byref(cnt), keys.ctypes.data_as(_ip),
+                flip.ctypes.data_as(_ip), _dptr(geom), _dptr(lam),
+                warm.ctypes.data_as(_ip)))
+        return {"keys": keys, "flip": flip, "geom": geom, "lam": lam,
+                "warm": warm}
+
+    def hulls(self, world: int = 0) -> np.ndarray:
+        out = np.empty((self._vert_counts[world], 4), dtype=np.float64)
+        _check(self._L.shp_batch_read_hulls(self._h, world, _dptr(out)))
+        return out
+
+    def stats(self) -> dict:
+        s = StepStatsC()
+        _check(self._L.shp_batch_last_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in StepStatsC._fields_}
+
+    def set_cache(self, world: int, keys, lam):
+        k = np.ascontiguousarray(keys, dtype=np.int32)
+        l = np.ascontiguousarray(lam, dtype=np.float64)
+        _check(self._L.shp_batch_set_cache(self._h, world, len(k),
+                                           k.ctypes.data_as(_ip), _dptr(l)))
+
+    def set_mlp(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        _check(self._L.shp_batch_set_mlp(self._h, p.ctypes.data_as(_fp)))
+
+
+def make_batch(worlds, config, **kw) -> Batch:
+    return Batch(worlds, config, **kw)
+
+
+# ---------------------------------------------------------------------------
+# Names of the reference's Physics.Engine / Physics.Engine.Main API.
+
+
+@dataclass
+class ContactBehavior:
+    """ContactBehavior (shapes/src/Physics/Contact/Types.hs:65-70)."""
+    contact_baumgarte: float = 0.0
+    contact_penetration_slop: float = 0.0
+
+
+@dataclass
+class EngineConfig:
+    """EngineConfig (Engine/Main.hs:34-37) plus the sweep count."""
+    timestep: float
+    contact_beh: ContactBehavior
+    iterations: int = 2
+
+
+def init_engine(scene: Scene, solver: int = SOLVER_ORDERED_F64,
+                warm_start: int = WARM_ZERO) -> Batch:
+    """initEngine (Engine/Main.hs:43-46): world + empty cache + externals."""
+    return make_batch(scene.worlds, scene.config, solver=solver,
+                      warm_start=warm_start)
+
+
+def update_world(state: Batch) -> None:
+    """updateWorld (Engine/Main.hs:71-86)."""
+    state.step(1)
+
+
+def step_world(state: Batch, steps: int) -> None:
+    """stepWorld (Engine/Main.hs:88-89)."""
+    state.step(steps)
+
+
+def run_world(dt: float, scene: Scene, steps: int) -> Batch:
+    """runWorld (Engine/Main.hs:97-98); returns the final state."""
+    cfg = SceneConfig(**{**scene.config.__dict__, "dt": dt})
+    state = make_batch(scene.worlds, cfg)
+    state.step(steps)
+    return state
+
+
+def read_phys_obj(state: Batch, i: int, world: int = 0) -> dict:
+    """readPhysObj (World.hs:120-121)."""
+    b = state.bodies(world)[i]
+    return {"pos": (b[0], b[1]), "rot_pos": b[2], "vel": (b[3], b[4]),
+            "rot_vel": b[5]}

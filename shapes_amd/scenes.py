@@ -1,0 +1,263 @@
+"""Seeded scene generators for the contact-physics workloads.
+
+A scene is a list of worlds; a world is a `WorldDesc` of structure-of-arrays
+body state plus CSR hull vertices in local space, counter-clockwise and
+centred on the centroid.  Body order is the world's slot order (the
+reference's `fromList` order, shapes/src/Physics/World.hs:144-151): the
+static floor is body 0.
+
+Random numbers come from numpy's Philox counter-based generator, so a seed
+gives the same scene on every machine.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class WorldDesc:
+    """Structure-of-arrays description of one world.
+
+    mass/inertia of 0 mean infinite (the reference's `toInvMass2`,
+    shapes/src/Physics/Constraint.hs:79-83).
+    """
+
+    px: np.ndarray
+    py: np.ndarray
+    theta: np.ndarray
+    vx: np.ndarray
+    vy: np.ndarray
+    omega: np.ndarray
+    mass: np.ndarray
+    inertia: np.ndarray
+    mu: np.ndarray
+    bounce: np.ndarray
+    hull_off: np.ndarray  # int32, n + 1
+    hull_x: np.ndarray
+    hull_y: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return int(self.px.shape[0])
+
+    @property
+    def n_dynamic(self) -> int:
+        return int(np.count_nonzero((self.mass != 0) | (self.inertia != 0)))
+
+    def copy(self) -> "WorldDesc":
+        return WorldDesc(**{k: v.copy() for k, v in self.__dict__.items()})
+
+
+@dataclass
+class SceneConfig:
+    """Engine parameters: the reference's EngineConfig + ContactBehavior
+    (shapes/src/Physics/Engine/Main.hs:34-37, Contact/Types.hs:65-70), the
+    constant acceleration External, and the sweep count the reference
+    hard-codes to 2 (Engine/Main.hs:81-82)."""
+
+    dt: float = 1.0 / 60.0
+    baumgarte: float = 0.01
+    slop: float = 0.02
+    gx: float = 0.0
+    gy: float = -9.8
+    iterations: int = 4
+    steps: int = 300
+
+
+@dataclass
+class Scene:
+    worlds: list
+    config: SceneConfig
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_bodies(self) -> int:
+        return sum(w.n for w in self.worlds)
+
+    @property
+    def n_dynamic(self) -> int:
+        return sum(w.n_dynamic for w in self.worlds)
+
+
+# ---------------------------------------------------------------- geometry
+
+def rectangle_vertices(w: float, h: float):
+    """rectangleVertices (shapes/src/Physics/Contact/ConvexHull.hs:135-145)."""
+    w2 = w / 2.0
+    h2 = h / 2.0
+    return [(w2, h2), (-w2, h2), (-w2, -h2), (w2, -h2)]
+
+
+def polygon_mass_properties(xs, ys, density=1.0):
+    """Area, centroid and second moment about the centroid of a CCW polygon
+    (closed-form shoelace sums)."""
+    x = np.asarray(xs, dtype=np.float64)
+    y = np.asarray(ys, dtype=np.float64)
+    x1 = np.roll(x, -1)
+    y1 = np.roll(y, -1)
+    cr = x * y1 - x1 * y
+    area = 0.5 * cr.sum()
+    cx = ((x + x1) * cr).sum() / (6.0 * area)
+    cy = ((y + y1) * cr).sum() / (6.0 * area)
+    i_origin = (cr * (x * x + x * x1 + x1 * x1 + y * y + y * y1 + y1 * y1)).sum() / 12.0
+    mass = density * area
+    inertia = density * i_origin - mass * (cx * cx + cy * cy)
+    return mass, (cx, cy), inertia
+
+
+def random_polygon(rng: np.random.Generator, radius_lo=0.3, radius_hi=1.0):
+    """Convex 3-8-gon: vertices on a circle of radius U[lo,hi] at angles
+    2*pi*k/n + U(-0.25,0.25)*(2*pi/n), re-centred on the centroid."""
+    n = int(rng.integers(3, 9))
+    step = 2.0 * math.pi / n
+    ang = np.arange(n) * step + rng.uniform(-0.25, 0.25, n) * step
+    ang.sort()
+    r = rng.uniform(radius_lo, radius_hi)
+    xs = r * np.cos(ang)
+    ys = r * np.sin(ang)
+    _, (cx, cy), _ = polygon_mass_properties(xs, ys)
+    return list(zip((xs - cx).tolist(), (ys - cy).tolist()))
+
+
+class _Builder:
+    def __init__(self):
+        self.cols = {k: [] for k in ("px", "py", "theta", "vx", "vy", "omega",
+                                     "mass", "inertia", "mu", "bounce")}
+        self.off = [0]
+        self.hx = []
+        self.hy = []
+
+    def add(self, verts, pos, vel=(0.0, 0.0), omega=0.0, theta=0.0,
+            mass=0.0, inertia=0.0, mu=0.5, bounce=0.0):
+        c = self.cols
+        c["px"].append(pos[0]); c["py"].append(pos[1]); c["theta"].append(theta)
+        c["vx"].append(vel[0]); c["vy"].append(vel[1]); c["omega"].append(omega)
+        c["mass"].append(mass); c["inertia"].append(inertia)
+        c["mu"].append(mu); c["bounce"].append(bounce)
+        for (x, y) in verts:
+            self.hx.append(x)
+            self.hy.append(y)
+        self.off.append(len(self.hx))
+
+    def add_dynamic(self, verts, pos, density=1.0, **kw):
+        m, _, inertia = polygon_mass_properties([v[0] for v in verts],
+                                                [v[1] for v in verts], density)
+        self.add(verts, pos, mass=m, inertia=inertia, **kw)
+
+    def build(self) -> WorldDesc:
+        arr = {k: np.asarray(v, dtype=np.float64) for k, v in self.cols.items()}
+        return WorldDesc(hull_off=np.asarray(self.off, dtype=np.int32),
+                         hull_x=np.asarray(self.hx, dtype=np.float64),
+                         hull_y=np.asarray(self.hy, dtype=np.float64), **arr)
+
+
+# ------------------------------------------------------------------ scenes
+
+def pyramid(seed: int = 1001, rows: int = 10, steps: int = 300,
+            iterations: int = 4) -> Scene:
+    """Pyramid of `rows` rows of 1x1 boxes (55 for 10 rows), mass 1,
+    x-jitter U(-0.01, 0.01), on a static 40x1 floor; g = -9.8, dt = 1/60,
+    friction 0.5, Baumgarte/slop 0.01/0.02 (shapes/src/Physics/Scenes/
+    Stacks.hs:87-88)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    b = _Builder()
+    b.add(rectangle_vertices(40.0, 1.0), (0.0, -0.5), mu=0.5)
+    box = rectangle_vertices(1.0, 1.0)
+    for r in range(rows):
+        for c in range(rows - r):
+            x = (c - (rows - 1 - r) / 2.0) * 1.0 + rng.uniform(-0.01, 0.01)
+            b.add(box, (x, 0.5 + r), mass=1.0, inertia=1.0 / 6.0, mu=0.5)
+    cfg = SceneConfig(dt=1.0 / 60.0, gy=-9.8, iterations=iterations, steps=steps)
+    return Scene([b.build()], cfg, name=f"pyramid{rows}",
+                 meta={"seed": seed})
+
+
+def reference_stacks(dims=(30, 30), spacing=0.0, steps: int = 10) -> Scene:
+    """The reference benchmark scene Stacks.makeScene (30,30) 0 with
+    runWorld 0.01 (shapes/bench/Main.hs:19-26; shapes/src/Physics/Scenes/
+    Stacks.hs:13-113): 0.2x0.2 boxes of (mass, inertia) = (2, 1) in columns
+    left to right, bottom to top, on a static 18x1 floor at (0,-6);
+    constant acceleration (0,-2); ContactBehavior 0.01 0.02; 2 sweeps.
+    Positions accumulate with the same floating-point additions as the
+    Haskell `iterate (+ w)` and `_2 %~ (+ (h + spacing))`."""
+    n_w, n_h = dims
+    w = h = 0.2
+    b = _Builder()
+    b.add(rectangle_vertices(18.0, 1.0), (0.0, -6.0), mu=0.2)
+    box = rectangle_vertices(w, h)
+    center, bottom = 0.0, -4.5
+    leftmost = center - (w * float(n_w - 1) / 2.0)
+    left = leftmost
+    for _ in range(n_w):
+        y = bottom
+        for _ in range(n_h):
+            b.add(box, (left, y), mass=2.0, inertia=1.0, mu=0.2)
+            y = y + (h + spacing)
+        left = left + w
+    cfg = SceneConfig(dt=0.01, gy=-2.0, iterations=2, steps=steps)
+    return Scene([b.build()], cfg, name="reference_stacks_30x30",
+                 meta={"dims": dims, "spacing": spacing})
+
+
+def _polygon_field(rng, b: _Builder, count: int, cols: int, spacing: float,
+                   y0: float, square_frac: float = 0.0, mu_range=(0.2, 0.8)):
+    x0 = -(cols - 1) * spacing / 2.0
+    for k in range(count):
+        r, c = divmod(k, cols)
+        if square_frac > 0.0 and rng.uniform() < square_frac:
+            side = rng.uniform(0.5, 1.0)
+            verts = rectangle_vertices(side, side)
+        else:
+            verts = random_polygon(rng)
+        pos = (x0 + c * spacing, y0 + r * spacing)
+        vel = (rng.uniform(-1.0, 1.0), rng.uniform(-1.0, 1.0))
+        b.add_dynamic(verts, pos, vel=vel, mu=rng.uniform(*mu_range))
+
+
+def polygon_pile(seed: int = 1002, count: int = 8192, steps: int = 600,
+                 iterations: int = 8, cols: int = 90,
+                 floor_width: float = 200.0) -> Scene:
+    """One large world: `count` random convex 3-8-gons (density 1) on a
+    seeded grid at spacing 2.1 (`cols` columns, starting at y = 2) with
+    v ~ U[-1,1]^2 and mu ~ U[0.2,0.8], dropped onto a static floor."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    b = _Builder()
+    b.add(rectangle_vertices(floor_width, 1.0), (0.0, -0.5), mu=0.5)
+    _polygon_field(rng, b, count, cols, 2.1, 2.0)
+    cfg = SceneConfig(dt=1.0 / 60.0, gy=-9.8, iterations=iterations, steps=steps)
+    return Scene([b.build()], cfg, name=f"polygon_pile{count}",
+                 meta={"seed": seed})
+
+
+def world_batch(seed: int = 1003, n_worlds: int = 4096, lo: int = 8,
+                hi: int = 1024, steps: int = 300, iterations: int = 6,
+                dt: float = 1.0 / 60.0) -> Scene:
+    """Independent worlds with log-uniform body counts in [lo, hi]; half
+    squares (side U[0.5,1]) and half random 3-8-gons, each world on its own
+    static floor, seeded initial velocities."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    worlds = []
+    counts = np.rint(np.exp(rng.uniform(math.log(lo), math.log(hi), n_worlds)))
+    for n in counts.astype(int):
+        cols = max(2, int(math.ceil(math.sqrt(n))))
+        b = _Builder()
+        b.add(rectangle_vertices(cols * 2.1 + 10.0, 1.0), (0.0, -0.5), mu=0.5)
+        _polygon_field(rng, b, int(n), cols, 2.1, 2.0, square_frac=0.5)
+        worlds.append(b.build())
+    cfg = SceneConfig(dt=dt, gy=-9.8, iterations=iterations, steps=steps)
+    return Scene(worlds, cfg, name=f"world_batch{n_worlds}",
+                 meta={"seed": seed})
+
+
+def random_world(seed: int, count: int = 40, cols: int = 8,
+                 spacing: float = 1.6, square_frac: float = 0.5) -> WorldDesc:
+    """Small mixed world for parity tests (tighter spacing -> early contacts)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    b = _Builder()
+    b.add(rectangle_vertices(cols * spacing + 6.0, 1.0), (0.0, -0.5), mu=0.5)
+    _polygon_field(rng, b, count, cols, spacing, 1.2, square_frac=square_frac)
+    return b.build()

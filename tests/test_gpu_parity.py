@@ -1,0 +1,132 @@
+"""Drop-in (ordered FP64) mode on the GPU against the CPU oracle.
+
+Broadphase pairs, contact feature keys and orientation must match exactly;
+contact geometry, lagrangians and body state must match bit for bit (the
+1e-9 relative bound of the drop-in contract is met with zero error).  Every
+call goes through the C ABI of libshapes_gpu.so.
+"""
+import numpy as np
+import pytest
+
+from shapes_amd import engine, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def assert_bitwise(name, got, want, step):
+    got = np.asarray(got)
+    want = np.asarray(want)
+    assert got.shape == want.shape, f"{name} shape {got.shape} != {want.shape} at step {step}"
+    if got.dtype.kind == "f":
+        diff = np.nonzero(bits(got).ravel() != bits(want).ravel())[0]
+    else:
+        diff = np.nonzero(got.ravel() != want.ravel())[0]
+    if len(diff):
+        k = diff[0]
+        raise AssertionError(f"{name} differs at step {step}, flat index {k}: "
+                             f"gpu {got.ravel()[k]!r} oracle {want.ravel()[k]!r} "
+                             f"({len(diff)} values differ)")
+
+
+def compare_state(gpu, ref, step, world=0):
+    assert_bitwise("pairs", gpu.pairs(world), ref.pairs(), step)
+    gc, rc = gpu.contacts(world), ref.contacts()
+    for k in ("keys", "flip", "geom", "lam", "warm"):
+        assert_bitwise(k, gc[k], rc[k], step)
+    assert_bitwise("bodies", gpu.bodies(world), ref.bodies(), step)
+
+
+def run_parity(orc, world, cfg, steps, every=1):
+    print(f"parity: {world.n} bodies, {steps} steps", flush=True)
+    gpu = engine.make_batch([world], cfg)
+    ref = orc.OracleEngine(world, cfg)
+    assert_bitwise("hulls", gpu.hulls(), ref.hulls(), 0)
+    done = 0
+    while done < steps:
+        k = min(every, steps - done)
+        gpu.step(k)
+        ref.step(k)
+        done += k
+        compare_state(gpu, ref, done)
+    assert_bitwise("hulls", gpu.hulls(), ref.hulls(), done)
+    gpu.close()
+    return ref
+
+
+def test_transform_matches_oracle(orc):
+    """sin/cos, the affine transform, sqrt and division of moveShapes on
+    2000 spinning bodies far apart (no contacts), bit for bit."""
+    rng = np.random.default_rng(0)
+    b = scenes._Builder()
+    for k in range(2000):
+        verts = scenes.random_polygon(rng)
+        b.add_dynamic(verts, ((k % 50) * 3.0, (k // 50) * 3.0 + 100.0),
+                      theta=rng.uniform(-50, 50), omega=rng.uniform(-20, 20),
+                      vel=(rng.uniform(-1, 1), rng.uniform(-1, 1)))
+    w = b.build()
+    cfg = scenes.SceneConfig(iterations=2)
+    run_parity(orc, w, cfg, steps=20, every=5)
+
+
+def test_pyramid_bitwise(orc):
+    """Pyramid of 55 unit boxes, 4 sweeps, 300 steps, every step compared."""
+    sc = scenes.pyramid()
+    ref = run_parity(orc, sc.worlds[0], sc.config, steps=300)
+    assert ref.stats()["contacts"] > 200
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_random_world_bitwise(orc, seed):
+    w = scenes.random_world(seed, count=60, spacing=1.25)
+    cfg = scenes.SceneConfig(iterations=4)
+    run_parity(orc, w, cfg, steps=150)
+
+
+def test_reference_bench_scene_bitwise(orc):
+    """The reference's own benchmark scene, Stacks.makeScene (30,30) 0 with
+    runWorld 0.01 ... 10 (shapes/bench/Main.hs:19-26): 901 bodies."""
+    sc = scenes.reference_stacks()
+    run_parity(orc, sc.worlds[0], sc.config, steps=10)
+
+
+def test_batch_of_worlds_bitwise(orc):
+    worlds = [scenes.random_world(100 + k, count=10 + 7 * k, spacing=1.3)
+              for k in range(8)]
+    cfg = scenes.SceneConfig(iterations=6)
+    gpu = engine.make_batch(worlds, cfg)
+    refs = [orc.OracleEngine(w, cfg) for w in worlds]
+    for step in range(1, 81):
+        gpu.step(1)
+        for k, r in enumerate(refs):
+            r.step(1)
+            if step % 10 == 0:
+                compare_state(gpu, r, step, world=k)
+    gpu.close()
+
+
+def test_polygon_pile_bitwise(orc):
+    """The 8192-polygon world at full size while the first rows land."""
+    sc = scenes.polygon_pile()
+    run_parity(orc, sc.worlds[0], sc.config, steps=60, every=20)
+
+
+def test_graph_replay_matches_eager():
+    """A captured HIP graph replayed step after step gives the same bits as
+    eager launches."""
+    worlds = [scenes.random_world(7 + k, count=40, spacing=1.3) for k in range(4)]
+    cfg = scenes.SceneConfig(iterations=4)
+    a = engine.make_batch(worlds, cfg)
+    b = engine.make_batch(worlds, cfg)
+    for _ in range(6):
+        a.step(10)
+        b.step_async(10, use_graph=True)
+        b.sync()
+        for k in range(len(worlds)):
+            assert_bitwise("bodies", b.bodies(k), a.bodies(k), 0)
+            assert_bitwise("lam", b.contacts(k)["lam"], a.contacts(k)["lam"], 0)
+    a.close()
+    b.close()
