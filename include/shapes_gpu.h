@@ -94,7 +94,17 @@ typedef struct shp_step_stats {
   int64_t learned_starts;
   int32_t max_levels;   /* deepest per-world schedule (levels or colours) */
   int32_t flags;        /* bit 0 pair overflow, 1 contact overflow,
-                           2 non-finite */
+                           2 non-finite, 3 schedule fell back to serial */
+  /* Solve quality after the last sweep (FP64 state), maxima over contacts
+   * of: penetration depth, |min(lambda_n, Jn.v + b)| and
+   * |lambda_t - clamp(lambda_t - Jt.v / mt, +-mu*lambda_n)|; for the last
+   * step and since creation. */
+  double max_depth;
+  double normal_residual;
+  double friction_residual;
+  double run_max_depth;
+  double run_normal_residual;
+  double run_friction_residual;
 } shp_step_stats;
 
 /* ---- engine lifecycle (initEngine / runEngine, Engine/Main.hs:43-98) ---- */
@@ -154,6 +164,22 @@ shp_status shp_batch_set_cache(shp_batch* b, int32_t world, int32_t count,
  * host; the device keeps BF16 weights for inference. */
 int32_t shp_mlp_param_count(void);
 shp_status shp_batch_set_mlp(shp_batch* b, const float* params);
+
+/* ---- per-phase timing (no counterpart in the reference, whose only
+ * timing is the criterion benchmark, shapes/bench/Main.hs:19-26) --------- */
+
+/* Phases of one step, in launch order: 0 reset, 1 broadphase count (with
+ * gravity), 2 pair scan, 3 broadphase fill, 4 SAT, 5 contact scan,
+ * 6 world ranges, 7 contacts (constraintGen + warm-start join), 8 solve,
+ * 9 quality, 10 cache store, 11 integrate (advance + moveShapes + AABBs). */
+#define SHP_NUM_PHASES 12
+/* Enable (1) or disable (0) HIP-event timing of each phase on the batch's
+ * stream; resets the accumulated times.  Timed steps synchronise once per
+ * step and are never replayed from a graph. */
+shp_status shp_batch_set_timing(shp_batch* b, int32_t enable);
+/* Accumulated milliseconds per phase and the number of timed steps. */
+shp_status shp_batch_read_timing(shp_batch* b, double* ms_per_phase,
+                                 int32_t* steps);
 
 const char* shp_last_error(void);
 const char* shp_version(void);

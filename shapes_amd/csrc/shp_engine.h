@@ -16,6 +16,9 @@
 #include "../../include/shapes_gpu.h"
 
 enum { CNT_FLAGS = 0, CNT_WARM = 1, CNT_LEARNED = 2, CNT_LEVELS = 3 };
+// solve-quality maxima (doubles stored as their bit patterns): last step,
+// then since creation
+enum { Q_DEPTH = 0, Q_NRES = 1, Q_FRES = 2, Q_RUN_DEPTH = 3, Q_RUN_NRES = 4, Q_RUN_FRES = 5 };
 enum {
   SHP_FLAG_PAIR_OVERFLOW = 1,
   SHP_FLAG_CONTACT_OVERFLOW = 2,
@@ -75,6 +78,7 @@ struct Ctx {
   const float* mlp_b;
   int mlp_ready;
   int32_t* counters;  // 8 ints: flags, warm hits, learned starts, levels
+  unsigned long long* quality;  // 8 entries, see Q_*
 };
 
 struct shp_batch {
@@ -89,6 +93,12 @@ struct shp_batch {
   bool graph_ready = false;
   int32_t last_flags = 0;
   std::vector<void*> allocs;
+  // per-phase event timing (shp_batch_set_timing)
+  bool timing = false;
+  bool events_made = false;
+  hipEvent_t events[SHP_NUM_PHASES + 1]{};
+  double phase_ms[SHP_NUM_PHASES]{};
+  int32_t timed_steps = 0;
 };
 
 shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds, long NB, long NV,
@@ -98,3 +108,4 @@ void launch_solver(shp_batch* b);
 void launch_serial_solver(shp_batch* b);
 void launch_ordered_solver(shp_batch* b);
 void launch_colored_solver(shp_batch* b);
+void launch_quality(shp_batch* b);

@@ -36,12 +36,12 @@
 #include "shp_geom.h"
 #include "shp_trig.h"
 #include "shp_engine.h"
+#include "shp_solve_common.h"
 
 using namespace shp;
 
 // ---------------------------------------------------------------- helpers
 
-__device__ __forceinline__ bool body_static(const Ctx& c, int g) { return c.is_static[g] != 0; }
 
 // Key = ObjectFeatureKey ((i, j), (fi, fj)) (Constraints/Contact.hs:194-197)
 // packed so that unsigned order equals the derived lexicographic order.
@@ -67,6 +67,7 @@ __device__ __forceinline__ HullRef hull_of(const Ctx& c, int g) {
 
 __global__ void k_reset(Ctx c) {
   if (threadIdx.x < 8) c.counters[threadIdx.x] = 0;
+  if (threadIdx.x < 3) c.quality[threadIdx.x] = 0ull;
 }
 
 // listToHull in local space (ConvexHull.hs:151-167): unit edge normals and
@@ -355,108 +356,12 @@ __global__ void k_contacts(Ctx c) {
   }
 }
 
-// ---- ordered FP64 solver ---------------------------------------------------
-
-struct Vel6 {
-  double v[6];
-};
-
-__device__ __forceinline__ void load_vel(const Ctx& c, int ga, int gb, double* v) {
-  v[0] = c.vx[ga];
-  v[1] = c.vy[ga];
-  v[2] = c.w[ga];
-  v[3] = c.vx[gb];
-  v[4] = c.vy[gb];
-  v[5] = c.w[gb];
-}
-
-// Static bodies (both inverse masses 0) receive v + (+-0) in the reference,
-// which leaves a finite velocity unchanged (-0 is normalised at load), so
-// their velocities are never written.
-__device__ __forceinline__ void store_vel(const Ctx& c, int ga, int gb, const double* v) {
-  if (!body_static(c, ga)) {
-    c.vx[ga] = v[0];
-    c.vy[ga] = v[1];
-    c.w[ga] = v[2];
-  }
-  if (!body_static(c, gb)) {
-    c.vx[gb] = v[3];
-    c.vy[gb] = v[4];
-    c.w[gb] = v[5];
-  }
-}
-
-// applyLagrangian (Constraint.hs:182-221): v_k + ((j_k * l) * im_k)
-__device__ __forceinline__ void apply_lag(double l, const double* j, const double* im, double* v) {
-#pragma unroll
-  for (int k = 0; k < 6; ++k) v[k] = v[k] + ((j[k] * l) * im[k]);
-}
-
-__device__ __forceinline__ void load_rows(const Ctx& c, int k, double* jn, double* jt) {
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    jn[q] = c.jn[q * c.contact_cap + k];
-    jt[q] = c.jt[q * c.contact_cap + k];
-  }
-}
-
-// useCache branch of applyCachedSlns (Solvers/Contact.hs:100-114):
-// applySln = applyFriction . applyNonPen (:55-65)
-__device__ __forceinline__ void warm_apply(const Ctx& c, int k) {
-  int ga = c.ga[k], gb = c.gb[k];
-  double v[6], jn[6], jt[6];
-  double im[6] = {c.iml[ga], c.iml[ga], c.imr[ga], c.iml[gb], c.iml[gb], c.imr[gb]};
-  load_vel(c, ga, gb, v);
-  load_rows(c, k, jn, jt);
-  apply_lag(c.ln[k], jn, im, v);
-  apply_lag(c.lt[k], jt, im, v);
-  store_vel(c, ga, gb, v);
-}
-
-// improveContactSln (Solvers/Contact.hs:124-143): contactLagrangian with
-// bounceB (Constraints/Contact.hs:232-253, Restitution.hs:34-47), then
-// solutionProcessor = positive / clampAbs (Constraints/Contact.hs:259-268,
-// SolutionProcessors.hs:427-447), then applySln.
-__device__ __forceinline__ void improve(const Ctx& c, int k) {
-  int ga = c.ga[k], gb = c.gb[k];
-  double v[6], jn[6], jt[6];
-  double im[6] = {c.iml[ga], c.iml[ga], c.imr[ga], c.iml[gb], c.iml[gb], c.imr[gb]};
-  load_vel(c, ga, gb, v);
-  load_rows(c, k, jn, jt);
-  d2 ra = mk(c.rax[k], c.ray[k]), rb = mk(c.rbx[k], c.rby[k]);
-  d2 nr = c.flip[k] ? mk(-c.cnx[k], -c.cny[k]) : mk(c.cnx[k], c.cny[k]);
-  d2 closing = add(add(add(neg(mk(v[0], v[1])), zcross(-v[2], ra)), mk(v[3], v[4])), zcross(v[5], rb));
-  double e = hs_min(c.e[ga], c.e[gb]);
-  double bb = hs_min(0.0, e * dot(closing, nr));
-  double bn = c.bn[k] + bb;
-  double new_n = (-(dot6(jn, v) + bn)) / c.mcn[k];
-  double new_t = (-(dot6(jt, v) + 0.0)) / c.mct[k];
-  double ln = c.ln[k], lt = c.lt[k];
-  double app_n = hs_max(new_n, -ln);
-  double acc_n = ln + app_n;
-  double mu = (c.mu[ga] + c.mu[gb]) / 2.0;
-  double thr = acc_n * mu;
-  double acc_t = lt + new_t;
-  double accp = acc_t > thr ? thr : (acc_t < -thr ? -thr : acc_t);
-  double app_t = accp - lt;
-  apply_lag(app_n, jn, im, v);
-  apply_lag(app_t, jt, im, v);
-  store_vel(c, ga, gb, v);
-  c.ln[k] = acc_n;
-  c.lt[k] = accp;
-}
-
-// Reference order, one lane per world: the warm-start pass then
-// `iterations` improveWorld sweeps (Solvers/Contact.hs:146-157), each over
-// the world's contacts in descending key order.
+// Reference order, one lane per world (used for worlds whose schedule does
+// not fit the level-parallel solver).
 __global__ void k_solve_serial(Ctx c) {
   int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= c.W) return;
-  int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
-  for (int k = c0; k < c1; ++k)
-    if (c.warm[k]) warm_apply(c, k);
-  for (int it = 0; it < c.iterations; ++it)
-    for (int k = c0; k < c1; ++k) improve(c, k);
+  solve_world_serial(c, c.wcont_off[w], c.wcont_off[w + 1]);
 }
 
 // This frame's lagrangians become the next frame's EngineCache
@@ -514,23 +419,55 @@ static inline double nz(double x) { return x == 0.0 ? 0.0 : x; }
 
 static int blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// With timing enabled, an event is recorded on the batch's stream before
+// every launch and after the last one, so phase k's duration is
+// event[k+1] - event[k] (kernel time plus the launch gap).
+static inline void mark(shp_batch* b, int k) {
+  if (b->timing) (void)hipEventRecord(b->events[k], b->stream);
+}
+
 static shp_status enqueue_step(shp_batch* b) {
   Ctx& c = b->ctx;
   hipStream_t s = b->stream;
+  mark(b, 0);
   k_reset<<<1, 64, 0, s>>>(c);
+  mark(b, 1);
   k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 0);
+  mark(b, 2);
   size_t tb = b->scan_tmp_bytes;
   HIPCHK(hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, c.cnt_rank, c.off_rank, c.NB + 1, s));
+  mark(b, 3);
   k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 1);
+  mark(b, 4);
   k_sat<<<blocks(c.pair_cap + 1, 256), 256, 0, s>>>(c);
+  mark(b, 5);
   tb = b->scan_tmp_bytes;
   HIPCHK(hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, c.ccount, c.coff, c.pair_cap + 1, s));
+  mark(b, 6);
   k_world_ranges<<<blocks(c.W + 1, 256), 256, 0, s>>>(c);
+  mark(b, 7);
   k_contacts<<<blocks(c.pair_cap, 256), 256, 0, s>>>(c);
+  mark(b, 8);
   launch_solver(b);
+  mark(b, 9);
+  launch_quality(b);
+  mark(b, 10);
   k_cache_store<<<blocks(std::max(c.contact_cap, c.W + 1), 256), 256, 0, s>>>(c);
+  mark(b, 11);
   k_integrate<<<blocks(c.NB, 256), 256, 0, s>>>(c, 1);
+  mark(b, 12);
   HIPCHK(hipGetLastError());
+  return SHP_OK;
+}
+
+static shp_status accumulate_timing(shp_batch* b) {
+  HIPCHK(hipEventSynchronize(b->events[SHP_NUM_PHASES]));
+  for (int k = 0; k < SHP_NUM_PHASES; ++k) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, b->events[k], b->events[k + 1]));
+    b->phase_ms[k] += ms;
+  }
+  b->timed_steps += 1;
   return SHP_OK;
 }
 
@@ -732,6 +669,7 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
   TRY(dalloc(b, &c.cache_lt, CC));
   TRY(dalloc(b, &c.cache_woff, n_worlds + 1));
   TRY(dalloc(b, &c.counters, 8));
+  TRY(dalloc(b, &c.quality, 8));
   TRY(solver_alloc(b));
 
   size_t t1 = 0, t2 = 0;
@@ -746,6 +684,7 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
   k_fill_i32<<<blocks(n_worlds + 1, 256), 256, 0, s>>>(c.wcont_off, 0, n_worlds + 1);
   k_fill_i32<<<blocks(n_worlds + 1, 256), 256, 0, s>>>(c.wpair_off, 0, n_worlds + 1);
   k_fill_i32<<<1, 64, 0, s>>>(c.counters, 0, 8);
+  k_fill_i32<<<1, 64, 0, s>>>((int32_t*)c.quality, 0, 16);
   // makeWorldObj places each shape with moveShape (World.hs:169-176)
   k_init_hulls<<<blocks(NB, 256), 256, 0, s>>>(c);
   k_integrate<<<blocks(NB, 256), 256, 0, s>>>(c, 0);
@@ -769,7 +708,10 @@ static shp_status capture_graph(shp_batch* b) {
   if (b->graph_ready) return SHP_OK;
   hipGraph_t graph;
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  bool timing = b->timing;
+  b->timing = false;  // events are never captured
   shp_status st = enqueue_step(b);
+  b->timing = timing;
   hipError_t e = hipStreamEndCapture(b->stream, &graph);
   if (st != SHP_OK) return st;
   if (e != hipSuccess) return fail(SHP_E_HIP, "graph capture failed");
@@ -785,6 +727,8 @@ shp_status shp_batch_destroy(shp_batch* b) {
   if (!b) return SHP_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->graph_ready) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->events_made)
+    for (int k = 0; k <= SHP_NUM_PHASES; ++k) (void)hipEventDestroy(b->events[k]);
   for (void* p : b->allocs) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -799,8 +743,29 @@ shp_status shp_batch_step_async(shp_batch* b, int32_t n_steps, int32_t use_graph
       HIPCHK(hipGraphLaunch(b->graph_exec, b->stream));
     } else {
       TRY(enqueue_step(b));
+      if (b->timing) TRY(accumulate_timing(b));
     }
   }
+  return SHP_OK;
+}
+
+shp_status shp_batch_set_timing(shp_batch* b, int32_t enable) {
+  if (!b) return fail(SHP_E_INVAL, "null batch");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (enable && !b->events_made) {
+    for (int k = 0; k <= SHP_NUM_PHASES; ++k) HIPCHK(hipEventCreate(&b->events[k]));
+    b->events_made = true;
+  }
+  b->timing = enable != 0;
+  for (int k = 0; k < SHP_NUM_PHASES; ++k) b->phase_ms[k] = 0.0;
+  b->timed_steps = 0;
+  return SHP_OK;
+}
+
+shp_status shp_batch_read_timing(shp_batch* b, double* ms_per_phase, int32_t* steps) {
+  if (!b || !ms_per_phase || !steps) return fail(SHP_E_INVAL, "bad argument");
+  for (int k = 0; k < SHP_NUM_PHASES; ++k) ms_per_phase[k] = b->phase_ms[k];
+  *steps = b->timed_steps;
   return SHP_OK;
 }
 
@@ -928,6 +893,11 @@ shp_status shp_batch_last_stats(shp_batch* b, shp_step_stats* out) {
   out->learned_starts = cnt[CNT_LEARNED];
   out->max_levels = cnt[CNT_LEVELS];
   out->flags = cnt[CNT_FLAGS] | b->last_flags;
+  unsigned long long q[8];
+  HIPCHK(hipMemcpy(q, c.quality, sizeof(q), hipMemcpyDeviceToHost));
+  double* dst[6] = {&out->max_depth, &out->normal_residual, &out->friction_residual,
+                    &out->run_max_depth, &out->run_normal_residual, &out->run_friction_residual};
+  for (int k = 0; k < 6; ++k) memcpy(dst[k], &q[k], sizeof(double));
   return SHP_OK;
 }
 

@@ -59,7 +59,11 @@ class ConfigC(C.Structure):
 class StepStatsC(C.Structure):
     _fields_ = [("pairs", C.c_int64), ("contacts", C.c_int64),
                 ("warm_hits", C.c_int64), ("learned_starts", C.c_int64),
-                ("max_levels", C.c_int32), ("flags", C.c_int32)]
+                ("max_levels", C.c_int32), ("flags", C.c_int32),
+                ("max_depth", C.c_double), ("normal_residual", C.c_double),
+                ("friction_residual", C.c_double), ("run_max_depth", C.c_double),
+                ("run_normal_residual", C.c_double),
+                ("run_friction_residual", C.c_double)]
 
 
 EXPORTS = [
@@ -68,8 +72,12 @@ EXPORTS = [
     "shp_batch_read_bodies", "shp_batch_read_pairs", "shp_batch_read_contacts",
     "shp_batch_read_hulls", "shp_batch_last_stats", "shp_batch_set_cache",
     "shp_mlp_param_count", "shp_batch_set_mlp", "shp_last_error",
-    "shp_version",
+    "shp_version", "shp_batch_set_timing", "shp_batch_read_timing",
 ]
+
+PHASES = ["reset", "broadphase_count", "scan_pairs", "broadphase_fill", "sat",
+          "scan_contacts", "world_ranges", "contacts", "solve", "quality",
+          "cache_store", "integrate"]
 
 _lib = None
 
@@ -100,6 +108,8 @@ def load_library(path: str = LIB_PATH):
     L.shp_batch_set_mlp.argtypes = [C.c_void_p, _fp]
     L.shp_last_error.restype = C.c_char_p
     L.shp_version.restype = C.c_char_p
+    L.shp_batch_set_timing.argtypes = [C.c_void_p, C.c_int32]
+    L.shp_batch_read_timing.argtypes = [C.c_void_p, _dp, _ip]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L
@@ -232,6 +242,17 @@ class Batch:
         l = np.ascontiguousarray(lam, dtype=np.float64)
         _check(self._L.shp_batch_set_cache(self._h, world, len(k),
                                            k.ctypes.data_as(_ip), _dptr(l)))
+
+    def set_timing(self, enable: bool = True):
+        _check(self._L.shp_batch_set_timing(self._h, int(enable)))
+
+    def timing(self) -> dict:
+        """Average milliseconds per step of each phase since set_timing."""
+        ms = np.zeros(len(PHASES), dtype=np.float64)
+        n = C.c_int32()
+        _check(self._L.shp_batch_read_timing(self._h, _dptr(ms), C.byref(n)))
+        k = max(n.value, 1)
+        return {"steps": n.value, **{p: float(v) / k for p, v in zip(PHASES, ms)}}
 
     def set_mlp(self, params: np.ndarray):
         p = np.ascontiguousarray(params, dtype=np.float32)
