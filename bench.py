@@ -141,13 +141,50 @@ def cpu_baseline(scene, sample_steps):
     }
 
 
+def run_mode(scene, solver, warm, steps, warmup, graph=True, model=None):
+    """Warm up on a separate instance, then time `steps` steps from the
+    initial state; returns (seconds, run stats)."""
+    def make():
+        b = engine.make_batch(scene.worlds, scene.config, solver=solver, warm_start=warm)
+        if warm == engine.WARM_LEARNED:
+            b.set_mlp(model)
+        return b
+
+    if warmup > 0:
+        wb = make()
+        wb.step_async(warmup, use_graph=graph)
+        wb.sync()
+        wb.close()
+    b = make()
+    b.sync()
+    t0 = time.perf_counter()
+    b.step_async(steps, use_graph=graph)
+    b.sync()
+    elapsed = time.perf_counter() - t0
+    st = b.stats()
+    b.close()
+    return elapsed, st
+
+
+def quality(st):
+    return {"max_penetration": st["run_max_depth"],
+            "normal_residual": st["run_normal_residual"],
+            "friction_residual": st["run_friction_residual"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="polygon_pile")
-    ap.add_argument("--solver", default="ordered", choices=list(SOLVERS))
+    ap.add_argument("--solver", default="colored", choices=list(SOLVERS))
+    ap.add_argument("--warm", default="zero", choices=["zero", "learned"])
+    ap.add_argument("--iterations", type=int, default=0,
+                    help="solver sweeps (default: the workload's)")
+    ap.add_argument("--model", default=os.path.join(ROOT, "shapes_amd", "warmstart_mlp.npy"))
+    ap.add_argument("--compare", action="store_true",
+                    help="also run the other solver / warm-start modes")
     ap.add_argument("--cpu-sample-steps", type=int, default=100)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="steps of the instrumented replay (default: --steps)")
@@ -161,26 +198,20 @@ def main():
         sys.exit(2)
 
     scene = make_scene(args.workload)
+    if args.iterations:
+        scene.config.iterations = args.iterations
     solver = SOLVERS[args.solver]
+    warm = engine.WARM_LEARNED if args.warm == "learned" else engine.WARM_ZERO
+    model = None
+    if warm == engine.WARM_LEARNED or args.compare:
+        if os.path.exists(args.model):
+            model = np.load(args.model)
+        elif warm == engine.WARM_LEARNED:
+            raise SystemExit(f"bench.py: no model at {args.model}")
     dyn = scene.n_dynamic
 
-    # warm-up on a separate instance (kernels loaded, graph path exercised)
-    if args.warmup > 0:
-        wb = engine.make_batch(scene.worlds, scene.config, solver=solver)
-        wb.step_async(args.warmup, use_graph=not args.no_graph)
-        wb.sync()
-        wb.close()
-
-    b = engine.make_batch(scene.worlds, scene.config, solver=solver)
-    b.sync()
-    t0 = time.perf_counter()
-    b.step_async(args.steps, use_graph=not args.no_graph)
-    b.sync()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    last = b.stats()
-    b.close()
-
+    elapsed, last = run_mode(scene, solver, warm, args.steps, args.warmup,
+                             graph=not args.no_graph, model=model)
     value = dyn * args.steps / elapsed
     prof_steps = args.profile_steps or args.steps
     roof, phases, avg = roofline(scene, args.solver, prof_steps)
@@ -206,15 +237,36 @@ def main():
             "iterations": scene.config.iterations,
             "dt": scene.config.dt,
             "solver": args.solver,
+            "warm_start": args.warm,
             "graph": not args.no_graph,
         },
+        "quality": quality(last),
         "roofline": roof,
         "phase_ms": phases,
         "avg_pairs": round(avg["pairs"], 1),
         "avg_contacts": round(avg["contacts"], 1),
         "last_step": {"pairs": last["pairs"], "contacts": last["contacts"],
-                      "warm_hits": last["warm_hits"], "max_levels": last["max_levels"]},
+                      "warm_hits": last["warm_hits"], "misses": last["misses"],
+                      "learned_starts": last["learned_starts"],
+                      "max_levels": last["max_levels"]},
     }
+    if args.compare:
+        modes = []
+        base_it = scene.config.iterations
+        plan = [("ordered", "zero", base_it), ("colored", "zero", base_it)]
+        if model is not None:
+            plan += [("colored", "learned", base_it // 2), ("colored", "learned", base_it // 4),
+                     ("colored", "zero", base_it // 2), ("colored", "zero", base_it // 4)]
+        for sv, wm, it in plan:
+            scene.config.iterations = it
+            el, st = run_mode(scene, SOLVERS[sv], engine.WARM_LEARNED if wm == "learned"
+                              else engine.WARM_ZERO, args.steps, args.warmup,
+                              graph=not args.no_graph, model=model)
+            modes.append({"solver": sv, "warm_start": wm, "iterations": it,
+                          "body_steps_per_s": round(dyn * args.steps / el, 1),
+                          "ms_per_step": round(1e3 * el / args.steps, 4), **quality(st)})
+        scene.config.iterations = base_it
+        out["modes"] = modes
     if not args.no_cpu and len(scene.worlds) == 1:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_sample_steps)
     print(json.dumps(out), flush=True)

@@ -92,6 +92,7 @@ typedef struct shp_step_stats {
   int64_t contacts;
   int64_t warm_hits;
   int64_t learned_starts;
+  int64_t misses;       /* contacts absent from last frame's cache */
   int32_t max_levels;   /* deepest per-world schedule (levels or colours) */
   int32_t flags;        /* bit 0 pair overflow, 1 contact overflow,
                            2 non-finite, 3 schedule fell back to serial */
@@ -160,10 +161,49 @@ shp_status shp_batch_set_cache(shp_batch* b, int32_t world, int32_t count,
 /* ---- learned warm start (no reference counterpart: the reference always
  * zero-starts new contacts, Solvers/Contact.hs:95) ------------------------ */
 
-/* Parameter blob layout: see shp_mlp_param_count.  Weights are FP32 on the
- * host; the device keeps BF16 weights for inference. */
+/* A 14-128-128-128-2 SiLU MLP.  Parameter blob (shp_mlp_param_count() =
+ * 35202 FP32 values, weights row-major [in][out]): W1 14x128, b1, W2 128x128,
+ * b2, W3 128x128, b3, W4 128x2, b4.  Inputs are 14 features of a contact
+ * that missed the cache (normal, depth, relative normal/tangent velocity,
+ * inverse masses and inertias, lever arms, mean friction); the two outputs
+ * times the inverse effective masses' reciprocals give the initial normal
+ * and friction impulses, projected onto lambda_n >= 0, |lambda_t| <=
+ * mu*lambda_n.  The device keeps BF16 weights and accumulates in FP32. */
 int32_t shp_mlp_param_count(void);
+/* Load a model into a batch; used when the batch's warm_start is
+ * SHP_WARM_LEARNED (without a model, misses start at zero). */
 shp_status shp_batch_set_mlp(shp_batch* b, const float* params);
+/* Record (features[14], targets[2]) of every cache miss after each step's
+ * solve, up to `capacity` samples (0 disables).  Targets are the final
+ * lambdas times the rows' inverse effective masses. */
+shp_status shp_batch_set_logging(shp_batch* b, int64_t capacity);
+/* Copy up to *count logged samples (feats n x 14, targets n x 2); NULL
+ * outputs return the number logged in *count. */
+shp_status shp_batch_read_log(shp_batch* b, float* feats, float* targets,
+                              int64_t* count);
+
+/* Training and evaluation of the model on the GPU (BF16 matrix cores, FP32
+ * accumulation, Adam on FP32 master weights). */
+typedef struct shp_mlp shp_mlp;
+shp_status shp_mlp_create(const float* params, shp_mlp** out);
+shp_status shp_mlp_destroy(shp_mlp* m);
+shp_status shp_mlp_get_params(shp_mlp* m, float* params);
+/* out[2*i + k] = output k of row i (n x 14 features) */
+shp_status shp_mlp_forward(shp_mlp* m, const float* feats, int32_t n,
+                           float* out);
+/* Gradient of the mean squared error over n rows w.r.t. every parameter
+ * (blob layout) and the loss itself. */
+shp_status shp_mlp_gradient(shp_mlp* m, const float* feats,
+                            const float* targets, int32_t n, float* grad,
+                            float* loss);
+/* Upload a training set of n samples. */
+shp_status shp_mlp_set_data(shp_mlp* m, const float* feats,
+                            const float* targets, int64_t n);
+/* `steps` Adam steps (learning rate lr) on mini-batches of `batch` rows
+ * (a multiple of 64) given by row indices (steps x batch) into the training
+ * set; loss_out[s] (may be NULL) is step s's batch loss before the update. */
+shp_status shp_mlp_train(shp_mlp* m, const int32_t* indices, int32_t steps,
+                         int32_t batch, float lr, float* loss_out);
 
 /* ---- per-phase timing (no counterpart in the reference, whose only
  * timing is the criterion benchmark, shapes/bench/Main.hs:19-26) --------- */

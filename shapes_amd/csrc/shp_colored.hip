@@ -128,8 +128,8 @@ __device__ void improve_f32(const Ctx& c, int k) {
 // One workgroup per world.
 __global__ __launch_bounds__(COLOR_THREADS) void k_solve_colored(Ctx c) {
   __shared__ uint32_t s_best[COLOR_BODIES];
-  __shared__ int s_off[COLOR_CAP + 2];
-  __shared__ int s_cur[COLOR_CAP + 2];
+  __shared__ int s_off[COLOR_CAP + 3];  // colours 0..COLOR_CAP (serial tail) + end
+  __shared__ int s_cur[COLOR_CAP + 3];
   __shared__ int s_left, s_taken;
   int w = blockIdx.x, tid = threadIdx.x;
   int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
@@ -227,6 +227,7 @@ __global__ __launch_bounds__(COLOR_THREADS) void k_solve_colored(Ctx c) {
         for (int i = s + (tail ? 0 : tid); i < e; i += step) {
           int p = c.sched_order[p0 + i];
           int k0 = c.coff[p], k1 = k0 + c.ccount[p];
+          if (k1 > c1) k1 = c1;  // contacts beyond the capacity were not stored
           for (int k = k0; k < k1; ++k) {
             if (pass == 0) {
               if (c.warm[k]) warm_apply_f32(c, k);

@@ -14,8 +14,9 @@
 #include <vector>
 
 #include "../../include/shapes_gpu.h"
+#include "shp_mlp.h"
 
-enum { CNT_FLAGS = 0, CNT_WARM = 1, CNT_LEARNED = 2, CNT_LEVELS = 3 };
+enum { CNT_FLAGS = 0, CNT_WARM = 1, CNT_LEARNED = 2, CNT_LEVELS = 3, CNT_MISS = 4 };
 // solve-quality maxima (doubles stored as their bit patterns): last step,
 // then since creation
 enum { Q_DEPTH = 0, Q_NRES = 1, Q_FRES = 2, Q_RUN_DEPTH = 3, Q_RUN_NRES = 4, Q_RUN_FRES = 5 };
@@ -73,10 +74,18 @@ struct Ctx {
   int max_levels;
   // coloured solver state (FP32 velocities, FP64 accumulators)
   float *fvx, *fvy, *fw;
-  // learned warm start (BF16 weights as uint16)
-  const uint16_t* mlp_w;
-  const float* mlp_b;
+  // learned warm start: padded FP32 master / BF16 weights, the step's
+  // cache misses (contact index + 14 features) and the training-data log
+  const float* mlp_w;
+  const uint16_t* mlp_wbf;
   int mlp_ready;
+  int collect_misses;
+  int miss_cap;
+  int32_t* miss_idx;
+  float* miss_feat;
+  long log_cap;
+  float *log_feat, *log_tgt;
+  unsigned long long* log_count;
   int32_t* counters;  // 8 ints: flags, warm hits, learned starts, levels
   unsigned long long* quality;  // 8 entries, see Q_*
 };
@@ -97,9 +106,13 @@ struct shp_batch {
   bool timing = false;
   bool events_made = false;
   hipEvent_t events[SHP_NUM_PHASES + 1]{};
+  MlpParams mlp{};
+  bool logging = false;
   double phase_ms[SHP_NUM_PHASES]{};
   int32_t timed_steps = 0;
 };
+
+void shp_set_error(const std::string& msg);
 
 shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds, long NB, long NV,
                        long pair_cap);

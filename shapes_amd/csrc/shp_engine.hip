@@ -349,9 +349,21 @@ __global__ void k_contacts(Ctx c) {
       c.warm[dst] = 1;
       atomicAdd(&c.counters[CNT_WARM], 1);
     } else {
+      // newCache (Solvers/Contact.hs:91-99): zero start; the learned warm
+      // start may replace it before the solve
       c.ln[dst] = 0.0;
       c.lt[dst] = 0.0;
       c.warm[dst] = 0;
+      if (c.collect_misses) {
+        int slot = atomicAdd(&c.counters[CNT_MISS], 1);
+        if (slot < c.miss_cap) {
+          c.miss_idx[slot] = dst;
+          double v[6] = {c.vx[ga], c.vy[ga], c.w[ga], c.vx[gb], c.vy[gb], c.w[gb]};
+          double sgn = flip ? -1.0 : 1.0;
+          mlp_features(sgn * nn.x, sgn * nn.y, depth, v, ra.x, ra.y, rb.x, rb.y, c.iml[ga], c.iml[gb],
+                       c.imr[ga], c.imr[gb], (c.mu[ga] + c.mu[gb]) / 2.0, c.miss_feat + (size_t)slot * MLP_IN);
+        }
+      }
     }
   }
 }
@@ -384,6 +396,8 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int n) {
 // ---------------------------------------------------------------- host side
 
 static thread_local std::string g_err;
+
+void shp_set_error(const std::string& msg) { g_err = msg; }
 
 static shp_status fail(shp_status s, const std::string& msg) {
   g_err = msg;
@@ -447,10 +461,12 @@ static shp_status enqueue_step(shp_batch* b) {
   k_world_ranges<<<blocks(c.W + 1, 256), 256, 0, s>>>(c);
   mark(b, 7);
   k_contacts<<<blocks(c.pair_cap, 256), 256, 0, s>>>(c);
+  if (c.warm_mode == SHP_WARM_LEARNED && c.mlp_ready) mlp_launch_infer(b);
   mark(b, 8);
   launch_solver(b);
   mark(b, 9);
   launch_quality(b);
+  if (b->logging) mlp_launch_log(b);
   mark(b, 10);
   k_cache_store<<<blocks(std::max(c.contact_cap, c.W + 1), 256), 256, 0, s>>>(c);
   mark(b, 11);
@@ -668,6 +684,10 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
   TRY(dalloc(b, &c.cache_ln, CC));
   TRY(dalloc(b, &c.cache_lt, CC));
   TRY(dalloc(b, &c.cache_woff, n_worlds + 1));
+  c.miss_cap = (int)std::min<long>(2 * pair_cap, 4 * NB + 4096);
+  TRY(dalloc(b, &c.miss_idx, c.miss_cap));
+  TRY(dalloc(b, &c.miss_feat, (long)c.miss_cap * MLP_IN));
+  TRY(dalloc(b, &c.log_count, 1));
   TRY(dalloc(b, &c.counters, 8));
   TRY(dalloc(b, &c.quality, 8));
   TRY(solver_alloc(b));
@@ -685,6 +705,7 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
   k_fill_i32<<<blocks(n_worlds + 1, 256), 256, 0, s>>>(c.wpair_off, 0, n_worlds + 1);
   k_fill_i32<<<1, 64, 0, s>>>(c.counters, 0, 8);
   k_fill_i32<<<1, 64, 0, s>>>((int32_t*)c.quality, 0, 16);
+  k_fill_i32<<<1, 64, 0, s>>>((int32_t*)c.log_count, 0, 2);
   // makeWorldObj places each shape with moveShape (World.hs:169-176)
   k_init_hulls<<<blocks(NB, 256), 256, 0, s>>>(c);
   k_integrate<<<blocks(NB, 256), 256, 0, s>>>(c, 0);
@@ -729,6 +750,7 @@ shp_status shp_batch_destroy(shp_batch* b) {
   if (b->graph_ready) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->events_made)
     for (int k = 0; k <= SHP_NUM_PHASES; ++k) (void)hipEventDestroy(b->events[k]);
+  mlp_params_free(&b->mlp);
   for (void* p : b->allocs) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -891,6 +913,7 @@ shp_status shp_batch_last_stats(shp_batch* b, shp_step_stats* out) {
   out->contacts = cr[0];
   out->warm_hits = cnt[CNT_WARM];
   out->learned_starts = cnt[CNT_LEARNED];
+  out->misses = cnt[CNT_MISS];
   out->max_levels = cnt[CNT_LEVELS];
   out->flags = cnt[CNT_FLAGS] | b->last_flags;
   unsigned long long q[8];
@@ -898,6 +921,54 @@ shp_status shp_batch_last_stats(shp_batch* b, shp_step_stats* out) {
   double* dst[6] = {&out->max_depth, &out->normal_residual, &out->friction_residual,
                     &out->run_max_depth, &out->run_normal_residual, &out->run_friction_residual};
   for (int k = 0; k < 6; ++k) memcpy(dst[k], &q[k], sizeof(double));
+  return SHP_OK;
+}
+
+shp_status shp_batch_set_mlp(shp_batch* b, const float* params) {
+  if (!b || !params) return fail(SHP_E_INVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (!b->mlp.w) TRY(mlp_params_alloc(&b->mlp));
+  TRY(mlp_params_upload(&b->mlp, params, b->stream));
+  b->ctx.mlp_w = b->mlp.w;
+  b->ctx.mlp_wbf = b->mlp.wbf;
+  b->ctx.mlp_ready = 1;
+  b->ctx.collect_misses = (b->cfg.warm_start == SHP_WARM_LEARNED) || b->logging;
+  b->graph_ready = false;  // kernel arguments changed
+  return SHP_OK;
+}
+
+shp_status shp_batch_set_logging(shp_batch* b, int64_t capacity) {
+  if (!b || capacity < 0) return fail(SHP_E_INVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  Ctx& c = b->ctx;
+  if (capacity > c.log_cap) {
+    TRY(dalloc(b, &c.log_feat, capacity * MLP_IN));
+    TRY(dalloc(b, &c.log_tgt, capacity * 2));
+    c.log_cap = capacity;
+  }
+  unsigned long long zero = 0;
+  HIPCHK(hipMemcpy(c.log_count, &zero, 8, hipMemcpyHostToDevice));
+  b->logging = capacity > 0;
+  c.collect_misses = (b->cfg.warm_start == SHP_WARM_LEARNED && c.mlp_ready) || b->logging;
+  b->graph_ready = false;
+  return SHP_OK;
+}
+
+shp_status shp_batch_read_log(shp_batch* b, float* feats, float* targets, int64_t* count) {
+  if (!b || !count) return fail(SHP_E_INVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  unsigned long long n = 0;
+  HIPCHK(hipMemcpy(&n, b->ctx.log_count, 8, hipMemcpyDeviceToHost));
+  if ((long)n > b->ctx.log_cap) n = b->ctx.log_cap;
+  if (feats && targets && *count > 0) {
+    long m = (long)n < *count ? (long)n : *count;
+    if (m > 0) {
+      HIPCHK(hipMemcpy(feats, b->ctx.log_feat, m * MLP_IN * 4, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(targets, b->ctx.log_tgt, m * 2 * 4, hipMemcpyDeviceToHost));
+    }
+    n = m;
+  }
+  *count = (int64_t)n;
   return SHP_OK;
 }
 

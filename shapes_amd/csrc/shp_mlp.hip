@@ -415,9 +415,85 @@ void mlp_launch_forward(const MlpParams& p, const uint16_t* X, int rows, float* 
   k_mlp_forward<<<grid, 256, 0, s>>>(mlp_view(p), X, rows, Y, tb, 0);
 }
 
-// ---- training handle -------------------------------------------------------
+// ---- engine hooks ------------------------------------------------------------
 
-static thread_local std::string g_mlp_err;
+// Learned warm start of the step's cache misses: tiles of 64 misses, the
+// features rounded to BF16, the forward pass on the matrix cores, then the
+// output head lambda_n = max(y0, 0) / mn, lambda_t = y1 / mt projected onto
+// |lambda_t| <= mu * lambda_n.  The contact is marked warm (2) so the
+// solver applies the predicted impulses like a cached solution; a
+// non-finite prediction keeps the zero start.
+__global__ __launch_bounds__(256) void k_mlp_infer(Ctx c, MlpDev m) {
+  __shared__ uint16_t sx[TILE * 16];
+  __shared__ uint16_t s1[TILE * 128];
+  __shared__ uint16_t s2[TILE * 128];
+  __shared__ float sy[TILE * 16];
+  int n = c.counters[CNT_MISS];
+  if (n > c.miss_cap) n = c.miss_cap;
+  for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
+    int row0 = t * TILE;
+    for (int i = threadIdx.x; i < TILE * 16; i += 256) {
+      int r = i >> 4, q = i & 15;
+      float f = (row0 + r < n && q < MLP_IN) ? c.miss_feat[(size_t)(row0 + r) * MLP_IN + q] : 0.0f;
+      sx[i] = f2bf(f);
+    }
+    __syncthreads();
+    forward_tile(m, sx, s1, s2, sy, nullptr, row0);
+    if (threadIdx.x < TILE && row0 + (int)threadIdx.x < n) {
+      int r = threadIdx.x;
+      int k = c.miss_idx[row0 + r];
+      double ln = (double)sy[r * 16];
+      ln = ln > 0.0 ? ln / c.mcn[k] : 0.0;
+      double lt = (double)sy[r * 16 + 1] / c.mct[k];
+      double mu = (c.mu[c.ga[k]] + c.mu[c.gb[k]]) / 2.0;
+      double thr = ln * mu;
+      lt = lt > thr ? thr : (lt < -thr ? -thr : lt);
+      if (ln - ln == 0.0 && lt - lt == 0.0) {
+        c.ln[k] = ln;
+        c.lt[k] = lt;
+        c.warm[k] = 2;
+        atomicAdd(&c.counters[CNT_LEARNED], 1);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// After the solve: append (features, lambda_n * mn, lambda_t * mt) of the
+// step's cache misses to the training log.
+__global__ void k_mlp_log(Ctx c) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = c.counters[CNT_MISS];
+  if (n > c.miss_cap) n = c.miss_cap;
+  if (s >= n) return;
+  unsigned long long pos = atomicAdd(c.log_count, 1ull);
+  if ((long)pos >= c.log_cap) return;
+  int k = c.miss_idx[s];
+  for (int q = 0; q < MLP_IN; ++q) c.log_feat[pos * MLP_IN + q] = c.miss_feat[(size_t)s * MLP_IN + q];
+  c.log_tgt[pos * 2] = (float)(c.ln[k] * c.mcn[k]);
+  c.log_tgt[pos * 2 + 1] = (float)(c.lt[k] * c.mct[k]);
+}
+
+void mlp_launch_infer(shp_batch* b) {
+  MlpDev m;
+  m.w1 = b->ctx.mlp_wbf + MLP_OFF_W1;
+  m.w2 = b->ctx.mlp_wbf + MLP_OFF_W2;
+  m.w3 = b->ctx.mlp_wbf + MLP_OFF_W3;
+  m.w4 = b->ctx.mlp_wbf + MLP_OFF_W4;
+  m.b1 = b->ctx.mlp_w + MLP_OFF_B1;
+  m.b2 = b->ctx.mlp_w + MLP_OFF_B2;
+  m.b3 = b->ctx.mlp_w + MLP_OFF_B3;
+  m.b4 = b->ctx.mlp_w + MLP_OFF_B4;
+  int tiles = (b->ctx.miss_cap + TILE - 1) / TILE;
+  int grid = tiles < 512 ? tiles : 512;
+  k_mlp_infer<<<grid, 256, 0, b->stream>>>(b->ctx, m);
+}
+
+void mlp_launch_log(shp_batch* b) {
+  k_mlp_log<<<(b->ctx.miss_cap + 255) / 256, 256, 0, b->stream>>>(b->ctx);
+}
+
+// ---- training handle -------------------------------------------------------
 
 struct shp_mlp {
   hipStream_t stream = nullptr;
@@ -438,7 +514,7 @@ struct shp_mlp {
 };
 
 static shp_status mlp_fail(shp_status s, const char* m) {
-  g_mlp_err = m;
+  shp_set_error(m);
   return s;
 }
 
@@ -498,8 +574,6 @@ static shp_status mlp_grad_rows(shp_mlp* h, int n) {
 extern "C" {
 
 int32_t shp_mlp_param_count(void) { return MLP_NPARAM; }
-
-const char* shp_mlp_last_error(void) { return g_mlp_err.c_str(); }
 
 shp_status shp_mlp_create(const float* params, shp_mlp** out) {
   if (!params || !out) return mlp_fail(SHP_E_INVAL, "null argument");
@@ -635,6 +709,8 @@ shp_status shp_mlp_train(shp_mlp* h, const int32_t* indices, int32_t steps, int3
                          float* loss_out) {
   if (!h || !indices || steps <= 0 || batch <= 0 || batch % TILE) return mlp_fail(SHP_E_INVAL, "bad argument");
   if (!h->dX) return mlp_fail(SHP_E_INVAL, "no training data");
+  for (long i = 0; i < (long)steps * batch; ++i)
+    if (indices[i] < 0 || indices[i] >= h->n_data) return mlp_fail(SHP_E_INVAL, "training index out of range");
   shp_status st = mlp_reserve(h, batch);
   if (st != SHP_OK) return st;
   int32_t* didx = nullptr;

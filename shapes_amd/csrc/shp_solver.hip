@@ -163,14 +163,3 @@ void launch_ordered_solver(shp_batch* b) {
   k_solve_levels<<<b->ctx.W, SOLVE_THREADS, 0, b->stream>>>(b->ctx);
 }
 
-extern "C" {
-
-int32_t shp_mlp_param_count(void) { return 0; }
-
-shp_status shp_batch_set_mlp(shp_batch* b, const float* params) {
-  (void)b;
-  (void)params;
-  return SHP_E_UNSUPPORTED;
-}
-
-}  // extern "C"

@@ -59,6 +59,7 @@ class ConfigC(C.Structure):
 class StepStatsC(C.Structure):
     _fields_ = [("pairs", C.c_int64), ("contacts", C.c_int64),
                 ("warm_hits", C.c_int64), ("learned_starts", C.c_int64),
+                ("misses", C.c_int64),
                 ("max_levels", C.c_int32), ("flags", C.c_int32),
                 ("max_depth", C.c_double), ("normal_residual", C.c_double),
                 ("friction_residual", C.c_double), ("run_max_depth", C.c_double),
@@ -73,6 +74,9 @@ EXPORTS = [
     "shp_batch_read_hulls", "shp_batch_last_stats", "shp_batch_set_cache",
     "shp_mlp_param_count", "shp_batch_set_mlp", "shp_last_error",
     "shp_version", "shp_batch_set_timing", "shp_batch_read_timing",
+    "shp_batch_set_logging", "shp_batch_read_log", "shp_mlp_create",
+    "shp_mlp_destroy", "shp_mlp_get_params", "shp_mlp_forward",
+    "shp_mlp_gradient", "shp_mlp_set_data", "shp_mlp_train",
 ]
 
 PHASES = ["reset", "broadphase_count", "scan_pairs", "broadphase_fill", "sat",
@@ -108,6 +112,15 @@ def load_library(path: str = LIB_PATH):
     L.shp_batch_set_mlp.argtypes = [C.c_void_p, _fp]
     L.shp_last_error.restype = C.c_char_p
     L.shp_version.restype = C.c_char_p
+    L.shp_batch_set_logging.argtypes = [C.c_void_p, C.c_int64]
+    L.shp_batch_read_log.argtypes = [C.c_void_p, _fp, _fp, C.POINTER(C.c_int64)]
+    L.shp_mlp_create.argtypes = [_fp, C.POINTER(C.c_void_p)]
+    L.shp_mlp_destroy.argtypes = [C.c_void_p]
+    L.shp_mlp_get_params.argtypes = [C.c_void_p, _fp]
+    L.shp_mlp_forward.argtypes = [C.c_void_p, _fp, C.c_int32, _fp]
+    L.shp_mlp_gradient.argtypes = [C.c_void_p, _fp, _fp, C.c_int32, _fp, _fp]
+    L.shp_mlp_set_data.argtypes = [C.c_void_p, _fp, _fp, C.c_int64]
+    L.shp_mlp_train.argtypes = [C.c_void_p, _ip, C.c_int32, C.c_int32, C.c_float, _fp]
     L.shp_batch_set_timing.argtypes = [C.c_void_p, C.c_int32]
     L.shp_batch_read_timing.argtypes = [C.c_void_p, _dp, _ip]
     for name in EXPORTS:
@@ -256,7 +269,88 @@ class Batch:
 
     def set_mlp(self, params: np.ndarray):
         p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == MLP_NPARAM
         _check(self._L.shp_batch_set_mlp(self._h, p.ctypes.data_as(_fp)))
+
+    def set_logging(self, capacity: int):
+        _check(self._L.shp_batch_set_logging(self._h, int(capacity)))
+
+    def read_log(self):
+        """(features n x 14, targets n x 2) logged since set_logging."""
+        n = C.c_int64(0)
+        _check(self._L.shp_batch_read_log(self._h, None, None, C.byref(n)))
+        f = np.empty((n.value, MLP_IN), dtype=np.float32)
+        t = np.empty((n.value, 2), dtype=np.float32)
+        if n.value:
+            _check(self._L.shp_batch_read_log(self._h, f.ctypes.data_as(_fp),
+                                              t.ctypes.data_as(_fp), C.byref(n)))
+        return f[:n.value], t[:n.value]
+
+
+MLP_IN = 14
+MLP_NPARAM = 35202
+
+
+class Mlp:
+    """The warm-start network on the GPU (shp_mlp_*): BF16 matrix-core
+    forward/backward, Adam on FP32 master weights."""
+
+    def __init__(self, params: np.ndarray):
+        self._L = load_library()
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == MLP_NPARAM
+        h = C.c_void_p()
+        _check(self._L.shp_mlp_create(p.ctypes.data_as(_fp), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.shp_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.empty(MLP_NPARAM, dtype=np.float32)
+        _check(self._L.shp_mlp_get_params(self._h, out.ctypes.data_as(_fp)))
+        return out
+
+    def forward(self, feats) -> np.ndarray:
+        f = np.ascontiguousarray(feats, dtype=np.float32)
+        out = np.empty((len(f), 2), dtype=np.float32)
+        _check(self._L.shp_mlp_forward(self._h, f.ctypes.data_as(_fp), len(f),
+                                       out.ctypes.data_as(_fp)))
+        return out
+
+    def gradient(self, feats, targets):
+        f = np.ascontiguousarray(feats, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        g = np.empty(MLP_NPARAM, dtype=np.float32)
+        loss = C.c_float()
+        _check(self._L.shp_mlp_gradient(self._h, f.ctypes.data_as(_fp),
+                                        t.ctypes.data_as(_fp), len(f),
+                                        g.ctypes.data_as(_fp), C.byref(loss)))
+        return g, loss.value
+
+    def set_data(self, feats, targets):
+        f = np.ascontiguousarray(feats, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        _check(self._L.shp_mlp_set_data(self._h, f.ctypes.data_as(_fp),
+                                        t.ctypes.data_as(_fp), len(f)))
+
+    def train(self, indices: np.ndarray, batch: int, lr: float,
+              want_loss: bool = False):
+        idx = np.ascontiguousarray(indices, dtype=np.int32)
+        steps = idx.size // batch
+        loss = np.empty(steps, dtype=np.float32) if want_loss else None
+        _check(self._L.shp_mlp_train(self._h, idx.ctypes.data_as(_ip), steps, batch,
+                                     float(lr),
+                                     loss.ctypes.data_as(_fp) if want_loss else None))
+        return loss
 
 
 def make_batch(worlds, config, **kw) -> Batch:
