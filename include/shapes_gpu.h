@@ -41,6 +41,11 @@ typedef int32_t shp_status;
 #define SHP_SOLVER_COLORED_F32 1 /* graph-coloured Gauss-Seidel, FP32 math,
                                     FP64 impulse accumulators */
 
+/* Broadphase algorithms (same pair set and order) */
+#define SHP_BROADPHASE_AUTO 0 /* grid when a world has more than 2048 bodies */
+#define SHP_BROADPHASE_SCAN 1 /* one wavefront per body scans j < i */
+#define SHP_BROADPHASE_GRID 2 /* uniform grid + explicit large static bodies */
+
 /* Warm-start modes for contacts absent from last frame's cache */
 #define SHP_WARM_ZERO 0    /* the reference: start at 0
                               (shapes/src/Physics/Solvers/Contact.hs:95) */
@@ -82,6 +87,7 @@ typedef struct shp_config {
   int32_t solver;             /* SHP_SOLVER_* */
   int32_t warm_start;         /* SHP_WARM_* */
   int32_t max_pairs_per_body; /* capacity; 0 = default (16) */
+  int32_t broadphase;         /* SHP_BROADPHASE_* */
 } shp_config;
 
 typedef struct shp_batch shp_batch;

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libshapes_gpu.so")
-SOURCES = ["shp_engine.hip", "shp_solver.hip", "shp_colored.hip", "shp_mlp.hip"]
+SOURCES = ["shp_engine.hip", "shp_solver.hip", "shp_colored.hip", "shp_mlp.hip", "shp_grid.hip"]
 HEADERS = ["shp_engine.h", "shp_geom.h", "shp_trig.h", "shp_solve_common.h", "shp_mlp.h",
            os.path.join("..", "..", "include", "shapes_gpu.h")]
 

@@ -69,9 +69,12 @@ struct Ctx {
   // scheduling (ordered: levels; coloured: colours)
   int32_t* sched_level;   // contact_cap
   int32_t* sched_order;   // contact_cap: contacts sorted by level in world
-  int32_t* sched_woff;    // W + 1: level-bucket offsets base per world
-  int32_t* sched_bucket;  // per world: level offsets (max_levels + 1 each)
-  int max_levels;
+  // contact constants packed in schedule order (FP64 records for the
+  // ordered solver, FP32 records + FP64 lambdas for the coloured one)
+  double* pk;
+  int32_t* pk_g;  // 2 x contact_cap: bodies a, b
+  int32_t* pk_k;  // contact index of a slot (coloured)
+  uint8_t* pk_f;  // flags: 1 a static, 2 b static, 4 warm, 8 pair continues
   // coloured solver state (FP32 velocities, FP64 accumulators)
   float *fvx, *fvy, *fw;
   // learned warm start: padded FP32 master / BF16 weights, the step's
@@ -86,6 +89,16 @@ struct Ctx {
   long log_cap;
   float *log_feat, *log_tgt;
   unsigned long long* log_count;
+  // uniform-grid broadphase (use_grid): per-world cell size and hash-table
+  // offsets, per-body cell, large static bodies listed per world
+  int use_grid;
+  int grid_buckets;
+  double* cell_h;
+  uint8_t* is_large;
+  int32_t* body_cell;
+  int32_t* tab_off;
+  int32_t *bucket_cnt, *bucket_start, *bucket_fill, *bucket_items;
+  int32_t *large_off, *large_ids;
   int32_t* counters;  // 8 ints: flags, warm hits, learned starts, levels
   unsigned long long* quality;  // 8 entries, see Q_*
 };
@@ -108,6 +121,10 @@ struct shp_batch {
   hipEvent_t events[SHP_NUM_PHASES + 1]{};
   MlpParams mlp{};
   bool logging = false;
+  int ppb = 16;             // pair capacity per body
+  bool solve_vlds = false;  // ordered solver keeps velocities in LDS
+  size_t solve_lds = 0;     // dynamic LDS bytes of the ordered solver
+  size_t color_lds = 0;     // dynamic LDS bytes of the coloured solver
   double phase_ms[SHP_NUM_PHASES]{};
   int32_t timed_steps = 0;
 };
@@ -117,8 +134,12 @@ void shp_set_error(const std::string& msg);
 shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds, long NB, long NV,
                        long pair_cap);
 shp_status solver_alloc(shp_batch* b);
+shp_status colored_alloc(shp_batch* b);
 void launch_solver(shp_batch* b);
 void launch_serial_solver(shp_batch* b);
 void launch_ordered_solver(shp_batch* b);
 void launch_colored_solver(shp_batch* b);
 void launch_quality(shp_batch* b);
+hipError_t launch_grid_count(shp_batch* b);
+hipError_t launch_grid_emit(shp_batch* b);
+size_t grid_scan_bytes(shp_batch* b);

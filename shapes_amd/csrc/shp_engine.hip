@@ -29,6 +29,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -268,12 +269,7 @@ __global__ void k_world_ranges(Ctx c) {
 // of the key in last frame's cache of the same world, which is strictly
 // descending, finds exactly the element descZipVector (Utils/Descending.hs:
 // 47-71) would pair it with.
-__global__ void k_contacts(Ctx c) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  int tp = total_pairs(c);
-  if (p >= tp) return;
-  int n = c.ccount[p];
-  if (n == 0) return;
+__device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
   int ga = c.pa[p], gb = c.pb[p];
   int w = c.body_world[ga];
   int b0 = c.body_off[w];
@@ -347,7 +343,7 @@ __global__ void k_contacts(Ctx c) {
       c.ln[dst] = c.cache_ln[found];
       c.lt[dst] = c.cache_lt[found];
       c.warm[dst] = 1;
-      atomicAdd(&c.counters[CNT_WARM], 1);
+      ++hits;
     } else {
       // newCache (Solvers/Contact.hs:91-99): zero start; the learned warm
       // start may replace it before the solve
@@ -366,6 +362,18 @@ __global__ void k_contacts(Ctx c) {
       }
     }
   }
+}
+
+
+__global__ void k_contacts(Ctx c) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int tp = total_pairs(c);
+  int n = p < tp ? c.ccount[p] : 0;
+  int hits = 0;
+  if (n > 0) contacts_of_pair(c, p, n, hits);
+  // one atomic per wavefront for the warm-start hit counter
+  for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off);
+  if ((threadIdx.x & 63) == 0 && hits) atomicAdd(&c.counters[CNT_WARM], hits);
 }
 
 // Reference order, one lane per world (used for worlds whose schedule does
@@ -446,12 +454,18 @@ static shp_status enqueue_step(shp_batch* b) {
   mark(b, 0);
   k_reset<<<1, 64, 0, s>>>(c);
   mark(b, 1);
-  k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 0);
+  if (c.use_grid)
+    HIPCHK(launch_grid_count(b));
+  else
+    k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 0);
   mark(b, 2);
   size_t tb = b->scan_tmp_bytes;
   HIPCHK(hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, c.cnt_rank, c.off_rank, c.NB + 1, s));
   mark(b, 3);
-  k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 1);
+  if (c.use_grid)
+    HIPCHK(launch_grid_emit(b));
+  else
+    k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 1);
   mark(b, 4);
   k_sat<<<blocks(c.pair_cap + 1, 256), 256, 0, s>>>(c);
   mark(b, 5);
@@ -514,6 +528,8 @@ shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, cons
     return fail(SHP_E_INVAL, "unknown solver mode");
   if (cfg->warm_start != SHP_WARM_ZERO && cfg->warm_start != SHP_WARM_LEARNED)
     return fail(SHP_E_INVAL, "unknown warm-start mode");
+  if (cfg->broadphase < SHP_BROADPHASE_AUTO || cfg->broadphase > SHP_BROADPHASE_GRID)
+    return fail(SHP_E_INVAL, "unknown broadphase mode");
   long NB = 0, NV = 0;
   for (int w = 0; w < n_worlds; ++w) {
     const shp_world_desc& d = worlds[w];
@@ -538,6 +554,7 @@ shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, cons
 
   shp_batch* b = new shp_batch();
   b->cfg = *cfg;
+  b->ppb = ppb;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) {
     delete b;
@@ -692,10 +709,63 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
   TRY(dalloc(b, &c.quality, 8));
   TRY(solver_alloc(b));
 
+  // grid broadphase set-up: cell size per world from the largest vertex
+  // radius of its dynamic bodies; static bodies larger than a cell are
+  // "large" and tested explicitly
+  {
+    int max_n = 0;
+    for (int w = 0; w < n_worlds; ++w) max_n = std::max(max_n, body_off[w + 1] - body_off[w]);
+    int mode = b->cfg.broadphase;
+    c.use_grid = mode == SHP_BROADPHASE_GRID || (mode == SHP_BROADPHASE_AUTO && max_n > 2048);
+    std::vector<double> rmax(NB, 0.0), cell(n_worlds, 1.0);
+    for (long gg = 0; gg < NB; ++gg)
+      for (int k = vert_off[gg]; k < vert_off[gg + 1]; ++k)
+        rmax[gg] = std::max(rmax[gg], std::sqrt(lx[k] * lx[k] + ly[k] * ly[k]));
+    std::vector<int32_t> tab_off(n_worlds + 1), large_off(n_worlds + 1), large_ids;
+    std::vector<uint8_t> large(NB, 0);
+    long tb_total = 0;
+    for (int w = 0; w < n_worlds; ++w) {
+      double hmax = 0.0;
+      for (int gg = body_off[w]; gg < body_off[w + 1]; ++gg)
+        if (!is_static[gg]) hmax = std::max(hmax, 2.0 * rmax[gg]);
+      cell[w] = hmax > 0.0 ? hmax * (1.0 + 1e-6) + 1e-9 : 1.0;
+      large_off[w] = (int32_t)large_ids.size();
+      for (int gg = body_off[w]; gg < body_off[w + 1]; ++gg)
+        if (is_static[gg] && 2.0 * rmax[gg] >= cell[w]) {
+          large[gg] = 1;
+          large_ids.push_back(gg);
+        }
+      int nw = body_off[w + 1] - body_off[w];
+      long t = 64;
+      while (t < 2L * nw) t <<= 1;
+      tab_off[w] = (int32_t)tb_total;
+      tb_total += t;
+    }
+    tab_off[n_worlds] = (int32_t)tb_total;
+    large_off[n_worlds] = (int32_t)large_ids.size();
+    c.grid_buckets = (int)tb_total;
+    TRY(dalloc(b, &c.cell_h, n_worlds));
+    HIPCHK(hipMemcpy(c.cell_h, cell.data(), n_worlds * 8, hipMemcpyHostToDevice));
+    TRY(dalloc(b, &c.is_large, NB));
+    HIPCHK(hipMemcpy(c.is_large, large.data(), NB, hipMemcpyHostToDevice));
+    TRY(dalloc(b, &c.body_cell, 2 * NB));
+    TRY(dalloc(b, &c.tab_off, n_worlds + 1));
+    HIPCHK(hipMemcpy(c.tab_off, tab_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
+    TRY(dalloc(b, &c.bucket_cnt, tb_total + 1));
+    TRY(dalloc(b, &c.bucket_start, tb_total + 1));
+    TRY(dalloc(b, &c.bucket_fill, tb_total));
+    TRY(dalloc(b, &c.bucket_items, NB));
+    TRY(dalloc(b, &c.large_off, n_worlds + 1));
+    HIPCHK(hipMemcpy(c.large_off, large_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
+    TRY(dalloc(b, &c.large_ids, std::max<size_t>(large_ids.size(), 1)));
+    if (!large_ids.empty())
+      HIPCHK(hipMemcpy(c.large_ids, large_ids.data(), large_ids.size() * 4, hipMemcpyHostToDevice));
+  }
+
   size_t t1 = 0, t2 = 0;
   HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, c.cnt_rank, c.off_rank, c.NB + 1, b->stream));
   HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, c.ccount, c.coff, c.pair_cap + 1, b->stream));
-  b->scan_tmp_bytes = std::max(t1, t2);
+  b->scan_tmp_bytes = std::max(std::max(t1, t2), grid_scan_bytes(b));
   TRY(dalloc(b, (uint8_t**)&b->scan_tmp, b->scan_tmp_bytes));
 
   hipStream_t s = b->stream;

@@ -1,0 +1,180 @@
+// shp_grid.hip - uniform-grid broadphase for large worlds.
+//
+// Output contract (identical to the wavefront scan in shp_engine.hip and to
+// culledKeys, shapes/src/Physics/Broadphase/Aabb.hs:168-183 / Grid.hs:67-86):
+// for every body i of a world, all j < i of the same world with inclusive
+// AABB overlap and not both static, segments in descending i, each segment
+// in descending j.
+//
+// Each world has a cell size h >= the AABB extent of any of its "small"
+// bodies (2 * the largest vertex radius of its dynamic bodies, plus a
+// margin), fixed at creation.  Static bodies larger than that ("large",
+// e.g. floors) are listed per world and tested explicitly.  A small body is
+// binned by the cell of its AABB's min corner into a per-world hash table
+// (counting sort, per-step).  Any small j overlapping small i has its min
+// corner in cells [cell(min_i) - 1, cell(max_i)] on each axis (at most 3x3
+// cells), because max_j >= min_i and extent_j < h.  A large body i scans
+// every j < i of its world.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "shp_engine.h"
+
+__device__ __forceinline__ int cell_of(double x, double h) {
+  double q = floor(x / h);
+  if (!(q == q)) return 0;
+  if (q > 1073741824.0) return 1073741824;
+  if (q < -1073741824.0) return -1073741824;
+  return (int)q;
+}
+
+__device__ __forceinline__ uint32_t cell_hash(int cx, int cy, uint32_t mask) {
+  uint32_t x = (uint32_t)cx * 0x8DA6B343u ^ (uint32_t)cy * 0xD8163841u;
+  x ^= x >> 15;
+  x *= 0x2C1B3C6Du;
+  x ^= x >> 12;
+  return x & mask;
+}
+
+__device__ __forceinline__ bool aabb_hit(const Ctx& c, double aminx, double amaxx, double aminy,
+                                         double amaxy, int gj) {
+  double bminx = c.bminx[gj], bmaxx = c.bmaxx[gj];
+  double bminy = c.bminy[gj], bmaxy = c.bmaxy[gj];
+  return !(bminx > amaxx || bmaxx < aminx) && !(bminy > amaxy || bmaxy < aminy);
+}
+
+// gravity (applyExternal) + binning of small bodies
+__global__ void k_grid_assign(Ctx c) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= c.NB) return;
+  if (c.iml[g] != 0.0) {
+    c.vx[g] = c.vx[g] + c.gx * c.dt;
+    c.vy[g] = c.vy[g] + c.gy * c.dt;
+  }
+  if (c.is_large[g]) return;
+  int w = c.body_world[g];
+  double h = c.cell_h[w];
+  int cx = cell_of(c.bminx[g], h), cy = cell_of(c.bminy[g], h);
+  c.body_cell[2 * g] = cx;
+  c.body_cell[2 * g + 1] = cy;
+  int t0 = c.tab_off[w];
+  uint32_t mask = (uint32_t)(c.tab_off[w + 1] - t0 - 1);
+  atomicAdd(&c.bucket_cnt[t0 + cell_hash(cx, cy, mask)], 1);
+}
+
+__global__ void k_grid_fill(Ctx c) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= c.NB || c.is_large[g]) return;
+  int w = c.body_world[g];
+  int t0 = c.tab_off[w];
+  uint32_t mask = (uint32_t)(c.tab_off[w + 1] - t0 - 1);
+  int b = t0 + cell_hash(c.body_cell[2 * g], c.body_cell[2 * g + 1], mask);
+  int pos = c.bucket_start[b] + atomicAdd(&c.bucket_fill[b], 1);
+  c.bucket_items[pos] = g;
+}
+
+// One lane per body i: count (fill = 0) or emit (fill = 1) its pairs.
+__global__ void k_grid_pairs(Ctx c, int fill) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= c.NB) return;
+  int w = c.body_world[g];
+  int b0 = c.body_off[w];
+  int li = g - b0;
+  int nw = c.body_off[w + 1] - b0;
+  int rank = b0 + (nw - 1 - li);
+  bool si = c.is_static[g] != 0;
+  double aminx = c.bminx[g], amaxx = c.bmaxx[g], aminy = c.bminy[g], amaxy = c.bmaxy[g];
+  int base = fill ? c.off_rank[rank] : 0;
+  int count = 0;
+  auto emit = [&](int gj) {
+    if (fill) {
+      int pos = base + count;
+      if (pos < c.pair_cap) {
+        c.pa[pos] = g;
+        c.pb[pos] = gj;
+      }
+    }
+    ++count;
+  };
+  if (c.is_large[g]) {
+    for (int gj = g - 1; gj >= b0; --gj)
+      if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) emit(gj);
+  } else {
+    for (int q = c.large_off[w]; q < c.large_off[w + 1]; ++q) {
+      int gj = c.large_ids[q];
+      if (gj >= g) break;  // ascending
+      if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) emit(gj);
+    }
+    double h = c.cell_h[w];
+    int cx0 = cell_of(aminx, h) - 1, cx1 = cell_of(amaxx, h);
+    int cy0 = cell_of(aminy, h) - 1, cy1 = cell_of(amaxy, h);
+    int t0 = c.tab_off[w];
+    uint32_t mask = (uint32_t)(c.tab_off[w + 1] - t0 - 1);
+    for (int cy = cy0; cy <= cy1; ++cy)
+      for (int cx = cx0; cx <= cx1; ++cx) {
+        int b = t0 + cell_hash(cx, cy, mask);
+        for (int q = c.bucket_start[b]; q < c.bucket_start[b + 1]; ++q) {
+          int gj = c.bucket_items[q];
+          if (gj >= g || c.body_cell[2 * gj] != cx || c.body_cell[2 * gj + 1] != cy) continue;
+          if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) emit(gj);
+        }
+      }
+  }
+  if (!fill) {
+    c.cnt_rank[rank] = count;
+    return;
+  }
+  if (base + count > c.pair_cap) {
+    atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_PAIR_OVERFLOW);
+    return;
+  }
+  // the segment in descending j (insertion sort; segments are short)
+  for (int a = base + 1; a < base + count; ++a) {
+    int v = c.pb[a];
+    int q = a - 1;
+    while (q >= base && c.pb[q] < v) {
+      c.pb[q + 1] = c.pb[q];
+      --q;
+    }
+    c.pb[q + 1] = v;
+  }
+}
+
+__global__ void k_zero_i32(int32_t* p, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+static int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// Grid broadphase, first half: binning and per-body pair counts into
+// cnt_rank (the caller then scans cnt_rank into off_rank).
+hipError_t launch_grid_count(shp_batch* b) {
+  Ctx& c = b->ctx;
+  hipStream_t s = b->stream;
+  k_zero_i32<<<nblk(c.grid_buckets + 1, 256), 256, 0, s>>>(c.bucket_cnt, c.grid_buckets + 1);
+  k_zero_i32<<<nblk(c.grid_buckets, 256), 256, 0, s>>>(c.bucket_fill, c.grid_buckets);
+  k_grid_assign<<<nblk(c.NB, 256), 256, 0, s>>>(c);
+  size_t tb = b->scan_tmp_bytes;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, c.bucket_cnt, c.bucket_start,
+                                                  c.grid_buckets + 1, s);
+  if (e != hipSuccess) return e;
+  k_grid_fill<<<nblk(c.NB, 256), 256, 0, s>>>(c);
+  k_grid_pairs<<<nblk(c.NB, 256), 256, 0, s>>>(c, 0);
+  return hipGetLastError();
+}
+
+// Second half: emit the pairs at off_rank.
+hipError_t launch_grid_emit(shp_batch* b) {
+  k_grid_pairs<<<nblk(b->ctx.NB, 256), 256, 0, b->stream>>>(b->ctx, 1);
+  return hipGetLastError();
+}
+
+size_t grid_scan_bytes(shp_batch* b) {
+  size_t t = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, t, b->ctx.bucket_cnt, b->ctx.bucket_start,
+                                         b->ctx.grid_buckets + 1, b->stream);
+  return t;
+}

@@ -26,6 +26,9 @@ SOLVER_ORDERED_F64 = 0
 SOLVER_COLORED_F32 = 1
 WARM_ZERO = 0
 WARM_LEARNED = 1
+BROADPHASE_AUTO = 0
+BROADPHASE_SCAN = 1
+BROADPHASE_GRID = 2
 
 STATUS = {0: "OK", 1: "E_INVAL", 2: "E_OOM", 3: "E_HIP", 4: "E_UNSUPPORTED",
           5: "E_OVERFLOW", 6: "E_NONFINITE"}
@@ -53,7 +56,7 @@ class ConfigC(C.Structure):
                 ("slop", C.c_double), ("gravity_x", C.c_double),
                 ("gravity_y", C.c_double), ("iterations", C.c_int32),
                 ("solver", C.c_int32), ("warm_start", C.c_int32),
-                ("max_pairs_per_body", C.c_int32)]
+                ("max_pairs_per_body", C.c_int32), ("broadphase", C.c_int32)]
 
 
 class StepStatsC(C.Structure):
@@ -145,7 +148,7 @@ class Batch:
 
     def __init__(self, worlds: Sequence[WorldDesc], config: SceneConfig,
                  solver: int = SOLVER_ORDERED_F64, warm_start: int = WARM_ZERO,
-                 max_pairs_per_body: int = 0):
+                 max_pairs_per_body: int = 0, broadphase: int = BROADPHASE_AUTO):
         L = load_library()
         self._L = L
         self._keep = []
@@ -167,7 +170,7 @@ class Batch:
             d.hull_y = _dptr(hy)
         cfg = ConfigC(config.dt, config.baumgarte, config.slop, config.gx,
                       config.gy, config.iterations, solver, warm_start,
-                      max_pairs_per_body)
+                      max_pairs_per_body, broadphase)
         h = C.c_void_p()
         _check(L.shp_batch_create(descs, len(worlds), C.byref(cfg), C.byref(h)))
         self._h = h

@@ -53,12 +53,14 @@ def test_create_rejects_bad_arguments():
     w = scenes.pyramid(rows=2).worlds[0]
     d, keep = _desc(w)
     h = C.c_void_p()
-    cfg = engine.ConfigC(1 / 60, 0.01, 0.02, 0, -9.8, 4, 0, 0, 0)
+    cfg = engine.ConfigC(1 / 60, 0.01, 0.02, 0, -9.8, 4, 0, 0, 0, 0)
     assert L.shp_batch_create(None, 1, C.byref(cfg), C.byref(h)) == 1
-    bad = engine.ConfigC(0.0, 0.01, 0.02, 0, -9.8, 4, 0, 0, 0)
+    bad = engine.ConfigC(0.0, 0.01, 0.02, 0, -9.8, 4, 0, 0, 0, 0)
     assert L.shp_batch_create(C.byref(d), 1, C.byref(bad), C.byref(h)) == 1
     assert b"dt" in L.shp_last_error()
-    bad = engine.ConfigC(1 / 60, 0.01, 0.02, 0, -9.8, 4, 7, 0, 0)
+    bad = engine.ConfigC(1 / 60, 0.01, 0.02, 0, -9.8, 4, 7, 0, 0, 0)
+    assert L.shp_batch_create(C.byref(d), 1, C.byref(bad), C.byref(h)) == 1
+    bad = engine.ConfigC(1 / 60, 0.01, 0.02, 0, -9.8, 4, 0, 0, 0, 5)
     assert L.shp_batch_create(C.byref(d), 1, C.byref(bad), C.byref(h)) == 1
     # a 9-gon is beyond the kernels' hull size
     b = scenes._Builder()

@@ -130,3 +130,30 @@ def test_graph_replay_matches_eager():
             assert_bitwise("lam", b.contacts(k)["lam"], a.contacts(k)["lam"], 0)
     a.close()
     b.close()
+
+
+def _run_grid(orc, worlds, cfg, steps, every):
+    gpu = engine.make_batch(worlds, cfg, broadphase=engine.BROADPHASE_GRID)
+    refs = [orc.OracleEngine(w, cfg) for w in worlds]
+    done = 0
+    while done < steps:
+        gpu.step(every)
+        for k, r in enumerate(refs):
+            r.step(every)
+            compare_state(gpu, r, done + every, world=k)
+        done += every
+    gpu.close()
+
+
+def test_grid_broadphase_bitwise_pile(orc):
+    """The uniform-grid broadphase gives the same pairs (set and order) and
+    therefore the same bits as the scan, on the 8192-polygon world."""
+    sc = scenes.polygon_pile()
+    _run_grid(orc, sc.worlds, sc.config, steps=60, every=20)
+
+
+def test_grid_broadphase_bitwise_worlds(orc):
+    worlds = [scenes.random_world(500 + k, count=30 + 9 * k, spacing=1.25) for k in range(6)]
+    worlds.append(scenes.pyramid().worlds[0])
+    worlds.append(scenes.reference_stacks().worlds[0])
+    _run_grid(orc, worlds, scenes.SceneConfig(iterations=3), steps=40, every=10)
