@@ -34,11 +34,13 @@ SOLVERS = {"ordered": engine.SOLVER_ORDERED_F64, "colored": engine.SOLVER_COLORE
 DTYPES = {"ordered": "f64", "colored": "f32"}
 
 
-def make_scene(name: str):
+def make_scene(name: str, worlds: int = 0):
     if name == "polygon_pile":
         return scenes.polygon_pile()
     if name == "world_batch":
-        return scenes.world_batch()
+        return scenes.world_batch_fast(n_worlds=worlds or 4096)
+    if name == "pile_batch":
+        return scenes.pile_batch(n_worlds=worlds or 1024)
     if name == "pyramid":
         return scenes.pyramid()
     if name == "reference_stacks":
@@ -79,9 +81,10 @@ def kernel_bytes(phase: str, st: dict, nb: int, nv: int, word: int) -> float:
     return 0.0
 
 
-def instrumented_run(scene, solver, steps):
+def instrumented_run(scene, solver, steps, broadphase="auto", ppb=0):
     w = scene.worlds
-    b = engine.make_batch(w, scene.config, solver=solver)
+    b = engine.make_batch(w, scene.config, solver=solver, broadphase=BROADPHASES[broadphase],
+                          max_pairs_per_body=ppb)
     b.set_timing(True)
     tot = {"pairs": 0, "contacts": 0}
     for _ in range(steps):
@@ -90,13 +93,14 @@ def instrumented_run(scene, solver, steps):
         tot["pairs"] += s["pairs"]
         tot["contacts"] += s["contacts"]
     t = b.timing()
+    t["solve_profile_us"] = b.solve_profile()
     b.close()
     n = max(t["steps"], 1)
     return t, {k: v / n for k, v in tot.items()}
 
 
-def roofline(scene, solver_name, steps):
-    t, st = instrumented_run(scene, SOLVERS[solver_name], steps)
+def roofline(scene, solver_name, steps, broadphase="auto", ppb=0):
+    t, st = instrumented_run(scene, SOLVERS[solver_name], steps, broadphase, ppb)
     nb = scene.n_bodies
     nv = sum(int(w.hull_off[-1]) for w in scene.worlds)
     word = 8 if solver_name == "ordered" else 4
@@ -116,7 +120,7 @@ def roofline(scene, solver_name, steps):
         "traffic": None,
         "algorithmic_bytes_per_launch": round(by),
         "avg_launch_ms": round(ms, 5),
-    }, {p: round(t[p], 5) for p in phases}, st
+    }, {**{p: round(t[p], 5) for p in phases}, "solve_profile_us": t["solve_profile_us"]}, st
 
 
 # ------------------------------------------------------------ CPU baseline
@@ -141,11 +145,36 @@ def cpu_baseline(scene, sample_steps):
     }
 
 
-def run_mode(scene, solver, warm, steps, warmup, graph=True, model=None):
+BROADPHASES = {"auto": engine.BROADPHASE_AUTO, "scan": engine.BROADPHASE_SCAN,
+               "grid": engine.BROADPHASE_GRID}
+
+
+def autotune(scene, solver, ppb, steps=20, warm_steps=10):
+    """Pick the broadphase schedule by measurement: each candidate steps the
+    workload from its initial state (warm_steps untimed, then `steps` timed
+    graph replays); the fastest wins."""
+    times = {}
+    for name in ("scan", "grid"):
+        b = engine.make_batch(scene.worlds, scene.config, solver=solver,
+                              broadphase=BROADPHASES[name], max_pairs_per_body=ppb)
+        b.step_async(warm_steps, use_graph=True)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step_async(steps, use_graph=True)
+        b.sync()
+        times[name] = (time.perf_counter() - t0) / steps * 1e3
+        b.close()
+    best = min(times, key=times.get)
+    return best, {k: round(v, 4) for k, v in times.items()}
+
+
+def run_mode(scene, solver, warm, steps, warmup, graph=True, model=None, broadphase="auto",
+             ppb=0):
     """Warm up on a separate instance, then time `steps` steps from the
     initial state; returns (seconds, run stats)."""
     def make():
-        b = engine.make_batch(scene.worlds, scene.config, solver=solver, warm_start=warm)
+        b = engine.make_batch(scene.worlds, scene.config, solver=solver, warm_start=warm,
+                              broadphase=BROADPHASES[broadphase], max_pairs_per_body=ppb)
         if warm == engine.WARM_LEARNED:
             b.set_mlp(model)
         return b
@@ -190,6 +219,11 @@ def main():
                     help="steps of the instrumented replay (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--broadphase", default="tune", choices=["tune"] + list(BROADPHASES))
+    ap.add_argument("--pairs-per-body", type=int, default=0,
+                    help="pair capacity per body (0: library default)")
+    ap.add_argument("--worlds", type=int, default=0,
+                    help="batch workloads: number of worlds (default: the config's)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -197,7 +231,7 @@ def main():
               "described in DESIGN.md (run with --gpus 1).", file=sys.stderr)
         sys.exit(2)
 
-    scene = make_scene(args.workload)
+    scene = make_scene(args.workload, args.worlds)
     if args.iterations:
         scene.config.iterations = args.iterations
     solver = SOLVERS[args.solver]
@@ -209,12 +243,17 @@ def main():
         elif warm == engine.WARM_LEARNED:
             raise SystemExit(f"bench.py: no model at {args.model}")
     dyn = scene.n_dynamic
+    ppb = args.pairs_per_body
+    tuning = None
+    bp = args.broadphase
+    if bp == "tune":
+        bp, tuning = autotune(scene, solver, ppb)
 
     elapsed, last = run_mode(scene, solver, warm, args.steps, args.warmup,
-                             graph=not args.no_graph, model=model)
+                             graph=not args.no_graph, model=model, broadphase=bp, ppb=ppb)
     value = dyn * args.steps / elapsed
     prof_steps = args.profile_steps or args.steps
-    roof, phases, avg = roofline(scene, args.solver, prof_steps)
+    roof, phases, avg = roofline(scene, args.solver, prof_steps, bp, ppb)
 
     out = {
         "metric": METRIC,
@@ -239,6 +278,8 @@ def main():
             "solver": args.solver,
             "warm_start": args.warm,
             "graph": not args.no_graph,
+            "broadphase": bp,
+            "autotune_ms_per_step": tuning,
         },
         "quality": quality(last),
         "roofline": roof,
@@ -261,7 +302,7 @@ def main():
             scene.config.iterations = it
             el, st = run_mode(scene, SOLVERS[sv], engine.WARM_LEARNED if wm == "learned"
                               else engine.WARM_ZERO, args.steps, args.warmup,
-                              graph=not args.no_graph, model=model)
+                              graph=not args.no_graph, model=model, broadphase=bp, ppb=ppb)
             modes.append({"solver": sv, "warm_start": wm, "iterations": it,
                           "body_steps_per_s": round(dyn * args.steps / el, 1),
                           "ms_per_step": round(1e3 * el / args.steps, 4), **quality(st)})

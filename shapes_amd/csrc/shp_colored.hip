@@ -245,10 +245,9 @@ __global__ __launch_bounds__(COLOR_THREADS) void k_solve_colored(Ctx c, int pool
   int np = p1 - p0;
   int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
   long need_col = 4L * nb + 6L * np, need_vel = 12L * nb;
-  if (nb >= (int)NO_BODY || np > 32767 || need_col > pool_bytes || need_vel > pool_bytes) {
-    if (tid == 0) solve_world_serial(c, c0, c1);  // exact FP64 fallback
-    return;
-  }
+  if (nb >= (int)NO_BODY || np > 32767 || need_col > pool_bytes || need_vel > pool_bytes)
+    return;  // k_solve_colored_global takes this world
+  tmark(c, 0);
   uint32_t* s_best = (uint32_t*)pool;
   uint32_t* m_ab = s_best + nb;                       // np: local a | local b << 16
   uint16_t* m_col = (uint16_t*)(m_ab + np);           // np: colour, 0xFFFF uncoloured
@@ -300,6 +299,7 @@ __global__ __launch_bounds__(COLOR_THREADS) void k_solve_colored(Ctx c, int pool
   }
   bool serial_tail = s_left > 0;
   int ncol = rounds + (serial_tail ? 1 : 0);
+  tmark(c, 1);
   if (tid == 0) atomicMax(&c.counters[CNT_LEVELS], ncol);
   for (int i = tid; i <= ncol + 1; i += COLOR_THREADS) s_cur[i] = 0;
   __syncthreads();
@@ -343,7 +343,9 @@ __global__ __launch_bounds__(COLOR_THREADS) void k_solve_colored(Ctx c, int pool
   }
   __threadfence_block();
   __syncthreads();
+  tmark(c, 2);
   for (int pass = 0; pass <= c.iterations; ++pass) {
+    if (pass == 1) tmark(c, 3);
     for (int l = 0; l < ncol; ++l) {
       int e = s_off[l + 1];
       bool tail = serial_tail && l == ncol - 1;
@@ -360,6 +362,7 @@ __global__ __launch_bounds__(COLOR_THREADS) void k_solve_colored(Ctx c, int pool
       __syncthreads();
     }
   }
+  tmark(c, 4);
   // lagrangians back to contact order; FP32 velocity change into FP64
   for (int i = tid; i < s_off[ncol]; i += COLOR_THREADS) {
     int s = c.sched_order[p0 + i];
@@ -379,6 +382,218 @@ __global__ __launch_bounds__(COLOR_THREADS) void k_solve_colored(Ctx c, int pool
     c.vy[g] = c.vy[g] + (double)(V[3 * i + 1] - (float)c.vy[g]);
     c.w[g] = c.w[g] + (double)(V[3 * i + 2] - (float)c.w[g]);
   }
+  __syncthreads();
+  tmark(c, 5);
+}
+
+
+// FP32 velocities in the global arrays (worlds too large for LDS)
+__device__ __forceinline__ void gload(const Ctx& c, int ga, int gb, float* v) {
+  v[0] = c.fvx[ga];
+  v[1] = c.fvy[ga];
+  v[2] = c.fw[ga];
+  v[3] = c.fvx[gb];
+  v[4] = c.fvy[gb];
+  v[5] = c.fw[gb];
+}
+
+__device__ __forceinline__ void gstore(const Ctx& c, int ga, int gb, int flags, const float* v) {
+  if (!(flags & 1)) {
+    c.fvx[ga] = v[0];
+    c.fvy[ga] = v[1];
+    c.fw[ga] = v[2];
+  }
+  if (!(flags & 2)) {
+    c.fvx[gb] = v[3];
+    c.fvy[gb] = v[4];
+    c.fw[gb] = v[5];
+  }
+}
+
+__device__ __forceinline__ void solve_one_g(const Ctx& c, const PackF& P, int s, const RecF& r, bool warm) {
+  float v[6];
+  gload(c, r.ga, r.gb, v);
+  if (warm) {
+    if (!(r.flags & 4)) return;
+    applyf((float)r.ln, r.jn, r.im, v);
+    applyf((float)r.lt, r.jt, r.im, v);
+    gstore(c, r.ga, r.gb, r.flags, v);
+    return;
+  }
+  float cx = ((-v[0] + v[2] * r.ray) + v[3]) - v[5] * r.rby;
+  float cy = ((-v[1] - v[2] * r.rax) + v[4]) + v[5] * r.rbx;
+  float bb = r.e * (cx * r.nrx + cy * r.nry);
+  bb = 0.0f <= bb ? 0.0f : bb;
+  float bn = r.bn + bb;
+  float new_n = -(dot6f(r.jn, v) + bn) / r.mcn;
+  float new_t = -dot6f(r.jt, v) / r.mct;
+  double app_n = hs_max((double)new_n, -r.ln);
+  double acc_n = r.ln + app_n;
+  double thr = acc_n * (double)r.mu;
+  double acc_t = r.lt + (double)new_t;
+  double accp = acc_t > thr ? thr : (acc_t < -thr ? -thr : acc_t);
+  double app_t = accp - r.lt;
+  applyf((float)app_n, r.jn, r.im, v);
+  applyf((float)app_t, r.jt, r.im, v);
+  gstore(c, r.ga, r.gb, r.flags, v);
+  P.lam[s] = acc_n;
+  P.lam[P.CC + s] = accp;
+}
+
+template <bool WARM>
+__device__ __forceinline__ void manifold_g(const Ctx& c, const PackF& P, int s, const RecF& r0) {
+  solve_one_g(c, P, s, r0, WARM);
+  if (r0.flags & 8) {
+    RecF r1;
+    load_recf(c, P, s + 1, r1);
+    solve_one_g(c, P, s + 1, r1, WARM);
+  }
+}
+
+// The same algorithm for worlds whose colouring or velocities do not fit in
+// LDS: 64-bit priorities per body, colours per pair and FP32 velocities in
+// global memory (L2-resident), 1024 lanes per world.  Worlds that fit are
+// left to k_solve_colored (the fit test is the same).
+#define COLOR_THREADS_G 1024
+
+__global__ __launch_bounds__(COLOR_THREADS_G) void k_solve_colored_global(Ctx c, int pool_bytes) {
+  __shared__ int s_off[COLOR_CAP + 3];
+  __shared__ int s_cur[COLOR_CAP + 3];
+  __shared__ int s_left, s_taken;
+  int w = blockIdx.x, tid = threadIdx.x;
+  int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
+  if (c1 <= c0) return;
+  int p0 = c.wpair_off[w], p1 = c.wpair_off[w + 1];
+  int np = p1 - p0;
+  int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
+  long need_col = 4L * nb + 6L * np, need_vel = 12L * nb;
+  if (!(nb >= (int)NO_BODY || np > 32767 || need_col > pool_bytes || need_vel > pool_bytes)) return;
+  unsigned long long* best = c.col_best + b0;
+  int32_t* col = c.sched_level;  // per pair: colour, -1 uncoloured, -2 none
+  if (tid == 0) s_left = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int q = tid; q < np; q += COLOR_THREADS_G) {
+    int p = p0 + q;
+    bool has = c.ccount[p] > 0 && c.coff[p] < c1;
+    col[p] = has ? -1 : -2;
+    mine += has;
+  }
+  atomicAdd(&s_left, mine);
+  __syncthreads();
+  int rounds = 0;
+  while (s_left > 0 && rounds < COLOR_CAP) {
+    for (int i = tid; i < nb; i += COLOR_THREADS_G) best[i] = 0ull;
+    if (tid == 0) s_taken = 0;
+    __syncthreads();
+    for (int q = tid; q < np; q += COLOR_THREADS_G) {
+      int p = p0 + q;
+      if (col[p] != -1) continue;
+      unsigned long long prio =
+          ((unsigned long long)mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) << 32) | (uint32_t)(q + 1);
+      int ga = c.pa[p], gb = c.pb[p];
+      if (!c.is_static[ga]) atomicMax(&best[ga - b0], prio);
+      if (!c.is_static[gb]) atomicMax(&best[gb - b0], prio);
+    }
+    __syncthreads();
+    int taken = 0;
+    for (int q = tid; q < np; q += COLOR_THREADS_G) {
+      int p = p0 + q;
+      if (col[p] != -1) continue;
+      unsigned long long prio =
+          ((unsigned long long)mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) << 32) | (uint32_t)(q + 1);
+      int ga = c.pa[p], gb = c.pb[p];
+      if ((c.is_static[ga] || best[ga - b0] == prio) && (c.is_static[gb] || best[gb - b0] == prio)) {
+        col[p] = rounds;
+        ++taken;
+      }
+    }
+    atomicAdd(&s_taken, taken);
+    __syncthreads();
+    if (tid == 0) s_left -= s_taken;
+    ++rounds;
+    __syncthreads();
+  }
+  bool serial_tail = s_left > 0;
+  int ncol = rounds + (serial_tail ? 1 : 0);
+  if (tid == 0) atomicMax(&c.counters[CNT_LEVELS], ncol);
+  for (int i = tid; i <= ncol + 1; i += COLOR_THREADS_G) s_cur[i] = 0;
+  __syncthreads();
+  for (int q = tid; q < np; q += COLOR_THREADS_G) {
+    int p = p0 + q;
+    int cl = col[p];
+    if (cl == -1) col[p] = cl = rounds;
+    if (cl >= 0) atomicAdd(&s_cur[cl], 1);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int l = 0; l <= ncol; ++l) {
+      int h = s_cur[l];
+      s_off[l] = acc;
+      s_cur[l] = acc;
+      acc += h;
+    }
+    s_off[ncol + 1] = acc;
+  }
+  __syncthreads();
+  PackF P = packf(c);
+  for (int q = tid; q < np; q += COLOR_THREADS_G) {
+    int p = p0 + q;
+    int cl = col[p];
+    if (cl < 0) continue;
+    int m = atomicAdd(&s_cur[cl], 1);
+    int k0 = c.coff[p], k1 = k0 + c.ccount[p];
+    if (k1 > c1) k1 = c1;
+    int s = 2 * (p0 + m);
+    c.sched_order[p0 + m] = s;
+    if (k0 < k1) pack_contact_f(c, P, k0, s, k1 - k0 > 1);
+    if (k0 + 1 < k1) pack_contact_f(c, P, k0 + 1, s + 1, 0);
+  }
+  for (int i = tid; i < nb; i += COLOR_THREADS_G) {
+    c.fvx[b0 + i] = (float)c.vx[b0 + i];
+    c.fvy[b0 + i] = (float)c.vy[b0 + i];
+    c.fw[b0 + i] = (float)c.w[b0 + i];
+  }
+  __threadfence_block();
+  __syncthreads();
+  // velocities as a flat FP32 array of this world: V[3*i + k] is not the
+  // global layout, so use per-component accessors through a local view
+  for (int pass = 0; pass <= c.iterations; ++pass) {
+    for (int l = 0; l < ncol; ++l) {
+      int e = s_off[l + 1];
+      bool tail = serial_tail && l == ncol - 1;
+      if (!tail || tid == 0) {
+        int step = tail ? 1 : COLOR_THREADS_G;
+        for (int i = s_off[l] + (tail ? 0 : tid); i < e; i += step) {
+          int s = c.sched_order[p0 + i];
+          RecF r;
+          load_recf(c, P, s, r);
+          if (pass == 0) manifold_g<true>(c, P, s, r);
+          else manifold_g<false>(c, P, s, r);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < s_off[ncol]; i += COLOR_THREADS_G) {
+    int s = c.sched_order[p0 + i];
+    int k = c.pk_k[s];
+    c.ln[k] = P.lam[s];
+    c.lt[k] = P.lam[P.CC + s];
+    if (c.pk_f[s] & 8) {
+      int k1 = c.pk_k[s + 1];
+      c.ln[k1] = P.lam[s + 1];
+      c.lt[k1] = P.lam[P.CC + s + 1];
+    }
+  }
+  for (int i = tid; i < nb; i += COLOR_THREADS_G) {
+    int g = b0 + i;
+    if (c.is_static[g]) continue;
+    c.vx[g] = c.vx[g] + (double)(c.fvx[g] - (float)c.vx[g]);
+    c.vy[g] = c.vy[g] + (double)(c.fvy[g] - (float)c.vy[g]);
+    c.w[g] = c.w[g] + (double)(c.fw[g] - (float)c.w[g]);
+  }
 }
 
 #define LDS_MAX_BYTES (160 * 1024 - 512)
@@ -397,6 +612,11 @@ shp_status colored_alloc(shp_batch* b) {
   }
   size_t fixed = (2 * (COLOR_CAP + 3) + 2) * 4;
   b->color_lds = std::min<size_t>(fixed + need, LDS_MAX_BYTES);
+  if (hipMalloc(&c.col_best, (size_t)c.NB * 8) != hipSuccess) return SHP_E_OOM;
+  b->allocs.push_back(c.col_best);
+  b->color_global = false;
+  for (int w = 0; w < c.W; ++w)
+    if (12L * (b->h_body_off[w + 1] - b->h_body_off[w]) > (long)(b->color_lds - fixed)) b->color_global = true;
   if (hipFuncSetAttribute((const void*)k_solve_colored, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)b->color_lds) != hipSuccess)
     return SHP_E_HIP;
@@ -406,6 +626,10 @@ shp_status colored_alloc(shp_batch* b) {
 void launch_colored_solver(shp_batch* b) {
   int pool = (int)(b->color_lds - (2 * (COLOR_CAP + 3) + 2) * 4);
   k_solve_colored<<<b->ctx.W, COLOR_THREADS, b->color_lds, b->stream>>>(b->ctx, pool);
+  // worlds that do not fit in LDS (by body count at creation, or by their
+  // pair count this step) are solved by the global-memory variant; it exits
+  // at once for the others
+  k_solve_colored_global<<<b->ctx.W, COLOR_THREADS_G, 0, b->stream>>>(b->ctx, pool);
 }
 
 // ---- solve quality -----------------------------------------------------------
@@ -429,10 +653,9 @@ __device__ __forceinline__ unsigned long long wave_max(unsigned long long x) {
 // and of the run.  Non-negative doubles order like their bit patterns, so the
 // maxima are integer maxima, reduced per wavefront before one atomic each.
 __global__ void k_quality(Ctx c) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
   int tc = c.wcont_off[c.W];
   unsigned long long qd = 0, qn = 0, qt = 0;
-  if (k < tc) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < tc; k += gridDim.x * blockDim.x) {
     int ga = c.ga[k], gb = c.gb[k];
     double v[6], jn[6], jt[6];
     load_vel(c, ga, gb, v);
@@ -450,9 +673,10 @@ __global__ void k_quality(Ctx c) {
     double t = lt - dot6(jt, v) / c.mct[k];
     double tcl = t > thr ? thr : (t < -thr ? -thr : t);
     double rt = fabs(lt - tcl);
-    qd = nonneg_bits(c.depth[k]);
-    qn = nonneg_bits(rn);
-    qt = nonneg_bits(rt);
+    unsigned long long a = nonneg_bits(c.depth[k]), b = nonneg_bits(rn), d = nonneg_bits(rt);
+    qd = a > qd ? a : qd;
+    qn = b > qn ? b : qn;
+    qt = d > qt ? d : qt;
   }
   qd = wave_max(qd);
   qn = wave_max(qn);
@@ -468,6 +692,7 @@ __global__ void k_quality(Ctx c) {
 }
 
 void launch_quality(shp_batch* b) {
-  int n = b->ctx.contact_cap;
-  k_quality<<<(n + 255) / 256, 256, 0, b->stream>>>(b->ctx);
+  long n = b->ctx.contact_cap;
+  int grid = (int)std::max<long>(1, std::min<long>((n + 255) / 256, 2048));
+  k_quality<<<grid, 256, 0, b->stream>>>(b->ctx);
 }

@@ -75,6 +75,10 @@ struct Ctx {
   int32_t* pk_g;  // 2 x contact_cap: bodies a, b
   int32_t* pk_k;  // contact index of a slot (coloured)
   uint8_t* pk_f;  // flags: 1 a static, 2 b static, 4 warm, 8 pair continues
+  unsigned long long* col_best;  // colouring priorities per body (large worlds)
+  // solver sub-phase timestamps of world 0 (wall_clock64), timing mode only
+  unsigned long long* tstamp;
+  unsigned long long* tstamp_store;
   // coloured solver state (FP32 velocities, FP64 accumulators)
   float *fvx, *fvy, *fw;
   // learned warm start: padded FP32 master / BF16 weights, the step's
@@ -125,7 +129,9 @@ struct shp_batch {
   bool solve_vlds = false;  // ordered solver keeps velocities in LDS
   size_t solve_lds = 0;     // dynamic LDS bytes of the ordered solver
   size_t color_lds = 0;     // dynamic LDS bytes of the coloured solver
+  bool color_global = false;  // some world needs the global-memory colour path
   double phase_ms[SHP_NUM_PHASES]{};
+  double solve_us[8]{};  // accumulated solver sub-phase durations (world 0)
   int32_t timed_steps = 0;
 };
 

@@ -225,14 +225,7 @@ __device__ __forceinline__ int total_pairs(const Ctx& c) {
 
 // SAT narrowphase, one lane per pair (generateContacts, Contact.hs:22-40,
 // for two hulls).  Writes up to two contacts into the pair's slots.
-__global__ void k_sat(Ctx c) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p > c.pair_cap) return;
-  int tp = total_pairs(c);
-  if (p >= tp) {
-    c.ccount[p] = 0;
-    return;
-  }
+__device__ __forceinline__ void sat_pair(const Ctx& c, int p) {
   HullRef A = hull_of(c, c.pa[p]);
   HullRef B = hull_of(c, c.pb[p]);
   Cgeom gm[2];
@@ -250,6 +243,24 @@ __global__ void k_sat(Ctx c) {
   }
   c.ccount[p] = n;
 }
+
+// grid-stride over this step's pairs (the launch is sized for the capacity)
+__global__ void k_sat(Ctx c) {
+  int tp = total_pairs(c);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < tp; p += gridDim.x * blockDim.x) sat_pair(c, p);
+}
+
+// Contact count of pair p for the contact scan: 0 beyond this step's pairs.
+struct PairCount {
+  const int32_t* ccount;
+  const int32_t* total;  // off_rank[NB]
+  int cap;
+  __host__ __device__ int operator()(int p) const {
+    int t = *total;
+    t = t < cap ? t : cap;
+    return p < t ? ccount[p] : 0;
+  }
+};
 
 // Per-world pair and contact ranges after the scans.
 __global__ void k_world_ranges(Ctx c) {
@@ -366,11 +377,12 @@ __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
 
 
 __global__ void k_contacts(Ctx c) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
   int tp = total_pairs(c);
-  int n = p < tp ? c.ccount[p] : 0;
   int hits = 0;
-  if (n > 0) contacts_of_pair(c, p, n, hits);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < tp; p += gridDim.x * blockDim.x) {
+    int n = c.ccount[p];
+    if (n > 0) contacts_of_pair(c, p, n, hits);
+  }
   // one atomic per wavefront for the warm-start hit counter
   for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off);
   if ((threadIdx.x & 63) == 0 && hits) atomicAdd(&c.counters[CNT_WARM], hits);
@@ -387,13 +399,16 @@ __global__ void k_solve_serial(Ctx c) {
 // This frame's lagrangians become the next frame's EngineCache
 // (Engine/Main.hs:32, wrapInitializer :61-69).
 __global__ void k_cache_store(Ctx c) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k <= c.W) c.cache_woff[k] = c.wcont_off[k];
+  int stride = gridDim.x * blockDim.x;
   int tc = c.wcont_off[c.W];
-  if (k >= tc) return;
-  c.cache_key[k] = c.key[k];
-  c.cache_ln[k] = c.ln[k];
-  c.cache_lt[k] = c.lt[k];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= c.W || k < tc; k += stride) {
+    if (k <= c.W) c.cache_woff[k] = c.wcont_off[k];
+    if (k < tc) {
+      c.cache_key[k] = c.key[k];
+      c.cache_ln[k] = c.ln[k];
+      c.cache_lt[k] = c.lt[k];
+    }
+  }
 }
 
 __global__ void k_fill_i32(int32_t* p, int32_t v, int n) {
@@ -440,6 +455,8 @@ static shp_status dalloc(shp_batch* b, T** p, size_t n) {
 static inline double nz(double x) { return x == 0.0 ? 0.0 : x; }
 
 static int blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
+// grid of a grid-stride kernel over at most n items
+static int gs_blocks(long n) { return std::max(1, std::min(blocks(n, 256), 4096)); }
 
 // With timing enabled, an event is recorded on the batch's stream before
 // every launch and after the last one, so phase k's duration is
@@ -467,14 +484,19 @@ static shp_status enqueue_step(shp_batch* b) {
   else
     k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 1);
   mark(b, 4);
-  k_sat<<<blocks(c.pair_cap + 1, 256), 256, 0, s>>>(c);
+  k_sat<<<gs_blocks(c.pair_cap), 256, 0, s>>>(c);
   mark(b, 5);
   tb = b->scan_tmp_bytes;
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, c.ccount, c.coff, c.pair_cap + 1, s));
+  {
+    hipcub::CountingInputIterator<int> idx(0);
+    hipcub::TransformInputIterator<int, PairCount, hipcub::CountingInputIterator<int>> cnt(
+        idx, PairCount{c.ccount, c.off_rank + c.NB, c.pair_cap});
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, cnt, c.coff, c.pair_cap + 1, s));
+  }
   mark(b, 6);
   k_world_ranges<<<blocks(c.W + 1, 256), 256, 0, s>>>(c);
   mark(b, 7);
-  k_contacts<<<blocks(c.pair_cap, 256), 256, 0, s>>>(c);
+  k_contacts<<<gs_blocks(c.pair_cap), 256, 0, s>>>(c);
   if (c.warm_mode == SHP_WARM_LEARNED && c.mlp_ready) mlp_launch_infer(b);
   mark(b, 8);
   launch_solver(b);
@@ -482,7 +504,7 @@ static shp_status enqueue_step(shp_batch* b) {
   launch_quality(b);
   if (b->logging) mlp_launch_log(b);
   mark(b, 10);
-  k_cache_store<<<blocks(std::max(c.contact_cap, c.W + 1), 256), 256, 0, s>>>(c);
+  k_cache_store<<<gs_blocks(std::max(c.contact_cap, c.W + 1)), 256, 0, s>>>(c);
   mark(b, 11);
   k_integrate<<<blocks(c.NB, 256), 256, 0, s>>>(c, 1);
   mark(b, 12);
@@ -492,6 +514,13 @@ static shp_status enqueue_step(shp_batch* b) {
 
 static shp_status accumulate_timing(shp_batch* b) {
   HIPCHK(hipEventSynchronize(b->events[SHP_NUM_PHASES]));
+  unsigned long long ts[8];
+  HIPCHK(hipMemcpy(ts, b->ctx.tstamp, sizeof(ts), hipMemcpyDeviceToHost));
+  // wall_clock64 runs at 100 MHz on CDNA3/4
+  for (int k = 0; k < 5; ++k)
+    if (ts[k + 1] >= ts[k] && ts[k] != 0) b->solve_us[k] += (double)(ts[k + 1] - ts[k]) * 0.01;
+  unsigned long long zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  HIPCHK(hipMemcpy(b->ctx.tstamp, zero, sizeof(zero), hipMemcpyHostToDevice));
   for (int k = 0; k < SHP_NUM_PHASES; ++k) {
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, b->events[k], b->events[k + 1]));
@@ -548,7 +577,7 @@ shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, cons
     NV += d.hull_off[d.n_bodies];
   }
   if (NB >= (1L << 30)) return fail(SHP_E_UNSUPPORTED, "too many bodies");
-  int ppb = cfg->max_pairs_per_body > 0 ? cfg->max_pairs_per_body : 16;
+  int ppb = cfg->max_pairs_per_body > 0 ? cfg->max_pairs_per_body : 8;
   long pair_cap = NB * (long)ppb + 1024;
   if (pair_cap >= (1L << 30)) return fail(SHP_E_UNSUPPORTED, "pair capacity too large");
 
@@ -764,7 +793,12 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
 
   size_t t1 = 0, t2 = 0;
   HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, c.cnt_rank, c.off_rank, c.NB + 1, b->stream));
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, c.ccount, c.coff, c.pair_cap + 1, b->stream));
+  {
+    hipcub::CountingInputIterator<int> idx(0);
+    hipcub::TransformInputIterator<int, PairCount, hipcub::CountingInputIterator<int>> cnt(
+        idx, PairCount{c.ccount, c.off_rank + c.NB, c.pair_cap});
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, cnt, c.coff, c.pair_cap + 1, b->stream));
+  }
   b->scan_tmp_bytes = std::max(std::max(t1, t2), grid_scan_bytes(b));
   TRY(dalloc(b, (uint8_t**)&b->scan_tmp, b->scan_tmp_bytes));
 
@@ -847,8 +881,14 @@ shp_status shp_batch_set_timing(shp_batch* b, int32_t enable) {
   if (enable && !b->events_made) {
     for (int k = 0; k <= SHP_NUM_PHASES; ++k) HIPCHK(hipEventCreate(&b->events[k]));
     b->events_made = true;
+    TRY(dalloc(b, &b->ctx.tstamp_store, 8));
+    unsigned long long zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    HIPCHK(hipMemcpy(b->ctx.tstamp_store, zero, sizeof(zero), hipMemcpyHostToDevice));
   }
   b->timing = enable != 0;
+  b->ctx.tstamp = b->timing ? b->ctx.tstamp_store : nullptr;
+  b->graph_ready = false;  // kernel arguments changed
+  for (int k = 0; k < 8; ++k) b->solve_us[k] = 0.0;
   for (int k = 0; k < SHP_NUM_PHASES; ++k) b->phase_ms[k] = 0.0;
   b->timed_steps = 0;
   return SHP_OK;
@@ -858,6 +898,13 @@ shp_status shp_batch_read_timing(shp_batch* b, double* ms_per_phase, int32_t* st
   if (!b || !ms_per_phase || !steps) return fail(SHP_E_INVAL, "bad argument");
   for (int k = 0; k < SHP_NUM_PHASES; ++k) ms_per_phase[k] = b->phase_ms[k];
   *steps = b->timed_steps;
+  return SHP_OK;
+}
+
+shp_status shp_batch_read_solve_profile(shp_batch* b, double* us) {
+  if (!b || !us) return fail(SHP_E_INVAL, "bad argument");
+  int n = b->timed_steps > 0 ? b->timed_steps : 1;
+  for (int k = 0; k < 5; ++k) us[k] = b->solve_us[k] / n;
   return SHP_OK;
 }
 

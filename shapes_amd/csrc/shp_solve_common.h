@@ -13,6 +13,11 @@ namespace shp {
 
 __device__ __forceinline__ bool body_static(const Ctx& c, int g) { return c.is_static[g] != 0; }
 
+// Solver sub-phase timestamp k of world 0 (timing mode only).
+__device__ __forceinline__ void tmark(const Ctx& c, int k) {
+  if (c.tstamp && blockIdx.x == 0 && threadIdx.x == 0) c.tstamp[k] = wall_clock64();
+}
+
 
 struct Vel6 {
   double v[6];

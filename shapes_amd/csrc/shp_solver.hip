@@ -309,6 +309,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve_levels(Ctx c, int pool_
     return;
   }
   int* last = pool;
+  tmark(c, 0);
   for (int i = tid; i < nb; i += SOLVE_THREADS) last[i] = 0;
   __syncthreads();
   if (tid < 64) {
@@ -319,6 +320,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve_levels(Ctx c, int pool_
     }
   }
   __syncthreads();
+  tmark(c, 1);
   int maxl = s_maxl;
   if (maxl > LEVEL_CAP) {
     if (tid == 0) {
@@ -360,8 +362,11 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve_levels(Ctx c, int pool_
   }
   __threadfence_block();
   __syncthreads();
+  tmark(c, 2);
   run_pass<VLDS, true>(c, c0, maxl, s_off, V);
+  tmark(c, 3);
   for (int it = 0; it < c.iterations; ++it) run_pass<VLDS, false>(c, c0, maxl, s_off, V);
+  tmark(c, 4);
   long CC = c.contact_cap;
   for (int s = tid; s < c1 - c0; s += SOLVE_THREADS) {
     int k = c.sched_order[c0 + s];
@@ -376,6 +381,8 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve_levels(Ctx c, int pool_
       c.w[b0 + i] = V.lds[3 * i + 2];
     }
   }
+  __syncthreads();
+  tmark(c, 5);
 }
 
 #define LDS_MAX_BYTES (160 * 1024 - 512)

@@ -77,6 +77,7 @@ EXPORTS = [
     "shp_batch_read_hulls", "shp_batch_last_stats", "shp_batch_set_cache",
     "shp_mlp_param_count", "shp_batch_set_mlp", "shp_last_error",
     "shp_version", "shp_batch_set_timing", "shp_batch_read_timing",
+    "shp_batch_read_solve_profile",
     "shp_batch_set_logging", "shp_batch_read_log", "shp_mlp_create",
     "shp_mlp_destroy", "shp_mlp_get_params", "shp_mlp_forward",
     "shp_mlp_gradient", "shp_mlp_set_data", "shp_mlp_train",
@@ -126,6 +127,7 @@ def load_library(path: str = LIB_PATH):
     L.shp_mlp_train.argtypes = [C.c_void_p, _ip, C.c_int32, C.c_int32, C.c_float, _fp]
     L.shp_batch_set_timing.argtypes = [C.c_void_p, C.c_int32]
     L.shp_batch_read_timing.argtypes = [C.c_void_p, _dp, _ip]
+    L.shp_batch_read_solve_profile.argtypes = [C.c_void_p, _dp]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L
@@ -269,6 +271,13 @@ class Batch:
         _check(self._L.shp_batch_read_timing(self._h, _dptr(ms), C.byref(n)))
         k = max(n.value, 1)
         return {"steps": n.value, **{p: float(v) / k for p, v in zip(PHASES, ms)}}
+
+    def solve_profile(self) -> list:
+        """Average microseconds per timed step of the solve's sub-phases
+        (world 0)."""
+        us = np.zeros(5, dtype=np.float64)
+        _check(self._L.shp_batch_read_solve_profile(self._h, _dptr(us)))
+        return [round(float(x), 2) for x in us]
 
     def set_mlp(self, params: np.ndarray):
         p = np.ascontiguousarray(params, dtype=np.float32)
