@@ -32,6 +32,7 @@ HBM_PEAK_GBS = 8000.0
 
 SOLVERS = {"ordered": engine.SOLVER_ORDERED_F64, "colored": engine.SOLVER_COLORED_F32}
 DTYPES = {"ordered": "f64", "colored": "f32"}
+DTYPES_BYTES = {"ordered": 8, "colored": 4}
 
 
 def make_scene(name: str, worlds: int = 0):
@@ -73,11 +74,13 @@ def kernel_bytes(phase: str, st: dict, nb: int, nv: int, word: int) -> float:
         return C * (50 + 2 * 4 * word + 24) + C * (8 + 8 + 1 + 5 * word + 12 * word +
                                                    word + 4 * word + 2 * word + 2 * 8 + 1)
     if phase == "solve":
-        # per contact and pass: rows, bias, radii, normal, masses and
-        # lambdas read, lambdas written; both bodies' velocities read and
-        # written, inverse masses and materials read
-        per = (12 + 1 + 4 + 2 + 2) * word + 2 * 8 + 2 * 8 + 8 + 2 * (6 * word) + 2 * 4 * word
-        return C * per * passes
+        # compulsory traffic of one launch: per contact its bodies and warm
+        # flag, the normal, the four cross terms, bias, lever arms, two
+        # effective masses and two lambdas read (15 FP64 words) and two
+        # lambdas written; per body the velocity read and written and the
+        # inverse masses and materials read.  Re-reads of the same records
+        # and velocities across sweeps are served by LDS and L2.
+        return C * (9 + 15 * 8 + 2 * 8) + nb * (6 * 8 + 4 * 8 + 1)
     return 0.0
 
 
@@ -99,6 +102,14 @@ def instrumented_run(scene, solver, steps, broadphase="auto", ppb=0):
     return t, {k: v / n for k, v in tot.items()}
 
 
+def step_bytes(nb: int, nv: int, contacts: float, word: int) -> float:
+    """Compulsory HBM bytes of one whole step: every body's state and
+    materials read, pos/theta/vel/omega written ((10 + 6) words), its local
+    hull read (2 words per vertex), and the warm-start cache read and
+    written (packed key + two FP64 lambdas = 24 B each way per contact)."""
+    return nb * 16 * word + nv * 2 * word + contacts * 48
+
+
 def roofline(scene, solver_name, steps, broadphase="auto", ppb=0):
     t, st = instrumented_run(scene, SOLVERS[solver_name], steps, broadphase, ppb)
     nb = scene.n_bodies
@@ -108,7 +119,7 @@ def roofline(scene, solver_name, steps, broadphase="auto", ppb=0):
     phases = [p for p in engine.PHASES if p not in ("reset",)]
     dom = max(phases, key=lambda p: t[p])
     ms = t[dom]
-    by = kernel_bytes(dom, st, nb, nv, word)
+    by = kernel_bytes(dom, st, nb, nv, 8 if dom == "solve" else word)
     achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {
         "kernel": dom,
@@ -125,23 +136,29 @@ def roofline(scene, solver_name, steps, broadphase="auto", ppb=0):
 
 # ------------------------------------------------------------ CPU baseline
 
-def cpu_baseline(scene, sample_steps):
+def cpu_baseline(scene, sample_steps, max_worlds=64):
+    """The C++ FP64 oracle on this host: one world on one thread, or for a
+    batch the first `max_worlds` worlds over the host's cores (at most 16)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     oracle.build()
-    w = scene.worlds[0]
-    e = oracle.OracleEngine(w, scene.config)
-    secs = oracle.run_engines([e], sample_steps, threads=1)
-    e.close()
+    worlds = scene.worlds[:max_worlds]
+    threads = 1 if len(worlds) == 1 else max(1, min(16, os.cpu_count() or 1, len(worlds)))
+    engines = [oracle.OracleEngine(w, scene.config) for w in worlds]
+    secs = oracle.run_engines(engines, sample_steps, threads=threads)
+    for e in engines:
+        e.close()
+    dyn = sum(w.n_dynamic for w in worlds)
+    what = (f"{scene.name} ({dyn} dynamic bodies)" if len(worlds) == 1 else
+            f"worlds 0-{len(worlds) - 1} of {scene.name} ({dyn} dynamic bodies)")
     return {
-        "value": round(w.n_dynamic * sample_steps / secs, 1),
+        "value": round(dyn * sample_steps / secs, 1),
         "unit": "body-steps/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"first {sample_steps} of {scene.config.steps} steps of "
-                  f"{scene.name} ({w.n_dynamic} dynamic bodies), C++ FP64 oracle, "
-                  f"1 thread, {secs:.1f} s",
+        "sample": f"first {sample_steps} of {scene.config.steps} steps of {what}, "
+                  f"C++ FP64 oracle, {threads} thread(s), {secs:.1f} s",
     }
 
 
@@ -227,8 +244,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; the sharded design is "
-              "described in DESIGN.md (run with --gpus 1).", file=sys.stderr)
+        print("bench.py: this build steps worlds on one GPU only; sharding worlds "
+              "or slabs over several GPUs is not implemented (run with --gpus 1).",
+              file=sys.stderr)
         sys.exit(2)
 
     scene = make_scene(args.workload, args.worlds)
@@ -254,6 +272,9 @@ def main():
     value = dyn * args.steps / elapsed
     prof_steps = args.profile_steps or args.steps
     roof, phases, avg = roofline(scene, args.solver, prof_steps, bp, ppb)
+    nv = sum(int(w.hull_off[-1]) for w in scene.worlds)
+    sb = step_bytes(scene.n_bodies, nv, avg["contacts"], DTYPES_BYTES[args.solver])
+    step_gbs = sb / (elapsed / args.steps) / 1e9
 
     out = {
         "metric": METRIC,
@@ -283,6 +304,9 @@ def main():
         },
         "quality": quality(last),
         "roofline": roof,
+        "step_roofline": {"bound": "hbm", "algorithmic_bytes_per_step": round(sb),
+                          "achieved": round(step_gbs, 3), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 6)},
         "phase_ms": phases,
         "avg_pairs": round(avg["pairs"], 1),
         "avg_contacts": round(avg["contacts"], 1),
@@ -308,7 +332,7 @@ def main():
                           "ms_per_step": round(1e3 * el / args.steps, 4), **quality(st)})
         scene.config.iterations = base_it
         out["modes"] = modes
-    if not args.no_cpu and len(scene.worlds) == 1:
+    if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_sample_steps)
     print(json.dumps(out), flush=True)
 

@@ -99,9 +99,10 @@ typedef struct shp_step_stats {
   int64_t warm_hits;
   int64_t learned_starts;
   int64_t misses;       /* contacts absent from last frame's cache */
-  int32_t max_levels;   /* deepest per-world schedule (levels or colours) */
+  int32_t max_levels;   /* most colours of a world (coloured solver) */
   int32_t flags;        /* bit 0 pair overflow, 1 contact overflow,
-                           2 non-finite, 3 schedule fell back to serial */
+                           2 non-finite, 3 coloured schedule overflow,
+                           4 ordered solver stopped waiting (defect) */
   /* Solve quality after the last sweep (FP64 state), maxima over contacts
    * of: penetration depth, |min(lambda_n, Jn.v + b)| and
    * |lambda_t - clamp(lambda_t - Jt.v / mt, +-mu*lambda_n)|; for the last
@@ -227,8 +228,9 @@ shp_status shp_batch_set_timing(shp_batch* b, int32_t enable);
 shp_status shp_batch_read_timing(shp_batch* b, double* ms_per_phase,
                                  int32_t* steps);
 /* Average microseconds per timed step of the solve's sub-phases in world 0:
- * ordered: 0 level assignment, 1 bucketing + packing, 2 warm-start pass,
- * 3 sweeps, 4 write-back; coloured: 0 colouring, 1 bucketing + packing +
+ * ordered: 0 per-body ranks + record packing, 1 counter clearing and
+ * velocity staging, 2 (empty), 3 warm-start pass and sweeps (one dataflow
+ * loop), 4 write-back; coloured: 0 colouring, 1 bucketing + packing +
  * velocity staging, 2 warm-start pass, 3 sweeps, 4 write-back. */
 shp_status shp_batch_read_solve_profile(shp_batch* b, double* us5);
 

@@ -24,7 +24,8 @@ enum {
   SHP_FLAG_PAIR_OVERFLOW = 1,
   SHP_FLAG_CONTACT_OVERFLOW = 2,
   SHP_FLAG_NONFINITE = 4,
-  SHP_FLAG_SCHEDULE_OVERFLOW = 8
+  SHP_FLAG_SCHEDULE_OVERFLOW = 8,
+  SHP_FLAG_SOLVER_STALL = 16
 };
 
 struct Ctx {
@@ -66,11 +67,18 @@ struct Ctx {
   uint64_t* cache_key;
   double *cache_ln, *cache_lt;
   int32_t* cache_woff;  // W + 1
-  // scheduling (ordered: levels; coloured: colours)
+  // coloured schedule: per-pair colour, pairs sorted by colour in world
   int32_t* sched_level;   // contact_cap
-  int32_t* sched_order;   // contact_cap: contacts sorted by level in world
-  // contact constants packed in schedule order (FP64 records for the
-  // ordered solver, FP32 records + FP64 lambdas for the coloured one)
+  int32_t* sched_order;   // contact_cap
+  // ordered (dataflow) schedule: every (contact, dynamic side) listed in
+  // its body's segment, per contact {rank a, degree a, rank b, degree b},
+  // per-body offsets/cursors/counters for worlds too large for LDS
+  int32_t* flow_list;  // 2 x contact_cap
+  int4* flow_rk;       // contact_cap
+  int32_t *flow_off, *flow_cur;  // NB
+  // contact constants packed for the solver (compact FP64 records in
+  // contact order for the ordered solver, FP32 records + FP64 lambdas in
+  // colour order for the coloured one)
   double* pk;
   int32_t* pk_g;  // 2 x contact_cap: bodies a, b
   int32_t* pk_k;  // contact index of a slot (coloured)
@@ -127,6 +135,8 @@ struct shp_batch {
   bool logging = false;
   int ppb = 16;             // pair capacity per body
   bool solve_vlds = false;  // ordered solver keeps velocities in LDS
+  bool solve_clds = false;  // ordered solver keeps body counters in LDS
+  int solve_threads = 256;  // ordered solver workgroup size
   size_t solve_lds = 0;     // dynamic LDS bytes of the ordered solver
   size_t color_lds = 0;     // dynamic LDS bytes of the coloured solver
   bool color_global = false;  // some world needs the global-memory colour path
@@ -142,7 +152,6 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
 shp_status solver_alloc(shp_batch* b);
 shp_status colored_alloc(shp_batch* b);
 void launch_solver(shp_batch* b);
-void launch_serial_solver(shp_batch* b);
 void launch_ordered_solver(shp_batch* b);
 void launch_colored_solver(shp_batch* b);
 void launch_quality(shp_batch* b);

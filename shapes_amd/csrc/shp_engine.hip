@@ -388,14 +388,6 @@ __global__ void k_contacts(Ctx c) {
   if ((threadIdx.x & 63) == 0 && hits) atomicAdd(&c.counters[CNT_WARM], hits);
 }
 
-// Reference order, one lane per world (used for worlds whose schedule does
-// not fit the level-parallel solver).
-__global__ void k_solve_serial(Ctx c) {
-  int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= c.W) return;
-  solve_world_serial(c, c.wcont_off[w], c.wcont_off[w + 1]);
-}
-
 // This frame's lagrangians become the next frame's EngineCache
 // (Engine/Main.hs:32, wrapInitializer :61-69).
 __global__ void k_cache_store(Ctx c) {
@@ -539,10 +531,6 @@ void launch_solver(shp_batch* b) {
   }
 }
 
-void launch_serial_solver(shp_batch* b) {
-  k_solve_serial<<<blocks(b->ctx.W, 64), 64, 0, b->stream>>>(b->ctx);
-}
-
 extern "C" {
 
 const char* shp_last_error(void) { return g_err.c_str(); }
@@ -625,7 +613,7 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
   c.solver = b->cfg.solver;
   c.warm_mode = b->cfg.warm_start;
 
-  // host staging in SoA, -0 normalised to +0 (see store_vel)
+  // host staging in SoA, -0 normalised to +0 (static velocities stay as loaded)
   std::vector<double> h[10];
   for (auto& v : h) v.resize(NB);
   std::vector<int32_t> body_world(NB), vert_off(NB + 1), body_off(n_worlds + 1);

@@ -1,25 +1,42 @@
-// shp_solver.hip - ordered FP64 (drop-in) solve schedule.
+// shp_solver.hip - ordered FP64 (drop-in) solve: dataflow over per-body
+// counters.
 //
-// The reference solves contacts one after another in descending key order
-// (improveWorld, shapes/src/Physics/Solvers/Contact.hs:146-157), after
-// applying cached lagrangians in the same order (applyCachedSlns, :74-121).
-// Contact c only depends on the latest earlier contact that touches one of
-// its dynamic bodies, so each world's contacts get a level
-//   level(c) = 1 + max(level of earlier contacts sharing a dynamic body)
-// and each level is solved in parallel.  Two contacts of one level touch
-// disjoint dynamic bodies, so the order inside a level cannot change any bit
-// and the result equals the sequential sweep bit for bit.  Static bodies
-// (both inverse masses 0) are read-only (see store_vel).
+// The reference applies the cached lagrangians in descending key order
+// (applyCachedSlns, shapes/src/Physics/Solvers/Contact.hs:74-121) and then
+// sweeps the contacts `iterations` times in the same order (improveWorld,
+// :146-157).  Written as one sequence, item (p, k) is pass p of contact k
+// (pass 0 is the warm start) and the items run in (p, k) order.  An item
+// reads and writes the velocities of its two bodies and the lagrangians of
+// its own contact only, so it depends on nothing but the latest earlier
+// items touching one of its dynamic bodies.  For a dynamic body a touched
+// by d(a) contacts, with contact k at rank r_a(k) among them in key order,
+// the items on a are numbered p * d(a) + r_a(k) in sequence order.  A
+// per-body counter holds how many items on the body have completed: item
+// (p, k) runs when counter[a] == p * d(a) + r_a(k) and the same holds for
+// b, and on completion it raises both counters by one (a release store
+// after its velocity stores; the waiting side loads with acquire).  Every
+// read therefore sees exactly the values of the sequential sweep and the
+// result is the sequential result bit for bit, whatever the timing.  Static
+// bodies (both inverse masses 0) are read-only and carry no counter.
+//
+// Lane t of the workgroup owns contacts t, t + T, t + 2T, ... and runs its
+// items in sequence order.  The earliest unfinished item of the world is
+// always ready (every earlier item is done) and it is its lane's current
+// item, and lanes wait on nothing but counters inside the loop, so the loop
+// always progresses.  As a bound that holds even if this argument were
+// broken by a defect, a lane that sees no item of the workgroup complete
+// for WAIT_LIMIT consecutive polls stops and raises SHP_FLAG_SOLVER_STALL.
 //
 // One workgroup per world:
-//   1. wave 0 assigns levels, 64 contacts at a time (readlane broadcasts)
-//   2. counting sort of the contacts by level -> schedule slots
-//   3. every contact's constants are packed in slot order (one coalesced
-//      record per slot, so a level's records are contiguous)
-//   4. velocities of the world's bodies are staged in LDS when they fit
-//   5. warm-start pass and sweeps, level by level; the next level's records
-//      are loaded before the barrier
-//   6. lagrangians and velocities are written back
+//   1. degrees d(a) by atomics, prefix sum -> per-body segments; every
+//      (contact, side) is listed in its body's segment; rank r_a(k) = number
+//      of entries of the segment that precede it in key order
+//   2. each contact's constants are packed into a compact record (the
+//      jacobian rows are rebuilt from the normal and the two cross terms,
+//      exactly: every other component is a sign change of the normal)
+//   3. counters are cleared, velocities staged in LDS when they fit
+//   4. dataflow loop over all items
+//   5. lagrangians and velocities are written back
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
@@ -32,73 +49,131 @@
 
 using namespace shp;
 
-#define LEVEL_CAP 2048
-#define SOLVE_THREADS 256
+#define WAIT_LIMIT (1u << 20)
+#define LDS_MAX_BYTES (160 * 1024 - 512)
 
-// packed record fields (doubles), stride contact_cap
+// compact record fields (doubles), stride contact_cap
 enum {
-  PK_JN = 0,
-  PK_JT = 6,
-  PK_BN = 12,
-  PK_RAX = 13,
-  PK_RAY = 14,
-  PK_RBX = 15,
-  PK_RBY = 16,
-  PK_NRX = 17,
-  PK_NRY = 18,
-  PK_MCN = 19,
-  PK_MCT = 20,
-  PK_MU = 21,
-  PK_E = 22,
-  PK_IMLA = 23,
-  PK_IMRA = 24,
-  PK_IMLB = 25,
-  PK_IMRB = 26,
-  PK_LN = 27,
-  PK_LT = 28,
-  PK_FIELDS = 29
+  FL_UX = 0,  // jn[3], jn[4]; jn[0..1] = -u, jt[3..4] = clockwise(u), jt[0..1] = -jt[3..4]
+  FL_UY,
+  FL_JN2,
+  FL_JN5,
+  FL_JT2,
+  FL_JT5,
+  FL_BN,
+  FL_RAX,
+  FL_RAY,
+  FL_RBX,
+  FL_RBY,
+  FL_MCN,
+  FL_MCT,
+  FL_MU,
+  FL_E,
+  FL_IMLA,
+  FL_IMRA,
+  FL_IMLB,
+  FL_IMRB,
+  FL_LN,
+  FL_LT,
+  FL_FIELDS
 };
 
-struct Rec {
-  double jn[6], jt[6];
-  double bn, rax, ray, rbx, rby, nrx, nry, mcn, mct, mu, e;
-  double im[6];
-  double ln, lt;
+struct FRec {
+  double ux, uy, jn2, jn5, jt2, jt5, bn, rax, ray, rbx, rby, mcn, mct, mu, e;
+  double ima, ira, imb, irb, ln, lt;
   int ga, gb, flags;  // flags: 1 a static, 2 b static, 4 warm
 };
 
-__device__ __forceinline__ void load_rec(const Ctx& c, int s, Rec& r) {
+__device__ __forceinline__ void load_frec(const Ctx& c, int k, FRec& r) {
   const double* P = c.pk;
-  long CC = c.contact_cap;
+  const long CC = c.contact_cap;
+  r.ux = P[FL_UX * CC + k];
+  r.uy = P[FL_UY * CC + k];
+  r.jn2 = P[FL_JN2 * CC + k];
+  r.jn5 = P[FL_JN5 * CC + k];
+  r.jt2 = P[FL_JT2 * CC + k];
+  r.jt5 = P[FL_JT5 * CC + k];
+  r.bn = P[FL_BN * CC + k];
+  r.rax = P[FL_RAX * CC + k];
+  r.ray = P[FL_RAY * CC + k];
+  r.rbx = P[FL_RBX * CC + k];
+  r.rby = P[FL_RBY * CC + k];
+  r.mcn = P[FL_MCN * CC + k];
+  r.mct = P[FL_MCT * CC + k];
+  r.mu = P[FL_MU * CC + k];
+  r.e = P[FL_E * CC + k];
+  r.ima = P[FL_IMLA * CC + k];
+  r.ira = P[FL_IMRA * CC + k];
+  r.imb = P[FL_IMLB * CC + k];
+  r.irb = P[FL_IMRB * CC + k];
+  r.ln = P[FL_LN * CC + k];
+  r.lt = P[FL_LT * CC + k];
+  r.ga = c.pk_g[k];
+  r.gb = c.pk_g[CC + k];
+  r.flags = c.pk_f[k];
+}
+
+// The full rows of contact k: non-flipped, jn = (-n, xa_n, n, xb_n) and
+// jt = (-cw(n), xa_t, cw(n), xb_t); flipped, the halves are swapped back
+// (shp_engine.hip, constraint generation), which gives the same shape with
+// u = -n.  So jn[3..4] = u, jn[0..1] = -u, jt[3..4] = clockwise(u) and
+// jt[0..1] = -jt[3..4] in both cases, and negation is exact.
+__device__ __forceinline__ void pack_flow(const Ctx& c, int k) {
+  const long CC = c.contact_cap;
+  const int ga = c.ga[k], gb = c.gb[k];
+  // every value is loaded before the first store, so the loads are not
+  // serialised behind stores the compiler cannot prove disjoint
+  double f[FL_FIELDS];
+  f[FL_UX] = c.jn[3 * CC + k];
+  f[FL_UY] = c.jn[4 * CC + k];
+  f[FL_JN2] = c.jn[2 * CC + k];
+  f[FL_JN5] = c.jn[5 * CC + k];
+  f[FL_JT2] = c.jt[2 * CC + k];
+  f[FL_JT5] = c.jt[5 * CC + k];
+  f[FL_BN] = c.bn[k];
+  f[FL_RAX] = c.rax[k];
+  f[FL_RAY] = c.ray[k];
+  f[FL_RBX] = c.rbx[k];
+  f[FL_RBY] = c.rby[k];
+  f[FL_MCN] = c.mcn[k];
+  f[FL_MCT] = c.mct[k];
+  f[FL_MU] = (c.mu[ga] + c.mu[gb]) / 2.0;  // pairMu (Friction.hs:46-48)
+  f[FL_E] = hs_min(c.e[ga], c.e[gb]);       // min bounciness
+  f[FL_IMLA] = c.iml[ga];
+  f[FL_IMRA] = c.imr[ga];
+  f[FL_IMLB] = c.iml[gb];
+  f[FL_IMRB] = c.imr[gb];
+  f[FL_LN] = c.ln[k];
+  f[FL_LT] = c.lt[k];
+  const uint8_t fl = (uint8_t)((c.is_static[ga] ? 1 : 0) | (c.is_static[gb] ? 2 : 0) | (c.warm[k] ? 4 : 0));
+  double* P = c.pk;
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    r.jn[q] = P[(PK_JN + q) * CC + s];
-    r.jt[q] = P[(PK_JT + q) * CC + s];
-  }
-  r.bn = P[PK_BN * CC + s];
-  r.rax = P[PK_RAX * CC + s];
-  r.ray = P[PK_RAY * CC + s];
-  r.rbx = P[PK_RBX * CC + s];
-  r.rby = P[PK_RBY * CC + s];
-  r.nrx = P[PK_NRX * CC + s];
-  r.nry = P[PK_NRY * CC + s];
-  r.mcn = P[PK_MCN * CC + s];
-  r.mct = P[PK_MCT * CC + s];
-  r.mu = P[PK_MU * CC + s];
-  r.e = P[PK_E * CC + s];
-  double iml_a = P[PK_IMLA * CC + s], imr_a = P[PK_IMRA * CC + s];
-  double iml_b = P[PK_IMLB * CC + s], imr_b = P[PK_IMRB * CC + s];
-  r.im[0] = iml_a;
-  r.im[1] = iml_a;
-  r.im[2] = imr_a;
-  r.im[3] = iml_b;
-  r.im[4] = iml_b;
-  r.im[5] = imr_b;
-  r.ln = P[PK_LN * CC + s];
-  r.lt = P[PK_LT * CC + s];
-  r.ga = c.pk_g[s];
-  r.gb = c.pk_g[CC + s];
-  r.flags = c.pk_f[s];
+  for (int q = 0; q < FL_FIELDS; ++q) P[q * CC + k] = f[q];
+  c.pk_g[k] = ga;
+  c.pk_g[CC + k] = gb;
+  c.pk_f[k] = fl;
+}
+
+__device__ __forceinline__ void rows(const FRec& r, double* jn, double* jt, double* im) {
+  d2 t = clockwise(mk(r.ux, r.uy));
+  jn[0] = -r.ux;
+  jn[1] = -r.uy;
+  jn[2] = r.jn2;
+  jn[3] = r.ux;
+  jn[4] = r.uy;
+  jn[5] = r.jn5;
+  jt[0] = -t.x;
+  jt[1] = -t.y;
+  jt[2] = r.jt2;
+  jt[3] = t.x;
+  jt[4] = t.y;
+  jt[5] = r.jt5;
+  im[0] = r.ima;
+  im[1] = r.ima;
+  im[2] = r.ira;
+  im[3] = r.imb;
+  im[4] = r.imb;
+  im[5] = r.irb;
 }
 
 // Velocity access: LDS (3 doubles per local body) or the global arrays.
@@ -135,294 +210,333 @@ struct Vel {
 
 // useCache (Solvers/Contact.hs:100-114): applySln with the cached lambdas
 template <bool VLDS>
-__device__ __forceinline__ void warm_rec(const Rec& r, const Vel<VLDS>& V) {
-  double v[6];
+__device__ __forceinline__ void warm_frec(const FRec& r, const Vel<VLDS>& V) {
+  double v[6], jn[6], jt[6], im[6];
+  rows(r, jn, jt, im);
   V.load(r.ga, v);
   V.load(r.gb, v + 3);
-  apply_lag(r.ln, r.jn, r.im, v);
-  apply_lag(r.lt, r.jt, r.im, v);
+  apply_lag(r.ln, jn, im, v);
+  apply_lag(r.lt, jt, im, v);
   if (!(r.flags & 1)) V.store(r.ga, v);
   if (!(r.flags & 2)) V.store(r.gb, v + 3);
 }
 
-// improveContactSln (Solvers/Contact.hs:124-143), operation for operation
-// as improve() in shp_solve_common.h, on a packed record.
+// improveContactSln (Solvers/Contact.hs:124-143): contactLagrangian with
+// bounceB (Constraints/Contact.hs:232-253, Restitution.hs:34-47), then
+// solutionProcessor = positive / clampAbs (Constraints/Contact.hs:259-268,
+// SolutionProcessors.hs:427-447), then applySln, on a compact record.  The
+// new lagrangians go to the record and to the packed array.
 template <bool VLDS>
-__device__ __forceinline__ void improve_rec(const Ctx& c, int s, const Rec& r, const Vel<VLDS>& V) {
-  double v[6];
+__device__ __forceinline__ void improve_frec(const Ctx& c, int k, FRec& r, const Vel<VLDS>& V) {
+  double v[6], jn[6], jt[6], im[6];
+  rows(r, jn, jt, im);
   V.load(r.ga, v);
   V.load(r.gb, v + 3);
-  d2 ra = mk(r.rax, r.ray), rb = mk(r.rbx, r.rby), nr = mk(r.nrx, r.nry);
+  d2 ra = mk(r.rax, r.ray), rb = mk(r.rbx, r.rby), nr = mk(r.ux, r.uy);
   d2 closing = add(add(add(neg(mk(v[0], v[1])), zcross(-v[2], ra)), mk(v[3], v[4])), zcross(v[5], rb));
   double bb = hs_min(0.0, r.e * dot(closing, nr));
   double bn = r.bn + bb;
-  double new_n = (-(dot6(r.jn, v) + bn)) / r.mcn;
-  double new_t = (-(dot6(r.jt, v) + 0.0)) / r.mct;
+  double new_n = (-(dot6(jn, v) + bn)) / r.mcn;
+  double new_t = (-(dot6(jt, v) + 0.0)) / r.mct;
   double app_n = hs_max(new_n, -r.ln);
   double acc_n = r.ln + app_n;
   double thr = acc_n * r.mu;
   double acc_t = r.lt + new_t;
   double accp = acc_t > thr ? thr : (acc_t < -thr ? -thr : acc_t);
   double app_t = accp - r.lt;
-  apply_lag(app_n, r.jn, r.im, v);
-  apply_lag(app_t, r.jt, r.im, v);
+  apply_lag(app_n, jn, im, v);
+  apply_lag(app_t, jt, im, v);
   if (!(r.flags & 1)) V.store(r.ga, v);
   if (!(r.flags & 2)) V.store(r.gb, v + 3);
-  long CC = c.contact_cap;
-  c.pk[PK_LN * CC + s] = acc_n;
-  c.pk[PK_LT * CC + s] = accp;
+  const long CC = c.contact_cap;
+  r.ln = acc_n;
+  r.lt = accp;
+  c.pk[FL_LN * CC + k] = acc_n;
+  c.pk[FL_LT * CC + k] = accp;
 }
 
-// Wave 0 assigns levels to contacts [c0, c1), 64 at a time.  `last` holds
-// per local body the level of the latest contact touching it.  Inside a
-// chunk, lane k is raised above every earlier lane j that shares a dynamic
-// body (lane j's level is final when broadcast); only the last lane that
-// touches a body updates `last`.  Returns the maximum level.
-__device__ int assign_levels(const Ctx& c, int c0, int c1, int b0, int* last) {
-  int lane = threadIdx.x & 63;
-  int maxl = 0;
-  for (int base = c0; base < c1; base += 64) {
-    int k = base + lane;
-    bool valid = k < c1;
-    int a = -1, b = -1;
-    if (valid) {
-      int ga = c.ga[k], gb = c.gb[k];
-      a = c.is_static[ga] ? -1 : ga - b0;
-      b = c.is_static[gb] ? -1 : gb - b0;
-    }
-    int la = a >= 0 ? last[a] : 0;
-    int lb = b >= 0 ? last[b] : 0;
-    int lvl = (la > lb ? la : lb) + 1;
-    bool a_last = true, b_last = true;
-    int cnt = c1 - base < 64 ? c1 - base : 64;
-    for (int j = 0; j < cnt; ++j) {
-      int aj = __builtin_amdgcn_readlane(a, j);
-      int bj = __builtin_amdgcn_readlane(b, j);
-      int lj = __builtin_amdgcn_readlane(lvl, j);
-      bool sa = a >= 0 && (a == aj || a == bj);
-      bool sb = b >= 0 && (b == aj || b == bj);
-      if (j < lane && (sa || sb) && lj + 1 > lvl) lvl = lj + 1;
-      if (j > lane) {
-        if (sa) a_last = false;
-        if (sb) b_last = false;
-      }
-    }
-    if (valid) {
-      c.sched_level[k] = lvl;
-      if (a >= 0 && a_last) last[a] = lvl;
-      if (b >= 0 && b_last) last[b] = lvl;
-      if (lvl > maxl) maxl = lvl;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    int o = __shfl_xor(maxl, off);
-    if (o > maxl) maxl = o;
-  }
-  return maxl;
+__device__ __forceinline__ int ld_acquire(int* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void st_release(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Pack contact k's constants into slot s (global slot index).
-__device__ __forceinline__ void pack_contact(const Ctx& c, int k, int s) {
-  double* P = c.pk;
-  long CC = c.contact_cap;
-  int ga = c.ga[k], gb = c.gb[k];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    P[(PK_JN + q) * CC + s] = c.jn[q * CC + k];
-    P[(PK_JT + q) * CC + s] = c.jt[q * CC + k];
+// Exclusive prefix sum of a[0, n) in place by the T threads of the
+// workgroup; wsum holds T / 64 ints of LDS.
+template <int T>
+__device__ void block_scan_exclusive(int* a, int n, int* wsum) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int chunk = (n + T - 1) / T;
+  const int lo = min(n, tid * chunk), hi = min(n, lo + chunk);
+  int s = 0;
+  for (int i = lo; i < hi; ++i) s += a[i];
+  int x = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
   }
-  P[PK_BN * CC + s] = c.bn[k];
-  P[PK_RAX * CC + s] = c.rax[k];
-  P[PK_RAY * CC + s] = c.ray[k];
-  P[PK_RBX * CC + s] = c.rbx[k];
-  P[PK_RBY * CC + s] = c.rby[k];
-  bool fl = c.flip[k] != 0;
-  P[PK_NRX * CC + s] = fl ? -c.cnx[k] : c.cnx[k];
-  P[PK_NRY * CC + s] = fl ? -c.cny[k] : c.cny[k];
-  P[PK_MCN * CC + s] = c.mcn[k];
-  P[PK_MCT * CC + s] = c.mct[k];
-  P[PK_MU * CC + s] = (c.mu[ga] + c.mu[gb]) / 2.0;  // pairMu (Friction.hs:46-48)
-  P[PK_E * CC + s] = hs_min(c.e[ga], c.e[gb]);        // min bounciness
-  P[PK_IMLA * CC + s] = c.iml[ga];
-  P[PK_IMRA * CC + s] = c.imr[ga];
-  P[PK_IMLB * CC + s] = c.iml[gb];
-  P[PK_IMRB * CC + s] = c.imr[gb];
-  P[PK_LN * CC + s] = c.ln[k];
-  P[PK_LT * CC + s] = c.lt[k];
-  c.pk_g[s] = ga;
-  c.pk_g[CC + s] = gb;
-  c.pk_f[s] = (uint8_t)((c.is_static[ga] ? 1 : 0) | (c.is_static[gb] ? 2 : 0) | (c.warm[k] ? 4 : 0));
-}
-
-// Run one pass (warm-start application or a sweep) level by level.  Each
-// thread owns slot s_off[l] + tid of every level (plus further slots of
-// wide levels); the record of its next slot is loaded before the barrier.
-template <bool VLDS, bool WARM>
-__device__ void run_pass(const Ctx& c, int c0, int maxl, const int* s_off, const Vel<VLDS>& V) {
-  int tid = threadIdx.x;
-  Rec r, rn;
-  int s = s_off[1] + tid;
-  bool has = 1 <= maxl && s < s_off[2];
-  if (has) load_rec(c, c0 + s, r);
-  for (int l = 1; l <= maxl; ++l) {
-    int sn = s_off[l + 1] + tid;
-    bool hasn = l < maxl && sn < s_off[l + 2];
-    if (hasn) load_rec(c, c0 + sn, rn);
-    if (has) {
-      if (WARM) {
-        if (r.flags & 4) warm_rec<VLDS>(r, V);
-      } else {
-        improve_rec<VLDS>(c, c0 + s, r, V);
-      }
-    }
-    for (int s2 = s + SOLVE_THREADS; s2 < s_off[l + 1]; s2 += SOLVE_THREADS) {
-      Rec r2;
-      load_rec(c, c0 + s2, r2);
-      if (WARM) {
-        if (r2.flags & 4) warm_rec<VLDS>(r2, V);
-      } else {
-        improve_rec<VLDS>(c, c0 + s2, r2, V);
-      }
-    }
-    __syncthreads();
-    r = rn;
-    has = hasn;
-    s = sn;
-  }
-}
-
-template <bool VLDS>
-__global__ __launch_bounds__(SOLVE_THREADS) void k_solve_levels(Ctx c, int pool_ints) {
-  extern __shared__ int lds[];
-  int* s_off = lds;                            // LEVEL_CAP + 2
-  int* s_cur = lds + (LEVEL_CAP + 2);          // LEVEL_CAP + 2
-  int* pool = lds + 2 * (LEVEL_CAP + 2) + 2;   // 8-byte aligned
-  __shared__ int s_maxl;
-  int w = blockIdx.x;
-  int tid = threadIdx.x;
-  int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
-  if (c1 <= c0) return;
-  int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
-  if (nb > pool_ints || (VLDS && 6 * nb > pool_ints)) {  // world too large: exact serial
-    if (tid == 0) solve_world_serial(c, c0, c1);
-    return;
-  }
-  int* last = pool;
-  tmark(c, 0);
-  for (int i = tid; i < nb; i += SOLVE_THREADS) last[i] = 0;
-  __syncthreads();
-  if (tid < 64) {
-    int m = assign_levels(c, c0, c1, b0, last);
-    if (tid == 0) {
-      s_maxl = m;
-      atomicMax(&c.counters[CNT_LEVELS], m);
-    }
-  }
-  __syncthreads();
-  tmark(c, 1);
-  int maxl = s_maxl;
-  if (maxl > LEVEL_CAP) {
-    if (tid == 0) {
-      atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_SCHEDULE_OVERFLOW);
-      solve_world_serial(c, c0, c1);
-    }
-    return;
-  }
-  for (int i = tid; i <= maxl + 1; i += SOLVE_THREADS) s_cur[i] = 0;
-  __syncthreads();
-  for (int k = c0 + tid; k < c1; k += SOLVE_THREADS) atomicAdd(&s_cur[c.sched_level[k]], 1);
+  if (lane == 63) wsum[wid] = x;
   __syncthreads();
   if (tid == 0) {
     int acc = 0;
-    for (int l = 0; l <= maxl + 1; ++l) {
-      int h = s_cur[l];
-      s_off[l] = acc;
-      s_cur[l] = acc;
-      acc += h;
+    for (int q = 0; q < T / 64; ++q) {
+      int t = wsum[q];
+      wsum[q] = acc;
+      acc += t;
     }
   }
   __syncthreads();
-  for (int k = c0 + tid; k < c1; k += SOLVE_THREADS) {
-    int pos = atomicAdd(&s_cur[c.sched_level[k]], 1);
-    c.sched_order[c0 + pos] = k;
-    pack_contact(c, k, c0 + pos);
+  int run = wsum[wid] + x - s;
+  for (int i = lo; i < hi; ++i) {
+    int t = a[i];
+    a[i] = run;
+    run += t;
   }
+  __syncthreads();
+}
+
+// Ranks of every (contact, dynamic side) of world contacts [c0, c0 + n):
+// flow_rk[k] = {r_a, d(a), r_b, d(b)}, with r = -1 for a static side.
+// off and cur hold nb ints each (LDS or global, local body index).
+template <int T>
+__device__ void compute_ranks(const Ctx& c, int c0, int n, int b0, int nb, int* off, int* cur,
+                              int* wsum) {
+  const int tid = threadIdx.x;
+  int* list = c.flow_list + 2L * c0;
+  for (int i = tid; i < nb; i += T) off[i] = 0;
+  __syncthreads();
+  for (int i = tid; i < n; i += T) {
+    int ga = c.ga[c0 + i], gb = c.gb[c0 + i];
+    if (!c.is_static[ga]) atomicAdd(&off[ga - b0], 1);
+    if (!c.is_static[gb]) atomicAdd(&off[gb - b0], 1);
+  }
+  __syncthreads();
+  block_scan_exclusive<T>(off, nb, wsum);
+  for (int i = tid; i < nb; i += T) cur[i] = off[i];
+  __syncthreads();
+  for (int i = tid; i < n; i += T) {
+    int ga = c.ga[c0 + i], gb = c.gb[c0 + i];
+    if (!c.is_static[ga]) list[atomicAdd(&cur[ga - b0], 1)] = 2 * i;
+    if (!c.is_static[gb]) list[atomicAdd(&cur[gb - b0], 1)] = 2 * i + 1;
+  }
+  __syncthreads();
+  // segment of body x: [off[x], cur[x]); entries are 2 * contact + side,
+  // so ascending entries are ascending key order
+  for (int i = tid; i < n; i += T) {
+    int4 rk = make_int4(-1, 0, -1, 0);
+    int g[2] = {c.ga[c0 + i], c.gb[c0 + i]};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (c.is_static[g[s]]) continue;
+      int x = g[s] - b0, lo = off[x], hi = cur[x], me = 2 * i + s, r = 0;
+      for (int j = lo; j < hi; ++j) r += list[j] < me;
+      if (s == 0) {
+        rk.x = r;
+        rk.y = hi - lo;
+      } else {
+        rk.z = r;
+        rk.w = hi - lo;
+      }
+    }
+    c.flow_rk[c0 + i] = rk;
+    pack_flow(c, c0 + i);
+  }
+  __syncthreads();
+}
+
+// The dataflow loop over all items of the world (see the file comment).
+// cnt: per-body counters (local index), cleared; progress: LDS int, 0.
+template <bool VLDS, int T>
+__device__ void run_flow(const Ctx& c, int c0, int n, int b0, int* cnt, const Vel<VLDS>& V,
+                         int* progress) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int P = c.iterations;
+  int i = tid, p = 0;
+  bool live = i < n;
+  FRec r;
+  int4 rk = make_int4(-1, 0, -1, 0);
+  if (live) {
+    load_frec(c, c0 + i, r);
+    rk = c.flow_rk[c0 + i];
+  }
+  unsigned spins = 0;
+  int seen = -1;
+  while (live) {
+    const int ta = p * rk.y + rk.x, tb = p * rk.w + rk.z;
+    const int la = r.ga - b0, lb = r.gb - b0;
+    bool ready = true;
+    if (rk.x >= 0) ready = ld_acquire(cnt + la) == ta;
+    if (ready && rk.z >= 0) ready = ld_acquire(cnt + lb) == tb;
+    if (ready) {
+      if (p == 0) {
+        if (r.flags & 4) warm_frec<VLDS>(r, V);
+      } else {
+        improve_frec<VLDS>(c, c0 + i, r, V);
+      }
+      if (rk.x >= 0) st_release(cnt + la, ta + 1);
+      if (rk.z >= 0) st_release(cnt + lb, tb + 1);
+      int ni = i + T, np = p;
+      if (ni >= n) {
+        ni = tid;
+        ++np;
+      }
+      if (np > P) {
+        live = false;
+      } else {
+        if (ni != i) {
+          load_frec(c, c0 + ni, r);
+          rk = c.flow_rk[c0 + ni];
+        }
+        i = ni;
+        p = np;
+      }
+      spins = 0;
+    }
+    const unsigned long long m = __ballot(ready);
+    if (m) {
+      if (lane == __ffsll((long long)m) - 1) atomicAdd(progress, __popcll(m));
+    } else {
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (!ready) {
+      int pr = __hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (pr != seen) {
+        seen = pr;
+        spins = 0;
+      } else if (++spins > WAIT_LIMIT) {
+        atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_SOLVER_STALL);
+        live = false;
+      }
+    }
+  }
+}
+
+// CLDS: per-body offsets/counters in LDS (else in flow_off/flow_cur);
+// VLDS: velocities in LDS.
+template <bool CLDS, bool VLDS, int T>
+__global__ __launch_bounds__(T) void k_solve_flow(Ctx c) {
+  extern __shared__ double lds_d[];
+  __shared__ int wsum[T / 64];
+  __shared__ int progress;
+  const int w = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
+  if (c1 <= c0) return;
+  const int n = c1 - c0;
+  const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
+  double* vel = lds_d;
+  int* ip = (int*)(lds_d + (VLDS ? 3 * nb : 0));
+  int* off = CLDS ? ip : c.flow_off + b0;
+  int* cur = CLDS ? ip + nb : c.flow_cur + b0;
+  tmark(c, 0);
+  compute_ranks<T>(c, c0, n, b0, nb, off, cur, wsum);
+  tmark(c, 1);
+  int* cnt = off;
+  for (int i = tid; i < nb; i += T) cnt[i] = 0;
+  if (tid == 0) progress = 0;
   Vel<VLDS> V;
-  V.lds = (double*)pool;
+  V.lds = vel;
   V.c = &c;
   V.b0 = b0;
   if (VLDS) {
-    __syncthreads();  // `last` is dead; the pool now holds velocities
-    for (int i = tid; i < nb; i += SOLVE_THREADS) {
-      V.lds[3 * i] = c.vx[b0 + i];
-      V.lds[3 * i + 1] = c.vy[b0 + i];
-      V.lds[3 * i + 2] = c.w[b0 + i];
+    for (int i = tid; i < nb; i += T) {
+      vel[3 * i] = c.vx[b0 + i];
+      vel[3 * i + 1] = c.vy[b0 + i];
+      vel[3 * i + 2] = c.w[b0 + i];
     }
   }
-  __threadfence_block();
   __syncthreads();
   tmark(c, 2);
-  run_pass<VLDS, true>(c, c0, maxl, s_off, V);
   tmark(c, 3);
-  for (int it = 0; it < c.iterations; ++it) run_pass<VLDS, false>(c, c0, maxl, s_off, V);
+  run_flow<VLDS, T>(c, c0, n, b0, cnt, V, &progress);
+  __syncthreads();
   tmark(c, 4);
-  long CC = c.contact_cap;
-  for (int s = tid; s < c1 - c0; s += SOLVE_THREADS) {
-    int k = c.sched_order[c0 + s];
-    c.ln[k] = c.pk[PK_LN * CC + c0 + s];
-    c.lt[k] = c.pk[PK_LT * CC + c0 + s];
+  const long CC = c.contact_cap;
+  for (int k = c0 + tid; k < c1; k += T) {
+    c.ln[k] = c.pk[FL_LN * CC + k];
+    c.lt[k] = c.pk[FL_LT * CC + k];
   }
   if (VLDS) {
-    for (int i = tid; i < nb; i += SOLVE_THREADS) {
+    for (int i = tid; i < nb; i += T) {
       if (c.is_static[b0 + i]) continue;
-      c.vx[b0 + i] = V.lds[3 * i];
-      c.vy[b0 + i] = V.lds[3 * i + 1];
-      c.w[b0 + i] = V.lds[3 * i + 2];
+      c.vx[b0 + i] = vel[3 * i];
+      c.vy[b0 + i] = vel[3 * i + 1];
+      c.w[b0 + i] = vel[3 * i + 2];
     }
   }
   __syncthreads();
   tmark(c, 5);
 }
 
-#define LDS_MAX_BYTES (160 * 1024 - 512)
+template <bool CLDS, bool VLDS>
+static const void* flow_kernel(int threads) {
+  return threads == 1024 ? (const void*)k_solve_flow<CLDS, VLDS, 1024>
+                         : (const void*)k_solve_flow<CLDS, VLDS, 256>;
+}
 
 shp_status solver_alloc(shp_batch* b) {
   Ctx& c = b->ctx;
-  long CC = c.contact_cap;
-  void** bufs[] = {(void**)&c.sched_level, (void**)&c.sched_order, (void**)&c.pk, (void**)&c.pk_g,
-                   (void**)&c.pk_f, (void**)&c.fvx, (void**)&c.fvy, (void**)&c.fw};
-  size_t sizes[] = {CC * 4UL, CC * 4UL, (size_t)PK_FIELDS * CC * 8, 2UL * CC * 4, (size_t)CC,
-                    (size_t)c.NB * 4, (size_t)c.NB * 4, (size_t)c.NB * 4};
-  for (int k = 0; k < 8; ++k) {
-    if (hipMalloc(bufs[k], sizes[k] ? sizes[k] : 4) != hipSuccess) return SHP_E_OOM;
+  const long CC = c.contact_cap;
+  void** bufs[] = {(void**)&c.sched_level, (void**)&c.sched_order, (void**)&c.pk,
+                   (void**)&c.pk_g,        (void**)&c.pk_f,        (void**)&c.fvx,
+                   (void**)&c.fvy,         (void**)&c.fw,          (void**)&c.flow_list,
+                   (void**)&c.flow_rk,     (void**)&c.flow_off,    (void**)&c.flow_cur};
+  // pk also holds the coloured solver's records (28 FP32 words + 2 FP64
+  // lambdas per slot, less than 29 doubles)
+  size_t sizes[] = {CC * 4UL,        CC * 4UL,        29UL * CC * 8,   2UL * CC * 4,
+                    (size_t)CC,      (size_t)c.NB * 4, (size_t)c.NB * 4, (size_t)c.NB * 4,
+                    2UL * CC * 4,    CC * 16UL,       (size_t)c.NB * 4, (size_t)c.NB * 4};
+  static_assert(FL_FIELDS <= 29, "compact record exceeds the packed buffer");
+  for (int k = 0; k < 12; ++k) {
+    if (hipMalloc(bufs[k], sizes[k] ? sizes[k] : 16) != hipSuccess) return SHP_E_OOM;
     b->allocs.push_back(*bufs[k]);
   }
   int max_nb = 0;
   for (int w = 0; w < c.W; ++w) max_nb = std::max(max_nb, b->h_body_off[w + 1] - b->h_body_off[w]);
-  // LDS: level offsets/cursors + a pool for `last` (4 B/body) and, when it
-  // fits, the velocities (24 B/body)
-  size_t fixed = (2 * (LEVEL_CAP + 2) + 2) * 4;
-  size_t vel = fixed + 24UL * max_nb;
-  size_t lst = fixed + 4UL * max_nb;
-  if (vel <= LDS_MAX_BYTES) {
+  // LDS per world: velocities (24 B/body) and offsets + cursors (8 B/body)
+  // when they fit; offsets + cursors alone; else both in global memory
+  size_t both = 32UL * max_nb, cnt_only = 8UL * max_nb;
+  if (both <= LDS_MAX_BYTES) {
+    b->solve_clds = true;
     b->solve_vlds = true;
-    b->solve_lds = vel;
-  } else {
+    b->solve_lds = both;
+  } else if (cnt_only <= LDS_MAX_BYTES) {
+    b->solve_clds = true;
     b->solve_vlds = false;
-    b->solve_lds = std::min<size_t>(std::max<size_t>(lst, fixed + 4 * 1024), LDS_MAX_BYTES);
+    b->solve_lds = cnt_only;
+  } else {
+    b->solve_clds = false;
+    b->solve_vlds = false;
+    b->solve_lds = 0;
   }
-  auto fn = b->solve_vlds ? (const void*)k_solve_levels<true> : (const void*)k_solve_levels<false>;
-  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->solve_lds) !=
-      hipSuccess)
+  // one lane per contact for worlds up to ~1k contacts per 256 lanes; a
+  // large world gets 16 waves
+  b->solve_threads = max_nb > 2048 ? 1024 : 256;
+  const void* fn = b->solve_clds ? (b->solve_vlds ? flow_kernel<true, true>(b->solve_threads)
+                                                  : flow_kernel<true, false>(b->solve_threads))
+                                 : flow_kernel<false, false>(b->solve_threads);
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)std::max<size_t>(b->solve_lds, 16)) != hipSuccess)
     return SHP_E_HIP;
   return colored_alloc(b);
 }
 
-void launch_ordered_solver(shp_batch* b) {
-  int pool_ints = (int)((b->solve_lds - (2 * (LEVEL_CAP + 2) + 2) * 4) / 4);
-  if (b->solve_vlds)
-    k_solve_levels<true><<<b->ctx.W, SOLVE_THREADS, b->solve_lds, b->stream>>>(b->ctx, pool_ints);
+template <int T>
+static void launch_flow(shp_batch* b) {
+  const size_t lds = std::max<size_t>(b->solve_lds, 16);
+  if (b->solve_clds && b->solve_vlds)
+    k_solve_flow<true, true, T><<<b->ctx.W, T, lds, b->stream>>>(b->ctx);
+  else if (b->solve_clds)
+    k_solve_flow<true, false, T><<<b->ctx.W, T, lds, b->stream>>>(b->ctx);
   else
-    k_solve_levels<false><<<b->ctx.W, SOLVE_THREADS, b->solve_lds, b->stream>>>(b->ctx, pool_ints);
+    k_solve_flow<false, false, T><<<b->ctx.W, T, lds, b->stream>>>(b->ctx);
+}
+
+void launch_ordered_solver(shp_batch* b) {
+  if (b->solve_threads == 1024)
+    launch_flow<1024>(b);
+  else
+    launch_flow<256>(b);
 }

@@ -33,6 +33,8 @@ def assert_bitwise(name, got, want, step):
 
 
 def compare_state(gpu, ref, step, world=0):
+    flags = gpu.stats()["flags"]
+    assert flags == 0, f"step flags {flags:#x} at step {step}"
     assert_bitwise("pairs", gpu.pairs(world), ref.pairs(), step)
     gc, rc = gpu.contacts(world), ref.contacts()
     for k in ("keys", "flip", "geom", "lam", "warm"):
@@ -157,3 +159,31 @@ def test_grid_broadphase_bitwise_worlds(orc):
     worlds.append(scenes.pyramid().worlds[0])
     worlds.append(scenes.reference_stacks().worlds[0])
     _run_grid(orc, worlds, scenes.SceneConfig(iterations=3), steps=40, every=10)
+
+
+def test_plank_chain_bitwise(orc):
+    """A dynamic 60x0.5 plank on 40 static posts under 50 loose boxes: the
+    plank touches ~130 contacts, so its items form one long chain through
+    every pass (the longest wait the dataflow schedule can see)."""
+    b = scenes._Builder()
+    post = scenes.rectangle_vertices(0.6, 1.0)
+    for k in range(40):
+        b.add(post, (-29.25 + 1.5 * k, 0.5), mu=0.6)
+    b.add_dynamic(scenes.rectangle_vertices(60.0, 0.5), (0.0, 1.245), mu=0.6)
+    box = scenes.rectangle_vertices(0.9, 0.9)
+    for k in range(50):
+        b.add_dynamic(box, (-29.4 + 1.2 * k, 1.94), mu=0.4)
+    cfg = scenes.SceneConfig(iterations=8)
+    ref = run_parity(orc, b.build(), cfg, steps=60, every=3)
+    assert ref.stats()["contacts"] > 150
+
+
+def test_large_world_global_counters_bitwise(orc):
+    """24000 overlapping polygons in one world: more bodies than the
+    per-body counters of LDS hold, so the ordered solver keeps them in
+    global memory."""
+    rng = np.random.Generator(np.random.Philox(77))
+    w = scenes.grid_world(rng, 24000, cols=200, spacing=1.05, y0=1.0, square_frac=0.5, vel=0.3)
+    cfg = scenes.SceneConfig(iterations=3)
+    ref = run_parity(orc, w, cfg, steps=4, every=2)
+    assert ref.stats()["contacts"] > 20000
