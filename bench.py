@@ -214,6 +214,9 @@ def run_mode(scene, solver, warm, steps, warmup, graph=True, model=None, broadph
 
 def quality(st):
     return {"max_penetration": st["run_max_depth"],
+            "mean_step_max_penetration": st["mean_max_depth"],
+            "mean_step_normal_residual": st["mean_normal_residual"],
+            "mean_step_friction_residual": st["mean_friction_residual"],
             "normal_residual": st["run_normal_residual"],
             "friction_residual": st["run_friction_residual"]}
 

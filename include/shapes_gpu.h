@@ -113,6 +113,10 @@ typedef struct shp_step_stats {
   double run_max_depth;
   double run_normal_residual;
   double run_friction_residual;
+  /* Means over the steps since creation of the three per-step maxima. */
+  double mean_max_depth;
+  double mean_normal_residual;
+  double mean_friction_residual;
 } shp_step_stats;
 
 /* ---- engine lifecycle (initEngine / runEngine, Engine/Main.hs:43-98) ---- */
@@ -228,9 +232,9 @@ shp_status shp_batch_set_timing(shp_batch* b, int32_t enable);
 shp_status shp_batch_read_timing(shp_batch* b, double* ms_per_phase,
                                  int32_t* steps);
 /* Average microseconds per timed step of the solve's sub-phases in world 0:
- * ordered: 0 per-body ranks + record packing, 1 counter clearing and
- * velocity staging, 2 (empty), 3 warm-start pass and sweeps (one dataflow
- * loop), 4 write-back; coloured: 0 colouring, 1 bucketing + packing +
+ * ordered: 0 per-body ranks, 1 counter clearing and velocity staging,
+ * 2 levels + level sort (level-synchronous worlds) + record packing, 3
+ * warm-start pass and sweeps, 4 write-back; coloured: 0 colouring, 1 bucketing + packing +
  * velocity staging, 2 warm-start pass, 3 sweeps, 4 write-back. */
 shp_status shp_batch_read_solve_profile(shp_batch* b, double* us5);
 

@@ -11,9 +11,18 @@
 // Jones-Plassmann rounds: in round r every uncoloured manifold whose hashed
 // priority is the largest among the uncoloured manifolds on each of its
 // dynamic bodies takes colour r.  Velocities and constraint rows are FP32
-// (velocities staged in LDS); the accumulated lagrangians stay FP64; each
-// body's FP64 velocity receives the FP32 change of the solve, so bodies
-// without contacts keep their FP64 velocity exactly.
+// (velocities staged in LDS when they fit); the accumulated lagrangians stay
+// FP64; each body's FP64 velocity receives the FP32 change of the solve, so
+// bodies without contacts keep their FP64 velocity exactly.
+//
+// Records are per manifold: the normal, materials, inverse masses and
+// bodies once, then per contact the two cross terms of each row, bias,
+// lever arms, effective masses and the FP64 lagrangians.  The rows are
+// rebuilt from them exactly (jn = (-u, jn2, u, jn5), jt = (-cw(u), jt2,
+// cw(u), jt5) with u the row normal; negation commutes with rounding to
+// FP32).  A lane keeps its slot of a colour in every pass, so the record
+// of its manifold of the next colour (lagrangians included) is loaded
+// before the barrier that ends the current colour.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,31 +34,31 @@
 using namespace shp;
 
 #define COLOR_CAP 256
-#define COLOR_THREADS 256
 #define NO_BODY 0xFFFFu
+#define LDS_MAX_BYTES (160 * 1024 - 512)
 
-// packed FP32 record fields, stride contact_cap; lambdas are FP64 and live
-// after the FP32 fields in the same buffer
+// per-contact FP32 fields (stride contact_cap), then the FP64 lagrangians,
+// then the per-manifold FP32 fields (stride pair_cap); manifold bodies in
+// pk_g (2 x pair_cap), flags in pk_f, the contact of a slot in pk_k
 enum {
-  PF_JN = 0,
-  PF_JT = 6,
-  PF_BN = 12,
-  PF_RAX = 13,
-  PF_RAY = 14,
-  PF_RBX = 15,
-  PF_RBY = 16,
-  PF_NRX = 17,
-  PF_NRY = 18,
-  PF_MCN = 19,
-  PF_MCT = 20,
-  PF_MU = 21,
-  PF_E = 22,
-  PF_IMLA = 23,
-  PF_IMRA = 24,
-  PF_IMLB = 25,
-  PF_IMRB = 26,
-  PF_FIELDS = 28  // even, so the FP64 lambdas that follow stay 8-byte aligned
+  CF_JN2 = 0,
+  CF_JN5,
+  CF_JT2,
+  CF_JT5,
+  CF_BN,
+  CF_RAX,
+  CF_RAY,
+  CF_RBX,
+  CF_RBY,
+  CF_MCN,
+  CF_MCT,
+  CF_FIELDS,
+  CF_LAMBDA = 12  // float offset of the lambdas (x contact_cap), 8-byte aligned
 };
+enum { MF_UX = 0, MF_UY, MF_MU, MF_E, MF_IMLA, MF_IMRA, MF_IMLB, MF_IMRB, MF_FIELDS };
+// manifold flags: 1 a static, 2 b static, 4 first contact warm, 8 second
+// contact present, 16 second contact warm
+enum { MFL_A_STATIC = 1, MFL_B_STATIC = 2, MFL_WARM0 = 4, MFL_TWO = 8, MFL_WARM1 = 16 };
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
   x ^= x >> 16;
@@ -64,92 +73,123 @@ __device__ __forceinline__ float dot6f(const float* a, const float* b) {
   return ((((a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]) + a[3] * b[3]) + a[4] * b[4]) + a[5] * b[5];
 }
 
-struct RecF {
-  float jn[6], jt[6];
-  float bn, rax, ray, rbx, rby, nrx, nry, mcn, mct, mu, e;
-  float im[6];
-  double ln, lt;
-  int ga, gb, flags;  // 1 a static, 2 b static, 4 warm, 8 second contact follows
-};
-
 struct PackF {
-  float* f;        // PF_FIELDS x CC
-  double* lam;     // 2 x CC (normal, friction)
-  long CC;
+  float* f;      // CF_FIELDS x CC, then lambdas, then MF_FIELDS x PC
+  double* lam;   // 2 x CC (normal, friction)
+  float* mf;     // MF_FIELDS x PC
+  long CC, PC;
 };
 
 __device__ __forceinline__ PackF packf(const Ctx& c) {
   PackF p;
   p.CC = c.contact_cap;
+  p.PC = c.pair_cap;
   p.f = (float*)c.pk;
-  p.lam = (double*)(p.f + (long)PF_FIELDS * p.CC);
+  p.lam = (double*)(p.f + (long)CF_LAMBDA * p.CC);
+  p.mf = p.f + (long)(CF_LAMBDA + 4) * p.CC;
   return p;
 }
 
-__device__ __forceinline__ void load_recf(const Ctx& c, const PackF& P, int s, RecF& r) {
-  long CC = P.CC;
+struct ConF {
+  float jn2, jn5, jt2, jt5, bn, rax, ray, rbx, rby, mcn, mct;
+  double ln, lt;
+};
+
+struct ManF {
+  float ux, uy, mu, e, ima, ira, imb, irb;
+  int ga, gb, flags;
+  ConF k[2];
+};
+
+// Manifold M (world offset included) and its contact slots 2M, 2M + 1.
+__device__ __forceinline__ void load_man(const Ctx& c, const PackF& P, int M, ManF& m) {
+  const long PC = P.PC, CC = P.CC;
+  m.ux = P.mf[MF_UX * PC + M];
+  m.uy = P.mf[MF_UY * PC + M];
+  m.mu = P.mf[MF_MU * PC + M];
+  m.e = P.mf[MF_E * PC + M];
+  m.ima = P.mf[MF_IMLA * PC + M];
+  m.ira = P.mf[MF_IMRA * PC + M];
+  m.imb = P.mf[MF_IMLB * PC + M];
+  m.irb = P.mf[MF_IMRB * PC + M];
+  m.ga = c.pk_g[M];
+  m.gb = c.pk_g[PC + M];
+  m.flags = c.pk_f[M];
+  const int nk = (m.flags & MFL_TWO) ? 2 : 1;
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    r.jn[q] = P.f[(PF_JN + q) * CC + s];
-    r.jt[q] = P.f[(PF_JT + q) * CC + s];
+  for (int j = 0; j < 2; ++j) {
+    if (j >= nk) break;
+    const long s = 2L * M + j;
+    ConF& k = m.k[j];
+    k.jn2 = P.f[CF_JN2 * CC + s];
+    k.jn5 = P.f[CF_JN5 * CC + s];
+    k.jt2 = P.f[CF_JT2 * CC + s];
+    k.jt5 = P.f[CF_JT5 * CC + s];
+    k.bn = P.f[CF_BN * CC + s];
+    k.rax = P.f[CF_RAX * CC + s];
+    k.ray = P.f[CF_RAY * CC + s];
+    k.rbx = P.f[CF_RBX * CC + s];
+    k.rby = P.f[CF_RBY * CC + s];
+    k.mcn = P.f[CF_MCN * CC + s];
+    k.mct = P.f[CF_MCT * CC + s];
+    k.ln = P.lam[s];
+    k.lt = P.lam[CC + s];
   }
-  r.bn = P.f[PF_BN * CC + s];
-  r.rax = P.f[PF_RAX * CC + s];
-  r.ray = P.f[PF_RAY * CC + s];
-  r.rbx = P.f[PF_RBX * CC + s];
-  r.rby = P.f[PF_RBY * CC + s];
-  r.nrx = P.f[PF_NRX * CC + s];
-  r.nry = P.f[PF_NRY * CC + s];
-  r.mcn = P.f[PF_MCN * CC + s];
-  r.mct = P.f[PF_MCT * CC + s];
-  r.mu = P.f[PF_MU * CC + s];
-  r.e = P.f[PF_E * CC + s];
-  float ia = P.f[PF_IMLA * CC + s], ra = P.f[PF_IMRA * CC + s];
-  float ib = P.f[PF_IMLB * CC + s], rb = P.f[PF_IMRB * CC + s];
-  r.im[0] = ia;
-  r.im[1] = ia;
-  r.im[2] = ra;
-  r.im[3] = ib;
-  r.im[4] = ib;
-  r.im[5] = rb;
-  r.ln = P.lam[s];
-  r.lt = P.lam[CC + s];
-  r.ga = c.pk_g[s];
-  r.gb = c.pk_g[CC + s];
-  r.flags = c.pk_f[s];
 }
 
-__device__ __forceinline__ void pack_contact_f(const Ctx& c, const PackF& P, int k, int s, int extra) {
-  long CC = P.CC;
-  int ga = c.ga[k], gb = c.gb[k];
+// Pack the <= 2 contacts [k0, k1) of a pair as manifold M.  Every value is
+// loaded before the first store.
+__device__ __forceinline__ void pack_man(const Ctx& c, const PackF& P, int k0, int k1, int M) {
+  const long PC = P.PC, CC = P.CC;
+  const int ga = c.ga[k0], gb = c.gb[k0];
+  float mf[MF_FIELDS];
+  mf[MF_UX] = (float)c.jn[3 * CC + k0];  // the row normal (flip applied)
+  mf[MF_UY] = (float)c.jn[4 * CC + k0];
+  mf[MF_MU] = (float)((c.mu[ga] + c.mu[gb]) / 2.0);
+  mf[MF_E] = (float)hs_min(c.e[ga], c.e[gb]);
+  mf[MF_IMLA] = (float)c.iml[ga];
+  mf[MF_IMRA] = (float)c.imr[ga];
+  mf[MF_IMLB] = (float)c.iml[gb];
+  mf[MF_IMRB] = (float)c.imr[gb];
+  const bool two = k1 - k0 > 1;
+  int flags = (c.is_static[ga] ? MFL_A_STATIC : 0) | (c.is_static[gb] ? MFL_B_STATIC : 0) |
+              (c.warm[k0] ? MFL_WARM0 : 0) | (two ? MFL_TWO : 0);
+  float cf[2][CF_FIELDS];
+  double lam[2][2];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    P.f[(PF_JN + q) * CC + s] = (float)c.jn[q * CC + k];
-    P.f[(PF_JT + q) * CC + s] = (float)c.jt[q * CC + k];
+  for (int j = 0; j < 2; ++j) {
+    if (j == 1 && !two) break;
+    const int k = k0 + j;
+    cf[j][CF_JN2] = (float)c.jn[2 * CC + k];
+    cf[j][CF_JN5] = (float)c.jn[5 * CC + k];
+    cf[j][CF_JT2] = (float)c.jt[2 * CC + k];
+    cf[j][CF_JT5] = (float)c.jt[5 * CC + k];
+    cf[j][CF_BN] = (float)c.bn[k];
+    cf[j][CF_RAX] = (float)c.rax[k];
+    cf[j][CF_RAY] = (float)c.ray[k];
+    cf[j][CF_RBX] = (float)c.rbx[k];
+    cf[j][CF_RBY] = (float)c.rby[k];
+    cf[j][CF_MCN] = (float)c.mcn[k];
+    cf[j][CF_MCT] = (float)c.mct[k];
+    lam[j][0] = c.ln[k];
+    lam[j][1] = c.lt[k];
+    if (j == 1 && c.warm[k]) flags |= MFL_WARM1;
   }
-  P.f[PF_BN * CC + s] = (float)c.bn[k];
-  P.f[PF_RAX * CC + s] = (float)c.rax[k];
-  P.f[PF_RAY * CC + s] = (float)c.ray[k];
-  P.f[PF_RBX * CC + s] = (float)c.rbx[k];
-  P.f[PF_RBY * CC + s] = (float)c.rby[k];
-  bool fl = c.flip[k] != 0;
-  P.f[PF_NRX * CC + s] = (float)(fl ? -c.cnx[k] : c.cnx[k]);
-  P.f[PF_NRY * CC + s] = (float)(fl ? -c.cny[k] : c.cny[k]);
-  P.f[PF_MCN * CC + s] = (float)c.mcn[k];
-  P.f[PF_MCT * CC + s] = (float)c.mct[k];
-  P.f[PF_MU * CC + s] = (float)((c.mu[ga] + c.mu[gb]) / 2.0);
-  P.f[PF_E * CC + s] = (float)hs_min(c.e[ga], c.e[gb]);
-  P.f[PF_IMLA * CC + s] = (float)c.iml[ga];
-  P.f[PF_IMRA * CC + s] = (float)c.imr[ga];
-  P.f[PF_IMLB * CC + s] = (float)c.iml[gb];
-  P.f[PF_IMRB * CC + s] = (float)c.imr[gb];
-  P.lam[s] = c.ln[k];
-  P.lam[CC + s] = c.lt[k];
-  c.pk_g[s] = ga;
-  c.pk_g[CC + s] = gb;
-  c.pk_k[s] = k;
-  c.pk_f[s] = (uint8_t)((c.is_static[ga] ? 1 : 0) | (c.is_static[gb] ? 2 : 0) | (c.warm[k] ? 4 : 0) |
-                        (extra ? 8 : 0));
+#pragma unroll
+  for (int q = 0; q < MF_FIELDS; ++q) P.mf[q * PC + M] = mf[q];
+  c.pk_g[M] = ga;
+  c.pk_g[PC + M] = gb;
+  c.pk_f[M] = (uint8_t)flags;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    if (j == 1 && !two) break;
+    const long s = 2L * M + j;
+#pragma unroll
+    for (int q = 0; q < CF_FIELDS; ++q) P.f[q * CC + s] = cf[j][q];
+    P.lam[s] = lam[j][0];
+    P.lam[CC + s] = lam[j][1];
+    c.pk_k[s] = k0 + j;
+  }
 }
 
 __device__ __forceinline__ void applyf(float l, const float* j, const float* im, float* v) {
@@ -157,372 +197,194 @@ __device__ __forceinline__ void applyf(float l, const float* j, const float* im,
   for (int q = 0; q < 6; ++q) v[q] = v[q] + ((j[q] * l) * im[q]);
 }
 
-__device__ __forceinline__ void vload(const float* V, int la, int lb, float* v) {
-  v[0] = V[3 * la];
-  v[1] = V[3 * la + 1];
-  v[2] = V[3 * la + 2];
-  v[3] = V[3 * lb];
-  v[4] = V[3 * lb + 1];
-  v[5] = V[3 * lb + 2];
-}
-
-__device__ __forceinline__ void vstore(float* V, int la, int lb, int flags, const float* v) {
-  if (!(flags & 1)) {
-    V[3 * la] = v[0];
-    V[3 * la + 1] = v[1];
-    V[3 * la + 2] = v[2];
-  }
-  if (!(flags & 2)) {
-    V[3 * lb] = v[3];
-    V[3 * lb + 1] = v[4];
-    V[3 * lb + 2] = v[5];
-  }
-}
-
-// useCache's applySln in FP32
-__device__ __forceinline__ void warm_f(const RecF& r, float* V, int b0) {
-  if (!(r.flags & 4)) return;
-  float v[6];
-  int la = r.ga - b0, lb = r.gb - b0;
-  vload(V, la, lb, v);
-  applyf((float)r.ln, r.jn, r.im, v);
-  applyf((float)r.lt, r.jt, r.im, v);
-  vstore(V, la, lb, r.flags, v);
-}
-
-// improveContactSln with FP32 velocities/rows and FP64 accumulators
-__device__ __forceinline__ void improve_f(const PackF& P, int s, const RecF& r, float* V, int b0) {
-  float v[6];
-  int la = r.ga - b0, lb = r.gb - b0;
-  vload(V, la, lb, v);
-  float cx = ((-v[0] + v[2] * r.ray) + v[3]) - v[5] * r.rby;  // bounceB closing velocity
-  float cy = ((-v[1] - v[2] * r.rax) + v[4]) + v[5] * r.rbx;
-  float bb = r.e * (cx * r.nrx + cy * r.nry);
-  bb = 0.0f <= bb ? 0.0f : bb;
-  float bn = r.bn + bb;
-  float new_n = -(dot6f(r.jn, v) + bn) / r.mcn;
-  float new_t = -dot6f(r.jt, v) / r.mct;
-  double app_n = hs_max((double)new_n, -r.ln);
-  double acc_n = r.ln + app_n;
-  double thr = acc_n * (double)r.mu;
-  double acc_t = r.lt + (double)new_t;
-  double accp = acc_t > thr ? thr : (acc_t < -thr ? -thr : acc_t);
-  double app_t = accp - r.lt;
-  applyf((float)app_n, r.jn, r.im, v);
-  applyf((float)app_t, r.jt, r.im, v);
-  vstore(V, la, lb, r.flags, v);
-  P.lam[s] = acc_n;
-  P.lam[P.CC + s] = accp;
-}
-
-// Solve one manifold (1 or 2 packed contacts starting at slot s).
-template <bool WARM>
-__device__ __forceinline__ void manifold(const Ctx& c, const PackF& P, int s, const RecF& r0,
-                                         float* V, int b0) {
-  if (WARM) warm_f(r0, V, b0);
-  else improve_f(P, s, r0, V, b0);
-  if (r0.flags & 8) {
-    RecF r1;
-    load_recf(c, P, s + 1, r1);
-    if (WARM) warm_f(r1, V, b0);
-    else improve_f(P, s + 1, r1, V, b0);
-  }
-}
-
-// One workgroup per world.  LDS pool: during colouring s_best (4 B/body)
-// and per-manifold bodies + colour (6 B/pair); during the solve FP32
-// velocities (12 B/body).
-__global__ __launch_bounds__(COLOR_THREADS) void k_solve_colored(Ctx c, int pool_bytes) {
-  extern __shared__ int lds[];
-  int* s_off = lds;                      // COLOR_CAP + 3
-  int* s_cur = lds + (COLOR_CAP + 3);    // COLOR_CAP + 3
-  unsigned char* pool = (unsigned char*)(lds + 2 * (COLOR_CAP + 3) + 2);
-  __shared__ int s_left, s_taken;
-  int w = blockIdx.x, tid = threadIdx.x;
-  int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
-  if (c1 <= c0) return;
-  int p0 = c.wpair_off[w], p1 = c.wpair_off[w + 1];
-  int np = p1 - p0;
-  int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
-  long need_col = 4L * nb + 6L * np, need_vel = 12L * nb;
-  if (nb >= (int)NO_BODY || np > 32767 || need_col > pool_bytes || need_vel > pool_bytes)
-    return;  // k_solve_colored_global takes this world
-  tmark(c, 0);
-  uint32_t* s_best = (uint32_t*)pool;
-  uint32_t* m_ab = s_best + nb;                       // np: local a | local b << 16
-  uint16_t* m_col = (uint16_t*)(m_ab + np);           // np: colour, 0xFFFF uncoloured
-  if (tid == 0) s_left = 0;
-  __syncthreads();
-  int mine = 0;
-  for (int q = tid; q < np; q += COLOR_THREADS) {
-    int p = p0 + q;
-    int ga = c.pa[p], gb = c.pb[p];
-    uint32_t a = c.is_static[ga] ? NO_BODY : (uint32_t)(ga - b0);
-    uint32_t b = c.is_static[gb] ? NO_BODY : (uint32_t)(gb - b0);
-    m_ab[q] = a | (b << 16);
-    bool has = c.ccount[p] > 0 && c.coff[p] < c1;  // contacts stored within capacity
-    m_col[q] = has ? 0xFFFF : 0xFFFE;
-    mine += has;
-  }
-  atomicAdd(&s_left, mine);
-  __syncthreads();
-  int rounds = 0;
-  while (s_left > 0 && rounds < COLOR_CAP) {
-    for (int i = tid; i < nb; i += COLOR_THREADS) s_best[i] = 0;
-    if (tid == 0) s_taken = 0;
-    __syncthreads();
-    for (int q = tid; q < np; q += COLOR_THREADS) {
-      if (m_col[q] != 0xFFFF) continue;
-      uint32_t prio = (((mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x1FFFFu) << 15) | (uint32_t)q) + 1u;
-      uint32_t ab = m_ab[q];
-      uint32_t a = ab & 0xFFFF, b = ab >> 16;
-      if (a != NO_BODY) atomicMax(&s_best[a], prio);
-      if (b != NO_BODY) atomicMax(&s_best[b], prio);
+// FP32 velocities: LDS (3 floats per local body) or fvx/fvy/fw.
+template <bool VLDS>
+struct VelF {
+  float* lds;
+  const Ctx* c;
+  int b0;
+  __device__ __forceinline__ void load(int g, float* v) const {
+    if (VLDS) {
+      const float* p = lds + 3 * (g - b0);
+      v[0] = p[0];
+      v[1] = p[1];
+      v[2] = p[2];
+    } else {
+      v[0] = c->fvx[g];
+      v[1] = c->fvy[g];
+      v[2] = c->fw[g];
     }
-    __syncthreads();
-    int taken = 0;
-    for (int q = tid; q < np; q += COLOR_THREADS) {
-      if (m_col[q] != 0xFFFF) continue;
-      uint32_t prio = (((mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x1FFFFu) << 15) | (uint32_t)q) + 1u;
-      uint32_t ab = m_ab[q];
-      uint32_t a = ab & 0xFFFF, b = ab >> 16;
-      if ((a == NO_BODY || s_best[a] == prio) && (b == NO_BODY || s_best[b] == prio)) {
-        m_col[q] = (uint16_t)rounds;
-        ++taken;
-      }
+  }
+  __device__ __forceinline__ void store(int g, const float* v) const {
+    if (VLDS) {
+      float* p = lds + 3 * (g - b0);
+      p[0] = v[0];
+      p[1] = v[1];
+      p[2] = v[2];
+    } else {
+      c->fvx[g] = v[0];
+      c->fvy[g] = v[1];
+      c->fw[g] = v[2];
     }
-    atomicAdd(&s_taken, taken);
-    __syncthreads();
-    if (tid == 0) s_left -= s_taken;
-    ++rounds;
-    __syncthreads();
   }
-  bool serial_tail = s_left > 0;
-  int ncol = rounds + (serial_tail ? 1 : 0);
-  tmark(c, 1);
-  if (tid == 0) atomicMax(&c.counters[CNT_LEVELS], ncol);
-  for (int i = tid; i <= ncol + 1; i += COLOR_THREADS) s_cur[i] = 0;
-  __syncthreads();
-  for (int q = tid; q < np; q += COLOR_THREADS) {
-    int cl = m_col[q];
-    if (cl == 0xFFFF) m_col[q] = (uint16_t)(cl = rounds);
-    if (cl != 0xFFFE) atomicAdd(&s_cur[cl], 1);
-  }
-  __syncthreads();
-  if (tid == 0) {
-    int acc = 0;
-    for (int l = 0; l <= ncol; ++l) {
-      int h = s_cur[l];
-      s_off[l] = acc;
-      s_cur[l] = acc;
-      acc += h;
+};
+
+// One manifold: warm start (pass 0: useCache's applySln) or one sweep
+// (improveContactSln with FP32 velocities/rows and FP64 accumulators) of
+// its contacts in key order.  M: manifold index for the lagrangian stores.
+template <bool VLDS>
+__device__ __forceinline__ void solve_man(const PackF& P, int M, ManF& m, bool warm,
+                                          const VelF<VLDS>& V) {
+  float v[6], im[6] = {m.ima, m.ima, m.ira, m.imb, m.imb, m.irb};
+  const float tx = m.uy, ty = -m.ux;  // clockwise(u)
+  V.load(m.ga, v);
+  V.load(m.gb, v + 3);
+  const int nk = (m.flags & MFL_TWO) ? 2 : 1;
+  bool changed = false;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    if (j >= nk) break;
+    ConF& k = m.k[j];
+    const float jn[6] = {-m.ux, -m.uy, k.jn2, m.ux, m.uy, k.jn5};
+    const float jt[6] = {-tx, -ty, k.jt2, tx, ty, k.jt5};
+    if (warm) {
+      if (!(m.flags & (j == 0 ? MFL_WARM0 : MFL_WARM1))) continue;
+      applyf((float)k.ln, jn, im, v);
+      applyf((float)k.lt, jt, im, v);
+      changed = true;
+      continue;
     }
-    s_off[ncol + 1] = acc;
+    float cx = ((-v[0] + v[2] * k.ray) + v[3]) - v[5] * k.rby;  // bounceB closing velocity
+    float cy = ((-v[1] - v[2] * k.rax) + v[4]) + v[5] * k.rbx;
+    float bb = m.e * (cx * m.ux + cy * m.uy);
+    bb = 0.0f <= bb ? 0.0f : bb;
+    float bn = k.bn + bb;
+    float new_n = -(dot6f(jn, v) + bn) / k.mcn;
+    float new_t = -dot6f(jt, v) / k.mct;
+    double app_n = hs_max((double)new_n, -k.ln);
+    double acc_n = k.ln + app_n;
+    double thr = acc_n * (double)m.mu;
+    double acc_t = k.lt + (double)new_t;
+    double accp = acc_t > thr ? thr : (acc_t < -thr ? -thr : acc_t);
+    double app_t = accp - k.lt;
+    applyf((float)app_n, jn, im, v);
+    applyf((float)app_t, jt, im, v);
+    k.ln = acc_n;
+    k.lt = accp;
+    const long s = 2L * M + j;
+    P.lam[s] = acc_n;
+    P.lam[P.CC + s] = accp;
+    changed = true;
   }
-  __syncthreads();
-  // manifold m of the colour order owns slots 2 (p0 + m) and + 1
-  PackF P = packf(c);
-  for (int q = tid; q < np; q += COLOR_THREADS) {
-    int cl = m_col[q];
-    if (cl == 0xFFFE) continue;
-    int m = atomicAdd(&s_cur[cl], 1);
-    int p = p0 + q;
+  if (!changed) return;
+  if (!(m.flags & MFL_A_STATIC)) V.store(m.ga, v);
+  if (!(m.flags & MFL_B_STATIC)) V.store(m.gb, v + 3);
+}
+
+// Colours are stored as int (-1 uncoloured, -2 no contacts) in global
+// memory and as uint16 (0xFFFF uncoloured, 0xFFFE no contacts) in LDS.
+__device__ __forceinline__ int colour_of(int x) { return x; }
+__device__ __forceinline__ int colour_of(uint16_t x) {
+  return x == 0xFFFFu ? -1 : (x == 0xFFFEu ? -2 : (int)x);
+}
+
+// Pack the coloured manifolds: colour l owns manifold indices
+// [p0 + s_off[l], p0 + s_off[l + 1]); scur holds the colour cursors.
+template <int T, class CT>
+__device__ void pack_colours(const Ctx& c, const PackF& P, int p0, int np, int c1,
+                             const CT* colour, int* scur) {
+  for (int q = threadIdx.x; q < np; q += T) {
+    const int cl = colour_of(colour[q]);
+    if (cl < 0) continue;
+    const int m = atomicAdd(&scur[cl], 1);
+    const int p = p0 + q;
     int k0 = c.coff[p], k1 = k0 + c.ccount[p];
     if (k1 > c1) k1 = c1;  // contacts beyond the capacity were not stored
-    int s = 2 * (p0 + m);
-    c.sched_order[p0 + m] = s;
-    if (k0 < k1) pack_contact_f(c, P, k0, s, k1 - k0 > 1);
-    if (k0 + 1 < k1) pack_contact_f(c, P, k0 + 1, s + 1, 0);
+    pack_man(c, P, k0, k1, p0 + m);
   }
-  __syncthreads();  // colouring data is dead: the pool now holds velocities
-  float* V = (float*)pool;
-  for (int i = tid; i < nb; i += COLOR_THREADS) {
-    V[3 * i] = (float)c.vx[b0 + i];
-    V[3 * i + 1] = (float)c.vy[b0 + i];
-    V[3 * i + 2] = (float)c.w[b0 + i];
-  }
-  __threadfence_block();
-  __syncthreads();
-  tmark(c, 2);
-  for (int pass = 0; pass <= c.iterations; ++pass) {
-    if (pass == 1) tmark(c, 3);
-    for (int l = 0; l < ncol; ++l) {
-      int e = s_off[l + 1];
-      bool tail = serial_tail && l == ncol - 1;
-      if (!tail || tid == 0) {
-        int step = tail ? 1 : COLOR_THREADS;
-        for (int i = s_off[l] + (tail ? 0 : tid); i < e; i += step) {
-          int s = c.sched_order[p0 + i];
-          RecF r;
-          load_recf(c, P, s, r);
-          if (pass == 0) manifold<true>(c, P, s, r, V, b0);
-          else manifold<false>(c, P, s, r, V, b0);
+}
+
+// Warm-start pass and sweeps, colour by colour.  Phase h = pass * ncol + l;
+// thread t takes manifolds s_off[l] + t, + T, ... of colour l (the last
+// colour serially by thread 0 when colouring stopped at COLOR_CAP).
+template <bool VLDS, int T>
+__device__ void sweep_colours(const Ctx& c, const PackF& P, int p0, int ncol, bool serial_tail,
+                              const int* s_off, const VelF<VLDS>& V) {
+  const int tid = threadIdx.x;
+  const int phases = (c.iterations + 1) * ncol;
+  auto first = [&](int h, int& M) {
+    const int l = h % ncol;
+    if (serial_tail && l == ncol - 1) return false;
+    M = s_off[l] + tid;
+    return M < s_off[l + 1];
+  };
+  ManF cur, nxt;
+  int Mc = 0, Mn = 0;
+  bool has = phases > 0 && first(0, Mc);
+  if (has) load_man(c, P, p0 + Mc, cur);
+  for (int h = 0; h < phases; ++h) {
+    const int l = h % ncol;
+    const bool warm = h < ncol;
+    const bool hasn = h + 1 < phases && first(h + 1, Mn);
+    if (hasn) load_man(c, P, p0 + Mn, nxt);
+    if (serial_tail && l == ncol - 1) {
+      if (tid == 0) {
+        for (int i = s_off[l]; i < s_off[l + 1]; ++i) {
+          ManF m;
+          load_man(c, P, p0 + i, m);
+          solve_man<VLDS>(P, p0 + i, m, warm, V);
         }
       }
-      __syncthreads();
-    }
-  }
-  tmark(c, 4);
-  // lagrangians back to contact order; FP32 velocity change into FP64
-  for (int i = tid; i < s_off[ncol]; i += COLOR_THREADS) {
-    int s = c.sched_order[p0 + i];
-    int k = c.pk_k[s];
-    c.ln[k] = P.lam[s];
-    c.lt[k] = P.lam[P.CC + s];
-    if (c.pk_f[s] & 8) {
-      int k1 = c.pk_k[s + 1];
-      c.ln[k1] = P.lam[s + 1];
-      c.lt[k1] = P.lam[P.CC + s + 1];
-    }
-  }
-  for (int i = tid; i < nb; i += COLOR_THREADS) {
-    int g = b0 + i;
-    if (c.is_static[g]) continue;
-    c.vx[g] = c.vx[g] + (double)(V[3 * i] - (float)c.vx[g]);
-    c.vy[g] = c.vy[g] + (double)(V[3 * i + 1] - (float)c.vy[g]);
-    c.w[g] = c.w[g] + (double)(V[3 * i + 2] - (float)c.w[g]);
-  }
-  __syncthreads();
-  tmark(c, 5);
-}
-
-
-// FP32 velocities in the global arrays (worlds too large for LDS)
-__device__ __forceinline__ void gload(const Ctx& c, int ga, int gb, float* v) {
-  v[0] = c.fvx[ga];
-  v[1] = c.fvy[ga];
-  v[2] = c.fw[ga];
-  v[3] = c.fvx[gb];
-  v[4] = c.fvy[gb];
-  v[5] = c.fw[gb];
-}
-
-__device__ __forceinline__ void gstore(const Ctx& c, int ga, int gb, int flags, const float* v) {
-  if (!(flags & 1)) {
-    c.fvx[ga] = v[0];
-    c.fvy[ga] = v[1];
-    c.fw[ga] = v[2];
-  }
-  if (!(flags & 2)) {
-    c.fvx[gb] = v[3];
-    c.fvy[gb] = v[4];
-    c.fw[gb] = v[5];
-  }
-}
-
-__device__ __forceinline__ void solve_one_g(const Ctx& c, const PackF& P, int s, const RecF& r, bool warm) {
-  float v[6];
-  gload(c, r.ga, r.gb, v);
-  if (warm) {
-    if (!(r.flags & 4)) return;
-    applyf((float)r.ln, r.jn, r.im, v);
-    applyf((float)r.lt, r.jt, r.im, v);
-    gstore(c, r.ga, r.gb, r.flags, v);
-    return;
-  }
-  float cx = ((-v[0] + v[2] * r.ray) + v[3]) - v[5] * r.rby;
-  float cy = ((-v[1] - v[2] * r.rax) + v[4]) + v[5] * r.rbx;
-  float bb = r.e * (cx * r.nrx + cy * r.nry);
-  bb = 0.0f <= bb ? 0.0f : bb;
-  float bn = r.bn + bb;
-  float new_n = -(dot6f(r.jn, v) + bn) / r.mcn;
-  float new_t = -dot6f(r.jt, v) / r.mct;
-  double app_n = hs_max((double)new_n, -r.ln);
-  double acc_n = r.ln + app_n;
-  double thr = acc_n * (double)r.mu;
-  double acc_t = r.lt + (double)new_t;
-  double accp = acc_t > thr ? thr : (acc_t < -thr ? -thr : acc_t);
-  double app_t = accp - r.lt;
-  applyf((float)app_n, r.jn, r.im, v);
-  applyf((float)app_t, r.jt, r.im, v);
-  gstore(c, r.ga, r.gb, r.flags, v);
-  P.lam[s] = acc_n;
-  P.lam[P.CC + s] = accp;
-}
-
-template <bool WARM>
-__device__ __forceinline__ void manifold_g(const Ctx& c, const PackF& P, int s, const RecF& r0) {
-  solve_one_g(c, P, s, r0, WARM);
-  if (r0.flags & 8) {
-    RecF r1;
-    load_recf(c, P, s + 1, r1);
-    solve_one_g(c, P, s + 1, r1, WARM);
-  }
-}
-
-// The same algorithm for worlds whose colouring or velocities do not fit in
-// LDS: 64-bit priorities per body, colours per pair and FP32 velocities in
-// global memory (L2-resident), 1024 lanes per world.  Worlds that fit are
-// left to k_solve_colored (the fit test is the same).
-#define COLOR_THREADS_G 1024
-
-__global__ __launch_bounds__(COLOR_THREADS_G) void k_solve_colored_global(Ctx c, int pool_bytes) {
-  __shared__ int s_off[COLOR_CAP + 3];
-  __shared__ int s_cur[COLOR_CAP + 3];
-  __shared__ int s_left, s_taken;
-  int w = blockIdx.x, tid = threadIdx.x;
-  int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
-  if (c1 <= c0) return;
-  int p0 = c.wpair_off[w], p1 = c.wpair_off[w + 1];
-  int np = p1 - p0;
-  int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
-  long need_col = 4L * nb + 6L * np, need_vel = 12L * nb;
-  if (!(nb >= (int)NO_BODY || np > 32767 || need_col > pool_bytes || need_vel > pool_bytes)) return;
-  unsigned long long* best = c.col_best + b0;
-  int32_t* col = c.sched_level;  // per pair: colour, -1 uncoloured, -2 none
-  if (tid == 0) s_left = 0;
-  __syncthreads();
-  int mine = 0;
-  for (int q = tid; q < np; q += COLOR_THREADS_G) {
-    int p = p0 + q;
-    bool has = c.ccount[p] > 0 && c.coff[p] < c1;
-    col[p] = has ? -1 : -2;
-    mine += has;
-  }
-  atomicAdd(&s_left, mine);
-  __syncthreads();
-  int rounds = 0;
-  while (s_left > 0 && rounds < COLOR_CAP) {
-    for (int i = tid; i < nb; i += COLOR_THREADS_G) best[i] = 0ull;
-    if (tid == 0) s_taken = 0;
-    __syncthreads();
-    for (int q = tid; q < np; q += COLOR_THREADS_G) {
-      int p = p0 + q;
-      if (col[p] != -1) continue;
-      unsigned long long prio =
-          ((unsigned long long)mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) << 32) | (uint32_t)(q + 1);
-      int ga = c.pa[p], gb = c.pb[p];
-      if (!c.is_static[ga]) atomicMax(&best[ga - b0], prio);
-      if (!c.is_static[gb]) atomicMax(&best[gb - b0], prio);
-    }
-    __syncthreads();
-    int taken = 0;
-    for (int q = tid; q < np; q += COLOR_THREADS_G) {
-      int p = p0 + q;
-      if (col[p] != -1) continue;
-      unsigned long long prio =
-          ((unsigned long long)mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) << 32) | (uint32_t)(q + 1);
-      int ga = c.pa[p], gb = c.pb[p];
-      if ((c.is_static[ga] || best[ga - b0] == prio) && (c.is_static[gb] || best[gb - b0] == prio)) {
-        col[p] = rounds;
-        ++taken;
+    } else {
+      if (has) solve_man<VLDS>(P, p0 + Mc, cur, warm, V);
+      for (int i = Mc + T; i < s_off[l + 1]; i += T) {
+        ManF m;
+        load_man(c, P, p0 + i, m);
+        solve_man<VLDS>(P, p0 + i, m, warm, V);
       }
     }
-    atomicAdd(&s_taken, taken);
     __syncthreads();
-    if (tid == 0) s_left -= s_taken;
-    ++rounds;
-    __syncthreads();
+    cur = nxt;
+    Mc = Mn;
+    has = hasn;
   }
-  bool serial_tail = s_left > 0;
-  int ncol = rounds + (serial_tail ? 1 : 0);
-  if (tid == 0) atomicMax(&c.counters[CNT_LEVELS], ncol);
-  for (int i = tid; i <= ncol + 1; i += COLOR_THREADS_G) s_cur[i] = 0;
+}
+
+// Lagrangians back to contact order; the FP32 velocity change into FP64.
+template <bool VLDS, int T>
+__device__ void colour_writeback(const Ctx& c, const PackF& P, int p0, int nman, int b0, int nb,
+                                 const VelF<VLDS>& V) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nman; i += T) {
+    const int M = p0 + i;
+    const int nk = (c.pk_f[M] & MFL_TWO) ? 2 : 1;
+    for (int j = 0; j < nk; ++j) {
+      const long s = 2L * M + j;
+      const int k = c.pk_k[s];
+      c.ln[k] = P.lam[s];
+      c.lt[k] = P.lam[P.CC + s];
+    }
+  }
+  for (int i = tid; i < nb; i += T) {
+    const int g = b0 + i;
+    if (c.is_static[g]) continue;
+    float v[3];
+    V.load(g, v);
+    c.vx[g] = c.vx[g] + (double)(v[0] - (float)c.vx[g]);
+    c.vy[g] = c.vy[g] + (double)(v[1] - (float)c.vy[g]);
+    c.w[g] = c.w[g] + (double)(v[2] - (float)c.w[g]);
+  }
+}
+
+// Colour-order bookkeeping shared by both kernels: per-colour counts of the
+// coloured manifolds -> s_off (exclusive) and s_cur (cursors).
+template <int T, class CT>
+__device__ void colour_offsets(const CT* colour, int np, int ncol, int* s_off, int* s_cur) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i <= ncol + 1; i += T) s_cur[i] = 0;
   __syncthreads();
-  for (int q = tid; q < np; q += COLOR_THREADS_G) {
-    int p = p0 + q;
-    int cl = col[p];
-    if (cl == -1) col[p] = cl = rounds;
+  for (int q = tid; q < np; q += T) {
+    const int cl = colour_of(colour[q]);
     if (cl >= 0) atomicAdd(&s_cur[cl], 1);
   }
   __syncthreads();
@@ -537,76 +399,212 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_solve_colored_global(Ctx c,
     s_off[ncol + 1] = acc;
   }
   __syncthreads();
-  PackF P = packf(c);
-  for (int q = tid; q < np; q += COLOR_THREADS_G) {
-    int p = p0 + q;
-    int cl = col[p];
-    if (cl < 0) continue;
-    int m = atomicAdd(&s_cur[cl], 1);
-    int k0 = c.coff[p], k1 = k0 + c.ccount[p];
-    if (k1 > c1) k1 = c1;
-    int s = 2 * (p0 + m);
-    c.sched_order[p0 + m] = s;
-    if (k0 < k1) pack_contact_f(c, P, k0, s, k1 - k0 > 1);
-    if (k0 + 1 < k1) pack_contact_f(c, P, k0 + 1, s + 1, 0);
+}
+
+// LDS variant: local body ids and manifold indices fit 16 and 15 bits, the
+// colouring (4 B per body, 6 B per manifold) and the velocities (12 B per
+// body) fit the pool
+__device__ __forceinline__ bool fits_lds(int nb, int np, int pool_bytes) {
+  const long need_col = 4L * nb + 6L * np, need_vel = 12L * nb;
+  return nb < (int)NO_BODY && np <= 32767 && need_col <= pool_bytes && need_vel <= pool_bytes;
+}
+
+// hashed priority of manifold q < 32768 of world w (distinct within a world)
+__device__ __forceinline__ uint32_t prio32(int q, int w) {
+  return (((mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x1FFFFu) << 15) | (uint32_t)q) + 1u;
+}
+
+// One workgroup per world whose colouring and FP32 velocities fit in LDS.
+// LDS pool: during colouring s_best (4 B/body) and per manifold its bodies
+// (4 B) and colour (2 B); during the solve FP32 velocities (12 B/body).
+template <int T>
+__global__ __launch_bounds__(T) void k_solve_colored(Ctx c, int pool_bytes) {
+  extern __shared__ int lds[];
+  int* s_off = lds;                      // COLOR_CAP + 3
+  int* s_cur = lds + (COLOR_CAP + 3);    // COLOR_CAP + 3
+  unsigned char* pool = (unsigned char*)(lds + 2 * (COLOR_CAP + 3) + 2);
+  __shared__ int s_left, s_taken;
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
+  if (c1 <= c0) return;
+  const int p0 = c.wpair_off[w], p1 = c.wpair_off[w + 1];
+  const int np = p1 - p0;
+  const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
+  if (!fits_lds(nb, np, pool_bytes)) return;  // k_solve_colored_global takes this world
+  tmark(c, 0);
+  uint32_t* s_best = (uint32_t*)pool;
+  uint32_t* m_ab = s_best + nb;               // np: local a | local b << 16
+  uint16_t* colour = (uint16_t*)(m_ab + np);  // np: colour, 0xFFFF uncoloured
+  if (tid == 0) s_left = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int q = tid; q < np; q += T) {
+    const int p = p0 + q;
+    const int ga = c.pa[p], gb = c.pb[p];
+    const uint32_t a = c.is_static[ga] ? NO_BODY : (uint32_t)(ga - b0);
+    const uint32_t b = c.is_static[gb] ? NO_BODY : (uint32_t)(gb - b0);
+    m_ab[q] = a | (b << 16);
+    const bool has = c.ccount[p] > 0 && c.coff[p] < c1;  // contacts stored within capacity
+    colour[q] = has ? 0xFFFFu : 0xFFFEu;
+    mine += has;
   }
+  atomicAdd(&s_left, mine);
+  __syncthreads();
+  int rounds = 0;
+  while (s_left > 0 && rounds < COLOR_CAP) {
+    for (int i = tid; i < nb; i += T) s_best[i] = 0;
+    if (tid == 0) s_taken = 0;
+    __syncthreads();
+    for (int q = tid; q < np; q += T) {
+      if (colour[q] != 0xFFFFu) continue;
+      const uint32_t prio = prio32(q, w);
+      const uint32_t ab = m_ab[q], a = ab & 0xFFFF, b = ab >> 16;
+      if (a != NO_BODY) atomicMax(&s_best[a], prio);
+      if (b != NO_BODY) atomicMax(&s_best[b], prio);
+    }
+    __syncthreads();
+    int taken = 0;
+    for (int q = tid; q < np; q += T) {
+      if (colour[q] != 0xFFFFu) continue;
+      const uint32_t prio = prio32(q, w);
+      const uint32_t ab = m_ab[q], a = ab & 0xFFFF, b = ab >> 16;
+      if ((a == NO_BODY || s_best[a] == prio) && (b == NO_BODY || s_best[b] == prio)) {
+        colour[q] = (uint16_t)rounds;
+        ++taken;
+      }
+    }
+    atomicAdd(&s_taken, taken);
+    __syncthreads();
+    if (tid == 0) s_left -= s_taken;
+    ++rounds;
+    __syncthreads();
+  }
+  const bool serial_tail = s_left > 0;
+  const int ncol = rounds + (serial_tail ? 1 : 0);
+  for (int q = tid; q < np; q += T)
+    if (colour[q] == 0xFFFFu) colour[q] = (uint16_t)rounds;
+  tmark(c, 1);
+  if (tid == 0) atomicMax(&c.counters[CNT_LEVELS], ncol);
+  colour_offsets<T>(colour, np, ncol, s_off, s_cur);
+  const PackF P = packf(c);
+  pack_colours<T>(c, P, p0, np, c1, colour, s_cur);
+  __syncthreads();  // colouring data is dead: the pool now holds velocities
+  VelF<true> V;
+  V.lds = (float*)pool;
+  V.c = &c;
+  V.b0 = b0;
+  for (int i = tid; i < nb; i += T) {
+    V.lds[3 * i] = (float)c.vx[b0 + i];
+    V.lds[3 * i + 1] = (float)c.vy[b0 + i];
+    V.lds[3 * i + 2] = (float)c.w[b0 + i];
+  }
+  __syncthreads();
+  tmark(c, 2);
+  tmark(c, 3);
+  sweep_colours<true, T>(c, P, p0, ncol, serial_tail, s_off, V);
+  tmark(c, 4);
+  colour_writeback<true, T>(c, P, p0, s_off[ncol], b0, nb, V);
+  __syncthreads();
+  tmark(c, 5);
+}
+
+// The same algorithm for worlds whose colouring or velocities do not fit in
+// LDS: 64-bit priorities per body, colours per pair and FP32 velocities in
+// global memory (L2-resident), 512 lanes per world.  Worlds that fit are
+// left to k_solve_colored (the fit test is the same).
+#define COLOR_THREADS_G 512
+
+__global__ __launch_bounds__(COLOR_THREADS_G) void k_solve_colored_global(Ctx c, int pool_bytes) {
+  __shared__ int s_off[COLOR_CAP + 3];
+  __shared__ int s_cur[COLOR_CAP + 3];
+  __shared__ int s_left, s_taken;
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
+  if (c1 <= c0) return;
+  const int p0 = c.wpair_off[w], p1 = c.wpair_off[w + 1];
+  const int np = p1 - p0;
+  const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
+  if (fits_lds(nb, np, pool_bytes)) return;
+  unsigned long long* best = c.col_best + b0;
+  int32_t* colour = c.sched_level + p0;  // per pair: colour, -1 uncoloured, -2 none
+  if (tid == 0) s_left = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int q = tid; q < np; q += COLOR_THREADS_G) {
+    const int p = p0 + q;
+    const bool has = c.ccount[p] > 0 && c.coff[p] < c1;
+    colour[q] = has ? -1 : -2;
+    mine += has;
+  }
+  atomicAdd(&s_left, mine);
+  __syncthreads();
+  int rounds = 0;
+  while (s_left > 0 && rounds < COLOR_CAP) {
+    for (int i = tid; i < nb; i += COLOR_THREADS_G) best[i] = 0ull;
+    if (tid == 0) s_taken = 0;
+    __syncthreads();
+    for (int q = tid; q < np; q += COLOR_THREADS_G) {
+      if (colour[q] != -1) continue;
+      const int p = p0 + q;
+      const unsigned long long prio =
+          ((unsigned long long)mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) << 32) | (uint32_t)(q + 1);
+      const int ga = c.pa[p], gb = c.pb[p];
+      if (!c.is_static[ga]) atomicMax(&best[ga - b0], prio);
+      if (!c.is_static[gb]) atomicMax(&best[gb - b0], prio);
+    }
+    __syncthreads();
+    int taken = 0;
+    for (int q = tid; q < np; q += COLOR_THREADS_G) {
+      if (colour[q] != -1) continue;
+      const int p = p0 + q;
+      const unsigned long long prio =
+          ((unsigned long long)mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) << 32) | (uint32_t)(q + 1);
+      const int ga = c.pa[p], gb = c.pb[p];
+      if ((c.is_static[ga] || best[ga - b0] == prio) && (c.is_static[gb] || best[gb - b0] == prio)) {
+        colour[q] = rounds;
+        ++taken;
+      }
+    }
+    atomicAdd(&s_taken, taken);
+    __syncthreads();
+    if (tid == 0) s_left -= s_taken;
+    ++rounds;
+    __syncthreads();
+  }
+  const bool serial_tail = s_left > 0;
+  const int ncol = rounds + (serial_tail ? 1 : 0);
+  for (int q = tid; q < np; q += COLOR_THREADS_G)
+    if (colour[q] == -1) colour[q] = rounds;
+  if (tid == 0) atomicMax(&c.counters[CNT_LEVELS], ncol);
+  colour_offsets<COLOR_THREADS_G>(colour, np, ncol, s_off, s_cur);
+  const PackF P = packf(c);
+  pack_colours<COLOR_THREADS_G>(c, P, p0, np, c1, colour, s_cur);
   for (int i = tid; i < nb; i += COLOR_THREADS_G) {
     c.fvx[b0 + i] = (float)c.vx[b0 + i];
     c.fvy[b0 + i] = (float)c.vy[b0 + i];
     c.fw[b0 + i] = (float)c.w[b0 + i];
   }
-  __threadfence_block();
   __syncthreads();
-  // velocities as a flat FP32 array of this world: V[3*i + k] is not the
-  // global layout, so use per-component accessors through a local view
-  for (int pass = 0; pass <= c.iterations; ++pass) {
-    for (int l = 0; l < ncol; ++l) {
-      int e = s_off[l + 1];
-      bool tail = serial_tail && l == ncol - 1;
-      if (!tail || tid == 0) {
-        int step = tail ? 1 : COLOR_THREADS_G;
-        for (int i = s_off[l] + (tail ? 0 : tid); i < e; i += step) {
-          int s = c.sched_order[p0 + i];
-          RecF r;
-          load_recf(c, P, s, r);
-          if (pass == 0) manifold_g<true>(c, P, s, r);
-          else manifold_g<false>(c, P, s, r);
-        }
-      }
-      __syncthreads();
-    }
-  }
-  for (int i = tid; i < s_off[ncol]; i += COLOR_THREADS_G) {
-    int s = c.sched_order[p0 + i];
-    int k = c.pk_k[s];
-    c.ln[k] = P.lam[s];
-    c.lt[k] = P.lam[P.CC + s];
-    if (c.pk_f[s] & 8) {
-      int k1 = c.pk_k[s + 1];
-      c.ln[k1] = P.lam[s + 1];
-      c.lt[k1] = P.lam[P.CC + s + 1];
-    }
-  }
-  for (int i = tid; i < nb; i += COLOR_THREADS_G) {
-    int g = b0 + i;
-    if (c.is_static[g]) continue;
-    c.vx[g] = c.vx[g] + (double)(c.fvx[g] - (float)c.vx[g]);
-    c.vy[g] = c.vy[g] + (double)(c.fvy[g] - (float)c.vy[g]);
-    c.w[g] = c.w[g] + (double)(c.fw[g] - (float)c.w[g]);
-  }
+  VelF<false> V;
+  V.lds = nullptr;
+  V.c = &c;
+  V.b0 = b0;
+  sweep_colours<false, COLOR_THREADS_G>(c, P, p0, ncol, serial_tail, s_off, V);
+  colour_writeback<false, COLOR_THREADS_G>(c, P, p0, s_off[ncol], b0, nb, V);
 }
-
-#define LDS_MAX_BYTES (160 * 1024 - 512)
 
 shp_status colored_alloc(shp_batch* b) {
   Ctx& c = b->ctx;
+  static_assert((CF_LAMBDA + 4) * 2 + MF_FIELDS <= 2 * 29 * 2, "records exceed the packed buffer");
   if (hipMalloc(&c.pk_k, (size_t)c.contact_cap * 4) != hipSuccess) return SHP_E_OOM;
   b->allocs.push_back(c.pk_k);
   long need = 0;
+  int max_nb = 0;
   for (int w = 0; w < c.W; ++w) {
     long nb = b->h_body_off[w + 1] - b->h_body_off[w];
+    max_nb = std::max<int>(max_nb, (int)nb);
     // pairs per world are unknown before a step: budget 8 per body (a world
-    // that needs more falls back to the exact serial solve)
+    // that needs more this step goes to the global-memory variant)
     long np = std::min<long>(32767, 8 * nb + 64);
     need = std::max(need, std::max(4 * nb + 6 * np, 12 * nb));
   }
@@ -614,18 +612,23 @@ shp_status colored_alloc(shp_batch* b) {
   b->color_lds = std::min<size_t>(fixed + need, LDS_MAX_BYTES);
   if (hipMalloc(&c.col_best, (size_t)c.NB * 8) != hipSuccess) return SHP_E_OOM;
   b->allocs.push_back(c.col_best);
-  b->color_global = false;
-  for (int w = 0; w < c.W; ++w)
-    if (12L * (b->h_body_off[w + 1] - b->h_body_off[w]) > (long)(b->color_lds - fixed)) b->color_global = true;
-  if (hipFuncSetAttribute((const void*)k_solve_colored, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)b->color_lds) != hipSuccess)
+  // 4 waves per world; a large world gets 8 (two manifold records per lane
+  // leave 8 waves without spills)
+  b->color_threads = max_nb > 2048 ? 512 : 256;
+  const void* fn = b->color_threads == 512 ? (const void*)k_solve_colored<512>
+                                           : (const void*)k_solve_colored<256>;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->color_lds) !=
+      hipSuccess)
     return SHP_E_HIP;
   return SHP_OK;
 }
 
 void launch_colored_solver(shp_batch* b) {
   int pool = (int)(b->color_lds - (2 * (COLOR_CAP + 3) + 2) * 4);
-  k_solve_colored<<<b->ctx.W, COLOR_THREADS, b->color_lds, b->stream>>>(b->ctx, pool);
+  if (b->color_threads == 512)
+    k_solve_colored<512><<<b->ctx.W, 512, b->color_lds, b->stream>>>(b->ctx, pool);
+  else
+    k_solve_colored<256><<<b->ctx.W, 256, b->color_lds, b->stream>>>(b->ctx, pool);
   // worlds that do not fit in LDS (by body count at creation, or by their
   // pair count this step) are solved by the global-memory variant; it exits
   // at once for the others
@@ -691,8 +694,22 @@ __global__ void k_quality(Ctx c) {
   }
 }
 
+// Run sums of the step's maxima (one thread, after k_quality): their
+// means over the run are less sensitive to a single impact than the run
+// maxima.
+__global__ void k_quality_sum(Ctx c) {
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < 3; ++k) {
+    double m = __longlong_as_double((long long)c.quality[Q_DEPTH + k]);
+    double s = __longlong_as_double((long long)c.quality[Q_SUM_DEPTH + k]);
+    c.quality[Q_SUM_DEPTH + k] = (unsigned long long)__double_as_longlong(s + m);
+  }
+  c.quality[Q_STEPS] += 1;
+}
+
 void launch_quality(shp_batch* b) {
   long n = b->ctx.contact_cap;
   int grid = (int)std::max<long>(1, std::min<long>((n + 255) / 256, 2048));
   k_quality<<<grid, 256, 0, b->stream>>>(b->ctx);
+  k_quality_sum<<<1, 64, 0, b->stream>>>(b->ctx);
 }

@@ -18,8 +18,21 @@
 
 enum { CNT_FLAGS = 0, CNT_WARM = 1, CNT_LEARNED = 2, CNT_LEVELS = 3, CNT_MISS = 4 };
 // solve-quality maxima (doubles stored as their bit patterns): last step,
-// then since creation
-enum { Q_DEPTH = 0, Q_NRES = 1, Q_FRES = 2, Q_RUN_DEPTH = 3, Q_RUN_NRES = 4, Q_RUN_FRES = 5 };
+// then since creation; then the sums of the steps' maxima and the number
+// of steps summed
+enum {
+  Q_DEPTH = 0,
+  Q_NRES = 1,
+  Q_FRES = 2,
+  Q_RUN_DEPTH = 3,
+  Q_RUN_NRES = 4,
+  Q_RUN_FRES = 5,
+  Q_SUM_DEPTH = 6,
+  Q_SUM_NRES = 7,
+  Q_SUM_FRES = 8,
+  Q_STEPS = 9,
+  Q_ENTRIES = 10
+};
 enum {
   SHP_FLAG_PAIR_OVERFLOW = 1,
   SHP_FLAG_CONTACT_OVERFLOW = 2,
@@ -76,6 +89,9 @@ struct Ctx {
   int32_t* flow_list;  // 2 x contact_cap
   int4* flow_rk;       // contact_cap
   int32_t *flow_off, *flow_cur;  // NB
+  // level of every contact and the contacts in level order
+  // (level-synchronous worlds), contact_cap each
+  int32_t *flow_lvl, *flow_order;
   // contact constants packed for the solver (compact FP64 records in
   // contact order for the ordered solver, FP32 records + FP64 lambdas in
   // colour order for the coloured one)
@@ -112,7 +128,7 @@ struct Ctx {
   int32_t *bucket_cnt, *bucket_start, *bucket_fill, *bucket_items;
   int32_t *large_off, *large_ids;
   int32_t* counters;  // 8 ints: flags, warm hits, learned starts, levels
-  unsigned long long* quality;  // 8 entries, see Q_*
+  unsigned long long* quality;  // Q_ENTRIES entries, see Q_*
 };
 
 struct shp_batch {
@@ -139,7 +155,7 @@ struct shp_batch {
   int solve_threads = 256;  // ordered solver workgroup size
   size_t solve_lds = 0;     // dynamic LDS bytes of the ordered solver
   size_t color_lds = 0;     // dynamic LDS bytes of the coloured solver
-  bool color_global = false;  // some world needs the global-memory colour path
+  int color_threads = 256;    // coloured solver workgroup size
   double phase_ms[SHP_NUM_PHASES]{};
   double solve_us[8]{};  // accumulated solver sub-phase durations (world 0)
   int32_t timed_steps = 0;

@@ -723,7 +723,7 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
   TRY(dalloc(b, &c.miss_feat, (long)c.miss_cap * MLP_IN));
   TRY(dalloc(b, &c.log_count, 1));
   TRY(dalloc(b, &c.counters, 8));
-  TRY(dalloc(b, &c.quality, 8));
+  TRY(dalloc(b, &c.quality, Q_ENTRIES));
   TRY(solver_alloc(b));
 
   // grid broadphase set-up: cell size per world from the largest vertex
@@ -796,7 +796,7 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
   k_fill_i32<<<blocks(n_worlds + 1, 256), 256, 0, s>>>(c.wcont_off, 0, n_worlds + 1);
   k_fill_i32<<<blocks(n_worlds + 1, 256), 256, 0, s>>>(c.wpair_off, 0, n_worlds + 1);
   k_fill_i32<<<1, 64, 0, s>>>(c.counters, 0, 8);
-  k_fill_i32<<<1, 64, 0, s>>>((int32_t*)c.quality, 0, 16);
+  k_fill_i32<<<1, 64, 0, s>>>((int32_t*)c.quality, 0, 2 * Q_ENTRIES);
   k_fill_i32<<<1, 64, 0, s>>>((int32_t*)c.log_count, 0, 2);
   // makeWorldObj places each shape with moveShape (World.hs:169-176)
   k_init_hulls<<<blocks(NB, 256), 256, 0, s>>>(c);
@@ -1021,11 +1021,18 @@ shp_status shp_batch_last_stats(shp_batch* b, shp_step_stats* out) {
   out->misses = cnt[CNT_MISS];
   out->max_levels = cnt[CNT_LEVELS];
   out->flags = cnt[CNT_FLAGS] | b->last_flags;
-  unsigned long long q[8];
+  unsigned long long q[Q_ENTRIES];
   HIPCHK(hipMemcpy(q, c.quality, sizeof(q), hipMemcpyDeviceToHost));
   double* dst[6] = {&out->max_depth, &out->normal_residual, &out->friction_residual,
                     &out->run_max_depth, &out->run_normal_residual, &out->run_friction_residual};
   for (int k = 0; k < 6; ++k) memcpy(dst[k], &q[k], sizeof(double));
+  double* mean[3] = {&out->mean_max_depth, &out->mean_normal_residual,
+                     &out->mean_friction_residual};
+  for (int k = 0; k < 3; ++k) {
+    double sum;
+    memcpy(&sum, &q[Q_SUM_DEPTH + k], sizeof(double));
+    *mean[k] = q[Q_STEPS] ? sum / (double)q[Q_STEPS] : 0.0;
+  }
   return SHP_OK;
 }
 

@@ -27,16 +27,30 @@
 // broken by a defect, a lane that sees no item of the workgroup complete
 // for WAIT_LIMIT consecutive polls stops and raises SHP_FLAG_SOLVER_STALL.
 //
+// Which lane runs which item only changes the speed.  Sequence order per
+// lane (lane t: contacts t, t + T, ...) is ideal while a world has no more
+// contacts than lanes; beyond that a lane's second contact holds up its
+// next pass.  That still pays while velocities sit in LDS.  A world whose
+// velocities live in global memory and that has more contacts than lanes
+// runs level-synchronously instead: the dataflow computes, integer work
+// only, every contact's level in one pass (1 + the largest level of the
+// earlier contacts on its dynamic bodies; contacts of one level share no
+// dynamic body), the contacts are sorted by level, and each pass runs
+// level by level with a barrier between levels, the next level's records
+// loaded before the barrier.  Both give the sequential result bit for bit.
+//
 // One workgroup per world:
 //   1. degrees d(a) by atomics, prefix sum -> per-body segments; every
 //      (contact, side) is listed in its body's segment; rank r_a(k) = number
 //      of entries of the segment that precede it in key order
-//   2. each contact's constants are packed into a compact record (the
+//   2. counters are cleared, velocities staged in LDS when they fit
+//   3. level-synchronous worlds: levels, counting sort by level
+//   4. each contact's constants are packed into a compact record (the
 //      jacobian rows are rebuilt from the normal and the two cross terms,
-//      exactly: every other component is a sign change of the normal)
-//   3. counters are cleared, velocities staged in LDS when they fit
-//   4. dataflow loop over all items
-//   5. lagrangians and velocities are written back
+//      exactly: every other component is a sign change of the normal), in
+//      contact order or in level order
+//   5. dataflow loop over all items, or the passes level by level
+//   6. lagrangians and velocities are written back
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
@@ -50,6 +64,7 @@
 using namespace shp;
 
 #define WAIT_LIMIT (1u << 20)
+#define LEVEL_CAP 4095
 #define LDS_MAX_BYTES (160 * 1024 - 512)
 
 // compact record fields (doubles), stride contact_cap
@@ -118,7 +133,7 @@ __device__ __forceinline__ void load_frec(const Ctx& c, int k, FRec& r) {
 // (shp_engine.hip, constraint generation), which gives the same shape with
 // u = -n.  So jn[3..4] = u, jn[0..1] = -u, jt[3..4] = clockwise(u) and
 // jt[0..1] = -jt[3..4] in both cases, and negation is exact.
-__device__ __forceinline__ void pack_flow(const Ctx& c, int k) {
+__device__ __forceinline__ void pack_flow(const Ctx& c, int k, int dst) {
   const long CC = c.contact_cap;
   const int ga = c.ga[k], gb = c.gb[k];
   // every value is loaded before the first store, so the loads are not
@@ -148,10 +163,10 @@ __device__ __forceinline__ void pack_flow(const Ctx& c, int k) {
   const uint8_t fl = (uint8_t)((c.is_static[ga] ? 1 : 0) | (c.is_static[gb] ? 2 : 0) | (c.warm[k] ? 4 : 0));
   double* P = c.pk;
 #pragma unroll
-  for (int q = 0; q < FL_FIELDS; ++q) P[q * CC + k] = f[q];
-  c.pk_g[k] = ga;
-  c.pk_g[CC + k] = gb;
-  c.pk_f[k] = fl;
+  for (int q = 0; q < FL_FIELDS; ++q) P[q * CC + dst] = f[q];
+  c.pk_g[dst] = ga;
+  c.pk_g[CC + dst] = gb;
+  c.pk_f[dst] = fl;
 }
 
 __device__ __forceinline__ void rows(const FRec& r, double* jn, double* jt, double* im) {
@@ -340,18 +355,62 @@ __device__ void compute_ranks(const Ctx& c, int c0, int n, int b0, int nb, int* 
       }
     }
     c.flow_rk[c0 + i] = rk;
-    pack_flow(c, c0 + i);
   }
   __syncthreads();
 }
 
-// The dataflow loop over all items of the world (see the file comment).
-// cnt: per-body counters (local index), cleared; progress: LDS int, 0.
+// Bounded waiting shared by the dataflow loops: every wave iteration in
+// which some lane completed an item adds to the workgroup's progress count;
+// a lane that sees the count unchanged for WAIT_LIMIT consecutive polls
+// stops (SHP_FLAG_SOLVER_STALL).  Returns false when the lane must stop.
+struct Waiter {
+  unsigned spins = 0;
+  int seen = -1;
+  __device__ __forceinline__ bool after_poll(const Ctx& c, bool ready, int* progress) {
+    const unsigned long long m = __ballot(ready);
+    if (m) {
+      if ((int)(threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicAdd(progress, __popcll(m));
+    } else {
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (ready) {
+      spins = 0;
+      return true;
+    }
+    int pr = __hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (pr != seen) {
+      seen = pr;
+      spins = 0;
+    } else if (++spins > WAIT_LIMIT) {
+      atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_SOLVER_STALL);
+      return false;
+    }
+    return true;
+  }
+};
+
+// Whether item (p, contact with ranks rk) may run: both counters at its
+// rank for pass p.  la/lb: local body indices.
+__device__ __forceinline__ bool item_ready(int* cnt, int la, int lb, int4 rk, int p) {
+  bool ready = true;
+  if (rk.x >= 0) ready = ld_acquire(cnt + la) == p * rk.y + rk.x;
+  if (ready && rk.z >= 0) ready = ld_acquire(cnt + lb) == p * rk.w + rk.z;
+  return ready;
+}
+
+__device__ __forceinline__ void item_done(int* cnt, int la, int lb, int4 rk, int p) {
+  if (rk.x >= 0) st_release(cnt + la, p * rk.y + rk.x + 1);
+  if (rk.z >= 0) st_release(cnt + lb, p * rk.w + rk.z + 1);
+}
+
+// Lane order "cyclic": lane t runs (p, t), (p, t + T), ... for p = 0, 1,
+// ..., which is sequence order.  With at most one contact per lane this is
+// the best order (a lane's items are then one contact's passes, which
+// depend on each other anyway).  cnt: per-body counters, cleared.
 template <bool VLDS, int T>
 __device__ void run_flow(const Ctx& c, int c0, int n, int b0, int* cnt, const Vel<VLDS>& V,
                          int* progress) {
   const int tid = threadIdx.x;
-  const int lane = tid & 63;
   const int P = c.iterations;
   int i = tid, p = 0;
   bool live = i < n;
@@ -361,22 +420,17 @@ __device__ void run_flow(const Ctx& c, int c0, int n, int b0, int* cnt, const Ve
     load_frec(c, c0 + i, r);
     rk = c.flow_rk[c0 + i];
   }
-  unsigned spins = 0;
-  int seen = -1;
+  Waiter wt;
   while (live) {
-    const int ta = p * rk.y + rk.x, tb = p * rk.w + rk.z;
     const int la = r.ga - b0, lb = r.gb - b0;
-    bool ready = true;
-    if (rk.x >= 0) ready = ld_acquire(cnt + la) == ta;
-    if (ready && rk.z >= 0) ready = ld_acquire(cnt + lb) == tb;
+    const bool ready = item_ready(cnt, la, lb, rk, p);
     if (ready) {
       if (p == 0) {
         if (r.flags & 4) warm_frec<VLDS>(r, V);
       } else {
         improve_frec<VLDS>(c, c0 + i, r, V);
       }
-      if (rk.x >= 0) st_release(cnt + la, ta + 1);
-      if (rk.z >= 0) st_release(cnt + lb, tb + 1);
+      item_done(cnt, la, lb, rk, p);
       int ni = i + T, np = p;
       if (ni >= n) {
         ni = tid;
@@ -392,24 +446,117 @@ __device__ void run_flow(const Ctx& c, int c0, int n, int b0, int* cnt, const Ve
         i = ni;
         p = np;
       }
-      spins = 0;
     }
-    const unsigned long long m = __ballot(ready);
-    if (m) {
-      if (lane == __ffsll((long long)m) - 1) atomicAdd(progress, __popcll(m));
-    } else {
-      __builtin_amdgcn_s_sleep(1);
-    }
-    if (!ready) {
-      int pr = __hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (pr != seen) {
-        seen = pr;
-        spins = 0;
-      } else if (++spins > WAIT_LIMIT) {
-        atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_SOLVER_STALL);
+    if (!wt.after_poll(c, ready, progress)) live = false;
+  }
+}
+
+// Level of every contact in one pass (the same counters, integer work
+// only, cyclic lane order): lvl[k] = 1 + max(level of the previous contact
+// on a, on b), with last[] holding per body the level of its latest
+// contact (cleared).  Returns the lane's largest level.
+template <int T>
+__device__ int run_levels(const Ctx& c, int c0, int n, int b0, int* cnt, int* last, int* lvl,
+                          int* progress) {
+  const int tid = threadIdx.x;
+  const int P = 0;
+  int i = tid, p = 0, maxl = 0;
+  bool live = i < n;
+  int la = 0, lb = 0;
+  int4 rk = make_int4(-1, 0, -1, 0);
+  if (live) {
+    la = c.ga[c0 + i] - b0;
+    lb = c.gb[c0 + i] - b0;
+    rk = c.flow_rk[c0 + i];
+  }
+  Waiter wt;
+  while (live) {
+    const bool ready = item_ready(cnt, la, lb, rk, p);
+    if (ready) {
+      int l = 0;
+      if (rk.x >= 0) l = last[la];
+      if (rk.z >= 0) l = max(l, last[lb]);
+      ++l;
+      if (rk.x >= 0) last[la] = l;
+      if (rk.z >= 0) last[lb] = l;
+      lvl[i] = l;
+      maxl = max(maxl, l);
+      item_done(cnt, la, lb, rk, p);
+      int ni = i + T, np = p;
+      if (ni >= n) {
+        ni = tid;
+        ++np;
+      }
+      if (np > P) {
         live = false;
+      } else {
+        if (ni != i) {
+          la = c.ga[c0 + ni] - b0;
+          lb = c.gb[c0 + ni] - b0;
+          rk = c.flow_rk[c0 + ni];
+        }
+        i = ni;
+        p = np;
       }
     }
+    if (!wt.after_poll(c, ready, progress)) live = false;
+  }
+  return maxl;
+}
+
+// Counting sort of the n contacts by level (levels 1..maxl): order[s]
+// gets the contact of slot s, and bins[l] (LDS, maxl + 1 ints) ends as the
+// end of level l's slots (level l = [bins[l - 1], bins[l]), bins[0] = 0).
+template <int T>
+__device__ void sort_by_level(const int* lvl, int n, int maxl, int* bins, int* order, int* wsum) {
+  const int tid = threadIdx.x;
+  for (int l = tid; l <= maxl; l += T) bins[l] = 0;
+  __syncthreads();
+  for (int q = tid; q < n; q += T) atomicAdd(&bins[lvl[q]], 1);
+  __syncthreads();
+  block_scan_exclusive<T>(bins, maxl + 1, wsum);
+  for (int q = tid; q < n; q += T) order[atomicAdd(&bins[lvl[q]], 1)] = q;
+  __syncthreads();
+}
+
+// Pass p over slot-ordered records, level by level.  Thread t takes slots
+// start(l) + t, + T, ... of every level, so a slot keeps its thread in
+// every pass and the lagrangians it wrote last pass are its own; the
+// record of its first slot of the next level (lagrangians included) is
+// loaded before the barrier, the velocities after it.
+template <bool VLDS, int T>
+__device__ void run_pass_levels(const Ctx& c, int c0, int p, int maxl, const int* bins,
+                                const Vel<VLDS>& V) {
+  const int tid = threadIdx.x;
+  FRec r, rn;
+  int s = tid;  // level 1 starts at slot 0
+  bool has = maxl >= 1 && s < bins[1];
+  if (has) load_frec(c, c0 + s, r);
+  for (int l = 1; l <= maxl; ++l) {
+    const int end = bins[l];
+    const int sn = end + tid;
+    const bool hasn = l < maxl && sn < bins[l + 1];
+    if (hasn) load_frec(c, c0 + sn, rn);
+    if (has) {
+      if (p == 0) {
+        if (r.flags & 4) warm_frec<VLDS>(r, V);
+      } else {
+        improve_frec<VLDS>(c, c0 + s, r, V);
+      }
+    }
+    for (int s2 = s + T; s2 < end; s2 += T) {
+      FRec r2;
+      load_frec(c, c0 + s2, r2);
+      if (p == 0) {
+        if (r2.flags & 4) warm_frec<VLDS>(r2, V);
+      } else {
+        improve_frec<VLDS>(c, c0 + s2, r2, V);
+      }
+    }
+    __syncthreads();
+    r = rn;
+    has = hasn;
+    s = sn;
   }
 }
 
@@ -420,6 +567,7 @@ __global__ __launch_bounds__(T) void k_solve_flow(Ctx c) {
   extern __shared__ double lds_d[];
   __shared__ int wsum[T / 64];
   __shared__ int progress;
+  __shared__ int s_maxl;
   const int w = blockIdx.x;
   const int tid = threadIdx.x;
   const int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
@@ -428,14 +576,23 @@ __global__ __launch_bounds__(T) void k_solve_flow(Ctx c) {
   const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
   double* vel = lds_d;
   int* ip = (int*)(lds_d + (VLDS ? 3 * nb : 0));
-  int* off = CLDS ? ip : c.flow_off + b0;
-  int* cur = CLDS ? ip + nb : c.flow_cur + b0;
+  int* bins = ip;  // LEVEL_CAP + 1
+  int* off = CLDS ? ip + (LEVEL_CAP + 1) : c.flow_off + b0;
+  int* cur = CLDS ? off + nb : c.flow_cur + b0;
   tmark(c, 0);
   compute_ranks<T>(c, c0, n, b0, nb, off, cur, wsum);
   tmark(c, 1);
   int* cnt = off;
-  for (int i = tid; i < nb; i += T) cnt[i] = 0;
-  if (tid == 0) progress = 0;
+  int* last = cur;
+  const bool levels = !VLDS && n > T;
+  for (int i = tid; i < nb; i += T) {
+    cnt[i] = 0;
+    if (levels) last[i] = 0;
+  }
+  if (tid == 0) {
+    progress = 0;
+    s_maxl = 0;
+  }
   Vel<VLDS> V;
   V.lds = vel;
   V.c = &c;
@@ -449,14 +606,40 @@ __global__ __launch_bounds__(T) void k_solve_flow(Ctx c) {
   }
   __syncthreads();
   tmark(c, 2);
+  int maxl = 0;
+  int* order = c.flow_order + c0;
+  if (levels) {
+    int m = run_levels<T>(c, c0, n, b0, cnt, last, c.flow_lvl + c0, &progress);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    if ((tid & 63) == 0) atomicMax(&s_maxl, m);
+    __syncthreads();
+    maxl = s_maxl;
+    if (maxl <= LEVEL_CAP) {
+      sort_by_level<T>(c.flow_lvl + c0, n, maxl, bins, order, wsum);  // ends with a barrier
+      for (int s = tid; s < n; s += T) pack_flow(c, c0 + order[s], c0 + s);
+    } else {  // too deep for the bins: the dataflow with its counters
+      for (int i = tid; i < nb; i += T) cnt[i] = 0;
+      if (tid == 0) progress = 0;
+      for (int k = c0 + tid; k < c1; k += T) pack_flow(c, k, k);
+    }
+  } else {
+    for (int k = c0 + tid; k < c1; k += T) pack_flow(c, k, k);
+  }
+  __syncthreads();
   tmark(c, 3);
-  run_flow<VLDS, T>(c, c0, n, b0, cnt, V, &progress);
+  const bool sync = levels && maxl <= LEVEL_CAP;
+  if (sync) {
+    for (int p = 0; p <= c.iterations; ++p) run_pass_levels<VLDS, T>(c, c0, p, maxl, bins, V);
+  } else {
+    run_flow<VLDS, T>(c, c0, n, b0, cnt, V, &progress);
+  }
   __syncthreads();
   tmark(c, 4);
   const long CC = c.contact_cap;
-  for (int k = c0 + tid; k < c1; k += T) {
-    c.ln[k] = c.pk[FL_LN * CC + k];
-    c.lt[k] = c.pk[FL_LT * CC + k];
+  for (int s = tid; s < n; s += T) {
+    const int k = sync ? c0 + order[s] : c0 + s;
+    c.ln[k] = c.pk[FL_LN * CC + c0 + s];
+    c.lt[k] = c.pk[FL_LT * CC + c0 + s];
   }
   if (VLDS) {
     for (int i = tid; i < nb; i += T) {
@@ -472,8 +655,8 @@ __global__ __launch_bounds__(T) void k_solve_flow(Ctx c) {
 
 template <bool CLDS, bool VLDS>
 static const void* flow_kernel(int threads) {
-  return threads == 1024 ? (const void*)k_solve_flow<CLDS, VLDS, 1024>
-                         : (const void*)k_solve_flow<CLDS, VLDS, 256>;
+  return threads == 512 ? (const void*)k_solve_flow<CLDS, VLDS, 512>
+                        : (const void*)k_solve_flow<CLDS, VLDS, 256>;
 }
 
 shp_status solver_alloc(shp_batch* b) {
@@ -482,22 +665,28 @@ shp_status solver_alloc(shp_batch* b) {
   void** bufs[] = {(void**)&c.sched_level, (void**)&c.sched_order, (void**)&c.pk,
                    (void**)&c.pk_g,        (void**)&c.pk_f,        (void**)&c.fvx,
                    (void**)&c.fvy,         (void**)&c.fw,          (void**)&c.flow_list,
-                   (void**)&c.flow_rk,     (void**)&c.flow_off,    (void**)&c.flow_cur};
+                   (void**)&c.flow_rk,     (void**)&c.flow_off,    (void**)&c.flow_cur,
+                   (void**)&c.flow_lvl,    (void**)&c.flow_order};
   // pk also holds the coloured solver's records (28 FP32 words + 2 FP64
   // lambdas per slot, less than 29 doubles)
+  // levels and the level order of the contacts (ordered solver only)
+  const long items = c.solver == SHP_SOLVER_ORDERED_F64 ? CC : 0;
   size_t sizes[] = {CC * 4UL,        CC * 4UL,        29UL * CC * 8,   2UL * CC * 4,
                     (size_t)CC,      (size_t)c.NB * 4, (size_t)c.NB * 4, (size_t)c.NB * 4,
-                    2UL * CC * 4,    CC * 16UL,       (size_t)c.NB * 4, (size_t)c.NB * 4};
+                    2UL * CC * 4,    CC * 16UL,       (size_t)c.NB * 4, (size_t)c.NB * 4,
+                    4UL * items,     4UL * items};
   static_assert(FL_FIELDS <= 29, "compact record exceeds the packed buffer");
-  for (int k = 0; k < 12; ++k) {
+  for (int k = 0; k < 14; ++k) {
     if (hipMalloc(bufs[k], sizes[k] ? sizes[k] : 16) != hipSuccess) return SHP_E_OOM;
     b->allocs.push_back(*bufs[k]);
   }
   int max_nb = 0;
   for (int w = 0; w < c.W; ++w) max_nb = std::max(max_nb, b->h_body_off[w + 1] - b->h_body_off[w]);
-  // LDS per world: velocities (24 B/body) and offsets + cursors (8 B/body)
-  // when they fit; offsets + cursors alone; else both in global memory
-  size_t both = 32UL * max_nb, cnt_only = 8UL * max_nb;
+  // LDS per world: level bins, then velocities (24 B/body) and offsets +
+  // cursors (8 B/body) when they fit; offsets + cursors alone; else both in
+  // global memory
+  const size_t bins = 4UL * (LEVEL_CAP + 1);
+  size_t both = bins + 32UL * max_nb, cnt_only = bins + 8UL * max_nb;
   if (both <= LDS_MAX_BYTES) {
     b->solve_clds = true;
     b->solve_vlds = true;
@@ -509,11 +698,11 @@ shp_status solver_alloc(shp_batch* b) {
   } else {
     b->solve_clds = false;
     b->solve_vlds = false;
-    b->solve_lds = 0;
+    b->solve_lds = bins;
   }
-  // one lane per contact for worlds up to ~1k contacts per 256 lanes; a
-  // large world gets 16 waves
-  b->solve_threads = max_nb > 2048 ? 1024 : 256;
+  // 4 waves per world; a large world gets 8 (a level-synchronous pass
+  // keeps two records per lane, so 8 waves leave it 256 registers per lane)
+  b->solve_threads = max_nb > 2048 ? 512 : 256;
   const void* fn = b->solve_clds ? (b->solve_vlds ? flow_kernel<true, true>(b->solve_threads)
                                                   : flow_kernel<true, false>(b->solve_threads))
                                  : flow_kernel<false, false>(b->solve_threads);
@@ -535,8 +724,8 @@ static void launch_flow(shp_batch* b) {
 }
 
 void launch_ordered_solver(shp_batch* b) {
-  if (b->solve_threads == 1024)
-    launch_flow<1024>(b);
+  if (b->solve_threads == 512)
+    launch_flow<512>(b);
   else
     launch_flow<256>(b);
 }

@@ -67,7 +67,8 @@ class StepStatsC(C.Structure):
                 ("max_depth", C.c_double), ("normal_residual", C.c_double),
                 ("friction_residual", C.c_double), ("run_max_depth", C.c_double),
                 ("run_normal_residual", C.c_double),
-                ("run_friction_residual", C.c_double)]
+                ("run_friction_residual", C.c_double), ("mean_max_depth", C.c_double),
+                ("mean_normal_residual", C.c_double), ("mean_friction_residual", C.c_double)]
 
 
 EXPORTS = [

@@ -99,7 +99,8 @@ def train(f, t, epochs: int = 8, batch: int = 4096, lr: float = 1e-3, seed: int 
 def evaluate(params, seed: int = EVAL_SEED, iters=(2, 4, 8), steps: int = 600,
              count: int = 8192, log=print):
     """Zero start vs learned start on the polygon pile with the coloured
-    solver: run maxima of penetration and residuals, and body-steps/s."""
+    solver: run maxima and run means of the per-step maxima of penetration
+    and residuals, and body-steps/s."""
     out = []
     for it in iters:
         for mode in ("zero", "learned"):
@@ -120,7 +121,10 @@ def evaluate(params, seed: int = EVAL_SEED, iters=(2, 4, 8), steps: int = 600,
                    "body_steps_per_s": round(sc.n_dynamic * steps / el, 1),
                    "max_penetration": s["run_max_depth"],
                    "normal_residual": s["run_normal_residual"],
-                   "friction_residual": s["run_friction_residual"]}
+                   "friction_residual": s["run_friction_residual"],
+                   "mean_step_max_penetration": s["mean_max_depth"],
+                   "mean_step_normal_residual": s["mean_normal_residual"],
+                   "mean_step_friction_residual": s["mean_friction_residual"]}
             log(json.dumps(row))
             out.append(row)
     return out

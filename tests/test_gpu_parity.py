@@ -161,21 +161,26 @@ def test_grid_broadphase_bitwise_worlds(orc):
     _run_grid(orc, worlds, scenes.SceneConfig(iterations=3), steps=40, every=10)
 
 
-def test_plank_chain_bitwise(orc):
-    """A dynamic 60x0.5 plank on 40 static posts under 50 loose boxes: the
-    plank touches ~130 contacts, so its items form one long chain through
-    every pass (the longest wait the dataflow schedule can see)."""
+@pytest.mark.parametrize("posts,boxes,steps", [(40, 50, 60), (240, 300, 12)])
+def test_plank_chain_bitwise(orc, posts, boxes, steps):
+    """A dynamic plank on static posts under loose boxes: the plank touches
+    every contact, so its items form one chain through all passes (the
+    longest wait the dataflow schedule can see).  The larger case has more
+    contacts than the solver has lanes and a chain deeper than the depth
+    sort takes, so it also runs the fallback lane order."""
     b = scenes._Builder()
     post = scenes.rectangle_vertices(0.6, 1.0)
-    for k in range(40):
-        b.add(post, (-29.25 + 1.5 * k, 0.5), mu=0.6)
-    b.add_dynamic(scenes.rectangle_vertices(60.0, 0.5), (0.0, 1.245), mu=0.6)
+    length = 1.5 * posts
+    for k in range(posts):
+        b.add(post, (-length / 2 + 0.75 + 1.5 * k, 0.5), mu=0.6)
+    b.add_dynamic(scenes.rectangle_vertices(length, 0.5), (0.0, 1.245), mu=0.6)
     box = scenes.rectangle_vertices(0.9, 0.9)
-    for k in range(50):
-        b.add_dynamic(box, (-29.4 + 1.2 * k, 1.94), mu=0.4)
+    for k in range(boxes):
+        b.add_dynamic(box, (-length / 2 + 0.6 + 1.2 * k * (length - 1.2) / (1.2 * boxes), 1.94),
+                      mu=0.4)
     cfg = scenes.SceneConfig(iterations=8)
-    ref = run_parity(orc, b.build(), cfg, steps=60, every=3)
-    assert ref.stats()["contacts"] > 150
+    ref = run_parity(orc, b.build(), cfg, steps=steps, every=3)
+    assert ref.stats()["contacts"] >= 2 * (posts + boxes)
 
 
 def test_large_world_global_counters_bitwise(orc):
