@@ -110,6 +110,21 @@ def step_bytes(nb: int, nv: int, contacts: float, word: int) -> float:
     return nb * 16 * word + nv * 2 * word + contacts * 48
 
 
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` measured with
+    rocprofv3 FETCH_SIZE and WRITE_SIZE counters (each scaled by the same
+    counter on a copy of known size), from profiles/pmc_<kernel>_<workload>.md;
+    None when no such measurement exists."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def roofline(scene, solver_name, steps, broadphase="auto", ppb=0):
     t, st = instrumented_run(scene, SOLVERS[solver_name], steps, broadphase, ppb)
     nb = scene.n_bodies
@@ -121,6 +136,7 @@ def roofline(scene, solver_name, steps, broadphase="auto", ppb=0):
     ms = t[dom]
     by = kernel_bytes(dom, st, nb, nv, 8 if dom == "solve" else word)
     achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    kname = f"solve_{solver_name}" if dom == "solve" else dom
     return {
         "kernel": dom,
         "bound": "hbm",
@@ -128,7 +144,7 @@ def roofline(scene, solver_name, steps, broadphase="auto", ppb=0):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 6),
-        "traffic": None,
+        "traffic": measured_traffic(kname, scene.name),
         "algorithmic_bytes_per_launch": round(by),
         "avg_launch_ms": round(ms, 5),
     }, {**{p: round(t[p], 5) for p in phases}, "solve_profile_us": t["solve_profile_us"]}, st

@@ -203,10 +203,15 @@ shp_status shp_mlp_get_params(shp_mlp* m, float* params);
 shp_status shp_mlp_forward(shp_mlp* m, const float* feats, int32_t n,
                            float* out);
 /* Gradient of the mean squared error over n rows w.r.t. every parameter
- * (blob layout) and the loss itself. */
+ * (blob layout) and the loss itself (with a Huber threshold set, the
+ * gradient is that of twice the mean Huber loss; the loss reported stays
+ * the squared error). */
 shp_status shp_mlp_gradient(shp_mlp* m, const float* feats,
                             const float* targets, int32_t n, float* grad,
                             float* loss);
+/* Training loss: squared error (huber_delta = 0, the default) or Huber with
+ * threshold huber_delta (robust to the heavy tail of impact impulses). */
+shp_status shp_mlp_set_loss(shp_mlp* m, float huber_delta);
 /* Upload a training set of n samples. */
 shp_status shp_mlp_set_data(shp_mlp* m, const float* feats,
                             const float* targets, int64_t n);

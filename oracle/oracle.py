@@ -22,10 +22,15 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 
+SOLVER_ORDERED = 0   # the reference's sequential FP64 sweep
+SOLVER_COLORED = 1   # restatement of the GPU's coloured FP32 schedule
+
+
 class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("baumgarte", C.c_double),
                 ("slop", C.c_double), ("gx", C.c_double), ("gy", C.c_double),
-                ("iterations", C.c_int32), ("trig", C.c_int32)]
+                ("iterations", C.c_int32), ("trig", C.c_int32),
+                ("solver", C.c_int32), ("world", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -94,7 +99,8 @@ def _i(a):
 class OracleEngine:
     """One world stepped by the oracle (updateWorld semantics)."""
 
-    def __init__(self, world, config, trig: int = TRIG_ENGINE):
+    def __init__(self, world, config, trig: int = TRIG_ENGINE, solver: int = SOLVER_ORDERED,
+                 world_index: int = 0):
         L = lib()
         keep = []
         args = []
@@ -108,7 +114,7 @@ class OracleEngine:
         hy, hyp = _d(world.hull_y)
         keep += [ho, hx, hy]
         prm = Params(config.dt, config.baumgarte, config.slop, config.gx,
-                     config.gy, config.iterations, trig)
+                     config.gy, config.iterations, trig, solver, world_index)
         self.n = world.n
         self._h = L.orc_engine_create(world.n, *args, hop, hxp, hyp, C.byref(prm))
         self._L = L

@@ -485,6 +485,7 @@ struct orc_engine {
   std::vector<Material> mats;
   std::vector<Hull> hulls;
   std::vector<std::pair<int, int>> pairs;  // last step
+  std::vector<int> pair_first;             // pairs + 1: first contact of each pair
   std::vector<Contact> contacts;           // last step == the cache
   orc_stats stats;
 };
@@ -512,7 +513,9 @@ void culled_keys(orc_engine* e, std::vector<std::pair<int, int>>& out) {
 // (Constraints/Contact.hs:207-215)
 void prepare_frame(orc_engine* e, std::vector<Contact>& out) {
   out.clear();
+  e->pair_first.clear();
   for (auto& pr : e->pairs) {
+    e->pair_first.push_back((int)out.size());
     ContactGeom g[2];
     bool sep;
     int k = hull_contacts(e->hulls[pr.first], e->hulls[pr.second], g, &sep);
@@ -525,6 +528,7 @@ void prepare_frame(orc_engine* e, std::vector<Contact>& out) {
       out.push_back(c);
     }
   }
+  e->pair_first.push_back((int)out.size());
 }
 
 // applyCachedSlns (Solvers/Contact.hs:74-121) with descZipVector
@@ -532,7 +536,7 @@ void prepare_frame(orc_engine* e, std::vector<Contact>& out) {
 // applying reused lagrangians immediately in contact order; new contacts
 // start at zero (:95).
 void apply_cached_slns(orc_engine* e, std::vector<Contact>& cs,
-                       const std::vector<Contact>& old) {
+                       const std::vector<Contact>& old, bool apply) {
   size_t that_i = 0;
   for (auto& c : cs) {
     Phys& a = e->objs[c.key.i];
@@ -549,7 +553,7 @@ void apply_cached_slns(orc_engine* e, std::vector<Contact>& cs,
       c.ln = old[that_i].ln;
       c.lt = old[that_i].lt;
       c.warm = true;
-      apply_sln(c.ln, c.lt, c.cc, a, b);
+      if (apply) apply_sln(c.ln, c.lt, c.cc, a, b);
       ++that_i;
     } else {
       c.ln = 0.0;
@@ -585,6 +589,176 @@ void improve_contact(orc_engine* e, Contact& c) {
   apply_sln(app_n, app_t, c.cc, a, b);
   c.ln = acc_n;
   c.lt = accp;
+}
+
+// ---- graph-coloured FP32 Gauss-Seidel (the GPU's SHP_SOLVER_COLORED_F32) ----
+//
+// Not a reference function: the same per-contact update as improve_contact
+// in FP32 (FP64 accumulators), in colour order.  Manifolds (pairs with
+// contacts) get colours from Jones-Plassmann rounds over hashed priorities;
+// a colour's manifolds share no dynamic body, so their order inside a
+// colour does not matter; the colours run in order, the warm-start
+// application (applyCachedSlns) too.
+
+uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+
+// Priority of manifold q (pair index in the world) of world w: 17 hash bits
+// above the index, distinct within a world of < 32768 pairs.
+uint32_t colour_prio(int q, int w) {
+  return (((mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x1FFFFu) << 15) | (uint32_t)q) + 1u;
+}
+
+const int COLOUR_CAP = 256;
+
+// colour[q] >= 0 for pairs with contacts; the last colour is the serial
+// tail (pair order) when COLOUR_CAP rounds were not enough.  Returns the
+// number of colours.
+int colour_pairs(orc_engine* e, std::vector<int>& colour, bool* tail) {
+  const int np = (int)e->pairs.size();
+  const int n = (int)e->objs.size();
+  colour.assign(np, -2);
+  int left = 0;
+  for (int q = 0; q < np; ++q)
+    if (e->pair_first[q + 1] > e->pair_first[q]) {
+      colour[q] = -1;
+      ++left;
+    }
+  std::vector<uint32_t> best(n);
+  int rounds = 0;
+  while (left > 0 && rounds < COLOUR_CAP) {
+    std::fill(best.begin(), best.end(), 0u);
+    for (int q = 0; q < np; ++q) {
+      if (colour[q] != -1) continue;
+      uint32_t pr = colour_prio(q, e->prm.world);
+      int a = e->pairs[q].first, b = e->pairs[q].second;
+      if (!is_static(e->objs[a]) && pr > best[a]) best[a] = pr;
+      if (!is_static(e->objs[b]) && pr > best[b]) best[b] = pr;
+    }
+    std::vector<int> take;
+    for (int q = 0; q < np; ++q) {
+      if (colour[q] != -1) continue;
+      uint32_t pr = colour_prio(q, e->prm.world);
+      int a = e->pairs[q].first, b = e->pairs[q].second;
+      if ((is_static(e->objs[a]) || best[a] == pr) && (is_static(e->objs[b]) || best[b] == pr))
+        take.push_back(q);
+    }
+    for (int q : take) colour[q] = rounds;
+    left -= (int)take.size();
+    ++rounds;
+  }
+  *tail = left > 0;
+  for (int q = 0; q < np; ++q)
+    if (colour[q] == -1) colour[q] = rounds;
+  return rounds + (*tail ? 1 : 0);
+}
+
+inline float dot6f(const float* a, const float* b) {
+  return ((((a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]) + a[3] * b[3]) + a[4] * b[4]) + a[5] * b[5];
+}
+
+inline void applyf(float l, const float* j, const float* im, float* v) {
+  for (int q = 0; q < 6; ++q) v[q] = v[q] + ((j[q] * l) * im[q]);
+}
+
+// Pass `pass` (0: warm start) over the contacts [k0, k1) of one manifold
+// between bodies a and b, FP32 velocities vf (3 per body).
+void solve_manifold_f32(orc_engine* e, std::vector<Contact>& cs, int k0, int k1, int pass,
+                        std::vector<float>& vf) {
+  const int ia = cs[k0].key.i, ib = cs[k0].key.j;
+  const Phys& A = e->objs[ia];
+  const Phys& B = e->objs[ib];
+  const Material& ma = e->mats[ia];
+  const Material& mb = e->mats[ib];
+  const float im[6] = {(float)A.iml, (float)A.iml, (float)A.imr,
+                       (float)B.iml, (float)B.iml, (float)B.imr};
+  const float mu = (float)((ma.mu + mb.mu) / 2.0);
+  const float eb = (float)hs_min(ma.bounce, mb.bounce);
+  float v[6] = {vf[3 * ia], vf[3 * ia + 1], vf[3 * ia + 2], vf[3 * ib], vf[3 * ib + 1], vf[3 * ib + 2]};
+  bool changed = false;
+  for (int k = k0; k < k1; ++k) {
+    Contact& c = cs[k];
+    float jn[6], jt[6];
+    for (int q = 0; q < 6; ++q) {
+      jn[q] = (float)c.cc.jn[q];
+      jt[q] = (float)c.cc.jt[q];
+    }
+    if (pass == 0) {
+      if (!c.warm) continue;
+      applyf((float)c.ln, jn, im, v);
+      applyf((float)c.lt, jt, im, v);
+      changed = true;
+      continue;
+    }
+    double imd[6];
+    im6(A, B, imd);
+    const float mcn = (float)eff_mass(c.cc.jn, imd), mct = (float)eff_mass(c.cc.jt, imd);
+    const float rax = (float)c.cc.ra.x, ray = (float)c.cc.ra.y;
+    const float rbx = (float)c.cc.rb.x, rby = (float)c.cc.rb.y;
+    const float nrx = (float)c.cc.nr.x, nry = (float)c.cc.nr.y;
+    float cx = ((-v[0] + v[2] * ray) + v[3]) - v[5] * rby;  // bounceB closing velocity
+    float cy = ((-v[1] - v[2] * rax) + v[4]) + v[5] * rbx;
+    float bb = eb * (cx * nrx + cy * nry);
+    bb = 0.0f <= bb ? 0.0f : bb;
+    float bn = (float)c.cc.bn + bb;
+    float new_n = -(dot6f(jn, v) + bn) / mcn;
+    float new_t = -dot6f(jt, v) / mct;
+    double app_n = hs_max((double)new_n, -c.ln);
+    double acc_n = c.ln + app_n;
+    double thr = acc_n * (double)mu;
+    double acc_t = c.lt + (double)new_t;
+    double accp = acc_t > thr ? thr : (acc_t < -thr ? -thr : acc_t);
+    double app_t = accp - c.lt;
+    applyf((float)app_n, jn, im, v);
+    applyf((float)app_t, jt, im, v);
+    c.ln = acc_n;
+    c.lt = accp;
+    changed = true;
+  }
+  if (!changed) return;
+  if (!is_static(A)) {
+    vf[3 * ia] = v[0];
+    vf[3 * ia + 1] = v[1];
+    vf[3 * ia + 2] = v[2];
+  }
+  if (!is_static(B)) {
+    vf[3 * ib] = v[3];
+    vf[3 * ib + 1] = v[4];
+    vf[3 * ib + 2] = v[5];
+  }
+}
+
+void solve_coloured(orc_engine* e, std::vector<Contact>& cs) {
+  std::vector<int> colour;
+  bool tail = false;
+  const int ncol = colour_pairs(e, colour, &tail);
+  const int n = (int)e->objs.size();
+  std::vector<float> vf(3 * n);
+  for (int i = 0; i < n; ++i) {
+    vf[3 * i] = (float)e->objs[i].vel.x;
+    vf[3 * i + 1] = (float)e->objs[i].vel.y;
+    vf[3 * i + 2] = (float)e->objs[i].w;
+  }
+  std::vector<std::vector<int>> members(ncol);
+  for (int q = 0; q < (int)colour.size(); ++q)
+    if (colour[q] >= 0) members[colour[q]].push_back(q);
+  for (int pass = 0; pass <= e->prm.iterations; ++pass)
+    for (int l = 0; l < ncol; ++l)
+      for (int q : members[l])
+        solve_manifold_f32(e, cs, e->pair_first[q], e->pair_first[q + 1], pass, vf);
+  for (int i = 0; i < n; ++i) {
+    Phys& o = e->objs[i];
+    if (is_static(o)) continue;
+    o.vel = v2(o.vel.x + (double)(vf[3 * i] - (float)o.vel.x),
+               o.vel.y + (double)(vf[3 * i + 1] - (float)o.vel.y));
+    o.w = o.w + (double)(vf[3 * i + 2] - (float)o.w);
+  }
 }
 
 void compute_stats(orc_engine* e) {
@@ -654,11 +828,17 @@ static void step_once(orc_engine* e) {
   std::vector<Contact> cs;
   prepare_frame(e, cs);
   // 4. applyCachedSlns against the previous frame's cache
-  apply_cached_slns(e, cs, e->contacts);
+  const bool coloured = p.solver == ORC_SOLVER_COLORED;
+  apply_cached_slns(e, cs, e->contacts, !coloured);
   e->contacts.swap(cs);
-  // 5. improveWorld sweeps in descending key order
-  for (int it = 0; it < p.iterations; ++it)
-    for (auto& c : e->contacts) improve_contact(e, c);
+  // 5. improveWorld sweeps in descending key order (or the coloured
+  //    schedule, warm start included)
+  if (coloured) {
+    solve_coloured(e, e->contacts);
+  } else {
+    for (int it = 0; it < p.iterations; ++it)
+      for (auto& c : e->contacts) improve_contact(e, c);
+  }
   compute_stats(e);
   residuals(e);
   // 6. advance (World.hs:194-196, Constraint.hs:225-229)

@@ -26,6 +26,11 @@ typedef struct orc_engine orc_engine;
 #define ORC_TRIG_ENGINE 0 /* correctly rounded shp_sincos (same bits as GPU) */
 #define ORC_TRIG_LIBM 1   /* host libm sin/cos, as GHC's sinDouble# calls */
 
+/* solve schedules */
+#define ORC_SOLVER_ORDERED 0 /* the reference: sequential, FP64 */
+#define ORC_SOLVER_COLORED 1 /* restatement of the GPU's graph-coloured FP32
+                                Gauss-Seidel (not a reference function) */
+
 typedef struct orc_params {
   double dt;
   double baumgarte;   /* ContactBehavior.contactBaumgarte */
@@ -33,6 +38,8 @@ typedef struct orc_params {
   double gx, gy;      /* constantAccel vector (External.hs:23-28) */
   int32_t iterations; /* improveWorld sweeps; the reference hard-codes 2 */
   int32_t trig;       /* ORC_TRIG_* */
+  int32_t solver;     /* ORC_SOLVER_* */
+  int32_t world;      /* the world's index in a GPU batch (colouring hash) */
 } orc_params;
 
 /* One world.  hull_off has n+1 entries (CSR into hull_x/hull_y); vertices

@@ -289,18 +289,27 @@ __device__ __forceinline__ int colour_of(uint16_t x) {
 }
 
 // Pack the coloured manifolds: colour l owns manifold indices
-// [p0 + s_off[l], p0 + s_off[l + 1]); scur holds the colour cursors.
+// [p0 + s_off[l], p0 + s_off[l + 1]); scur holds the colour cursors.  The
+// order inside a colour is free, except in the serial tail colour (tail >=
+// 0), whose manifolds may share bodies: thread 0 packs those in pair order.
 template <int T, class CT>
 __device__ void pack_colours(const Ctx& c, const PackF& P, int p0, int np, int c1,
-                             const CT* colour, int* scur) {
-  for (int q = threadIdx.x; q < np; q += T) {
-    const int cl = colour_of(colour[q]);
-    if (cl < 0) continue;
-    const int m = atomicAdd(&scur[cl], 1);
+                             const CT* colour, int* scur, int tail) {
+  auto pack = [&](int q, int m) {
     const int p = p0 + q;
     int k0 = c.coff[p], k1 = k0 + c.ccount[p];
     if (k1 > c1) k1 = c1;  // contacts beyond the capacity were not stored
     pack_man(c, P, k0, k1, p0 + m);
+  };
+  for (int q = threadIdx.x; q < np; q += T) {
+    const int cl = colour_of(colour[q]);
+    if (cl < 0 || cl == tail) continue;
+    pack(q, atomicAdd(&scur[cl], 1));
+  }
+  if (tail >= 0 && threadIdx.x == 0) {
+    int m = scur[tail];
+    for (int q = 0; q < np; ++q)
+      if (colour_of(colour[q]) == tail) pack(q, m++);
   }
 }
 
@@ -487,7 +496,7 @@ __global__ __launch_bounds__(T) void k_solve_colored(Ctx c, int pool_bytes) {
   if (tid == 0) atomicMax(&c.counters[CNT_LEVELS], ncol);
   colour_offsets<T>(colour, np, ncol, s_off, s_cur);
   const PackF P = packf(c);
-  pack_colours<T>(c, P, p0, np, c1, colour, s_cur);
+  pack_colours<T>(c, P, p0, np, c1, colour, s_cur, serial_tail ? ncol - 1 : -1);
   __syncthreads();  // colouring data is dead: the pool now holds velocities
   VelF<true> V;
   V.lds = (float*)pool;
@@ -578,7 +587,7 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_solve_colored_global(Ctx c,
   if (tid == 0) atomicMax(&c.counters[CNT_LEVELS], ncol);
   colour_offsets<COLOR_THREADS_G>(colour, np, ncol, s_off, s_cur);
   const PackF P = packf(c);
-  pack_colours<COLOR_THREADS_G>(c, P, p0, np, c1, colour, s_cur);
+  pack_colours<COLOR_THREADS_G>(c, P, p0, np, c1, colour, s_cur, serial_tail ? ncol - 1 : -1);
   for (int i = tid; i < nb; i += COLOR_THREADS_G) {
     c.fvx[b0 + i] = (float)c.vx[b0 + i];
     c.fvy[b0 + i] = (float)c.vy[b0 + i];

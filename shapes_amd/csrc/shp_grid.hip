@@ -75,64 +75,92 @@ __global__ void k_grid_fill(Ctx c) {
   c.bucket_items[pos] = g;
 }
 
-// One lane per body i: count (fill = 0) or emit (fill = 1) its pairs.
-__global__ void k_grid_pairs(Ctx c, int fill) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= c.NB) return;
-  int w = c.body_world[g];
-  int b0 = c.body_off[w];
-  int li = g - b0;
-  int nw = c.body_off[w + 1] - b0;
-  int rank = b0 + (nw - 1 - li);
-  bool si = c.is_static[g] != 0;
-  double aminx = c.bminx[g], amaxx = c.bmaxx[g], aminy = c.bminy[g], amaxy = c.bmaxy[g];
-  int base = fill ? c.off_rank[rank] : 0;
-  int count = 0;
-  auto emit = [&](int gj) {
-    if (fill) {
-      int pos = base + count;
-      if (pos < c.pair_cap) {
-        c.pa[pos] = g;
-        c.pb[pos] = gj;
-      }
-    }
-    ++count;
-  };
+// Sixteen lanes per body i: count (fill = 0) or emit (fill = 1) its pairs.
+// A small body's lanes 0..8 take one cell each of the <= 3x3 cells that can
+// hold a partner's min corner, lanes 9..15 the world's large bodies; a
+// large body's lanes stride over every j < i.  Each lane visits its
+// candidates twice when emitting (count, then write at the group's prefix
+// sum); lane 0 then sorts the segment into descending j.
+#define GRID_LANES 16
+
+template <class F>
+__device__ __forceinline__ void grid_visit(const Ctx& c, int g, int sl, F&& hit) {
+  const int w = c.body_world[g];
+  const int b0 = c.body_off[w];
+  const bool si = c.is_static[g] != 0;
+  const double aminx = c.bminx[g], amaxx = c.bmaxx[g], aminy = c.bminy[g], amaxy = c.bmaxy[g];
   if (c.is_large[g]) {
-    for (int gj = g - 1; gj >= b0; --gj)
-      if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) emit(gj);
-  } else {
-    for (int q = c.large_off[w]; q < c.large_off[w + 1]; ++q) {
-      int gj = c.large_ids[q];
+    for (int gj = g - 1 - sl; gj >= b0; gj -= GRID_LANES)
+      if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) hit(gj);
+    return;
+  }
+  if (sl >= 9) {
+    for (int q = c.large_off[w] + (sl - 9); q < c.large_off[w + 1]; q += GRID_LANES - 9) {
+      const int gj = c.large_ids[q];
       if (gj >= g) break;  // ascending
-      if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) emit(gj);
+      if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) hit(gj);
     }
-    double h = c.cell_h[w];
-    int cx0 = cell_of(aminx, h) - 1, cx1 = cell_of(amaxx, h);
-    int cy0 = cell_of(aminy, h) - 1, cy1 = cell_of(amaxy, h);
-    int t0 = c.tab_off[w];
-    uint32_t mask = (uint32_t)(c.tab_off[w + 1] - t0 - 1);
-    for (int cy = cy0; cy <= cy1; ++cy)
-      for (int cx = cx0; cx <= cx1; ++cx) {
-        int b = t0 + cell_hash(cx, cy, mask);
-        for (int q = c.bucket_start[b]; q < c.bucket_start[b + 1]; ++q) {
-          int gj = c.bucket_items[q];
-          if (gj >= g || c.body_cell[2 * gj] != cx || c.body_cell[2 * gj + 1] != cy) continue;
-          if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) emit(gj);
-        }
-      }
+    return;
   }
+  const double h = c.cell_h[w];
+  const int cx0 = cell_of(aminx, h) - 1, cx1 = cell_of(amaxx, h);
+  const int cy0 = cell_of(aminy, h) - 1, cy1 = cell_of(amaxy, h);
+  // a small body's AABB spans at most 2 cells per axis, so ncells <= 9 and
+  // each of lanes 0..8 takes at most one cell; the stride keeps any other
+  // case correct
+  const long ncx = (long)cx1 - cx0 + 1, ncy = (long)cy1 - cy0 + 1;
+  const int t0 = c.tab_off[w];
+  const uint32_t mask = (uint32_t)(c.tab_off[w + 1] - t0 - 1);
+  for (long ci = sl; ci < ncx * ncy; ci += 9) {
+    const int cx = (int)(cx0 + ci % ncx), cy = (int)(cy0 + ci / ncx);
+    const int b = t0 + cell_hash(cx, cy, mask);
+    for (int q = c.bucket_start[b]; q < c.bucket_start[b + 1]; ++q) {
+      const int gj = c.bucket_items[q];
+      if (gj >= g || c.body_cell[2 * gj] != cx || c.body_cell[2 * gj + 1] != cy) continue;
+      if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) hit(gj);
+    }
+  }
+}
+
+__global__ void k_grid_pairs(Ctx c, int fill) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = (int)(t / GRID_LANES);  // the same for the 16 lanes of a group
+  const int sl = (int)(t % GRID_LANES);
+  if (g >= c.NB) return;
+  int count = 0;
+  grid_visit(c, g, sl, [&](int) { ++count; });
+  int incl = count;
+  for (int o = 1; o < GRID_LANES; o <<= 1) {
+    const int y = __shfl_up(incl, o, GRID_LANES);
+    if (sl >= o) incl += y;
+  }
+  const int total = __shfl(incl, GRID_LANES - 1, GRID_LANES);
+  const int w = c.body_world[g];
+  const int b0 = c.body_off[w];
+  const int nw = c.body_off[w + 1] - b0;
+  const int rank = b0 + (nw - 1 - (g - b0));
   if (!fill) {
-    c.cnt_rank[rank] = count;
+    if (sl == 0) c.cnt_rank[rank] = total;
     return;
   }
-  if (base + count > c.pair_cap) {
-    atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_PAIR_OVERFLOW);
+  const int base = c.off_rank[rank];
+  if (base + total > c.pair_cap) {
+    if (sl == 0) atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_PAIR_OVERFLOW);
     return;
   }
+  int pos = base + incl - count;
+  grid_visit(c, g, sl, [&](int gj) {
+    c.pa[pos] = g;
+    c.pb[pos] = gj;
+    ++pos;
+  });
+  // the group's stores precede lane 0's sort in this wavefront's program
+  // order; the fence keeps the compiler from moving them
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (sl != 0) return;
   // the segment in descending j (insertion sort; segments are short)
-  for (int a = base + 1; a < base + count; ++a) {
-    int v = c.pb[a];
+  for (int a = base + 1; a < base + total; ++a) {
+    const int v = c.pb[a];
     int q = a - 1;
     while (q >= base && c.pb[q] < v) {
       c.pb[q + 1] = c.pb[q];
@@ -162,13 +190,13 @@ hipError_t launch_grid_count(shp_batch* b) {
                                                   c.grid_buckets + 1, s);
   if (e != hipSuccess) return e;
   k_grid_fill<<<nblk(c.NB, 256), 256, 0, s>>>(c);
-  k_grid_pairs<<<nblk(c.NB, 256), 256, 0, s>>>(c, 0);
+  k_grid_pairs<<<nblk((long)c.NB * GRID_LANES, 256), 256, 0, s>>>(c, 0);
   return hipGetLastError();
 }
 
 // Second half: emit the pairs at off_rank.
 hipError_t launch_grid_emit(shp_batch* b) {
-  k_grid_pairs<<<nblk(b->ctx.NB, 256), 256, 0, b->stream>>>(b->ctx, 1);
+  k_grid_pairs<<<nblk((long)b->ctx.NB * GRID_LANES, 256), 256, 0, b->stream>>>(b->ctx, 1);
   return hipGetLastError();
 }
 

@@ -190,13 +190,15 @@ __global__ __launch_bounds__(256) void k_mlp_forward(MlpDev m, const uint16_t* X
   }
 }
 
-// Backward of the mean squared error over `rows` rows (rows % 64 == 0)
-// given the forward buffers.  dY = 2 (Y - T) / n_real for real rows.
+// Backward of the mean loss over `rows` rows (rows % 64 == 0) given the
+// forward buffers: squared error (huber = 0), dY = 2 (Y - T) / n_real, or
+// twice the Huber loss with threshold `huber`, dY = 2 clamp(Y - T, +-huber)
+// / n_real (the same gradient where |Y - T| <= huber).
 // Workgroup g accumulates the weight gradients of its tiles in MFMA
 // accumulators and writes partial g (MLP_NPARAM_PAD floats).
 __global__ __launch_bounds__(256) void k_mlp_backward(MlpDev m, const uint16_t* X, const float* T,
                                                       int rows, int n_real, TrainBufs tb,
-                                                      float* partial) {
+                                                      float* partial, float huber) {
   __shared__ uint16_t s_dy[TILE * 16];
   __shared__ uint16_t s_da[TILE * 128];
   __shared__ uint16_t s_db[TILE * 128];
@@ -217,7 +219,11 @@ __global__ __launch_bounds__(256) void k_mlp_backward(MlpDev m, const uint16_t* 
     for (int i = threadIdx.x; i < TILE * 16; i += 256) {
       int r = i >> 4, cc = i & 15;
       float d = 0.f;
-      if (cc < 2 && row0 + r < n_real) d = (tb.y[(size_t)(row0 + r) * 16 + cc] - T[(size_t)(row0 + r) * 2 + cc]) * inv_n;
+      if (cc < 2 && row0 + r < n_real) {
+        float e = tb.y[(size_t)(row0 + r) * 16 + cc] - T[(size_t)(row0 + r) * 2 + cc];
+        if (huber > 0.f) e = fminf(fmaxf(e, -huber), huber);
+        d = e * inv_n;
+      }
       s_dy[i] = f2bf(d);
       if (cc < 2) atomicAdd(&s_bias[3 * 128 + cc], d);
     }
@@ -511,6 +517,7 @@ struct shp_mlp {
   uint16_t* dX = nullptr;
   float* dT = nullptr;
   long n_data = 0;
+  float huber = 0.f;  // 0: squared error
 };
 
 static shp_status mlp_fail(shp_status s, const char* m) {
@@ -565,7 +572,7 @@ static shp_status mlp_grad_rows(shp_mlp* h, int n) {
   MlpDev m = mlp_view(h->p);
   k_mlp_forward<<<fg, 256, 0, h->stream>>>(m, h->X, rows, h->tb.y, h->tb, 1);
   int parts = tiles < h->parts ? tiles : h->parts;
-  k_mlp_backward<<<parts, 256, 0, h->stream>>>(m, h->X, h->T, rows, n, h->tb, h->partial);
+  k_mlp_backward<<<parts, 256, 0, h->stream>>>(m, h->X, h->T, rows, n, h->tb, h->partial, h->huber);
   k_mlp_reduce<<<(MLP_NPARAM_PAD + 255) / 256, 256, 0, h->stream>>>(h->partial, parts, h->grad);
   MCHK(hipGetLastError());
   return SHP_OK;
@@ -676,6 +683,13 @@ shp_status shp_mlp_gradient(shp_mlp* h, const float* feats, const float* targets
       }
     *loss_out = (float)(l / n);
   }
+  return SHP_OK;
+}
+
+shp_status shp_mlp_set_loss(shp_mlp* h, float huber_delta) {
+  if (!h || !(huber_delta >= 0.f))
+    return mlp_fail(SHP_E_INVAL, "huber_delta must be >= 0");
+  h->huber = huber_delta;
   return SHP_OK;
 }
 

@@ -81,7 +81,7 @@ EXPORTS = [
     "shp_batch_read_solve_profile",
     "shp_batch_set_logging", "shp_batch_read_log", "shp_mlp_create",
     "shp_mlp_destroy", "shp_mlp_get_params", "shp_mlp_forward",
-    "shp_mlp_gradient", "shp_mlp_set_data", "shp_mlp_train",
+    "shp_mlp_gradient", "shp_mlp_set_data", "shp_mlp_train", "shp_mlp_set_loss",
 ]
 
 PHASES = ["reset", "broadphase_count", "scan_pairs", "broadphase_fill", "sat",
@@ -126,6 +126,7 @@ def load_library(path: str = LIB_PATH):
     L.shp_mlp_gradient.argtypes = [C.c_void_p, _fp, _fp, C.c_int32, _fp, _fp]
     L.shp_mlp_set_data.argtypes = [C.c_void_p, _fp, _fp, C.c_int64]
     L.shp_mlp_train.argtypes = [C.c_void_p, _ip, C.c_int32, C.c_int32, C.c_float, _fp]
+    L.shp_mlp_set_loss.argtypes = [C.c_void_p, C.c_float]
     L.shp_batch_set_timing.argtypes = [C.c_void_p, C.c_int32]
     L.shp_batch_read_timing.argtypes = [C.c_void_p, _dp, _ip]
     L.shp_batch_read_solve_profile.argtypes = [C.c_void_p, _dp]
@@ -348,6 +349,10 @@ class Mlp:
                                         t.ctypes.data_as(_fp), len(f),
                                         g.ctypes.data_as(_fp), C.byref(loss)))
         return g, loss.value
+
+    def set_loss(self, huber_delta: float = 0.0):
+        """Squared error (0) or Huber loss with this threshold."""
+        _check(self._L.shp_mlp_set_loss(self._h, float(huber_delta)))
 
     def set_data(self, feats, targets):
         f = np.ascontiguousarray(feats, dtype=np.float32)

@@ -73,9 +73,13 @@ def collect(samples: int, count: int = 8192, steps: int = 600, iterations: int =
 
 
 def train(f, t, epochs: int = 8, batch: int = 4096, lr: float = 1e-3, seed: int = 0,
-          log=print):
-    t = np.clip(t, -20.0, 20.0)
+          huber: float = 0.0, clip: float = 20.0, log=print):
+    """Adam on mini-batches; targets clipped to +-clip (0: no clip); loss
+    squared error, or Huber with threshold `huber`."""
+    if clip > 0:
+        t = np.clip(t, -clip, clip)
     net = engine.Mlp(init_params(seed))
+    net.set_loss(huber)
     net.set_data(f, t)
     rng = np.random.default_rng(seed)
     n = (len(f) // batch) * batch
@@ -144,6 +148,8 @@ def main(argv=None):
     t.add_argument("--epochs", type=int, default=8)
     t.add_argument("--batch", type=int, default=4096)
     t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--huber", type=float, default=0.0)
+    t.add_argument("--clip", type=float, default=20.0)
     e = sub.add_parser("eval")
     e.add_argument("--model", default=DEFAULT_MODEL)
     e.add_argument("--steps", type=int, default=600)
@@ -156,7 +162,7 @@ def main(argv=None):
     elif a.cmd == "train":
         d = np.load(a.data)
         p, hist = train(d["feats"], d["targets"], epochs=a.epochs, batch=a.batch, lr=a.lr,
-                        log=log)
+                        huber=a.huber, clip=a.clip, log=log)
         np.save(a.out, p)
         log(f"train: wrote {a.out}")
     else:

@@ -63,3 +63,50 @@ def test_ordered_and_colored_quality_on_batch():
         b.close()
     print(out)
     assert out["colored"]["run_max_depth"] < 2.0 * out["ordered"]["run_max_depth"] + 0.02
+
+
+def _colored_vs_restatement(orc, worlds, cfg, steps, every):
+    """GPU coloured solver against the oracle's restatement of the same
+    schedule (Jones-Plassmann colours, FP32 rows and velocities, FP64
+    accumulators, colour order): every bit of state must agree."""
+    gpu = engine.make_batch(worlds, cfg, solver=engine.SOLVER_COLORED_F32)
+    refs = [orc.OracleEngine(w, cfg, solver=orc.SOLVER_COLORED, world_index=k)
+            for k, w in enumerate(worlds)]
+    done = 0
+    while done < steps:
+        gpu.step(every)
+        done += every
+        assert gpu.stats()["flags"] == 0
+        for k, r in enumerate(refs):
+            r.step(every)
+            gb, rb = gpu.bodies(k), r.bodies()
+            diff = np.nonzero(gb.view(np.uint64).ravel() != rb.view(np.uint64).ravel())[0]
+            assert len(diff) == 0, (f"world {k} step {done}: {len(diff)} values differ, first "
+                                    f"{gb.ravel()[diff[0]]!r} vs {rb.ravel()[diff[0]]!r}")
+            gl, rl = gpu.contacts(k)["lam"], r.contacts()["lam"]
+            assert np.array_equal(gl.view(np.uint64), rl.view(np.uint64)), f"lambdas, world {k}"
+    gpu.close()
+
+
+def test_colored_matches_restatement_pyramid(orc):
+    sc = scenes.pyramid()
+    _colored_vs_restatement(orc, sc.worlds, sc.config, steps=300, every=10)
+
+
+def test_colored_matches_restatement_batch(orc):
+    worlds = [scenes.random_world(700 + k, count=20 + 11 * k, spacing=1.25) for k in range(6)]
+    worlds.append(scenes.reference_stacks().worlds[0])
+    _colored_vs_restatement(orc, worlds, scenes.SceneConfig(iterations=6), steps=60, every=10)
+
+
+def test_colored_matches_restatement_polygon_pile(orc):
+    """The 8192-polygon world: 8 waves per world, LDS velocities."""
+    sc = scenes.polygon_pile()
+    _colored_vs_restatement(orc, sc.worlds, sc.config, steps=90, every=30)
+
+
+def test_colored_matches_restatement_global_path(orc):
+    """A world too large for LDS: colouring and velocities in global memory."""
+    rng = np.random.Generator(np.random.Philox(78))
+    w = scenes.grid_world(rng, 24000, cols=200, spacing=1.05, y0=1.0, square_frac=0.5, vel=0.3)
+    _colored_vs_restatement(orc, [w], scenes.SceneConfig(iterations=3), steps=4, every=2)

@@ -104,3 +104,34 @@ def test_learned_start_counts_misses():
     assert s["learned_starts"] == (c["warm"] == 2).sum()
     assert s["learned_starts"] > 0
     lb.close()
+
+
+def _pile_quality(model, iterations, steps=600):
+    sc = scenes.polygon_pile(iterations=iterations)
+    ws = engine.WARM_LEARNED if model is not None else engine.WARM_ZERO
+    b = engine.make_batch(sc.worlds, sc.config, solver=engine.SOLVER_COLORED_F32, warm_start=ws)
+    if model is not None:
+        b.set_mlp(model)
+    b.step(steps)
+    s = b.stats()
+    b.close()
+    return s
+
+
+def test_trained_model_beats_zero_start():
+    """The committed network (shapes_amd/warmstart_mlp.npy, trained by the
+    library's own kernels on 5M contacts of polygon piles with other seeds)
+    against the zero start of new contacts on the 8192-polygon pile at 4 and
+    2 sweeps: lower run means of the per-step maximum normal residual and
+    penetration.  The bound for any predictor of new contacts' impulses is
+    measured by tests/test_warmstart_bound.py."""
+    from shapes_amd import warmstart
+    model = np.load(warmstart.DEFAULT_MODEL)
+    for it in (4, 2):
+        z = _pile_quality(None, it)
+        m = _pile_quality(model, it)
+        print(f"{it} sweeps: zero start mean normal residual {z['mean_normal_residual']:.4f} "
+              f"penetration {z['mean_max_depth']:.4f}; learned {m['mean_normal_residual']:.4f} "
+              f"{m['mean_max_depth']:.4f}")
+        assert m["mean_normal_residual"] < 0.97 * z["mean_normal_residual"]
+        assert m["mean_max_depth"] < z["mean_max_depth"]
