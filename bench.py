@@ -237,6 +237,11 @@ def quality(st):
             "friction_residual": st["run_friction_residual"]}
 
 
+def note(msg: str) -> None:
+    """Progress of a long run on stderr (stdout carries the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,7 +273,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    t_start = time.perf_counter()
     scene = make_scene(args.workload, args.worlds)
+    note(f"scene {scene.name}: {scene.n_bodies} bodies in {len(scene.worlds)} world(s), "
+         f"built in {time.perf_counter() - t_start:.1f} s")
     if args.iterations:
         scene.config.iterations = args.iterations
     solver = SOLVERS[args.solver]
@@ -285,12 +293,15 @@ def main():
     bp = args.broadphase
     if bp == "tune":
         bp, tuning = autotune(scene, solver, ppb)
+        note(f"autotuned broadphase: {bp} {tuning}")
 
     elapsed, last = run_mode(scene, solver, warm, args.steps, args.warmup,
                              graph=not args.no_graph, model=model, broadphase=bp, ppb=ppb)
     value = dyn * args.steps / elapsed
+    note(f"timed run: {args.steps} steps in {elapsed:.2f} s")
     prof_steps = args.profile_steps or args.steps
     roof, phases, avg = roofline(scene, args.solver, prof_steps, bp, ppb)
+    note(f"instrumented replay: {prof_steps} steps")
     nv = sum(int(w.hull_off[-1]) for w in scene.worlds)
     sb = step_bytes(scene.n_bodies, nv, avg["contacts"], DTYPES_BYTES[args.solver])
     step_gbs = sb / (elapsed / args.steps) / 1e9
@@ -353,6 +364,7 @@ def main():
         out["modes"] = modes
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_sample_steps)
+        note("cpu baseline done")
     print(json.dumps(out), flush=True)
 
 

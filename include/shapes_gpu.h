@@ -88,6 +88,9 @@ typedef struct shp_config {
   int32_t warm_start;         /* SHP_WARM_* */
   int32_t max_pairs_per_body; /* capacity; 0 = default (8) */
   int32_t broadphase;         /* SHP_BROADPHASE_* */
+  int32_t solver_threads;     /* lanes per world in the solve: 0 = by world
+                                 size, or 256 or 512 (a schedule choice; the
+                                 ordered results do not depend on it) */
 } shp_config;
 
 typedef struct shp_batch shp_batch;

@@ -609,10 +609,11 @@ uint32_t mix32(uint32_t x) {
   return x;
 }
 
-// Priority of manifold q (pair index in the world) of world w: 17 hash bits
-// above the index, distinct within a world of < 32768 pairs.
-uint32_t colour_prio(int q, int w) {
-  return (((mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x1FFFFu) << 15) | (uint32_t)q) + 1u;
+// Priority of manifold q (pair index in the world) of world w: 17 hash bits,
+// then the index (distinct within a world).
+uint64_t colour_prio(int q, int w) {
+  return ((uint64_t)(mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x1FFFFu) << 32) |
+         (uint64_t)((uint32_t)q + 1u);
 }
 
 const int COLOUR_CAP = 256;
@@ -630,13 +631,13 @@ int colour_pairs(orc_engine* e, std::vector<int>& colour, bool* tail) {
       colour[q] = -1;
       ++left;
     }
-  std::vector<uint32_t> best(n);
+  std::vector<uint64_t> best(n);
   int rounds = 0;
   while (left > 0 && rounds < COLOUR_CAP) {
-    std::fill(best.begin(), best.end(), 0u);
+    std::fill(best.begin(), best.end(), 0ull);
     for (int q = 0; q < np; ++q) {
       if (colour[q] != -1) continue;
-      uint32_t pr = colour_prio(q, e->prm.world);
+      uint64_t pr = colour_prio(q, e->prm.world);
       int a = e->pairs[q].first, b = e->pairs[q].second;
       if (!is_static(e->objs[a]) && pr > best[a]) best[a] = pr;
       if (!is_static(e->objs[b]) && pr > best[b]) best[b] = pr;
@@ -644,7 +645,7 @@ int colour_pairs(orc_engine* e, std::vector<int>& colour, bool* tail) {
     std::vector<int> take;
     for (int q = 0; q < np; ++q) {
       if (colour[q] != -1) continue;
-      uint32_t pr = colour_prio(q, e->prm.world);
+      uint64_t pr = colour_prio(q, e->prm.world);
       int a = e->pairs[q].first, b = e->pairs[q].second;
       if ((is_static(e->objs[a]) || best[a] == pr) && (is_static(e->objs[b]) || best[b] == pr))
         take.push_back(q);

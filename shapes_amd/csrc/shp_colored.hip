@@ -327,6 +327,9 @@ __device__ void sweep_colours(const Ctx& c, const PackF& P, int p0, int ncol, bo
     M = s_off[l] + tid;
     return M < s_off[l + 1];
   };
+  // with one colour the next phase is the same manifold, whose lagrangians
+  // this phase changes: load it when it is used instead
+  const bool prefetch = ncol > 1;
   ManF cur, nxt;
   int Mc = 0, Mn = 0;
   bool has = phases > 0 && first(0, Mc);
@@ -335,7 +338,7 @@ __device__ void sweep_colours(const Ctx& c, const PackF& P, int p0, int ncol, bo
     const int l = h % ncol;
     const bool warm = h < ncol;
     const bool hasn = h + 1 < phases && first(h + 1, Mn);
-    if (hasn) load_man(c, P, p0 + Mn, nxt);
+    if (hasn && prefetch) load_man(c, P, p0 + Mn, nxt);
     if (serial_tail && l == ncol - 1) {
       if (tid == 0) {
         for (int i = s_off[l]; i < s_off[l + 1]; ++i) {
@@ -353,9 +356,12 @@ __device__ void sweep_colours(const Ctx& c, const PackF& P, int p0, int ncol, bo
       }
     }
     __syncthreads();
-    cur = nxt;
     Mc = Mn;
     has = hasn;
+    if (prefetch)
+      cur = nxt;
+    else if (has)
+      load_man(c, P, p0 + Mc, cur);
   }
 }
 
@@ -418,9 +424,16 @@ __device__ __forceinline__ bool fits_lds(int nb, int np, int pool_bytes) {
   return nb < (int)NO_BODY && np <= 32767 && need_col <= pool_bytes && need_vel <= pool_bytes;
 }
 
-// hashed priority of manifold q < 32768 of world w (distinct within a world)
+// Hashed priority of manifold q of world w: 17 hash bits, then the index
+// (distinct within a world).  prio32 packs it for q < 32768 (LDS colouring),
+// prio64 for any q (global colouring); both order manifolds alike, so the
+// colours do not depend on which kernel colours a world.
 __device__ __forceinline__ uint32_t prio32(int q, int w) {
   return (((mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x1FFFFu) << 15) | (uint32_t)q) + 1u;
+}
+__device__ __forceinline__ unsigned long long prio64(int q, int w) {
+  return ((unsigned long long)(mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x1FFFFu) << 32) |
+         (unsigned long long)((uint32_t)q + 1u);
 }
 
 // One workgroup per world whose colouring and FP32 velocities fit in LDS.
@@ -555,8 +568,7 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_solve_colored_global(Ctx c,
     for (int q = tid; q < np; q += COLOR_THREADS_G) {
       if (colour[q] != -1) continue;
       const int p = p0 + q;
-      const unsigned long long prio =
-          ((unsigned long long)mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) << 32) | (uint32_t)(q + 1);
+      const unsigned long long prio = prio64(q, w);
       const int ga = c.pa[p], gb = c.pb[p];
       if (!c.is_static[ga]) atomicMax(&best[ga - b0], prio);
       if (!c.is_static[gb]) atomicMax(&best[gb - b0], prio);
@@ -566,8 +578,7 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_solve_colored_global(Ctx c,
     for (int q = tid; q < np; q += COLOR_THREADS_G) {
       if (colour[q] != -1) continue;
       const int p = p0 + q;
-      const unsigned long long prio =
-          ((unsigned long long)mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) << 32) | (uint32_t)(q + 1);
+      const unsigned long long prio = prio64(q, w);
       const int ga = c.pa[p], gb = c.pb[p];
       if ((c.is_static[ga] || best[ga - b0] == prio) && (c.is_static[gb] || best[gb - b0] == prio)) {
         colour[q] = rounds;
@@ -623,7 +634,7 @@ shp_status colored_alloc(shp_batch* b) {
   b->allocs.push_back(c.col_best);
   // 4 waves per world; a large world gets 8 (two manifold records per lane
   // leave 8 waves without spills)
-  b->color_threads = max_nb > 2048 ? 512 : 256;
+  b->color_threads = b->cfg.solver_threads ? b->cfg.solver_threads : (max_nb > 2048 ? 512 : 256);
   const void* fn = b->color_threads == 512 ? (const void*)k_solve_colored<512>
                                            : (const void*)k_solve_colored<256>;
   if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->color_lds) !=

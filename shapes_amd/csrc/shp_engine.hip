@@ -547,6 +547,8 @@ shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, cons
     return fail(SHP_E_INVAL, "unknown warm-start mode");
   if (cfg->broadphase < SHP_BROADPHASE_AUTO || cfg->broadphase > SHP_BROADPHASE_GRID)
     return fail(SHP_E_INVAL, "unknown broadphase mode");
+  if (cfg->solver_threads != 0 && cfg->solver_threads != 256 && cfg->solver_threads != 512)
+    return fail(SHP_E_INVAL, "solver_threads must be 0, 256 or 512");
   long NB = 0, NV = 0;
   for (int w = 0; w < n_worlds; ++w) {
     const shp_world_desc& d = worlds[w];

@@ -702,7 +702,7 @@ shp_status solver_alloc(shp_batch* b) {
   }
   // 4 waves per world; a large world gets 8 (a level-synchronous pass
   // keeps two records per lane, so 8 waves leave it 256 registers per lane)
-  b->solve_threads = max_nb > 2048 ? 512 : 256;
+  b->solve_threads = b->cfg.solver_threads ? b->cfg.solver_threads : (max_nb > 2048 ? 512 : 256);
   const void* fn = b->solve_clds ? (b->solve_vlds ? flow_kernel<true, true>(b->solve_threads)
                                                   : flow_kernel<true, false>(b->solve_threads))
                                  : flow_kernel<false, false>(b->solve_threads);

@@ -56,7 +56,8 @@ class ConfigC(C.Structure):
                 ("slop", C.c_double), ("gravity_x", C.c_double),
                 ("gravity_y", C.c_double), ("iterations", C.c_int32),
                 ("solver", C.c_int32), ("warm_start", C.c_int32),
-                ("max_pairs_per_body", C.c_int32), ("broadphase", C.c_int32)]
+                ("max_pairs_per_body", C.c_int32), ("broadphase", C.c_int32),
+                ("solver_threads", C.c_int32)]
 
 
 class StepStatsC(C.Structure):
@@ -152,7 +153,8 @@ class Batch:
 
     def __init__(self, worlds: Sequence[WorldDesc], config: SceneConfig,
                  solver: int = SOLVER_ORDERED_F64, warm_start: int = WARM_ZERO,
-                 max_pairs_per_body: int = 0, broadphase: int = BROADPHASE_AUTO):
+                 max_pairs_per_body: int = 0, broadphase: int = BROADPHASE_AUTO,
+                 solver_threads: int = 0):
         L = load_library()
         self._L = L
         self._keep = []
@@ -174,7 +176,7 @@ class Batch:
             d.hull_y = _dptr(hy)
         cfg = ConfigC(config.dt, config.baumgarte, config.slop, config.gx,
                       config.gy, config.iterations, solver, warm_start,
-                      max_pairs_per_body, broadphase)
+                      max_pairs_per_body, broadphase, solver_threads)
         h = C.c_void_p()
         _check(L.shp_batch_create(descs, len(worlds), C.byref(cfg), C.byref(h)))
         self._h = h

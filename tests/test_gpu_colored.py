@@ -65,11 +65,12 @@ def test_ordered_and_colored_quality_on_batch():
     assert out["colored"]["run_max_depth"] < 2.0 * out["ordered"]["run_max_depth"] + 0.02
 
 
-def _colored_vs_restatement(orc, worlds, cfg, steps, every):
+def _colored_vs_restatement(orc, worlds, cfg, steps, every, threads=0):
     """GPU coloured solver against the oracle's restatement of the same
     schedule (Jones-Plassmann colours, FP32 rows and velocities, FP64
     accumulators, colour order): every bit of state must agree."""
-    gpu = engine.make_batch(worlds, cfg, solver=engine.SOLVER_COLORED_F32)
+    gpu = engine.make_batch(worlds, cfg, solver=engine.SOLVER_COLORED_F32,
+                            solver_threads=threads)
     refs = [orc.OracleEngine(w, cfg, solver=orc.SOLVER_COLORED, world_index=k)
             for k, w in enumerate(worlds)]
     done = 0
@@ -99,10 +100,23 @@ def test_colored_matches_restatement_batch(orc):
     _colored_vs_restatement(orc, worlds, scenes.SceneConfig(iterations=6), steps=60, every=10)
 
 
-def test_colored_matches_restatement_polygon_pile(orc):
-    """The 8192-polygon world: 8 waves per world, LDS velocities."""
+@pytest.mark.parametrize("threads", [256, 512])
+def test_colored_matches_restatement_polygon_pile(orc, threads):
+    """The 8192-polygon world (LDS velocities), with 4 and 8 waves: colours
+    wider than the lanes, so lanes take several manifolds of a colour."""
     sc = scenes.polygon_pile()
-    _colored_vs_restatement(orc, sc.worlds, sc.config, steps=90, every=30)
+    _colored_vs_restatement(orc, sc.worlds, sc.config, steps=40, every=2, threads=threads)
+
+
+def test_colored_matches_restatement_single_colour(orc):
+    """Three boxes resting on one floor: every manifold shares only the
+    static floor, so the world has one colour and each pass solves the same
+    manifolds again."""
+    b = scenes._Builder()
+    b.add(scenes.rectangle_vertices(20.0, 1.0), (0.0, -0.5), mu=0.5)
+    for x in (-5.0, 0.0, 5.0):
+        b.add_dynamic(scenes.rectangle_vertices(1.0, 1.0), (x, 0.49), mu=0.5)
+    _colored_vs_restatement(orc, [b.build()], scenes.SceneConfig(iterations=4), steps=60, every=5)
 
 
 def test_colored_matches_restatement_global_path(orc):
