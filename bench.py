@@ -1,16 +1,26 @@
 """Benchmark of the contact-physics step on one MI355X.
 
-Workload (default): one large world of 8192 random convex 3-8-gons dropped
-onto a 200-wide static floor, dt = 1/60, 8 sweeps, 600 steps
-(scenes.polygon_pile).  A "step" is one updateWorld of that world.  The
-timed region replays K steps from the initial state after W warm-up steps
-on a separate instance of the same scene.
+Default workload: one large world of 8192 random convex 3-8-gons dropped
+onto a 200-wide static floor, dt = 1/60, 600 steps (scenes.polygon_pile).
+A "step" is one updateWorld of that world; body-steps/s counts dynamic
+bodies.  Each timed region replays K steps from the initial state (as HIP
+graphs) after W warm-up steps on a separate instance of the same scene.
 
-Prints ONE JSON line: body-steps/s of dynamic bodies, the roofline of the
-dominant kernel (algorithmic bytes per launch / its average duration,
-measured with HIP events on the batch's stream in an instrumented replay of
-the same steps), the solver quality of the run, and the CPU oracle timed on
-this host on a bounded sample of the same workload.
+Default mode ("headline"): body-steps/s at fixed penetration/residual.  The
+plain GPU solver (coloured, zero start for new contacts, the workload's 8
+sweeps) sets the quality: the run means of the per-step maximum
+penetration and normal residual.  The learned warm start then runs at 8,
+7, 6, ... sweeps while both means stay within 2% of the plain run's; the
+value is the body-steps/s of the fewest sweeps that pass (at least the
+workload's sweep count when none below it does), next to the plain run's.
+
+"--mode single" times one configuration (--solver, --warm, --iterations).
+
+Prints ONE JSON line: the value, the roofline of the dominant kernel
+(algorithmic bytes per launch / its average duration, measured with HIP
+events on the batch's stream in an instrumented replay of the same steps),
+the solve quality, and the CPU oracle timed on this host on a bounded
+sample of the same workload.
 """
 from __future__ import annotations
 
@@ -84,10 +94,13 @@ def kernel_bytes(phase: str, st: dict, nb: int, nv: int, word: int) -> float:
     return 0.0
 
 
-def instrumented_run(scene, solver, steps, broadphase="auto", ppb=0):
+def instrumented_run(scene, solver, steps, broadphase="auto", ppb=0, warm=engine.WARM_ZERO,
+                     model=None, threads=0):
     w = scene.worlds
     b = engine.make_batch(w, scene.config, solver=solver, broadphase=BROADPHASES[broadphase],
-                          max_pairs_per_body=ppb)
+                          max_pairs_per_body=ppb, warm_start=warm, solver_threads=threads)
+    if warm == engine.WARM_LEARNED:
+        b.set_mlp(model)
     b.set_timing(True)
     tot = {"pairs": 0, "contacts": 0}
     for _ in range(steps):
@@ -125,8 +138,10 @@ def measured_traffic(kernel: str, workload: str):
     return None
 
 
-def roofline(scene, solver_name, steps, broadphase="auto", ppb=0):
-    t, st = instrumented_run(scene, SOLVERS[solver_name], steps, broadphase, ppb)
+def roofline(scene, solver_name, steps, broadphase="auto", ppb=0, warm=engine.WARM_ZERO,
+             model=None, threads=0):
+    t, st = instrumented_run(scene, SOLVERS[solver_name], steps, broadphase, ppb, warm, model,
+                             threads)
     nb = scene.n_bodies
     nv = sum(int(w.hull_off[-1]) for w in scene.worlds)
     word = 8 if solver_name == "ordered" else 4
@@ -183,31 +198,35 @@ BROADPHASES = {"auto": engine.BROADPHASE_AUTO, "scan": engine.BROADPHASE_SCAN,
 
 
 def autotune(scene, solver, ppb, steps=20, warm_steps=10):
-    """Pick the broadphase schedule by measurement: each candidate steps the
-    workload from its initial state (warm_steps untimed, then `steps` timed
-    graph replays); the fastest wins."""
+    """Pick the schedule by measurement: broadphase (wavefront scan or
+    uniform grid) x lanes per world in the solve (256 or 512).  Each
+    candidate steps the workload from its initial state (warm_steps untimed,
+    then `steps` timed graph replays); the fastest wins."""
     times = {}
     for name in ("scan", "grid"):
-        b = engine.make_batch(scene.worlds, scene.config, solver=solver,
-                              broadphase=BROADPHASES[name], max_pairs_per_body=ppb)
-        b.step_async(warm_steps, use_graph=True)
-        b.sync()
-        t0 = time.perf_counter()
-        b.step_async(steps, use_graph=True)
-        b.sync()
-        times[name] = (time.perf_counter() - t0) / steps * 1e3
-        b.close()
+        for threads in (256, 512):
+            b = engine.make_batch(scene.worlds, scene.config, solver=solver,
+                                  broadphase=BROADPHASES[name], max_pairs_per_body=ppb,
+                                  solver_threads=threads)
+            b.step_async(warm_steps, use_graph=True)
+            b.sync()
+            t0 = time.perf_counter()
+            b.step_async(steps, use_graph=True)
+            b.sync()
+            times[(name, threads)] = (time.perf_counter() - t0) / steps * 1e3
+            b.close()
     best = min(times, key=times.get)
-    return best, {k: round(v, 4) for k, v in times.items()}
+    return best, {f"{k[0]}/{k[1]}": round(v, 4) for k, v in times.items()}
 
 
 def run_mode(scene, solver, warm, steps, warmup, graph=True, model=None, broadphase="auto",
-             ppb=0):
+             ppb=0, threads=0):
     """Warm up on a separate instance, then time `steps` steps from the
     initial state; returns (seconds, run stats)."""
     def make():
         b = engine.make_batch(scene.worlds, scene.config, solver=solver, warm_start=warm,
-                              broadphase=BROADPHASES[broadphase], max_pairs_per_body=ppb)
+                              broadphase=BROADPHASES[broadphase], max_pairs_per_body=ppb,
+                              solver_threads=threads)
         if warm == engine.WARM_LEARNED:
             b.set_mlp(model)
         return b
@@ -242,25 +261,74 @@ def note(msg: str) -> None:
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
+QUALITY_TOL = 0.02
+
+
+def matches(q, ref, tol=QUALITY_TOL):
+    """Run means of the per-step maximum penetration and normal residual
+    within `tol` of the reference run's."""
+    return (q["mean_step_max_penetration"] <= ref["mean_step_max_penetration"] * (1 + tol) and
+            q["mean_step_normal_residual"] <= ref["mean_step_normal_residual"] * (1 + tol))
+
+
+def headline(scene, args, model, bp, ppb, dyn, threads):
+    """Plain coloured solver at the workload's sweeps, then the learned
+    warm start at decreasing sweeps while its quality matches."""
+    base_it = scene.config.iterations
+    graph = not args.no_graph
+    el, st = run_mode(scene, engine.SOLVER_COLORED_F32, engine.WARM_ZERO, args.steps,
+                      args.warmup, graph=graph, broadphase=bp, ppb=ppb, threads=threads)
+    plain = {"warm_start": "zero", "iterations": base_it,
+             "body_steps_per_s": round(dyn * args.steps / el, 1),
+             "ms_per_step": round(1e3 * el / args.steps, 4), **quality(st)}
+    note(f"plain coloured at {base_it} sweeps: {plain['body_steps_per_s']:.0f} body-steps/s")
+    tried = []
+    chosen = None
+    for it in range(base_it, 0, -1):
+        scene.config.iterations = it
+        el_l, st_l = run_mode(scene, engine.SOLVER_COLORED_F32, engine.WARM_LEARNED, args.steps,
+                              args.warmup, graph=graph, model=model, broadphase=bp, ppb=ppb,
+                              threads=threads)
+        row = {"warm_start": "learned", "iterations": it,
+               "body_steps_per_s": round(dyn * args.steps / el_l, 1),
+               "ms_per_step": round(1e3 * el_l / args.steps, 4), **quality(st_l)}
+        row["matches_plain"] = matches(row, plain)
+        tried.append(row)
+        note(f"learned at {it} sweeps: {row['body_steps_per_s']:.0f} body-steps/s, "
+             f"matches plain: {row['matches_plain']}")
+        if row["matches_plain"]:
+            chosen = (it, el_l, st_l, row)
+        elif it < base_it:
+            break
+    if chosen is None:  # even the full sweep count does not match: report it
+        row = next(r for r in tried if r["iterations"] == base_it)
+        chosen = (base_it, dyn * args.steps / row["body_steps_per_s"], None, row)
+    scene.config.iterations = chosen[0]
+    return plain, tried, chosen
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="polygon_pile")
+    ap.add_argument("--mode", default="headline", choices=["headline", "single"],
+                    help="headline: learned vs plain at fixed quality; single: one configuration")
     ap.add_argument("--solver", default="colored", choices=list(SOLVERS))
     ap.add_argument("--warm", default="zero", choices=["zero", "learned"])
     ap.add_argument("--iterations", type=int, default=0,
                     help="solver sweeps (default: the workload's)")
     ap.add_argument("--model", default=os.path.join(ROOT, "shapes_amd", "warmstart_mlp.npy"))
-    ap.add_argument("--compare", action="store_true",
-                    help="also run the other solver / warm-start modes")
-    ap.add_argument("--cpu-sample-steps", type=int, default=100)
+    ap.add_argument("--cpu-sample-steps", type=int, default=300)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="steps of the instrumented replay (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--broadphase", default="tune", choices=["tune"] + list(BROADPHASES))
+    ap.add_argument("--broadphase", default="tune", choices=["tune"] + list(BROADPHASES),
+                    help="tune: pick broadphase and solver lanes by measurement")
+    ap.add_argument("--solver-threads", type=int, default=0, choices=[0, 256, 512],
+                    help="lanes per world in the solve when not tuned (0: by world size)")
     ap.add_argument("--pairs-per-body", type=int, default=0,
                     help="pair capacity per body (0: library default)")
     ap.add_argument("--worlds", type=int, default=0,
@@ -279,31 +347,48 @@ def main():
          f"built in {time.perf_counter() - t_start:.1f} s")
     if args.iterations:
         scene.config.iterations = args.iterations
-    solver = SOLVERS[args.solver]
-    warm = engine.WARM_LEARNED if args.warm == "learned" else engine.WARM_ZERO
+    headline_mode = args.mode == "headline"
+    solver_name = "colored" if headline_mode else args.solver
+    solver = SOLVERS[solver_name]
+    warm = engine.WARM_LEARNED if (headline_mode or args.warm == "learned") else engine.WARM_ZERO
     model = None
-    if warm == engine.WARM_LEARNED or args.compare:
-        if os.path.exists(args.model):
-            model = np.load(args.model)
-        elif warm == engine.WARM_LEARNED:
+    if warm == engine.WARM_LEARNED:
+        if not os.path.exists(args.model):
             raise SystemExit(f"bench.py: no model at {args.model}")
+        model = np.load(args.model)
     dyn = scene.n_dynamic
     ppb = args.pairs_per_body
     tuning = None
     bp = args.broadphase
+    threads = args.solver_threads
     if bp == "tune":
-        bp, tuning = autotune(scene, solver, ppb)
-        note(f"autotuned broadphase: {bp} {tuning}")
+        (bp, threads), tuning = autotune(scene, solver, ppb)
+        note(f"autotuned schedule: {bp}, {threads} lanes {tuning}")
 
-    elapsed, last = run_mode(scene, solver, warm, args.steps, args.warmup,
-                             graph=not args.no_graph, model=model, broadphase=bp, ppb=ppb)
+    extra = {}
+    if headline_mode:
+        plain, tried, chosen = headline(scene, args, model, bp, ppb, dyn, threads)
+        it, elapsed, _, row = chosen
+        q = {k: v for k, v in row.items()
+             if k not in ("warm_start", "iterations", "body_steps_per_s", "ms_per_step",
+                          "matches_plain")}
+        extra = {"plain_gpu": plain, "learned_runs": tried,
+                 "vs_plain_gpu": round(row["body_steps_per_s"] / plain["body_steps_per_s"], 4),
+                 "quality_tolerance": QUALITY_TOL,
+                 "quality_matched": bool(row.get("matches_plain"))}
+        st_last = None
+    else:
+        elapsed, st_last = run_mode(scene, solver, warm, args.steps, args.warmup,
+                                    graph=not args.no_graph, model=model, broadphase=bp,
+                                    ppb=ppb, threads=threads)
+        q = quality(st_last)
     value = dyn * args.steps / elapsed
     note(f"timed run: {args.steps} steps in {elapsed:.2f} s")
     prof_steps = args.profile_steps or args.steps
-    roof, phases, avg = roofline(scene, args.solver, prof_steps, bp, ppb)
+    roof, phases, avg = roofline(scene, solver_name, prof_steps, bp, ppb, warm, model, threads)
     note(f"instrumented replay: {prof_steps} steps")
     nv = sum(int(w.hull_off[-1]) for w in scene.worlds)
-    sb = step_bytes(scene.n_bodies, nv, avg["contacts"], DTYPES_BYTES[args.solver])
+    sb = step_bytes(scene.n_bodies, nv, avg["contacts"], DTYPES_BYTES[solver_name])
     step_gbs = sb / (elapsed / args.steps) / 1e9
 
     out = {
@@ -317,7 +402,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": DTYPES[args.solver],
+        "dtype": DTYPES[solver_name],
         "data": "synthetic",
         "config": {
             "workload": scene.name,
@@ -326,13 +411,16 @@ def main():
             "dynamic_bodies": dyn,
             "iterations": scene.config.iterations,
             "dt": scene.config.dt,
-            "solver": args.solver,
-            "warm_start": args.warm,
+            "solver": solver_name,
+            "warm_start": "learned" if warm == engine.WARM_LEARNED else "zero",
+            "mode": args.mode,
             "graph": not args.no_graph,
             "broadphase": bp,
+            "solver_threads": threads,
             "autotune_ms_per_step": tuning,
         },
-        "quality": quality(last),
+        "quality": q,
+        **extra,
         "roofline": roof,
         "step_roofline": {"bound": "hbm", "algorithmic_bytes_per_step": round(sb),
                           "achieved": round(step_gbs, 3), "peak": HBM_PEAK_GBS,
@@ -340,28 +428,12 @@ def main():
         "phase_ms": phases,
         "avg_pairs": round(avg["pairs"], 1),
         "avg_contacts": round(avg["contacts"], 1),
-        "last_step": {"pairs": last["pairs"], "contacts": last["contacts"],
-                      "warm_hits": last["warm_hits"], "misses": last["misses"],
-                      "learned_starts": last["learned_starts"],
-                      "max_levels": last["max_levels"]},
     }
-    if args.compare:
-        modes = []
-        base_it = scene.config.iterations
-        plan = [("ordered", "zero", base_it), ("colored", "zero", base_it)]
-        if model is not None:
-            plan += [("colored", "learned", base_it // 2), ("colored", "learned", base_it // 4),
-                     ("colored", "zero", base_it // 2), ("colored", "zero", base_it // 4)]
-        for sv, wm, it in plan:
-            scene.config.iterations = it
-            el, st = run_mode(scene, SOLVERS[sv], engine.WARM_LEARNED if wm == "learned"
-                              else engine.WARM_ZERO, args.steps, args.warmup,
-                              graph=not args.no_graph, model=model, broadphase=bp, ppb=ppb)
-            modes.append({"solver": sv, "warm_start": wm, "iterations": it,
-                          "body_steps_per_s": round(dyn * args.steps / el, 1),
-                          "ms_per_step": round(1e3 * el / args.steps, 4), **quality(st)})
-        scene.config.iterations = base_it
-        out["modes"] = modes
+    if st_last is not None:
+        out["last_step"] = {"pairs": st_last["pairs"], "contacts": st_last["contacts"],
+                            "warm_hits": st_last["warm_hits"], "misses": st_last["misses"],
+                            "learned_starts": st_last["learned_starts"],
+                            "max_levels": st_last["max_levels"]}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_sample_steps)
         note("cpu baseline done")

@@ -140,7 +140,8 @@ struct shp_batch {
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
   hipGraphExec_t graph_exec{};
-  bool graph_ready = false;
+  bool graph_made = false;   // graph_exec holds an instantiated graph
+  bool graph_ready = false;  // ... and it matches the current kernel arguments
   int32_t last_flags = 0;
   std::vector<void*> allocs;
   // per-phase event timing (shp_batch_set_timing)

@@ -821,6 +821,11 @@ static shp_status check_flags(shp_batch* b) {
 
 static shp_status capture_graph(shp_batch* b) {
   if (b->graph_ready) return SHP_OK;
+  if (b->graph_made) {  // kernel arguments changed since the last capture
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipGraphExecDestroy(b->graph_exec));
+    b->graph_made = false;
+  }
   hipGraph_t graph;
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   bool timing = b->timing;
@@ -830,8 +835,10 @@ static shp_status capture_graph(shp_batch* b) {
   hipError_t e = hipStreamEndCapture(b->stream, &graph);
   if (st != SHP_OK) return st;
   if (e != hipSuccess) return fail(SHP_E_HIP, "graph capture failed");
-  HIPCHK(hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0));
-  HIPCHK(hipGraphDestroy(graph));
+  hipError_t ie = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) return fail(SHP_E_HIP, "graph instantiation failed");
+  b->graph_made = true;
   b->graph_ready = true;
   return SHP_OK;
 }
@@ -841,7 +848,7 @@ extern "C" {
 shp_status shp_batch_destroy(shp_batch* b) {
   if (!b) return SHP_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->graph_ready) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph_made) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->events_made)
     for (int k = 0; k <= SHP_NUM_PHASES; ++k) (void)hipEventDestroy(b->events[k]);
   mlp_params_free(&b->mlp);

@@ -75,43 +75,48 @@ __global__ void k_grid_fill(Ctx c) {
   c.bucket_items[pos] = g;
 }
 
-// Sixteen lanes per body i: count (fill = 0) or emit (fill = 1) its pairs.
-// A small body's lanes 0..8 take one cell each of the <= 3x3 cells that can
-// hold a partner's min corner, lanes 9..15 the world's large bodies; a
-// large body's lanes stride over every j < i.  Each lane visits its
-// candidates twice when emitting (count, then write at the group's prefix
-// sum); lane 0 then sorts the segment into descending j.
-#define GRID_LANES 16
+static int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
-template <class F>
+// LANES lanes per body i (1 or 16): count (fill = 0) or emit (fill = 1)
+// its pairs.  With 16, a small body's lanes 0..8 take one cell each of the
+// <= 3x3 cells that can hold a partner's min corner and lanes 9..15 the
+// world's large bodies, and a large body's lanes stride over every j < i:
+// for batches too small to fill the GPU with one lane per body (the
+// candidates are dependent loads).  With 1 lane per body the lane does all
+// of it: for large batches, where the GPU is full anyway.  Each lane visits
+// its candidates twice when emitting with 16 lanes (count, then write at
+// the group's prefix sum), once with 1; lane 0 then sorts the segment into
+// descending j.
+template <int LANES, class F>
 __device__ __forceinline__ void grid_visit(const Ctx& c, int g, int sl, F&& hit) {
+  constexpr int CELL_LANES = LANES < 9 ? LANES : 9;
   const int w = c.body_world[g];
   const int b0 = c.body_off[w];
   const bool si = c.is_static[g] != 0;
   const double aminx = c.bminx[g], amaxx = c.bmaxx[g], aminy = c.bminy[g], amaxy = c.bmaxy[g];
   if (c.is_large[g]) {
-    for (int gj = g - 1 - sl; gj >= b0; gj -= GRID_LANES)
+    for (int gj = g - 1 - sl; gj >= b0; gj -= LANES)
       if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) hit(gj);
     return;
   }
-  if (sl >= 9) {
-    for (int q = c.large_off[w] + (sl - 9); q < c.large_off[w + 1]; q += GRID_LANES - 9) {
+  if (LANES == 1 || sl >= CELL_LANES) {
+    const int l0 = LANES == 1 ? 0 : sl - CELL_LANES, ls = LANES == 1 ? 1 : LANES - CELL_LANES;
+    for (int q = c.large_off[w] + l0; q < c.large_off[w + 1]; q += ls) {
       const int gj = c.large_ids[q];
       if (gj >= g) break;  // ascending
       if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) hit(gj);
     }
-    return;
+    if (LANES != 1) return;
   }
   const double h = c.cell_h[w];
   const int cx0 = cell_of(aminx, h) - 1, cx1 = cell_of(amaxx, h);
   const int cy0 = cell_of(aminy, h) - 1, cy1 = cell_of(amaxy, h);
-  // a small body's AABB spans at most 2 cells per axis, so ncells <= 9 and
-  // each of lanes 0..8 takes at most one cell; the stride keeps any other
-  // case correct
+  // a small body's AABB spans at most 2 cells per axis, so there are <= 9
+  // cells; the stride keeps any other case correct
   const long ncx = (long)cx1 - cx0 + 1, ncy = (long)cy1 - cy0 + 1;
   const int t0 = c.tab_off[w];
   const uint32_t mask = (uint32_t)(c.tab_off[w + 1] - t0 - 1);
-  for (long ci = sl; ci < ncx * ncy; ci += 9) {
+  for (long ci = sl; ci < ncx * ncy; ci += CELL_LANES) {
     const int cx = (int)(cx0 + ci % ncx), cy = (int)(cy0 + ci / ncx);
     const int b = t0 + cell_hash(cx, cy, mask);
     for (int q = c.bucket_start[b]; q < c.bucket_start[b + 1]; ++q) {
@@ -122,41 +127,61 @@ __device__ __forceinline__ void grid_visit(const Ctx& c, int g, int sl, F&& hit)
   }
 }
 
+template <int LANES>
 __global__ void k_grid_pairs(Ctx c, int fill) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = (int)(t / GRID_LANES);  // the same for the 16 lanes of a group
-  const int sl = (int)(t % GRID_LANES);
+  const int g = (int)(t / LANES);  // the same for the LANES lanes of a group
+  const int sl = (int)(t % LANES);
   if (g >= c.NB) return;
-  int count = 0;
-  grid_visit(c, g, sl, [&](int) { ++count; });
-  int incl = count;
-  for (int o = 1; o < GRID_LANES; o <<= 1) {
-    const int y = __shfl_up(incl, o, GRID_LANES);
-    if (sl >= o) incl += y;
-  }
-  const int total = __shfl(incl, GRID_LANES - 1, GRID_LANES);
   const int w = c.body_world[g];
   const int b0 = c.body_off[w];
   const int nw = c.body_off[w + 1] - b0;
   const int rank = b0 + (nw - 1 - (g - b0));
-  if (!fill) {
-    if (sl == 0) c.cnt_rank[rank] = total;
-    return;
+  int base = 0, pos = 0, total = 0;
+  if (LANES == 1 && fill) {
+    // one lane: write while visiting (the count pass sized the segment)
+    base = c.off_rank[rank];
+    pos = base;
+    grid_visit<LANES>(c, g, sl, [&](int gj) {
+      if (pos < c.pair_cap) {
+        c.pa[pos] = g;
+        c.pb[pos] = gj;
+      }
+      ++pos;
+    });
+    total = pos - base;
+    if (pos > c.pair_cap) {
+      atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_PAIR_OVERFLOW);
+      return;
+    }
+  } else {
+    int count = 0;
+    grid_visit<LANES>(c, g, sl, [&](int) { ++count; });
+    int incl = count;
+    for (int o = 1; o < LANES; o <<= 1) {
+      const int y = __shfl_up(incl, o, LANES);
+      if (sl >= o) incl += y;
+    }
+    total = LANES == 1 ? incl : __shfl(incl, LANES - 1, LANES);
+    if (!fill) {
+      if (sl == 0) c.cnt_rank[rank] = total;
+      return;
+    }
+    base = c.off_rank[rank];
+    if (base + total > c.pair_cap) {
+      if (sl == 0) atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_PAIR_OVERFLOW);
+      return;
+    }
+    pos = base + incl - count;
+    grid_visit<LANES>(c, g, sl, [&](int gj) {
+      c.pa[pos] = g;
+      c.pb[pos] = gj;
+      ++pos;
+    });
+    // the group's stores precede lane 0's sort in this wavefront's program
+    // order; the fence keeps the compiler from moving them
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
-  const int base = c.off_rank[rank];
-  if (base + total > c.pair_cap) {
-    if (sl == 0) atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_PAIR_OVERFLOW);
-    return;
-  }
-  int pos = base + incl - count;
-  grid_visit(c, g, sl, [&](int gj) {
-    c.pa[pos] = g;
-    c.pb[pos] = gj;
-    ++pos;
-  });
-  // the group's stores precede lane 0's sort in this wavefront's program
-  // order; the fence keeps the compiler from moving them
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   if (sl != 0) return;
   // the segment in descending j (insertion sort; segments are short)
   for (int a = base + 1; a < base + total; ++a) {
@@ -170,12 +195,22 @@ __global__ void k_grid_pairs(Ctx c, int fill) {
   }
 }
 
+// 16 lanes per body while that still fits about one wave of work per SIMD
+// slot of the GPU several times over; beyond, one lane per body.
+#define GRID_WIDE_MAX_BODIES 131072
+
+static void launch_grid_pairs(shp_batch* b, int fill) {
+  const Ctx& c = b->ctx;
+  if (c.NB <= GRID_WIDE_MAX_BODIES)
+    k_grid_pairs<16><<<nblk((long)c.NB * 16, 256), 256, 0, b->stream>>>(c, fill);
+  else
+    k_grid_pairs<1><<<nblk(c.NB, 256), 256, 0, b->stream>>>(c, fill);
+}
+
 __global__ void k_zero_i32(int32_t* p, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
-
-static int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 // Grid broadphase, first half: binning and per-body pair counts into
 // cnt_rank (the caller then scans cnt_rank into off_rank).
@@ -190,13 +225,13 @@ hipError_t launch_grid_count(shp_batch* b) {
                                                   c.grid_buckets + 1, s);
   if (e != hipSuccess) return e;
   k_grid_fill<<<nblk(c.NB, 256), 256, 0, s>>>(c);
-  k_grid_pairs<<<nblk((long)c.NB * GRID_LANES, 256), 256, 0, s>>>(c, 0);
+  launch_grid_pairs(b, 0);
   return hipGetLastError();
 }
 
 // Second half: emit the pairs at off_rank.
 hipError_t launch_grid_emit(shp_batch* b) {
-  k_grid_pairs<<<nblk((long)b->ctx.NB * GRID_LANES, 256), 256, 0, b->stream>>>(b->ctx, 1);
+  launch_grid_pairs(b, 1);
   return hipGetLastError();
 }
 

@@ -192,3 +192,19 @@ def test_large_world_global_counters_bitwise(orc):
     cfg = scenes.SceneConfig(iterations=3)
     ref = run_parity(orc, w, cfg, steps=4, every=2)
     assert ref.stats()["contacts"] > 20000
+
+
+def test_grid_broadphase_bitwise_large_batch(orc):
+    """A batch of ~150k bodies takes the one-lane-per-body grid kernel;
+    pairs and state of a sample of its worlds match the oracle bit for bit."""
+    sc = scenes.world_batch_fast(n_worlds=720)
+    assert sc.n_bodies > 131072
+    gpu = engine.make_batch(sc.worlds, sc.config, broadphase=engine.BROADPHASE_GRID)
+    sample = list(range(0, 720, 37)) + [719]
+    refs = {k: orc.OracleEngine(sc.worlds[k], sc.config) for k in sample}
+    for step in (5, 10):
+        gpu.step(5)
+        for k, r in refs.items():
+            r.step(5)
+            compare_state(gpu, r, step, world=k)
+    gpu.close()
