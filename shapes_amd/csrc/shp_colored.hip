@@ -674,8 +674,9 @@ __device__ __forceinline__ unsigned long long wave_max(unsigned long long x) {
 // complementarity residual |min(lambda_n, Jn.v + b)| and friction residual
 // |lambda_t - clamp(lambda_t - Jt.v / mt, +-mu*lambda_n)|; maxima of the step
 // and of the run.  Non-negative doubles order like their bit patterns, so the
-// maxima are integer maxima, reduced per wavefront before one atomic each.
-__global__ void k_quality(Ctx c) {
+// maxima are integer maxima, reduced per workgroup before one atomic each.
+#define QUALITY_THREADS 256
+__global__ __launch_bounds__(QUALITY_THREADS) void k_quality(Ctx c) {
   int tc = c.wcont_off[c.W];
   unsigned long long qd = 0, qn = 0, qt = 0;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < tc; k += gridDim.x * blockDim.x) {
@@ -701,10 +702,25 @@ __global__ void k_quality(Ctx c) {
     qn = b > qn ? b : qn;
     qt = d > qt ? d : qt;
   }
+  // wavefront, then workgroup maxima: one set of atomics per workgroup
+  __shared__ unsigned long long s_q[3][QUALITY_THREADS / 64];
+  const int wid = threadIdx.x >> 6;
   qd = wave_max(qd);
   qn = wave_max(qn);
   qt = wave_max(qt);
-  if ((threadIdx.x & 63) == 0 && (qd | qn | qt)) {
+  if ((threadIdx.x & 63) == 0) {
+    s_q[0][wid] = qd;
+    s_q[1][wid] = qn;
+    s_q[2][wid] = qt;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int q = 1; q < QUALITY_THREADS / 64; ++q) {
+    qd = s_q[0][q] > qd ? s_q[0][q] : qd;
+    qn = s_q[1][q] > qn ? s_q[1][q] : qn;
+    qt = s_q[2][q] > qt ? s_q[2][q] : qt;
+  }
+  if (qd | qn | qt) {
     atomicMax(&c.quality[Q_DEPTH], qd);
     atomicMax(&c.quality[Q_NRES], qn);
     atomicMax(&c.quality[Q_FRES], qt);
@@ -730,6 +746,6 @@ __global__ void k_quality_sum(Ctx c) {
 void launch_quality(shp_batch* b) {
   long n = b->ctx.contact_cap;
   int grid = (int)std::max<long>(1, std::min<long>((n + 255) / 256, 2048));
-  k_quality<<<grid, 256, 0, b->stream>>>(b->ctx);
+  k_quality<<<grid, QUALITY_THREADS, 0, b->stream>>>(b->ctx);
   k_quality_sum<<<1, 64, 0, b->stream>>>(b->ctx);
 }

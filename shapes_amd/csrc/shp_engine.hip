@@ -383,9 +383,14 @@ __global__ void k_contacts(Ctx c) {
     int n = c.ccount[p];
     if (n > 0) contacts_of_pair(c, p, n, hits);
   }
-  // one atomic per wavefront for the warm-start hit counter
+  // one atomic per workgroup for the warm-start hit counter
+  __shared__ int s_hits;
+  if (threadIdx.x == 0) s_hits = 0;
+  __syncthreads();
   for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off);
-  if ((threadIdx.x & 63) == 0 && hits) atomicAdd(&c.counters[CNT_WARM], hits);
+  if ((threadIdx.x & 63) == 0 && hits) atomicAdd(&s_hits, hits);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_hits) atomicAdd(&c.counters[CNT_WARM], s_hits);
 }
 
 // This frame's lagrangians become the next frame's EngineCache
