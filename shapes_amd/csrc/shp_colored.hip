@@ -288,28 +288,23 @@ __device__ __forceinline__ int colour_of(uint16_t x) {
   return x == 0xFFFFu ? -1 : (x == 0xFFFEu ? -2 : (int)x);
 }
 
-// Pack the coloured manifolds: colour l owns manifold indices
-// [p0 + s_off[l], p0 + s_off[l + 1]); scur holds the colour cursors.  The
-// order inside a colour is free, except in the serial tail colour (tail >=
-// 0), whose manifolds may share bodies: thread 0 packs those in pair order.
+// Slots of the coloured manifolds: colour l owns manifold indices
+// [p0 + s_off[l], p0 + s_off[l + 1]); scur holds the colour cursors;
+// slot[q] (pair q of the world) gets its manifold index, or -1.  The order
+// inside a colour is free, except in the serial tail colour (tail >= 0),
+// whose manifolds may share bodies: thread 0 places those in pair order.
 template <int T, class CT>
-__device__ void pack_colours(const Ctx& c, const PackF& P, int p0, int np, int c1,
-                             const CT* colour, int* scur, int tail) {
-  auto pack = [&](int q, int m) {
-    const int p = p0 + q;
-    int k0 = c.coff[p], k1 = k0 + c.ccount[p];
-    if (k1 > c1) k1 = c1;  // contacts beyond the capacity were not stored
-    pack_man(c, P, k0, k1, p0 + m);
-  };
+__device__ void assign_slots(int p0, int np, const CT* colour, int* scur, int tail,
+                             int32_t* slot) {
   for (int q = threadIdx.x; q < np; q += T) {
     const int cl = colour_of(colour[q]);
-    if (cl < 0 || cl == tail) continue;
-    pack(q, atomicAdd(&scur[cl], 1));
+    if (cl == tail && tail >= 0) continue;
+    slot[q] = cl < 0 ? -1 : p0 + atomicAdd(&scur[cl], 1);
   }
   if (tail >= 0 && threadIdx.x == 0) {
     int m = scur[tail];
     for (int q = 0; q < np; ++q)
-      if (colour_of(colour[q]) == tail) pack(q, m++);
+      if (colour_of(colour[q]) == tail) slot[q] = p0 + m++;
   }
 }
 
@@ -416,13 +411,13 @@ __device__ void colour_offsets(const CT* colour, int np, int ncol, int* s_off, i
   __syncthreads();
 }
 
-// LDS variant: local body ids and manifold indices fit 16 and 15 bits, the
-// colouring (4 B per body, 6 B per manifold) and the velocities (12 B per
-// body) fit the pool
-__device__ __forceinline__ bool fits_lds(int nb, int np, int pool_bytes) {
-  const long need_col = 4L * nb + 6L * np, need_vel = 12L * nb;
-  return nb < (int)NO_BODY && np <= 32767 && need_col <= pool_bytes && need_vel <= pool_bytes;
+// Colouring in LDS: local body ids and manifold indices fit 16 and 15 bits
+// and the colouring (4 B per body, 6 B per manifold) fits the pool.
+__device__ __forceinline__ bool colour_fits_lds(int nb, int np, int pool_bytes) {
+  return nb < (int)NO_BODY && np <= 32767 && 4L * nb + 6L * np <= pool_bytes;
 }
+// FP32 velocities in LDS: 12 B per body.
+__device__ __forceinline__ bool vel_fits_lds(int nb, int pool_bytes) { return 12L * nb <= pool_bytes; }
 
 // Hashed priority of manifold q of world w: 17 hash bits, then the index
 // (distinct within a world).  prio32 packs it for q < 32768 (LDS colouring),
@@ -436,11 +431,27 @@ __device__ __forceinline__ unsigned long long prio64(int q, int w) {
          (unsigned long long)((uint32_t)q + 1u);
 }
 
-// One workgroup per world whose colouring and FP32 velocities fit in LDS.
-// LDS pool: during colouring s_best (4 B/body) and per manifold its bodies
-// (4 B) and colour (2 B); during the solve FP32 velocities (12 B/body).
+// After colouring world w (ncol colours, serial tail or not): per-colour
+// offsets into col_off[w], the colour count into col_info[w], and each
+// pair's manifold slot into sched_order (pair index).
+template <int T, class CT>
+__device__ void colour_finish(const Ctx& c, int w, int p0, int np, const CT* colour, int rounds,
+                              bool serial_tail, int* s_off, int* s_cur) {
+  const int ncol = rounds + (serial_tail ? 1 : 0);
+  if (threadIdx.x == 0) atomicMax(&c.counters[CNT_LEVELS], ncol);
+  colour_offsets<T>(colour, np, ncol, s_off, s_cur);
+  assign_slots<T>(p0, np, colour, s_cur, serial_tail ? ncol - 1 : -1, c.sched_order + p0);
+  int* off = c.col_off + (long)w * (COLOR_CAP + 3);
+  for (int l = threadIdx.x; l <= ncol; l += T) off[l] = s_off[l];
+  if (threadIdx.x == 0) c.col_info[w] = ncol | (serial_tail ? 1 << 16 : 0);
+}
+
+// Jones-Plassmann colouring of one world's manifolds in LDS (one workgroup
+// per world whose colouring fits; the others are left to
+// k_colour_global).  LDS pool: s_best (4 B/body), per pair its local bodies
+// (4 B) and colour (2 B: 0xFFFF uncoloured, 0xFFFE no contacts).
 template <int T>
-__global__ __launch_bounds__(T) void k_solve_colored(Ctx c, int pool_bytes) {
+__global__ __launch_bounds__(T) void k_colour_lds(Ctx c, int pool_bytes) {
   extern __shared__ int lds[];
   int* s_off = lds;                      // COLOR_CAP + 3
   int* s_cur = lds + (COLOR_CAP + 3);    // COLOR_CAP + 3
@@ -448,15 +459,19 @@ __global__ __launch_bounds__(T) void k_solve_colored(Ctx c, int pool_bytes) {
   __shared__ int s_left, s_taken;
   const int w = blockIdx.x, tid = threadIdx.x;
   const int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
-  if (c1 <= c0) return;
   const int p0 = c.wpair_off[w], p1 = c.wpair_off[w + 1];
   const int np = p1 - p0;
   const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
-  if (!fits_lds(nb, np, pool_bytes)) return;  // k_solve_colored_global takes this world
+  if (c1 <= c0) {  // nothing to solve: no manifold slots for the packer
+    for (int q = tid; q < np; q += T) c.sched_order[p0 + q] = -1;
+    if (tid == 0) c.col_info[w] = 0;
+    return;
+  }
+  if (!colour_fits_lds(nb, np, pool_bytes)) return;  // k_colour_global takes this world
   tmark(c, 0);
   uint32_t* s_best = (uint32_t*)pool;
   uint32_t* m_ab = s_best + nb;               // np: local a | local b << 16
-  uint16_t* colour = (uint16_t*)(m_ab + np);  // np: colour, 0xFFFF uncoloured
+  uint16_t* colour = (uint16_t*)(m_ab + np);  // np
   if (tid == 0) s_left = 0;
   __syncthreads();
   int mine = 0;
@@ -502,41 +517,19 @@ __global__ __launch_bounds__(T) void k_solve_colored(Ctx c, int pool_bytes) {
     __syncthreads();
   }
   const bool serial_tail = s_left > 0;
-  const int ncol = rounds + (serial_tail ? 1 : 0);
   for (int q = tid; q < np; q += T)
     if (colour[q] == 0xFFFFu) colour[q] = (uint16_t)rounds;
+  __syncthreads();
+  colour_finish<T>(c, w, p0, np, colour, rounds, serial_tail, s_off, s_cur);
   tmark(c, 1);
-  if (tid == 0) atomicMax(&c.counters[CNT_LEVELS], ncol);
-  colour_offsets<T>(colour, np, ncol, s_off, s_cur);
-  const PackF P = packf(c);
-  pack_colours<T>(c, P, p0, np, c1, colour, s_cur, serial_tail ? ncol - 1 : -1);
-  __syncthreads();  // colouring data is dead: the pool now holds velocities
-  VelF<true> V;
-  V.lds = (float*)pool;
-  V.c = &c;
-  V.b0 = b0;
-  for (int i = tid; i < nb; i += T) {
-    V.lds[3 * i] = (float)c.vx[b0 + i];
-    V.lds[3 * i + 1] = (float)c.vy[b0 + i];
-    V.lds[3 * i + 2] = (float)c.w[b0 + i];
-  }
-  __syncthreads();
-  tmark(c, 2);
-  tmark(c, 3);
-  sweep_colours<true, T>(c, P, p0, ncol, serial_tail, s_off, V);
-  tmark(c, 4);
-  colour_writeback<true, T>(c, P, p0, s_off[ncol], b0, nb, V);
-  __syncthreads();
-  tmark(c, 5);
 }
 
-// The same algorithm for worlds whose colouring or velocities do not fit in
-// LDS: 64-bit priorities per body, colours per pair and FP32 velocities in
-// global memory (L2-resident), 512 lanes per world.  Worlds that fit are
-// left to k_solve_colored (the fit test is the same).
+// The same colouring for worlds whose colouring does not fit in LDS:
+// 64-bit priorities per body (col_best) and colours per pair (sched_level)
+// in global memory (L2-resident).
 #define COLOR_THREADS_G 512
 
-__global__ __launch_bounds__(COLOR_THREADS_G) void k_solve_colored_global(Ctx c, int pool_bytes) {
+__global__ __launch_bounds__(COLOR_THREADS_G) void k_colour_global(Ctx c, int pool_bytes) {
   __shared__ int s_off[COLOR_CAP + 3];
   __shared__ int s_cur[COLOR_CAP + 3];
   __shared__ int s_left, s_taken;
@@ -546,7 +539,7 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_solve_colored_global(Ctx c,
   const int p0 = c.wpair_off[w], p1 = c.wpair_off[w + 1];
   const int np = p1 - p0;
   const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
-  if (fits_lds(nb, np, pool_bytes)) return;
+  if (colour_fits_lds(nb, np, pool_bytes)) return;
   unsigned long long* best = c.col_best + b0;
   int32_t* colour = c.sched_level + p0;  // per pair: colour, -1 uncoloured, -2 none
   if (tid == 0) s_left = 0;
@@ -592,67 +585,127 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_solve_colored_global(Ctx c,
     __syncthreads();
   }
   const bool serial_tail = s_left > 0;
-  const int ncol = rounds + (serial_tail ? 1 : 0);
   for (int q = tid; q < np; q += COLOR_THREADS_G)
     if (colour[q] == -1) colour[q] = rounds;
-  if (tid == 0) atomicMax(&c.counters[CNT_LEVELS], ncol);
-  colour_offsets<COLOR_THREADS_G>(colour, np, ncol, s_off, s_cur);
-  const PackF P = packf(c);
-  pack_colours<COLOR_THREADS_G>(c, P, p0, np, c1, colour, s_cur, serial_tail ? ncol - 1 : -1);
-  for (int i = tid; i < nb; i += COLOR_THREADS_G) {
-    c.fvx[b0 + i] = (float)c.vx[b0 + i];
-    c.fvy[b0 + i] = (float)c.vy[b0 + i];
-    c.fw[b0 + i] = (float)c.w[b0 + i];
-  }
   __syncthreads();
-  VelF<false> V;
-  V.lds = nullptr;
+  colour_finish<COLOR_THREADS_G>(c, w, p0, np, colour, rounds, serial_tail, s_off, s_cur);
+}
+
+// Every coloured pair's <= 2 contacts become its manifold record, over the
+// whole batch at once.
+__global__ void k_pack_manifolds(Ctx c) {
+  const int tp = c.wpair_off[c.W];
+  const PackF P = packf(c);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < tp; p += gridDim.x * blockDim.x) {
+    const int M = c.sched_order[p];
+    if (M < 0) continue;
+    const int k0 = c.coff[p];
+    int k1 = k0 + c.ccount[p];
+    if (k1 > c.contact_cap) k1 = c.contact_cap;  // contacts beyond the capacity were not stored
+    pack_man(c, P, k0, k1, M);
+  }
+}
+
+// Warm-start pass, sweeps and write-back of one world per workgroup, with
+// FP32 velocities in LDS (VLDS, worlds whose velocities fit) or in global
+// memory (the others).
+template <bool VLDS, int T>
+__global__ __launch_bounds__(T) void k_sweep_colours(Ctx c, int pool_bytes) {
+  extern __shared__ int lds[];
+  __shared__ int s_off[COLOR_CAP + 3];
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int info = c.col_info[w];
+  const int ncol = info & 0xFFFF;
+  if (ncol == 0) return;
+  const bool serial_tail = (info >> 16) & 1;
+  const int p0 = c.wpair_off[w];
+  const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
+  if (vel_fits_lds(nb, pool_bytes) != VLDS) return;
+  const int* off = c.col_off + (long)w * (COLOR_CAP + 3);
+  for (int l = tid; l <= ncol; l += T) s_off[l] = off[l];
+  VelF<VLDS> V;
+  V.lds = VLDS ? (float*)lds : nullptr;
   V.c = &c;
   V.b0 = b0;
-  sweep_colours<false, COLOR_THREADS_G>(c, P, p0, ncol, serial_tail, s_off, V);
-  colour_writeback<false, COLOR_THREADS_G>(c, P, p0, s_off[ncol], b0, nb, V);
+  for (int i = tid; i < nb; i += T) {
+    const float vx = (float)c.vx[b0 + i], vy = (float)c.vy[b0 + i], vw = (float)c.w[b0 + i];
+    if (VLDS) {
+      V.lds[3 * i] = vx;
+      V.lds[3 * i + 1] = vy;
+      V.lds[3 * i + 2] = vw;
+    } else {
+      c.fvx[b0 + i] = vx;
+      c.fvy[b0 + i] = vy;
+      c.fw[b0 + i] = vw;
+    }
+  }
+  __syncthreads();
+  tmark(c, 2);
+  tmark(c, 3);
+  const PackF P = packf(c);
+  sweep_colours<VLDS, T>(c, P, p0, ncol, serial_tail, s_off, V);
+  tmark(c, 4);
+  colour_writeback<VLDS, T>(c, P, p0, s_off[ncol], b0, nb, V);
+  __syncthreads();
+  tmark(c, 5);
 }
 
 shp_status colored_alloc(shp_batch* b) {
   Ctx& c = b->ctx;
   static_assert((CF_LAMBDA + 4) * 2 + MF_FIELDS <= 2 * 29 * 2, "records exceed the packed buffer");
-  if (hipMalloc(&c.pk_k, (size_t)c.contact_cap * 4) != hipSuccess) return SHP_E_OOM;
-  b->allocs.push_back(c.pk_k);
+  void** bufs[] = {(void**)&c.pk_k, (void**)&c.col_best, (void**)&c.col_off, (void**)&c.col_info};
+  size_t sizes[] = {(size_t)c.contact_cap * 4, (size_t)c.NB * 8,
+                    (size_t)c.W * (COLOR_CAP + 3) * 4, (size_t)c.W * 4};
+  for (int k = 0; k < 4; ++k) {
+    if (hipMalloc(bufs[k], sizes[k] ? sizes[k] : 16) != hipSuccess) return SHP_E_OOM;
+    b->allocs.push_back(*bufs[k]);
+  }
   long need = 0;
   int max_nb = 0;
   for (int w = 0; w < c.W; ++w) {
     long nb = b->h_body_off[w + 1] - b->h_body_off[w];
     max_nb = std::max<int>(max_nb, (int)nb);
     // pairs per world are unknown before a step: budget 8 per body (a world
-    // that needs more this step goes to the global-memory variant)
+    // that needs more this step is coloured in global memory)
     long np = std::min<long>(32767, 8 * nb + 64);
     need = std::max(need, std::max(4 * nb + 6 * np, 12 * nb));
   }
   size_t fixed = (2 * (COLOR_CAP + 3) + 2) * 4;
   b->color_lds = std::min<size_t>(fixed + need, LDS_MAX_BYTES);
-  if (hipMalloc(&c.col_best, (size_t)c.NB * 8) != hipSuccess) return SHP_E_OOM;
-  b->allocs.push_back(c.col_best);
-  // 4 waves per world; a large world gets 8 (two manifold records per lane
-  // leave 8 waves without spills)
+  // 4 waves per world; a large world gets 8
   b->color_threads = b->cfg.solver_threads ? b->cfg.solver_threads : (max_nb > 2048 ? 512 : 256);
-  const void* fn = b->color_threads == 512 ? (const void*)k_solve_colored<512>
-                                           : (const void*)k_solve_colored<256>;
-  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->color_lds) !=
-      hipSuccess)
-    return SHP_E_HIP;
+  const size_t vel_lds = b->color_lds - fixed;
+  // colouring is light on registers: a large world colours with 16 waves
+  const void* fns[] = {b->color_threads == 512 ? (const void*)k_colour_lds<1024>
+                                                : (const void*)k_colour_lds<256>,
+                       b->color_threads == 512 ? (const void*)k_sweep_colours<true, 512>
+                                                : (const void*)k_sweep_colours<true, 256>};
+  const size_t ldsz[] = {b->color_lds, vel_lds};
+  for (int k = 0; k < 2; ++k)
+    if (hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsz[k]) !=
+        hipSuccess)
+      return SHP_E_HIP;
   return SHP_OK;
 }
 
 void launch_colored_solver(shp_batch* b) {
-  int pool = (int)(b->color_lds - (2 * (COLOR_CAP + 3) + 2) * 4);
+  const Ctx& c = b->ctx;
+  const int pool = (int)(b->color_lds - (2 * (COLOR_CAP + 3) + 2) * 4);
+  const size_t vel_lds = (size_t)pool;
   if (b->color_threads == 512)
-    k_solve_colored<512><<<b->ctx.W, 512, b->color_lds, b->stream>>>(b->ctx, pool);
+    k_colour_lds<1024><<<c.W, 1024, b->color_lds, b->stream>>>(c, pool);
   else
-    k_solve_colored<256><<<b->ctx.W, 256, b->color_lds, b->stream>>>(b->ctx, pool);
-  // worlds that do not fit in LDS (by body count at creation, or by their
-  // pair count this step) are solved by the global-memory variant; it exits
-  // at once for the others
-  k_solve_colored_global<<<b->ctx.W, COLOR_THREADS_G, 0, b->stream>>>(b->ctx, pool);
+    k_colour_lds<256><<<c.W, 256, b->color_lds, b->stream>>>(c, pool);
+  // worlds whose colouring does not fit in LDS (by body count, or by this
+  // step's pair count); it exits at once for the others
+  k_colour_global<<<c.W, COLOR_THREADS_G, 0, b->stream>>>(c, pool);
+  const long tp = c.pair_cap;
+  k_pack_manifolds<<<std::max(1L, std::min((tp + 255) / 256, 4096L)), 256, 0, b->stream>>>(c);
+  if (b->color_threads == 512)
+    k_sweep_colours<true, 512><<<c.W, 512, vel_lds, b->stream>>>(c, pool);
+  else
+    k_sweep_colours<true, 256><<<c.W, 256, vel_lds, b->stream>>>(c, pool);
+  k_sweep_colours<false, COLOR_THREADS_G><<<c.W, COLOR_THREADS_G, 0, b->stream>>>(c, pool);
 }
 
 // ---- solve quality -----------------------------------------------------------

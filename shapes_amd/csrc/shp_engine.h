@@ -100,6 +100,8 @@ struct Ctx {
   int32_t* pk_k;  // contact index of a slot (coloured)
   uint8_t* pk_f;  // flags: 1 a static, 2 b static, 4 warm, 8 pair continues
   unsigned long long* col_best;  // colouring priorities per body (large worlds)
+  int32_t* col_off;   // per world: COLOR_CAP + 3 colour offsets (manifolds)
+  int32_t* col_info;  // per world: colours | serial tail << 16 (0: nothing to solve)
   // solver sub-phase timestamps of world 0 (wall_clock64), timing mode only
   unsigned long long* tstamp;
   unsigned long long* tstamp_store;

@@ -30,12 +30,21 @@ __device__ __forceinline__ int cell_of(double x, double h) {
   return (int)q;
 }
 
+// Bucket of cell (cx, cy): the low 16 bits of each coordinate interleaved
+// (Morton order), masked to the world's table.  Nearby cells land in
+// nearby buckets, so a body's 3x3 neighbourhood reads a few cache lines of
+// the table; cells that alias share a bucket and are told apart by their
+// stored cell coordinates.
+__device__ __forceinline__ uint32_t spread_bits(uint32_t x) {
+  x &= 0xFFFFu;
+  x = (x | (x << 8)) & 0x00FF00FFu;
+  x = (x | (x << 4)) & 0x0F0F0F0Fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
 __device__ __forceinline__ uint32_t cell_hash(int cx, int cy, uint32_t mask) {
-  uint32_t x = (uint32_t)cx * 0x8DA6B343u ^ (uint32_t)cy * 0xD8163841u;
-  x ^= x >> 15;
-  x *= 0x2C1B3C6Du;
-  x ^= x >> 12;
-  return x & mask;
+  return (spread_bits((uint32_t)cx) | (spread_bits((uint32_t)cy) << 1)) & mask;
 }
 
 __device__ __forceinline__ bool aabb_hit(const Ctx& c, double aminx, double amaxx, double aminy,
