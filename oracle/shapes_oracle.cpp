@@ -617,10 +617,16 @@ uint64_t colour_prio(int q, int w) {
 }
 
 const int COLOUR_CAP = 256;
+const int GREEDY_COLOURS = 32;                          // colours from the body masks
+const int JP_ROUNDS = COLOUR_CAP - GREEDY_COLOURS - 3;  // rounds before the serial tail
 
-// colour[q] >= 0 for pairs with contacts; the last colour is the serial
-// tail (pair order) when COLOUR_CAP rounds were not enough.  Returns the
-// number of colours.
+// colour[q] >= 0 for pairs with contacts.  Jones-Plassmann rounds select the
+// manifolds whose priority is the largest among the uncoloured ones on each
+// of their dynamic bodies; a selected manifold takes the smallest colour
+// that no manifold on its dynamic bodies has (colours below 32, tracked per
+// body), else 32 + the round.  The last colour is the serial tail (pair
+// order) when JP_ROUNDS rounds were not enough.  Returns the number of
+// colours.
 int colour_pairs(orc_engine* e, std::vector<int>& colour, bool* tail) {
   const int np = (int)e->pairs.size();
   const int n = (int)e->objs.size();
@@ -632,8 +638,9 @@ int colour_pairs(orc_engine* e, std::vector<int>& colour, bool* tail) {
       ++left;
     }
   std::vector<uint64_t> best(n);
-  int rounds = 0;
-  while (left > 0 && rounds < COLOUR_CAP) {
+  std::vector<uint32_t> used(n, 0u);
+  int rounds = 0, top = 0;
+  while (left > 0 && rounds < JP_ROUNDS) {
     std::fill(best.begin(), best.end(), 0ull);
     for (int q = 0; q < np; ++q) {
       if (colour[q] != -1) continue;
@@ -650,14 +657,30 @@ int colour_pairs(orc_engine* e, std::vector<int>& colour, bool* tail) {
       if ((is_static(e->objs[a]) || best[a] == pr) && (is_static(e->objs[b]) || best[b] == pr))
         take.push_back(q);
     }
-    for (int q : take) colour[q] = rounds;
+    for (int q : take) {
+      int a = e->pairs[q].first, b = e->pairs[q].second;
+      const bool sa = is_static(e->objs[a]), sb = is_static(e->objs[b]);
+      const uint32_t u = (sa ? 0u : used[a]) | (sb ? 0u : used[b]);
+      int l = GREEDY_COLOURS + rounds;
+      for (int k = 0; k < GREEDY_COLOURS; ++k)
+        if (!((u >> k) & 1u)) {
+          l = k;
+          break;
+        }
+      colour[q] = l;
+      if (l < GREEDY_COLOURS) {
+        if (!sa) used[a] |= 1u << l;
+        if (!sb) used[b] |= 1u << l;
+      }
+      top = std::max(top, l + 1);
+    }
     left -= (int)take.size();
     ++rounds;
   }
   *tail = left > 0;
   for (int q = 0; q < np; ++q)
-    if (colour[q] == -1) colour[q] = rounds;
-  return rounds + (*tail ? 1 : 0);
+    if (colour[q] == -1) colour[q] = top;
+  return top + (*tail ? 1 : 0);
 }
 
 inline float dot6f(const float* a, const float* b) {

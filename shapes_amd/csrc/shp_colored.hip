@@ -8,9 +8,11 @@
 // contacts; one lane solves a pair's <= 2 points in key order) are coloured
 // so that no two manifolds of a colour share a dynamic body, then each
 // colour is solved in parallel.  Colours come from deterministic
-// Jones-Plassmann rounds: in round r every uncoloured manifold whose hashed
-// priority is the largest among the uncoloured manifolds on each of its
-// dynamic bodies takes colour r.  Velocities and constraint rows are FP32
+// Jones-Plassmann rounds: in each round every uncoloured manifold whose
+// hashed priority is the largest among the uncoloured manifolds on each of
+// its dynamic bodies is selected (the selected manifolds share no dynamic
+// body) and takes the smallest colour that no manifold on its dynamic
+// bodies has (a 32-bit mask per body; past 32 colours, colour 32 + round).  Velocities and constraint rows are FP32
 // (velocities staged in LDS when they fit); the accumulated lagrangians stay
 // FP64; each body's FP64 velocity receives the FP32 change of the solve, so
 // bodies without contacts keep their FP64 velocity exactly.
@@ -34,6 +36,12 @@
 using namespace shp;
 
 #define COLOR_CAP 256
+// colours 0..31 are assigned smallest-free from the per-body masks; a
+// manifold whose bodies already use all 32 takes GREEDY_COLOURS + round,
+// so the Jones-Plassmann rounds stop at JP_ROUNDS to keep every colour and
+// the serial tail below COLOR_CAP (and below the uint8 sentinels)
+#define GREEDY_COLOURS 32
+#define JP_ROUNDS (COLOR_CAP - GREEDY_COLOURS - 3)
 #define NO_BODY 0xFFFFu
 #define LDS_MAX_BYTES (160 * 1024 - 512)
 
@@ -282,10 +290,16 @@ __device__ __forceinline__ void solve_man(const PackF& P, int M, ManF& m, bool w
 }
 
 // Colours are stored as int (-1 uncoloured, -2 no contacts) in global
-// memory and as uint16 (0xFFFF uncoloured, 0xFFFE no contacts) in LDS.
+// memory and as uint8 (0xFF uncoloured, 0xFE no contacts) in LDS.
 __device__ __forceinline__ int colour_of(int x) { return x; }
-__device__ __forceinline__ int colour_of(uint16_t x) {
-  return x == 0xFFFFu ? -1 : (x == 0xFFFEu ? -2 : (int)x);
+__device__ __forceinline__ int colour_of(uint8_t x) {
+  return x == 0xFFu ? -1 : (x == 0xFEu ? -2 : (int)x);
+}
+
+// Colour of a manifold selected in Jones-Plassmann round r whose dynamic
+// bodies use the colours in `used`.
+__device__ __forceinline__ int greedy_colour(uint32_t used, int r) {
+  return used != 0xFFFFFFFFu ? __builtin_ctz(~used) : GREEDY_COLOURS + r;
 }
 
 // Slots of the coloured manifolds: colour l owns manifold indices
@@ -310,7 +324,7 @@ __device__ void assign_slots(int p0, int np, const CT* colour, int* scur, int ta
 
 // Warm-start pass and sweeps, colour by colour.  Phase h = pass * ncol + l;
 // thread t takes manifolds s_off[l] + t, + T, ... of colour l (the last
-// colour serially by thread 0 when colouring stopped at COLOR_CAP).
+// colour serially by thread 0 when colouring stopped after JP_ROUNDS).
 template <bool VLDS, int T>
 __device__ void sweep_colours(const Ctx& c, const PackF& P, int p0, int ncol, bool serial_tail,
                               const int* s_off, const VelF<VLDS>& V) {
@@ -412,9 +426,9 @@ __device__ void colour_offsets(const CT* colour, int np, int ncol, int* s_off, i
 }
 
 // Colouring in LDS: local body ids and manifold indices fit 16 and 15 bits
-// and the colouring (4 B per body, 6 B per manifold) fits the pool.
+// and the colouring (8 B per body, 5 B per manifold) fits the pool.
 __device__ __forceinline__ bool colour_fits_lds(int nb, int np, int pool_bytes) {
-  return nb < (int)NO_BODY && np <= 32767 && 4L * nb + 6L * np <= pool_bytes;
+  return nb < (int)NO_BODY && np <= 32767 && 8L * nb + 5L * np <= pool_bytes;
 }
 // FP32 velocities in LDS: 12 B per body.
 __device__ __forceinline__ bool vel_fits_lds(int nb, int pool_bytes) { return 12L * nb <= pool_bytes; }
@@ -431,13 +445,13 @@ __device__ __forceinline__ unsigned long long prio64(int q, int w) {
          (unsigned long long)((uint32_t)q + 1u);
 }
 
-// After colouring world w (ncol colours, serial tail or not): per-colour
-// offsets into col_off[w], the colour count into col_info[w], and each
-// pair's manifold slot into sched_order (pair index).
+// After colouring world w (colours 0..top-1, then the serial tail colour
+// top if any): per-colour offsets into col_off[w], the colour count into
+// col_info[w], and each pair's manifold slot into sched_order (pair index).
 template <int T, class CT>
-__device__ void colour_finish(const Ctx& c, int w, int p0, int np, const CT* colour, int rounds,
+__device__ void colour_finish(const Ctx& c, int w, int p0, int np, const CT* colour, int top,
                               bool serial_tail, int* s_off, int* s_cur) {
-  const int ncol = rounds + (serial_tail ? 1 : 0);
+  const int ncol = top + (serial_tail ? 1 : 0);
   if (threadIdx.x == 0) atomicMax(&c.counters[CNT_LEVELS], ncol);
   colour_offsets<T>(colour, np, ncol, s_off, s_cur);
   assign_slots<T>(p0, np, colour, s_cur, serial_tail ? ncol - 1 : -1, c.sched_order + p0);
@@ -448,15 +462,16 @@ __device__ void colour_finish(const Ctx& c, int w, int p0, int np, const CT* col
 
 // Jones-Plassmann colouring of one world's manifolds in LDS (one workgroup
 // per world whose colouring fits; the others are left to
-// k_colour_global).  LDS pool: s_best (4 B/body), per pair its local bodies
-// (4 B) and colour (2 B: 0xFFFF uncoloured, 0xFFFE no contacts).
+// k_colour_global).  LDS pool: s_best and s_used (4 B/body each), per pair
+// its local bodies (4 B) and colour (1 B: 0xFF uncoloured, 0xFE no
+// contacts).
 template <int T>
 __global__ __launch_bounds__(T) void k_colour_lds(Ctx c, int pool_bytes) {
   extern __shared__ int lds[];
   int* s_off = lds;                      // COLOR_CAP + 3
   int* s_cur = lds + (COLOR_CAP + 3);    // COLOR_CAP + 3
   unsigned char* pool = (unsigned char*)(lds + 2 * (COLOR_CAP + 3) + 2);
-  __shared__ int s_left, s_taken;
+  __shared__ int s_left, s_taken, s_top;
   const int w = blockIdx.x, tid = threadIdx.x;
   const int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
   const int p0 = c.wpair_off[w], p1 = c.wpair_off[w + 1];
@@ -470,9 +485,14 @@ __global__ __launch_bounds__(T) void k_colour_lds(Ctx c, int pool_bytes) {
   if (!colour_fits_lds(nb, np, pool_bytes)) return;  // k_colour_global takes this world
   tmark(c, 0);
   uint32_t* s_best = (uint32_t*)pool;
-  uint32_t* m_ab = s_best + nb;               // np: local a | local b << 16
-  uint16_t* colour = (uint16_t*)(m_ab + np);  // np
-  if (tid == 0) s_left = 0;
+  uint32_t* s_used = s_best + nb;             // nb: colours taken at the body
+  uint32_t* m_ab = s_used + nb;               // np: local a | local b << 16
+  uint8_t* colour = (uint8_t*)(m_ab + np);    // np
+  if (tid == 0) {
+    s_left = 0;
+    s_top = 0;
+  }
+  for (int i = tid; i < nb; i += T) s_used[i] = 0;
   __syncthreads();
   int mine = 0;
   for (int q = tid; q < np; q += T) {
@@ -482,57 +502,68 @@ __global__ __launch_bounds__(T) void k_colour_lds(Ctx c, int pool_bytes) {
     const uint32_t b = c.is_static[gb] ? NO_BODY : (uint32_t)(gb - b0);
     m_ab[q] = a | (b << 16);
     const bool has = c.ccount[p] > 0 && c.coff[p] < c1;  // contacts stored within capacity
-    colour[q] = has ? 0xFFFFu : 0xFFFEu;
+    colour[q] = has ? 0xFFu : 0xFEu;
     mine += has;
   }
   atomicAdd(&s_left, mine);
   __syncthreads();
   int rounds = 0;
-  while (s_left > 0 && rounds < COLOR_CAP) {
+  while (s_left > 0 && rounds < JP_ROUNDS) {
     for (int i = tid; i < nb; i += T) s_best[i] = 0;
     if (tid == 0) s_taken = 0;
     __syncthreads();
     for (int q = tid; q < np; q += T) {
-      if (colour[q] != 0xFFFFu) continue;
+      if (colour[q] != 0xFFu) continue;
       const uint32_t prio = prio32(q, w);
       const uint32_t ab = m_ab[q], a = ab & 0xFFFF, b = ab >> 16;
       if (a != NO_BODY) atomicMax(&s_best[a], prio);
       if (b != NO_BODY) atomicMax(&s_best[b], prio);
     }
     __syncthreads();
-    int taken = 0;
+    // the selected manifolds share no dynamic body: each updates the masks
+    // of its own bodies
+    int taken = 0, top = 0;
     for (int q = tid; q < np; q += T) {
-      if (colour[q] != 0xFFFFu) continue;
+      if (colour[q] != 0xFFu) continue;
       const uint32_t prio = prio32(q, w);
       const uint32_t ab = m_ab[q], a = ab & 0xFFFF, b = ab >> 16;
       if ((a == NO_BODY || s_best[a] == prio) && (b == NO_BODY || s_best[b] == prio)) {
-        colour[q] = (uint16_t)rounds;
+        const uint32_t used = (a != NO_BODY ? s_used[a] : 0u) | (b != NO_BODY ? s_used[b] : 0u);
+        const int l = greedy_colour(used, rounds);
+        colour[q] = (uint8_t)l;
+        if (l < GREEDY_COLOURS) {
+          if (a != NO_BODY) s_used[a] = s_used[a] | (1u << l);
+          if (b != NO_BODY) s_used[b] = s_used[b] | (1u << l);
+        }
+        top = max(top, l + 1);
         ++taken;
       }
     }
     atomicAdd(&s_taken, taken);
+    atomicMax(&s_top, top);
     __syncthreads();
     if (tid == 0) s_left -= s_taken;
     ++rounds;
     __syncthreads();
   }
   const bool serial_tail = s_left > 0;
+  const int top = s_top;
   for (int q = tid; q < np; q += T)
-    if (colour[q] == 0xFFFFu) colour[q] = (uint16_t)rounds;
+    if (colour[q] == 0xFFu) colour[q] = (uint8_t)top;
   __syncthreads();
-  colour_finish<T>(c, w, p0, np, colour, rounds, serial_tail, s_off, s_cur);
+  colour_finish<T>(c, w, p0, np, colour, top, serial_tail, s_off, s_cur);
   tmark(c, 1);
 }
 
 // The same colouring for worlds whose colouring does not fit in LDS:
-// 64-bit priorities per body (col_best) and colours per pair (sched_level)
-// in global memory (L2-resident).
+// 64-bit priorities and colour masks per body (col_best, col_used) and
+// colours per pair (sched_level) in global memory (L2-resident).
 #define COLOR_THREADS_G 512
 
 __global__ __launch_bounds__(COLOR_THREADS_G) void k_colour_global(Ctx c, int pool_bytes) {
   __shared__ int s_off[COLOR_CAP + 3];
   __shared__ int s_cur[COLOR_CAP + 3];
-  __shared__ int s_left, s_taken;
+  __shared__ int s_left, s_taken, s_top;
   const int w = blockIdx.x, tid = threadIdx.x;
   const int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
   if (c1 <= c0) return;
@@ -541,8 +572,13 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_colour_global(Ctx c, int po
   const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
   if (colour_fits_lds(nb, np, pool_bytes)) return;
   unsigned long long* best = c.col_best + b0;
+  uint32_t* used_at = c.col_used + b0;
   int32_t* colour = c.sched_level + p0;  // per pair: colour, -1 uncoloured, -2 none
-  if (tid == 0) s_left = 0;
+  if (tid == 0) {
+    s_left = 0;
+    s_top = 0;
+  }
+  for (int i = tid; i < nb; i += COLOR_THREADS_G) used_at[i] = 0u;
   __syncthreads();
   int mine = 0;
   for (int q = tid; q < np; q += COLOR_THREADS_G) {
@@ -554,7 +590,7 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_colour_global(Ctx c, int po
   atomicAdd(&s_left, mine);
   __syncthreads();
   int rounds = 0;
-  while (s_left > 0 && rounds < COLOR_CAP) {
+  while (s_left > 0 && rounds < JP_ROUNDS) {
     for (int i = tid; i < nb; i += COLOR_THREADS_G) best[i] = 0ull;
     if (tid == 0) s_taken = 0;
     __syncthreads();
@@ -567,28 +603,38 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_colour_global(Ctx c, int po
       if (!c.is_static[gb]) atomicMax(&best[gb - b0], prio);
     }
     __syncthreads();
-    int taken = 0;
+    int taken = 0, top = 0;
     for (int q = tid; q < np; q += COLOR_THREADS_G) {
       if (colour[q] != -1) continue;
       const int p = p0 + q;
       const unsigned long long prio = prio64(q, w);
       const int ga = c.pa[p], gb = c.pb[p];
-      if ((c.is_static[ga] || best[ga - b0] == prio) && (c.is_static[gb] || best[gb - b0] == prio)) {
-        colour[q] = rounds;
+      const bool sa = c.is_static[ga] != 0, sb = c.is_static[gb] != 0;
+      if ((sa || best[ga - b0] == prio) && (sb || best[gb - b0] == prio)) {
+        const uint32_t used = (sa ? 0u : used_at[ga - b0]) | (sb ? 0u : used_at[gb - b0]);
+        const int l = greedy_colour(used, rounds);
+        colour[q] = l;
+        if (l < GREEDY_COLOURS) {
+          if (!sa) used_at[ga - b0] = used_at[ga - b0] | (1u << l);
+          if (!sb) used_at[gb - b0] = used_at[gb - b0] | (1u << l);
+        }
+        top = max(top, l + 1);
         ++taken;
       }
     }
     atomicAdd(&s_taken, taken);
+    atomicMax(&s_top, top);
     __syncthreads();
     if (tid == 0) s_left -= s_taken;
     ++rounds;
     __syncthreads();
   }
   const bool serial_tail = s_left > 0;
+  const int top = s_top;
   for (int q = tid; q < np; q += COLOR_THREADS_G)
-    if (colour[q] == -1) colour[q] = rounds;
+    if (colour[q] == -1) colour[q] = top;
   __syncthreads();
-  colour_finish<COLOR_THREADS_G>(c, w, p0, np, colour, rounds, serial_tail, s_off, s_cur);
+  colour_finish<COLOR_THREADS_G>(c, w, p0, np, colour, top, serial_tail, s_off, s_cur);
 }
 
 // Every coloured pair's <= 2 contacts become its manifold record, over the
@@ -653,10 +699,11 @@ __global__ __launch_bounds__(T) void k_sweep_colours(Ctx c, int pool_bytes) {
 shp_status colored_alloc(shp_batch* b) {
   Ctx& c = b->ctx;
   static_assert((CF_LAMBDA + 4) * 2 + MF_FIELDS <= 2 * 29 * 2, "records exceed the packed buffer");
-  void** bufs[] = {(void**)&c.pk_k, (void**)&c.col_best, (void**)&c.col_off, (void**)&c.col_info};
-  size_t sizes[] = {(size_t)c.contact_cap * 4, (size_t)c.NB * 8,
+  void** bufs[] = {(void**)&c.pk_k, (void**)&c.col_best, (void**)&c.col_used, (void**)&c.col_off,
+                   (void**)&c.col_info};
+  size_t sizes[] = {(size_t)c.contact_cap * 4, (size_t)c.NB * 8, (size_t)c.NB * 4,
                     (size_t)c.W * (COLOR_CAP + 3) * 4, (size_t)c.W * 4};
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 5; ++k) {
     if (hipMalloc(bufs[k], sizes[k] ? sizes[k] : 16) != hipSuccess) return SHP_E_OOM;
     b->allocs.push_back(*bufs[k]);
   }
@@ -668,7 +715,7 @@ shp_status colored_alloc(shp_batch* b) {
     // pairs per world are unknown before a step: budget 8 per body (a world
     // that needs more this step is coloured in global memory)
     long np = std::min<long>(32767, 8 * nb + 64);
-    need = std::max(need, std::max(4 * nb + 6 * np, 12 * nb));
+    need = std::max(need, std::max(8 * nb + 5 * np, 12 * nb));
   }
   size_t fixed = (2 * (COLOR_CAP + 3) + 2) * 4;
   b->color_lds = std::min<size_t>(fixed + need, LDS_MAX_BYTES);

@@ -17,6 +17,9 @@
 #include "shp_mlp.h"
 
 enum { CNT_FLAGS = 0, CNT_WARM = 1, CNT_LEARNED = 2, CNT_LEVELS = 3, CNT_MISS = 4 };
+// grid broadphase: partners per body kept by the count pass for the emit
+// pass (bodies with more are visited again)
+enum { GRID_SLOTS = 4 };
 // solve-quality maxima (doubles stored as their bit patterns): last step,
 // then since creation; then the sums of the steps' maxima and the number
 // of steps summed
@@ -100,6 +103,7 @@ struct Ctx {
   int32_t* pk_k;  // contact index of a slot (coloured)
   uint8_t* pk_f;  // flags: 1 a static, 2 b static, 4 warm, 8 pair continues
   unsigned long long* col_best;  // colouring priorities per body (large worlds)
+  uint32_t* col_used;            // colours taken at each body (large worlds)
   int32_t* col_off;   // per world: COLOR_CAP + 3 colour offsets (manifolds)
   int32_t* col_info;  // per world: colours | serial tail << 16 (0: nothing to solve)
   // solver sub-phase timestamps of world 0 (wall_clock64), timing mode only
@@ -127,7 +131,12 @@ struct Ctx {
   uint8_t* is_large;
   int32_t* body_cell;
   int32_t* tab_off;
-  int32_t *bucket_cnt, *bucket_start, *bucket_fill, *bucket_items;
+  int32_t *bucket_cnt, *bucket_start, *bucket_fill;
+  // per binned body in bucket order, 48 B: AABB (min x, max x, min y,
+  // max y), cell (x, y), body index, static flag
+  double* grid_rec;
+  // the count pass's first GRID_SLOTS partners of each body
+  int32_t* grid_slots;
   int32_t *large_off, *large_ids;
   int32_t* counters;  // 8 ints: flags, warm hits, learned starts, levels
   unsigned long long* quality;  // Q_ENTRIES entries, see Q_*

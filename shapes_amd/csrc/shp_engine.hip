@@ -102,69 +102,88 @@ __global__ void k_init_hulls(Ctx c) {
 // moveShapes (World.hs:163-167): T(pos) R(theta) applied to the local
 // vertices (Transform.hs:34-38, Linear.hs:217-220), unit edge normals
 // (ConvexHull.hs:184-195) and the AABB fold (Broadphase/Aabb.hs:81-110).
-__global__ void k_integrate(Ctx c, int advance) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= c.NB) return;
-  double px = c.px[g], py = c.py[g], th = c.th[g];
-  if (advance) {
-    double dt = c.dt;
-    px = (dt * c.vx[g]) + px;
-    py = (dt * c.vy[g]) + py;
-    th = (dt * c.w[g]) + th;
-    c.px[g] = px;
-    c.py[g] = py;
-    c.th[g] = th;
-  }
-  double s, co;
-  shp_sincos(th, &s, &co);
+// A workgroup takes 256 consecutive bodies: one lane per body builds the
+// transform, one lane per vertex transforms it and its successor and writes
+// the vertex and edge normal (coalesced), and one lane per body folds its
+// AABB in vertex order from LDS.
+constexpr int INTEGRATE_THREADS = 256;
+constexpr int INTEGRATE_VERTS = INTEGRATE_THREADS * SHP_MAXV;
+
+__global__ __launch_bounds__(INTEGRATE_THREADS) void k_integrate(Ctx c, int advance) {
+  __shared__ double s_m[6][INTEGRATE_THREADS];
+  __shared__ double s_wx[INTEGRATE_VERTS], s_wy[INTEGRATE_VERTS];
+  __shared__ uint8_t s_body[INTEGRATE_VERTS];
+  __shared__ int s_voff[INTEGRATE_THREADS + 1];
+  const int t = threadIdx.x;
+  const int g0 = blockIdx.x * INTEGRATE_THREADS;
+  const int nbk = min(INTEGRATE_THREADS, c.NB - g0);
+  const int g = g0 + t;
   const double one = 1.0, zer = 0.0;
-  double m00 = (one * co + zer * s) + px * zer;
-  double m01 = (one * (-s) + zer * co) + px * zer;
-  double m02 = (one * zer + zer * zer) + px * one;
-  double m10 = (zer * co + one * s) + py * zer;
-  double m11 = (zer * (-s) + one * co) + py * zer;
-  double m12 = (zer * zer + one * zer) + py * one;
-  int o = c.vert_off[g], n = c.vert_off[g + 1] - o;
-  double vx[SHP_MAXV], vy[SHP_MAXV];
-#pragma unroll
-  for (int k = 0; k < SHP_MAXV; ++k) {
-    if (k < n) {
-      double lx = c.lx[o + k], ly = c.ly[o + k];
-      vx[k] = (m00 * lx + m01 * ly) + m02 * one;
-      vy[k] = (m10 * lx + m11 * ly) + m12 * one;
-      c.wx[o + k] = vx[k];
-      c.wy[o + k] = vy[k];
+  if (t < nbk) {
+    double px = c.px[g], py = c.py[g], th = c.th[g];
+    if (advance) {
+      double dt = c.dt;
+      px = (dt * c.vx[g]) + px;
+      py = (dt * c.vy[g]) + py;
+      th = (dt * c.w[g]) + th;
+      c.px[g] = px;
+      c.py[g] = py;
+      c.th[g] = th;
     }
+    double s, co;
+    shp_sincos(th, &s, &co);
+    s_m[0][t] = (one * co + zer * s) + px * zer;
+    s_m[1][t] = (one * (-s) + zer * co) + px * zer;
+    s_m[2][t] = (one * zer + zer * zer) + px * one;
+    s_m[3][t] = (zer * co + one * s) + py * zer;
+    s_m[4][t] = (zer * (-s) + one * co) + py * zer;
+    s_m[5][t] = (zer * zer + one * zer) + py * one;
+    s_voff[t] = c.vert_off[g];
+    if (t == nbk - 1) s_voff[nbk] = c.vert_off[g + 1];
+    double chk = px + py + th + c.vx[g] + c.vy[g] + c.w[g];
+    if (!(chk - chk == 0.0)) atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_NONFINITE);
   }
-  double bminx = vx[0], bmaxx = vx[0], bminy = vy[0], bmaxy = vy[0];
-#pragma unroll
-  for (int k = 0; k < SHP_MAXV; ++k) {
-    if (k < n) {
-      int k1 = (k < n - 1) ? k + 1 : 0;
-      double ex = 0.0, ey = 0.0;
-#pragma unroll
-      for (int q = 0; q < SHP_MAXV; ++q)
-        if (q == k1) {
-          ex = vx[q];
-          ey = vy[q];
-        }
-      d2 nn = normalize(clockwise(sub(mk(ex, ey), mk(vx[k], vy[k]))));
-      c.wnx[o + k] = nn.x;
-      c.wny[o + k] = nn.y;
-      if (k > 0) {
-        bminx = bminx < vx[k] ? bminx : vx[k];
-        bmaxx = bmaxx > vx[k] ? bmaxx : vx[k];
-        bminy = bminy < vy[k] ? bminy : vy[k];
-        bmaxy = bmaxy > vy[k] ? bmaxy : vy[k];
-      }
+  __syncthreads();
+  const int v0 = s_voff[0], nv = s_voff[nbk] - v0;
+  if (t < nbk)
+    for (int i = s_voff[t] - v0; i < s_voff[t + 1] - v0; ++i) s_body[i] = (uint8_t)t;
+  __syncthreads();
+  for (int i = t; i < nv; i += INTEGRATE_THREADS) {
+    const int b = s_body[i];
+    const int last = s_voff[b + 1] - v0 - 1;
+    const int i1 = i < last ? i + 1 : s_voff[b] - v0;
+    const double m00 = s_m[0][b], m01 = s_m[1][b], m02 = s_m[2][b];
+    const double m10 = s_m[3][b], m11 = s_m[4][b], m12 = s_m[5][b];
+    const double lx = c.lx[v0 + i], ly = c.ly[v0 + i];
+    const double lx1 = c.lx[v0 + i1], ly1 = c.ly[v0 + i1];
+    const double x = (m00 * lx + m01 * ly) + m02 * one;
+    const double y = (m10 * lx + m11 * ly) + m12 * one;
+    const double x1 = (m00 * lx1 + m01 * ly1) + m02 * one;
+    const double y1 = (m10 * lx1 + m11 * ly1) + m12 * one;
+    c.wx[v0 + i] = x;
+    c.wy[v0 + i] = y;
+    s_wx[i] = x;
+    s_wy[i] = y;
+    d2 nn = normalize(clockwise(sub(mk(x1, y1), mk(x, y))));
+    c.wnx[v0 + i] = nn.x;
+    c.wny[v0 + i] = nn.y;
+  }
+  __syncthreads();
+  if (t < nbk) {
+    const int o = s_voff[t] - v0, e = s_voff[t + 1] - v0;
+    double bminx = s_wx[o], bmaxx = s_wx[o], bminy = s_wy[o], bmaxy = s_wy[o];
+    for (int i = o + 1; i < e; ++i) {
+      const double x = s_wx[i], y = s_wy[i];
+      bminx = bminx < x ? bminx : x;
+      bmaxx = bmaxx > x ? bmaxx : x;
+      bminy = bminy < y ? bminy : y;
+      bmaxy = bmaxy > y ? bmaxy : y;
     }
+    c.bminx[g] = bminx;
+    c.bmaxx[g] = bmaxx;
+    c.bminy[g] = bminy;
+    c.bmaxy[g] = bmaxy;
   }
-  c.bminx[g] = bminx;
-  c.bmaxx[g] = bmaxx;
-  c.bminy[g] = bminy;
-  c.bmaxy[g] = bmaxy;
-  double chk = px + py + th + c.vx[g] + c.vy[g] + c.w[g];
-  if (!(chk - chk == 0.0)) atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_NONFINITE);
 }
 
 // Broadphase: one wavefront per body i scans j = i-1 .. 0 of its world in
@@ -503,7 +522,7 @@ static shp_status enqueue_step(shp_batch* b) {
   mark(b, 10);
   k_cache_store<<<gs_blocks(std::max(c.contact_cap, c.W + 1)), 256, 0, s>>>(c);
   mark(b, 11);
-  k_integrate<<<blocks(c.NB, 256), 256, 0, s>>>(c, 1);
+  k_integrate<<<blocks(c.NB, INTEGRATE_THREADS), INTEGRATE_THREADS, 0, s>>>(c, 1);
   mark(b, 12);
   HIPCHK(hipGetLastError());
   return SHP_OK;
@@ -778,7 +797,8 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
     TRY(dalloc(b, &c.bucket_cnt, tb_total + 1));
     TRY(dalloc(b, &c.bucket_start, tb_total + 1));
     TRY(dalloc(b, &c.bucket_fill, tb_total));
-    TRY(dalloc(b, &c.bucket_items, NB));
+    TRY(dalloc(b, &c.grid_rec, 6L * NB));
+    TRY(dalloc(b, &c.grid_slots, (size_t)GRID_SLOTS * NB));
     TRY(dalloc(b, &c.large_off, n_worlds + 1));
     HIPCHK(hipMemcpy(c.large_off, large_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
     TRY(dalloc(b, &c.large_ids, std::max<size_t>(large_ids.size(), 1)));
@@ -807,7 +827,7 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
   k_fill_i32<<<1, 64, 0, s>>>((int32_t*)c.log_count, 0, 2);
   // makeWorldObj places each shape with moveShape (World.hs:169-176)
   k_init_hulls<<<blocks(NB, 256), 256, 0, s>>>(c);
-  k_integrate<<<blocks(NB, 256), 256, 0, s>>>(c, 0);
+  k_integrate<<<blocks(NB, INTEGRATE_THREADS), INTEGRATE_THREADS, 0, s>>>(c, 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
   return SHP_OK;

@@ -89,20 +89,38 @@ struct Overlap {
   double depth;
 };
 
+// A hull's world vertices held in registers (the projections below read
+// every vertex of the other hull once per edge).
+struct HullV {
+  double x[SHP_MAXV], y[SHP_MAXV];
+  int n;
+};
+
+__device__ __forceinline__ void load_hull_v(const HullRef& h, HullV& r) {
+  r.n = h.n;
+#pragma unroll
+  for (int k = 0; k < SHP_MAXV; ++k)
+    if (k < h.n) {
+      r.x[k] = h.wx[k];
+      r.y[k] = h.wy[k];
+    }
+}
+
 // overlap + minOverlap fold (Contact/SAT.hs:103-138; extentAlong',
 // ConvexHull.hs:81-92): first separating edge wins; otherwise the first
 // strictly smallest depth.  Returns false if separated.
-__device__ __forceinline__ bool min_overlap(const HullRef& se, const HullRef& sp,
-                                            Overlap* out) {
+__device__ __forceinline__ bool min_overlap(const HullRef& se, const HullV& sp, Overlap* out) {
   Overlap best{0, 0, 0.0};
   for (int e = 0; e < se.n; ++e) {
     d2 dir = se.nrm(e);
     double smin = dot(se.v(se.emin[e]), dir);
     double smax = dot(se.v(se.emax[e]), dir);
-    double pmin = dot(sp.v(0), dir), pmax = pmin;
+    double pmin = dot(mk(sp.x[0], sp.y[0]), dir), pmax = pmin;
     int imin = 0;
-    for (int k = 1; k < sp.n; ++k) {
-      double d = dot(sp.v(k), dir);
+#pragma unroll
+    for (int k = 1; k < SHP_MAXV; ++k) {
+      if (k >= sp.n) break;
+      double d = dot(mk(sp.x[k], sp.y[k]), dir);
       if (d < pmin) {
         pmin = d;
         imin = k;
@@ -196,8 +214,11 @@ __device__ __forceinline__ int clip_contacts(const HullRef& se, const HullRef& s
 __device__ __forceinline__ int hull_contacts(const HullRef& A, const HullRef& B, Cgeom* out,
                                              int* flip_out) {
   Overlap oab, oba;
-  if (!min_overlap(A, B, &oab)) return 0;
-  if (!min_overlap(B, A, &oba)) return 0;
+  HullV va, vb;
+  load_hull_v(A, va);
+  load_hull_v(B, vb);
+  if (!min_overlap(A, vb, &oab)) return 0;
+  if (!min_overlap(B, va, &oba)) return 0;
   if (oab.depth < oba.depth) {
     *flip_out = 0;
     return clip_contacts(A, B, oab, false, out);

@@ -11,7 +11,9 @@
 // margin), fixed at creation.  Static bodies larger than that ("large",
 // e.g. floors) are listed per world and tested explicitly.  A small body is
 // binned by the cell of its AABB's min corner into a per-world hash table
-// (counting sort, per-step).  Any small j overlapping small i has its min
+// (counting sort, per-step), which stores a 48-byte record per body in
+// bucket order (AABB, cell, index, static flag), so a bucket's candidates
+// are read contiguously.  Any small j overlapping small i has its min
 // corner in cells [cell(min_i) - 1, cell(max_i)] on each axis (at most 3x3
 // cells), because max_j >= min_i and extent_j < h.  A large body i scans
 // every j < i of its world.
@@ -73,21 +75,39 @@ __global__ void k_grid_assign(Ctx c) {
   atomicAdd(&c.bucket_cnt[t0 + cell_hash(cx, cy, mask)], 1);
 }
 
+struct GridRec {
+  double minx, maxx, miny, maxy;
+  int cx, cy, g, st;
+};
+
+__device__ __forceinline__ GridRec load_rec(const Ctx& c, int q) {
+  const double2* p = (const double2*)(c.grid_rec + 6L * q);
+  const double2 x = p[0], y = p[1];
+  const int4 m = *(const int4*)(p + 2);
+  return GridRec{x.x, x.y, y.x, y.y, m.x, m.y, m.z, m.w};
+}
+
 __global__ void k_grid_fill(Ctx c) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= c.NB || c.is_large[g]) return;
   int w = c.body_world[g];
   int t0 = c.tab_off[w];
   uint32_t mask = (uint32_t)(c.tab_off[w + 1] - t0 - 1);
-  int b = t0 + cell_hash(c.body_cell[2 * g], c.body_cell[2 * g + 1], mask);
+  const int cx = c.body_cell[2 * g], cy = c.body_cell[2 * g + 1];
+  int b = t0 + cell_hash(cx, cy, mask);
   int pos = c.bucket_start[b] + atomicAdd(&c.bucket_fill[b], 1);
-  c.bucket_items[pos] = g;
+  double2* p = (double2*)(c.grid_rec + 6L * pos);
+  p[0] = make_double2(c.bminx[g], c.bmaxx[g]);
+  p[1] = make_double2(c.bminy[g], c.bmaxy[g]);
+  *(int4*)(p + 2) = make_int4(cx, cy, g, c.is_static[g] != 0);
 }
 
 static int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 // LANES lanes per body i (1 or 16): count (fill = 0) or emit (fill = 1)
-// its pairs.  With 16, a small body's lanes 0..8 take one cell each of the
+// its pairs.  With one lane, the count pass also keeps the first
+// GRID_SLOTS partners, and the emit pass writes those without a second
+// visit when they are all.  With 16, a small body's lanes 0..8 take one cell each of the
 // <= 3x3 cells that can hold a partner's min corner and lanes 9..15 the
 // world's large bodies, and a large body's lanes stride over every j < i:
 // for batches too small to fill the GPU with one lane per body (the
@@ -129,9 +149,10 @@ __device__ __forceinline__ void grid_visit(const Ctx& c, int g, int sl, F&& hit)
     const int cx = (int)(cx0 + ci % ncx), cy = (int)(cy0 + ci / ncx);
     const int b = t0 + cell_hash(cx, cy, mask);
     for (int q = c.bucket_start[b]; q < c.bucket_start[b + 1]; ++q) {
-      const int gj = c.bucket_items[q];
-      if (gj >= g || c.body_cell[2 * gj] != cx || c.body_cell[2 * gj + 1] != cy) continue;
-      if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) hit(gj);
+      const GridRec r = load_rec(c, q);
+      if (r.g >= g || r.cx != cx || r.cy != cy) continue;
+      if (!(si && r.st) && !(r.minx > amaxx || r.maxx < aminx) && !(r.miny > amaxy || r.maxy < aminy))
+        hit(r.g);
     }
   }
 }
@@ -147,22 +168,52 @@ __global__ void k_grid_pairs(Ctx c, int fill) {
   const int nw = c.body_off[w + 1] - b0;
   const int rank = b0 + (nw - 1 - (g - b0));
   int base = 0, pos = 0, total = 0;
-  if (LANES == 1 && fill) {
-    // one lane: write while visiting (the count pass sized the segment)
-    base = c.off_rank[rank];
-    pos = base;
+  int32_t* slots = c.grid_slots + (long)g * GRID_SLOTS;
+  if (LANES == 1 && !fill) {
+    int count = 0;
     grid_visit<LANES>(c, g, sl, [&](int gj) {
-      if (pos < c.pair_cap) {
-        c.pa[pos] = g;
-        c.pb[pos] = gj;
-      }
-      ++pos;
+      if (count < GRID_SLOTS) slots[count] = gj;
+      ++count;
     });
-    total = pos - base;
-    if (pos > c.pair_cap) {
+    c.cnt_rank[rank] = count;
+    return;
+  }
+  if (LANES == 1) {
+    // one lane: the count pass sized the segment
+    base = c.off_rank[rank];
+    total = c.cnt_rank[rank];
+    if (base + total > c.pair_cap) {
       atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_PAIR_OVERFLOW);
       return;
     }
+    if (total <= GRID_SLOTS) {
+      // the kept partners, in descending order
+      int v[GRID_SLOTS];
+#pragma unroll
+      for (int k = 0; k < GRID_SLOTS; ++k) v[k] = k < total ? slots[k] : -1;
+#pragma unroll
+      for (int a = 1; a < GRID_SLOTS; ++a)
+#pragma unroll
+        for (int q = a; q > 0; --q)
+          if (v[q - 1] < v[q]) {
+            const int t = v[q - 1];
+            v[q - 1] = v[q];
+            v[q] = t;
+          }
+#pragma unroll
+      for (int k = 0; k < GRID_SLOTS; ++k)
+        if (k < total) {
+          c.pa[base + k] = g;
+          c.pb[base + k] = v[k];
+        }
+      return;
+    }
+    pos = base;
+    grid_visit<LANES>(c, g, sl, [&](int gj) {
+      c.pa[pos] = g;
+      c.pb[pos] = gj;
+      ++pos;
+    });
   } else {
     int count = 0;
     grid_visit<LANES>(c, g, sl, [&](int) { ++count; });
