@@ -199,12 +199,14 @@ BROADPHASES = {"auto": engine.BROADPHASE_AUTO, "scan": engine.BROADPHASE_SCAN,
 
 def autotune(scene, solver, ppb, steps=20, warm_steps=10):
     """Pick the schedule by measurement: broadphase (wavefront scan or
-    uniform grid) x lanes per world in the solve (256 or 512).  Each
-    candidate steps the workload from its initial state (warm_steps untimed,
-    then `steps` timed graph replays); the fastest wins."""
+    uniform grid) x lanes per world in the solve (256 or 512, and 1024 for
+    the coloured solver).  Each candidate steps the workload from its
+    initial state (warm_steps untimed, then `steps` timed graph replays);
+    the fastest wins."""
     times = {}
+    lanes = (256, 512, 1024) if solver == engine.SOLVER_COLORED_F32 else (256, 512)
     for name in ("scan", "grid"):
-        for threads in (256, 512):
+        for threads in lanes:
             b = engine.make_batch(scene.worlds, scene.config, solver=solver,
                                   broadphase=BROADPHASES[name], max_pairs_per_body=ppb,
                                   solver_threads=threads)
@@ -327,7 +329,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--broadphase", default="tune", choices=["tune"] + list(BROADPHASES),
                     help="tune: pick broadphase and solver lanes by measurement")
-    ap.add_argument("--solver-threads", type=int, default=0, choices=[0, 256, 512],
+    ap.add_argument("--solver-threads", type=int, default=0, choices=[0, 256, 512, 1024],
                     help="lanes per world in the solve when not tuned (0: by world size)")
     ap.add_argument("--pairs-per-body", type=int, default=0,
                     help="pair capacity per body (0: library default)")
@@ -362,7 +364,10 @@ def main():
     bp = args.broadphase
     threads = args.solver_threads
     if bp == "tune":
-        (bp, threads), tuning = autotune(scene, solver, ppb)
+        # time each candidate in the middle of the run, where the contacts
+        # of a settled scene set the cost, not on the first falling steps
+        (bp, threads), tuning = autotune(scene, solver, ppb, steps=30,
+                                         warm_steps=min(args.steps // 2, 300))
         note(f"autotuned schedule: {bp}, {threads} lanes {tuning}")
 
     extra = {}

@@ -89,8 +89,9 @@ typedef struct shp_config {
   int32_t max_pairs_per_body; /* capacity; 0 = default (8) */
   int32_t broadphase;         /* SHP_BROADPHASE_* */
   int32_t solver_threads;     /* lanes per world in the solve: 0 = by world
-                                 size, or 256 or 512 (a schedule choice; the
-                                 ordered results do not depend on it) */
+                                 size, or 256 or 512, or 1024 for the coloured
+                                 solver (a schedule choice; the results do
+                                 not depend on it) */
 } shp_config;
 
 typedef struct shp_batch shp_batch;

@@ -617,14 +617,14 @@ uint64_t colour_prio(int q, int w) {
 }
 
 const int COLOUR_CAP = 256;
-const int GREEDY_COLOURS = 32;                          // colours from the body masks
+const int GREEDY_COLOURS = 16;                          // colours from the body masks
 const int JP_ROUNDS = COLOUR_CAP - GREEDY_COLOURS - 3;  // rounds before the serial tail
 
 // colour[q] >= 0 for pairs with contacts.  Jones-Plassmann rounds select the
 // manifolds whose priority is the largest among the uncoloured ones on each
 // of their dynamic bodies; a selected manifold takes the smallest colour
-// that no manifold on its dynamic bodies has (colours below 32, tracked per
-// body), else 32 + the round.  The last colour is the serial tail (pair
+// that no manifold on its dynamic bodies has (colours below 16, tracked per
+// body), else 16 + the round.  The last colour is the serial tail (pair
 // order) when JP_ROUNDS rounds were not enough.  Returns the number of
 // colours.
 int colour_pairs(orc_engine* e, std::vector<int>& colour, bool* tail) {

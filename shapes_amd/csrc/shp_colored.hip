@@ -12,7 +12,7 @@
 // hashed priority is the largest among the uncoloured manifolds on each of
 // its dynamic bodies is selected (the selected manifolds share no dynamic
 // body) and takes the smallest colour that no manifold on its dynamic
-// bodies has (a 32-bit mask per body; past 32 colours, colour 32 + round).  Velocities and constraint rows are FP32
+// bodies has (a 16-bit mask per body; past 16 colours, colour 16 + round).  Velocities and constraint rows are FP32
 // (velocities staged in LDS when they fit); the accumulated lagrangians stay
 // FP64; each body's FP64 velocity receives the FP32 change of the solve, so
 // bodies without contacts keep their FP64 velocity exactly.
@@ -36,11 +36,11 @@
 using namespace shp;
 
 #define COLOR_CAP 256
-// colours 0..31 are assigned smallest-free from the per-body masks; a
-// manifold whose bodies already use all 32 takes GREEDY_COLOURS + round,
+// colours 0..15 are assigned smallest-free from the per-body masks; a
+// manifold whose bodies already use all 16 takes GREEDY_COLOURS + round,
 // so the Jones-Plassmann rounds stop at JP_ROUNDS to keep every colour and
-// the serial tail below COLOR_CAP (and below the uint8 sentinels)
-#define GREEDY_COLOURS 32
+// the serial tail below COLOR_CAP (and below the uint8 sentinel 0xFF)
+#define GREEDY_COLOURS 16
 #define JP_ROUNDS (COLOR_CAP - GREEDY_COLOURS - 3)
 #define NO_BODY 0xFFFFu
 #define LDS_MAX_BYTES (160 * 1024 - 512)
@@ -289,17 +289,43 @@ __device__ __forceinline__ void solve_man(const PackF& P, int M, ManF& m, bool w
   if (!(m.flags & MFL_B_STATIC)) V.store(m.gb, v + 3);
 }
 
-// Colours are stored as int (-1 uncoloured, -2 no contacts) in global
-// memory and as uint8 (0xFF uncoloured, 0xFE no contacts) in LDS.
+// Colours per pair, in global memory: -1 uncoloured, -2 no contacts.
 __device__ __forceinline__ int colour_of(int x) { return x; }
-__device__ __forceinline__ int colour_of(uint8_t x) {
-  return x == 0xFFu ? -1 : (x == 0xFEu ? -2 : (int)x);
-}
 
 // Colour of a manifold selected in Jones-Plassmann round r whose dynamic
 // bodies use the colours in `used`.
 __device__ __forceinline__ int greedy_colour(uint32_t used, int r) {
-  return used != 0xFFFFFFFFu ? __builtin_ctz(~used) : GREEDY_COLOURS + r;
+  return used != 0xFFFFu ? __builtin_ctz(~used) : GREEDY_COLOURS + r;
+}
+
+__device__ __forceinline__ int wave_sum(int x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+__device__ __forceinline__ int wave_max(int x) {
+  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o));
+  return x;
+}
+
+// atomicAdd(&ctr[key], 1) for every lane with key >= 0, one atomic per
+// distinct key of the wavefront (lanes of equal key take consecutive
+// values); returns the lane's old value, or -1.  Every lane of the
+// wavefront calls it.
+__device__ __forceinline__ int wave_count(int* ctr, int key) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(key >= 0);
+  int res = -1;
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int k = __shfl(key, leader);
+    const unsigned long long grp = __ballot(key == k) & pending;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&ctr[k], (int)__popcll(grp));
+    base = __shfl(base, leader);
+    if ((grp >> lane) & 1ull) res = base + (int)__popcll(grp & ((1ull << lane) - 1ull));
+    pending &= ~grp;
+  }
+  return res;
 }
 
 // Slots of the coloured manifolds: colour l owns manifold indices
@@ -310,10 +336,12 @@ __device__ __forceinline__ int greedy_colour(uint32_t used, int r) {
 template <int T, class CT>
 __device__ void assign_slots(int p0, int np, const CT* colour, int* scur, int tail,
                              int32_t* slot) {
-  for (int q = threadIdx.x; q < np; q += T) {
-    const int cl = colour_of(colour[q]);
-    if (cl == tail && tail >= 0) continue;
-    slot[q] = cl < 0 ? -1 : p0 + atomicAdd(&scur[cl], 1);
+  for (int q0 = 0; q0 < np; q0 += T) {
+    const int q = q0 + threadIdx.x;
+    const int cl = q < np ? colour_of(colour[q]) : -1;
+    const bool in_tail = cl == tail && tail >= 0;
+    const int m = wave_count(scur, in_tail ? -1 : cl);
+    if (q < np && !in_tail) slot[q] = cl < 0 ? -1 : p0 + m;
   }
   if (tail >= 0 && threadIdx.x == 0) {
     int m = scur[tail];
@@ -330,6 +358,23 @@ __device__ void sweep_colours(const Ctx& c, const PackF& P, int p0, int ncol, bo
                               const int* s_off, const VelF<VLDS>& V) {
   const int tid = threadIdx.x;
   const int phases = (c.iterations + 1) * ncol;
+  if constexpr (T > 512) {
+    // 16 waves leave room for one record per lane: loaded where it is used
+    for (int h = 0; h < phases; ++h) {
+      const int l = h % ncol;
+      const bool warm = h < ncol;
+      const bool tail = serial_tail && l == ncol - 1;
+      const int i0 = tail ? (tid == 0 ? s_off[l] : s_off[l + 1]) : s_off[l] + tid;
+      const int di = tail ? 1 : T;
+      for (int i = i0; i < s_off[l + 1]; i += di) {
+        ManF m;
+        load_man(c, P, p0 + i, m);
+        solve_man<VLDS>(P, p0 + i, m, warm, V);
+      }
+      __syncthreads();
+    }
+    return;
+  }
   auto first = [&](int h, int& M) {
     const int l = h % ncol;
     if (serial_tail && l == ncol - 1) return false;
@@ -407,9 +452,9 @@ __device__ void colour_offsets(const CT* colour, int np, int ncol, int* s_off, i
   const int tid = threadIdx.x;
   for (int i = tid; i <= ncol + 1; i += T) s_cur[i] = 0;
   __syncthreads();
-  for (int q = tid; q < np; q += T) {
-    const int cl = colour_of(colour[q]);
-    if (cl >= 0) atomicAdd(&s_cur[cl], 1);
+  for (int q0 = 0; q0 < np; q0 += T) {
+    const int q = q0 + tid;
+    (void)wave_count(s_cur, q < np ? colour_of(colour[q]) : -1);
   }
   __syncthreads();
   if (tid == 0) {
@@ -425,10 +470,10 @@ __device__ void colour_offsets(const CT* colour, int np, int ncol, int* s_off, i
   __syncthreads();
 }
 
-// Colouring in LDS: local body ids and manifold indices fit 16 and 15 bits
-// and the colouring (8 B per body, 5 B per manifold) fits the pool.
-__device__ __forceinline__ bool colour_fits_lds(int nb, int np, int pool_bytes) {
-  return nb < (int)NO_BODY && np <= 32767 && 8L * nb + 5L * np <= pool_bytes;
+// Colouring in LDS: local body ids and pair indices fit 16 and 15 bits and
+// the colouring (6 B per body, 7 B per manifold) fits the pool.
+__device__ __forceinline__ bool colour_fits_lds(int nb, int np, int nm, int pool_bytes) {
+  return nb < (int)NO_BODY && np <= 32767 && 4L * nb + 2L * ((nb + 1) & ~1) + 7L * nm <= pool_bytes;
 }
 // FP32 velocities in LDS: 12 B per body.
 __device__ __forceinline__ bool vel_fits_lds(int nb, int pool_bytes) { return 12L * nb <= pool_bytes; }
@@ -460,62 +505,82 @@ __device__ void colour_finish(const Ctx& c, int w, int p0, int np, const CT* col
   if (threadIdx.x == 0) c.col_info[w] = ncol | (serial_tail ? 1 << 16 : 0);
 }
 
-// Jones-Plassmann colouring of one world's manifolds in LDS (one workgroup
-// per world whose colouring fits; the others are left to
-// k_colour_global).  LDS pool: s_best and s_used (4 B/body each), per pair
-// its local bodies (4 B) and colour (1 B: 0xFF uncoloured, 0xFE no
-// contacts).
+// Number of manifolds (pairs with stored contacts) of a world, summed over
+// the workgroup; every thread gets it.
 template <int T>
-__global__ __launch_bounds__(T) void k_colour_lds(Ctx c, int pool_bytes) {
-  extern __shared__ int lds[];
-  int* s_off = lds;                      // COLOR_CAP + 3
-  int* s_cur = lds + (COLOR_CAP + 3);    // COLOR_CAP + 3
-  unsigned char* pool = (unsigned char*)(lds + 2 * (COLOR_CAP + 3) + 2);
-  __shared__ int s_left, s_taken, s_top;
-  const int w = blockIdx.x, tid = threadIdx.x;
-  const int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
-  const int p0 = c.wpair_off[w], p1 = c.wpair_off[w + 1];
-  const int np = p1 - p0;
-  const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
-  if (c1 <= c0) {  // nothing to solve: no manifold slots for the packer
-    for (int q = tid; q < np; q += T) c.sched_order[p0 + q] = -1;
-    if (tid == 0) c.col_info[w] = 0;
-    return;
+__device__ int count_manifolds(const Ctx& c, int p0, int np, int c1, int* s_cnt) {
+  if (threadIdx.x == 0) *s_cnt = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int q = threadIdx.x; q < np; q += T) {
+    const int p = p0 + q;
+    mine += c.ccount[p] > 0 && c.coff[p] < c1;  // contacts stored within capacity
   }
-  if (!colour_fits_lds(nb, np, pool_bytes)) return;  // k_colour_global takes this world
-  tmark(c, 0);
+  mine = wave_sum(mine);
+  if ((threadIdx.x & 63) == 0) atomicAdd(s_cnt, mine);
+  __syncthreads();
+  return *s_cnt;
+}
+
+// Jones-Plassmann colouring of world w's manifolds in LDS.  The world's
+// manifolds are first gathered into a list (any order: the colours depend
+// only on the priorities).  LDS pool: s_best (4 B/body), s_used (2 B/body),
+// per manifold its local bodies (4 B), pair index (2 B) and colour (1 B:
+// 0xFF uncoloured), room for `cap` manifolds.  Colours go to sched_level per
+// pair (-2 for pairs without contacts).
+template <int T>
+__device__ void colour_world_lds(const Ctx& c, int w, int p0, int np, int c1, int b0, int nb,
+                                 int cap, unsigned char* pool, int* s_off, int* s_cur, int* sm) {
+  int &s_left = sm[0], &s_taken = sm[1], &s_top = sm[2], &s_fill = sm[3];
+  const int tid = threadIdx.x;
   uint32_t* s_best = (uint32_t*)pool;
-  uint32_t* s_used = s_best + nb;             // nb: colours taken at the body
-  uint32_t* m_ab = s_used + nb;               // np: local a | local b << 16
-  uint8_t* colour = (uint8_t*)(m_ab + np);    // np
+  uint16_t* s_used = (uint16_t*)(s_best + nb);                  // nb: colours taken at the body
+  uint32_t* m_ab = (uint32_t*)(s_used + ((nb + 1) & ~1));       // cap: local a | local b << 16
+  uint16_t* m_q = (uint16_t*)(m_ab + cap);                      // cap: pair index
+  uint8_t* m_col = (uint8_t*)(m_q + cap);                       // cap
+  int32_t* colour = c.sched_level + p0;                         // per pair
   if (tid == 0) {
-    s_left = 0;
     s_top = 0;
+    s_fill = 0;
   }
   for (int i = tid; i < nb; i += T) s_used[i] = 0;
   __syncthreads();
-  int mine = 0;
-  for (int q = tid; q < np; q += T) {
-    const int p = p0 + q;
-    const int ga = c.pa[p], gb = c.pb[p];
-    const uint32_t a = c.is_static[ga] ? NO_BODY : (uint32_t)(ga - b0);
-    const uint32_t b = c.is_static[gb] ? NO_BODY : (uint32_t)(gb - b0);
-    m_ab[q] = a | (b << 16);
-    const bool has = c.ccount[p] > 0 && c.coff[p] < c1;  // contacts stored within capacity
-    colour[q] = has ? 0xFFu : 0xFEu;
-    mine += has;
+  for (int q0 = 0; q0 < np; q0 += T) {
+    const int q = q0 + tid;
+    bool has = false;
+    if (q < np) {
+      const int p = p0 + q;
+      has = c.ccount[p] > 0 && c.coff[p] < c1;
+      if (!has) colour[q] = -2;
+    }
+    const unsigned long long bal = __ballot(has);
+    int base = 0;
+    if ((tid & 63) == 0 && bal) base = atomicAdd(&s_fill, (int)__popcll(bal));
+    base = __shfl(base, 0);
+    if (has) {
+      const int m = base + (int)__popcll(bal & ((1ull << (tid & 63)) - 1ull));
+      const int p = p0 + q;
+      const int ga = c.pa[p], gb = c.pb[p];
+      const uint32_t a = c.is_static[ga] ? NO_BODY : (uint32_t)(ga - b0);
+      const uint32_t b = c.is_static[gb] ? NO_BODY : (uint32_t)(gb - b0);
+      m_ab[m] = a | (b << 16);
+      m_q[m] = (uint16_t)q;
+      m_col[m] = 0xFFu;
+    }
   }
-  atomicAdd(&s_left, mine);
+  __syncthreads();
+  const int nm = s_fill;
+  if (tid == 0) s_left = nm;
   __syncthreads();
   int rounds = 0;
   while (s_left > 0 && rounds < JP_ROUNDS) {
     for (int i = tid; i < nb; i += T) s_best[i] = 0;
     if (tid == 0) s_taken = 0;
     __syncthreads();
-    for (int q = tid; q < np; q += T) {
-      if (colour[q] != 0xFFu) continue;
-      const uint32_t prio = prio32(q, w);
-      const uint32_t ab = m_ab[q], a = ab & 0xFFFF, b = ab >> 16;
+    for (int m = tid; m < nm; m += T) {
+      if (m_col[m] != 0xFFu) continue;
+      const uint32_t prio = prio32(m_q[m], w);
+      const uint32_t ab = m_ab[m], a = ab & 0xFFFF, b = ab >> 16;
       if (a != NO_BODY) atomicMax(&s_best[a], prio);
       if (b != NO_BODY) atomicMax(&s_best[b], prio);
     }
@@ -523,24 +588,28 @@ __global__ __launch_bounds__(T) void k_colour_lds(Ctx c, int pool_bytes) {
     // the selected manifolds share no dynamic body: each updates the masks
     // of its own bodies
     int taken = 0, top = 0;
-    for (int q = tid; q < np; q += T) {
-      if (colour[q] != 0xFFu) continue;
-      const uint32_t prio = prio32(q, w);
-      const uint32_t ab = m_ab[q], a = ab & 0xFFFF, b = ab >> 16;
+    for (int m = tid; m < nm; m += T) {
+      if (m_col[m] != 0xFFu) continue;
+      const uint32_t prio = prio32(m_q[m], w);
+      const uint32_t ab = m_ab[m], a = ab & 0xFFFF, b = ab >> 16;
       if ((a == NO_BODY || s_best[a] == prio) && (b == NO_BODY || s_best[b] == prio)) {
         const uint32_t used = (a != NO_BODY ? s_used[a] : 0u) | (b != NO_BODY ? s_used[b] : 0u);
         const int l = greedy_colour(used, rounds);
-        colour[q] = (uint8_t)l;
+        m_col[m] = (uint8_t)l;
         if (l < GREEDY_COLOURS) {
-          if (a != NO_BODY) s_used[a] = s_used[a] | (1u << l);
-          if (b != NO_BODY) s_used[b] = s_used[b] | (1u << l);
+          if (a != NO_BODY) s_used[a] = (uint16_t)(s_used[a] | (1u << l));
+          if (b != NO_BODY) s_used[b] = (uint16_t)(s_used[b] | (1u << l));
         }
         top = max(top, l + 1);
         ++taken;
       }
     }
-    atomicAdd(&s_taken, taken);
-    atomicMax(&s_top, top);
+    taken = wave_sum(taken);
+    top = wave_max(top);
+    if ((tid & 63) == 0) {
+      atomicAdd(&s_taken, taken);
+      atomicMax(&s_top, top);
+    }
     __syncthreads();
     if (tid == 0) s_left -= s_taken;
     ++rounds;
@@ -548,29 +617,19 @@ __global__ __launch_bounds__(T) void k_colour_lds(Ctx c, int pool_bytes) {
   }
   const bool serial_tail = s_left > 0;
   const int top = s_top;
-  for (int q = tid; q < np; q += T)
-    if (colour[q] == 0xFFu) colour[q] = (uint8_t)top;
+  for (int m = tid; m < nm; m += T) colour[m_q[m]] = m_col[m] == 0xFFu ? top : (int)m_col[m];
   __syncthreads();
   colour_finish<T>(c, w, p0, np, colour, top, serial_tail, s_off, s_cur);
-  tmark(c, 1);
 }
 
 // The same colouring for worlds whose colouring does not fit in LDS:
 // 64-bit priorities and colour masks per body (col_best, col_used) and
 // colours per pair (sched_level) in global memory (L2-resident).
-#define COLOR_THREADS_G 512
-
-__global__ __launch_bounds__(COLOR_THREADS_G) void k_colour_global(Ctx c, int pool_bytes) {
-  __shared__ int s_off[COLOR_CAP + 3];
-  __shared__ int s_cur[COLOR_CAP + 3];
-  __shared__ int s_left, s_taken, s_top;
-  const int w = blockIdx.x, tid = threadIdx.x;
-  const int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
-  if (c1 <= c0) return;
-  const int p0 = c.wpair_off[w], p1 = c.wpair_off[w + 1];
-  const int np = p1 - p0;
-  const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
-  if (colour_fits_lds(nb, np, pool_bytes)) return;
+template <int T>
+__device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int c1, int b0, int nb,
+                                    int* s_off, int* s_cur, int* sm) {
+  int &s_left = sm[0], &s_taken = sm[1], &s_top = sm[2];
+  const int tid = threadIdx.x;
   unsigned long long* best = c.col_best + b0;
   uint32_t* used_at = c.col_used + b0;
   int32_t* colour = c.sched_level + p0;  // per pair: colour, -1 uncoloured, -2 none
@@ -578,23 +637,24 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_colour_global(Ctx c, int po
     s_left = 0;
     s_top = 0;
   }
-  for (int i = tid; i < nb; i += COLOR_THREADS_G) used_at[i] = 0u;
+  for (int i = tid; i < nb; i += T) used_at[i] = 0u;
   __syncthreads();
   int mine = 0;
-  for (int q = tid; q < np; q += COLOR_THREADS_G) {
+  for (int q = tid; q < np; q += T) {
     const int p = p0 + q;
     const bool has = c.ccount[p] > 0 && c.coff[p] < c1;
     colour[q] = has ? -1 : -2;
     mine += has;
   }
-  atomicAdd(&s_left, mine);
+  mine = wave_sum(mine);
+  if ((tid & 63) == 0) atomicAdd(&s_left, mine);
   __syncthreads();
   int rounds = 0;
   while (s_left > 0 && rounds < JP_ROUNDS) {
-    for (int i = tid; i < nb; i += COLOR_THREADS_G) best[i] = 0ull;
+    for (int i = tid; i < nb; i += T) best[i] = 0ull;
     if (tid == 0) s_taken = 0;
     __syncthreads();
-    for (int q = tid; q < np; q += COLOR_THREADS_G) {
+    for (int q = tid; q < np; q += T) {
       if (colour[q] != -1) continue;
       const int p = p0 + q;
       const unsigned long long prio = prio64(q, w);
@@ -604,7 +664,7 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_colour_global(Ctx c, int po
     }
     __syncthreads();
     int taken = 0, top = 0;
-    for (int q = tid; q < np; q += COLOR_THREADS_G) {
+    for (int q = tid; q < np; q += T) {
       if (colour[q] != -1) continue;
       const int p = p0 + q;
       const unsigned long long prio = prio64(q, w);
@@ -622,8 +682,12 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_colour_global(Ctx c, int po
         ++taken;
       }
     }
-    atomicAdd(&s_taken, taken);
-    atomicMax(&s_top, top);
+    taken = wave_sum(taken);
+    top = wave_max(top);
+    if ((tid & 63) == 0) {
+      atomicAdd(&s_taken, taken);
+      atomicMax(&s_top, top);
+    }
     __syncthreads();
     if (tid == 0) s_left -= s_taken;
     ++rounds;
@@ -631,10 +695,45 @@ __global__ __launch_bounds__(COLOR_THREADS_G) void k_colour_global(Ctx c, int po
   }
   const bool serial_tail = s_left > 0;
   const int top = s_top;
-  for (int q = tid; q < np; q += COLOR_THREADS_G)
+  for (int q = tid; q < np; q += T)
     if (colour[q] == -1) colour[q] = top;
   __syncthreads();
-  colour_finish<COLOR_THREADS_G>(c, w, p0, np, colour, top, serial_tail, s_off, s_cur);
+  colour_finish<T>(c, w, p0, np, colour, top, serial_tail, s_off, s_cur);
+}
+
+// Colouring, one workgroup per world, the largest worlds first: in LDS when
+// the world's colouring fits (by its pair count, else by its manifold
+// count), else in global memory.
+template <int T>
+__global__ __launch_bounds__(T) void k_colour(Ctx c, int pool_bytes) {
+  extern __shared__ int lds[];
+  int* s_off = lds;                      // COLOR_CAP + 3
+  int* s_cur = lds + (COLOR_CAP + 3);    // COLOR_CAP + 3
+  unsigned char* pool = (unsigned char*)(lds + 2 * (COLOR_CAP + 3) + 2);
+  __shared__ int sm[4];
+  const int w = c.world_order[blockIdx.x], tid = threadIdx.x;
+  const int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
+  const int p0 = c.wpair_off[w], p1 = c.wpair_off[w + 1];
+  const int np = p1 - p0;
+  const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
+  if (c1 <= c0) {  // nothing to solve: no manifold slots for the packer
+    for (int q = tid; q < np; q += T) c.sched_order[p0 + q] = -1;
+    if (tid == 0) c.col_info[w] = 0;
+    return;
+  }
+  tmark(c, 0);
+  // room for every pair as a manifold, or else for the manifolds counted
+  int cap = np;
+  bool in_lds = colour_fits_lds(nb, np, np, pool_bytes);
+  if (!in_lds) {
+    cap = count_manifolds<T>(c, p0, np, c1, &sm[0]);
+    in_lds = colour_fits_lds(nb, np, cap, pool_bytes);
+  }
+  if (in_lds)
+    colour_world_lds<T>(c, w, p0, np, c1, b0, nb, cap, pool, s_off, s_cur, sm);
+  else
+    colour_world_global<T>(c, w, p0, np, c1, b0, nb, s_off, s_cur, sm);
+  tmark(c, 1);
 }
 
 // Every coloured pair's <= 2 contacts become its manifold record, over the
@@ -652,25 +751,18 @@ __global__ void k_pack_manifolds(Ctx c) {
   }
 }
 
-// Warm-start pass, sweeps and write-back of one world per workgroup, with
-// FP32 velocities in LDS (VLDS, worlds whose velocities fit) or in global
-// memory (the others).
+// Warm-start pass, sweeps and write-back of world w, with FP32 velocities
+// in LDS (VLDS) or in global memory.
 template <bool VLDS, int T>
-__global__ __launch_bounds__(T) void k_sweep_colours(Ctx c, int pool_bytes) {
-  extern __shared__ int lds[];
-  __shared__ int s_off[COLOR_CAP + 3];
-  const int w = blockIdx.x, tid = threadIdx.x;
-  const int info = c.col_info[w];
-  const int ncol = info & 0xFFFF;
-  if (ncol == 0) return;
-  const bool serial_tail = (info >> 16) & 1;
+__device__ void sweep_world(const Ctx& c, int w, int ncol, bool serial_tail, float* vlds,
+                            int* s_off) {
+  const int tid = threadIdx.x;
   const int p0 = c.wpair_off[w];
   const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
-  if (vel_fits_lds(nb, pool_bytes) != VLDS) return;
   const int* off = c.col_off + (long)w * (COLOR_CAP + 3);
   for (int l = tid; l <= ncol; l += T) s_off[l] = off[l];
   VelF<VLDS> V;
-  V.lds = VLDS ? (float*)lds : nullptr;
+  V.lds = vlds;
   V.c = &c;
   V.b0 = b0;
   for (int i = tid; i < nb; i += T) {
@@ -696,6 +788,24 @@ __global__ __launch_bounds__(T) void k_sweep_colours(Ctx c, int pool_bytes) {
   tmark(c, 5);
 }
 
+// One workgroup per world, the largest worlds first; velocities in LDS
+// when they fit.
+template <int T>
+__global__ __launch_bounds__(T) void k_sweep_colours(Ctx c, int pool_bytes) {
+  extern __shared__ int lds[];
+  __shared__ int s_off[COLOR_CAP + 3];
+  const int w = c.world_order[blockIdx.x];
+  const int info = c.col_info[w];
+  const int ncol = info & 0xFFFF;
+  if (ncol == 0) return;
+  const bool serial_tail = (info >> 16) & 1;
+  const int nb = c.body_off[w + 1] - c.body_off[w];
+  if (vel_fits_lds(nb, pool_bytes))
+    sweep_world<true, T>(c, w, ncol, serial_tail, (float*)lds, s_off);
+  else
+    sweep_world<false, T>(c, w, ncol, serial_tail, nullptr, s_off);
+}
+
 shp_status colored_alloc(shp_batch* b) {
   Ctx& c = b->ctx;
   static_assert((CF_LAMBDA + 4) * 2 + MF_FIELDS <= 2 * 29 * 2, "records exceed the packed buffer");
@@ -715,7 +825,7 @@ shp_status colored_alloc(shp_batch* b) {
     // pairs per world are unknown before a step: budget 8 per body (a world
     // that needs more this step is coloured in global memory)
     long np = std::min<long>(32767, 8 * nb + 64);
-    need = std::max(need, std::max(8 * nb + 5 * np, 12 * nb));
+    need = std::max(need, std::max(6 * nb + 2 + 7 * np, 12 * nb));
   }
   size_t fixed = (2 * (COLOR_CAP + 3) + 2) * 4;
   b->color_lds = std::min<size_t>(fixed + need, LDS_MAX_BYTES);
@@ -723,10 +833,11 @@ shp_status colored_alloc(shp_batch* b) {
   b->color_threads = b->cfg.solver_threads ? b->cfg.solver_threads : (max_nb > 2048 ? 512 : 256);
   const size_t vel_lds = b->color_lds - fixed;
   // colouring is light on registers: a large world colours with 16 waves
-  const void* fns[] = {b->color_threads == 512 ? (const void*)k_colour_lds<1024>
-                                                : (const void*)k_colour_lds<256>,
-                       b->color_threads == 512 ? (const void*)k_sweep_colours<true, 512>
-                                                : (const void*)k_sweep_colours<true, 256>};
+  const int t = b->color_threads;
+  const void* fns[] = {t >= 512 ? (const void*)k_colour<1024> : (const void*)k_colour<256>,
+                       t == 1024  ? (const void*)k_sweep_colours<1024>
+                       : t == 512 ? (const void*)k_sweep_colours<512>
+                                  : (const void*)k_sweep_colours<256>};
   const size_t ldsz[] = {b->color_lds, vel_lds};
   for (int k = 0; k < 2; ++k)
     if (hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsz[k]) !=
@@ -739,20 +850,18 @@ void launch_colored_solver(shp_batch* b) {
   const Ctx& c = b->ctx;
   const int pool = (int)(b->color_lds - (2 * (COLOR_CAP + 3) + 2) * 4);
   const size_t vel_lds = (size_t)pool;
-  if (b->color_threads == 512)
-    k_colour_lds<1024><<<c.W, 1024, b->color_lds, b->stream>>>(c, pool);
+  if (b->color_threads >= 512)
+    k_colour<1024><<<c.W, 1024, b->color_lds, b->stream>>>(c, pool);
   else
-    k_colour_lds<256><<<c.W, 256, b->color_lds, b->stream>>>(c, pool);
-  // worlds whose colouring does not fit in LDS (by body count, or by this
-  // step's pair count); it exits at once for the others
-  k_colour_global<<<c.W, COLOR_THREADS_G, 0, b->stream>>>(c, pool);
+    k_colour<256><<<c.W, 256, b->color_lds, b->stream>>>(c, pool);
   const long tp = c.pair_cap;
   k_pack_manifolds<<<std::max(1L, std::min((tp + 255) / 256, 4096L)), 256, 0, b->stream>>>(c);
-  if (b->color_threads == 512)
-    k_sweep_colours<true, 512><<<c.W, 512, vel_lds, b->stream>>>(c, pool);
+  if (b->color_threads == 1024)
+    k_sweep_colours<1024><<<c.W, 1024, vel_lds, b->stream>>>(c, pool);
+  else if (b->color_threads == 512)
+    k_sweep_colours<512><<<c.W, 512, vel_lds, b->stream>>>(c, pool);
   else
-    k_sweep_colours<true, 256><<<c.W, 256, vel_lds, b->stream>>>(c, pool);
-  k_sweep_colours<false, COLOR_THREADS_G><<<c.W, COLOR_THREADS_G, 0, b->stream>>>(c, pool);
+    k_sweep_colours<256><<<c.W, 256, vel_lds, b->stream>>>(c, pool);
 }
 
 // ---- solve quality -----------------------------------------------------------

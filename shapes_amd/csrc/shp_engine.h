@@ -55,6 +55,7 @@ struct Ctx {
   uint8_t* is_static;
   int32_t* body_world;
   int32_t* body_off;  // W + 1
+  int32_t* world_order;  // W: worlds by descending body count (per-world workgroups)
   // hulls (NV; vert_off NB + 1)
   int32_t* vert_off;
   double *lx, *ly;

@@ -571,8 +571,9 @@ shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, cons
     return fail(SHP_E_INVAL, "unknown warm-start mode");
   if (cfg->broadphase < SHP_BROADPHASE_AUTO || cfg->broadphase > SHP_BROADPHASE_GRID)
     return fail(SHP_E_INVAL, "unknown broadphase mode");
-  if (cfg->solver_threads != 0 && cfg->solver_threads != 256 && cfg->solver_threads != 512)
-    return fail(SHP_E_INVAL, "solver_threads must be 0, 256 or 512");
+  if (cfg->solver_threads != 0 && cfg->solver_threads != 256 && cfg->solver_threads != 512 &&
+      !(cfg->solver_threads == 1024 && cfg->solver == SHP_SOLVER_COLORED_F32))
+    return fail(SHP_E_INVAL, "solver_threads must be 0, 256 or 512 (or 1024, coloured solver)");
   long NB = 0, NV = 0;
   for (int w = 0; w < n_worlds; ++w) {
     const shp_world_desc& d = worlds[w];
@@ -688,6 +689,15 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
   HIPCHK(hipMemcpy(c.vert_off, vert_off.data(), (NB + 1) * 4, hipMemcpyHostToDevice));
   TRY(dalloc(b, &c.body_off, n_worlds + 1));
   HIPCHK(hipMemcpy(c.body_off, body_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
+  // per-world solver workgroups take the largest worlds first, so the
+  // longest solves start at once instead of trailing the batch
+  std::vector<int32_t> order(n_worlds);
+  for (int w = 0; w < n_worlds; ++w) order[w] = w;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+    return body_off[x + 1] - body_off[x] > body_off[y + 1] - body_off[y];
+  });
+  TRY(dalloc(b, &c.world_order, n_worlds));
+  HIPCHK(hipMemcpy(c.world_order, order.data(), n_worlds * 4, hipMemcpyHostToDevice));
   TRY(dalloc(b, &c.lx, NV));
   TRY(dalloc(b, &c.ly, NV));
   HIPCHK(hipMemcpy(c.lx, lx.data(), NV * 8, hipMemcpyHostToDevice));

@@ -568,7 +568,7 @@ __global__ __launch_bounds__(T) void k_solve_flow(Ctx c) {
   __shared__ int wsum[T / 64];
   __shared__ int progress;
   __shared__ int s_maxl;
-  const int w = blockIdx.x;
+  const int w = c.world_order[blockIdx.x];
   const int tid = threadIdx.x;
   const int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
   if (c1 <= c0) return;

@@ -101,10 +101,11 @@ def test_colored_matches_restatement_batch(orc):
     _colored_vs_restatement(orc, worlds, scenes.SceneConfig(iterations=6), steps=60, every=10)
 
 
-@pytest.mark.parametrize("threads", [256, 512])
+@pytest.mark.parametrize("threads", [256, 512, 1024])
 def test_colored_matches_restatement_polygon_pile(orc, threads):
-    """The 8192-polygon world (LDS velocities), with 4 and 8 waves: colours
-    wider than the lanes, so lanes take several manifolds of a colour."""
+    """The 8192-polygon world (LDS velocities), with 4, 8 and 16 waves:
+    colours wider than the lanes, so lanes take several manifolds of a
+    colour."""
     sc = scenes.polygon_pile()
     _colored_vs_restatement(orc, sc.worlds, sc.config, steps=40, every=2, threads=threads)
 
