@@ -61,6 +61,18 @@ def make_scene(name: str, worlds: int = 0):
 
 # --------------------------------------------------------------- roofline
 
+# the kernels a phase launches (its HIP-event time covers them all)
+PHASE_KERNELS = {
+    ("solve", "colored"): ["k_colour", "k_pack_manifolds", "k_sweep_colours"],
+    ("solve", "ordered"): ["k_solve_flow"],
+    ("sat", None): ["k_sat"],
+    ("contacts", None): ["k_contacts"],
+    ("integrate", None): ["k_integrate"],
+    ("broadphase_count", None): ["k_grid_assign", "k_grid_fill", "k_grid_pairs", "k_broadphase"],
+    ("broadphase_fill", None): ["k_grid_pairs", "k_broadphase"],
+}
+
+
 def kernel_bytes(phase: str, st: dict, nb: int, nv: int, word: int) -> float:
     """Algorithmic (compulsory) HBM bytes of one launch of a phase, from
     the step's counts: nb bodies, nv hull vertices, st['pairs'],
@@ -154,6 +166,7 @@ def roofline(scene, solver_name, steps, broadphase="auto", ppb=0, warm=engine.WA
     kname = f"solve_{solver_name}" if dom == "solve" else dom
     return {
         "kernel": dom,
+        "kernels": PHASE_KERNELS.get((dom, solver_name), PHASE_KERNELS.get((dom, None), [])),
         "bound": "hbm",
         "achieved": round(achieved, 3),
         "peak": HBM_PEAK_GBS,
@@ -163,6 +176,20 @@ def roofline(scene, solver_name, steps, broadphase="auto", ppb=0, warm=engine.WA
         "algorithmic_bytes_per_launch": round(by),
         "avg_launch_ms": round(ms, 5),
     }, {**{p: round(t[p], 5) for p in phases}, "solve_profile_us": t["solve_profile_us"]}, st
+
+
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain device-to-device copy of a known size: what FETCH_SIZE and
+    WRITE_SIZE read per dispatch for it gives their scale to bytes."""
+    import torch
+    x = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    x.fill_(1)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        y = x.clone()
+        del y
+    torch.cuda.synchronize()
+    print(f"copied {nbytes} bytes x{reps}", flush=True)
 
 
 # ------------------------------------------------------------ CPU baseline
@@ -335,7 +362,14 @@ def main():
                     help="pair capacity per body (0: library default)")
     ap.add_argument("--worlds", type=int, default=0,
                     help="batch workloads: number of worlds (default: the config's)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device three times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build steps worlds on one GPU only; sharding worlds "

@@ -132,7 +132,8 @@ struct Ctx {
   uint8_t* is_large;
   int32_t* body_cell;
   int32_t* tab_off;
-  int32_t *bucket_cnt, *bucket_start, *bucket_fill;
+  int32_t *bucket_cnt, *bucket_start;
+  int32_t* body_slot;  // per binned body: its rank within its bucket
   // per binned body in bucket order, 48 B: AABB (min x, max x, min y,
   // max y), cell (x, y), body index, static flag
   double* grid_rec;

@@ -306,6 +306,7 @@ __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
   d2 xa = mk(c.px[ga], c.py[ga]), xb = mk(c.px[gb], c.py[gb]);
   double im[6] = {c.iml[ga], c.iml[ga], c.imr[ga], c.iml[gb], c.iml[gb], c.imr[gb]};
   int cache_lo = c.cache_woff[w], cache_hi = c.cache_woff[w + 1];
+  int c_lo = c.wcont_off[w], c_hi = c.wcont_off[w + 1];
   for (int k = 0; k < n; ++k) {
     int dst = c.coff[p] + k;
     if (dst >= c.contact_cap) continue;
@@ -356,18 +357,47 @@ __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
     c.rby[dst] = rb.y;
     c.mcn[dst] = eff_mass(jn, im);
     c.mct[dst] = eff_mass(jt, im);
-    // warm start lookup in the descending cache segment
-    int lo = cache_lo, hi = cache_hi;  // search [lo, hi)
+    // warm start lookup in the descending cache segment [cache_lo,
+    // cache_hi): the first entry <= key, then equality.  This frame's
+    // contacts come in the same order and mostly persist, so the search
+    // starts at the contact's proportional position and gallops outwards.
     int found = -1;
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      uint64_t km = c.cache_key[mid];
-      if (km == key) {
-        found = mid;
-        break;
+    if (cache_hi > cache_lo) {
+      const int nc = cache_hi - cache_lo, nw = c_hi - c_lo;
+      int g = cache_lo + (int)((long)(dst - c_lo) * nc / (nw > 0 ? nw : 1));
+      g = g < cache_lo ? cache_lo : (g >= cache_hi ? cache_hi - 1 : g);
+      int L, R;  // the first entry <= key lies in [L, R] (R = cache_hi: none)
+      if (c.cache_key[g] > key) {
+        L = g + 1;
+        R = cache_hi;
+        for (int step = 1;; step <<= 1) {
+          const int t = g + step;
+          if (t >= cache_hi) break;
+          if (c.cache_key[t] <= key) {
+            R = t;
+            break;
+          }
+          L = t + 1;
+        }
+      } else {
+        L = cache_lo;
+        R = g;
+        for (int step = 1;; step <<= 1) {
+          const int t = g - step;
+          if (t < cache_lo) break;
+          if (c.cache_key[t] > key) {
+            L = t + 1;
+            break;
+          }
+          R = t;
+        }
       }
-      if (km > key) lo = mid + 1;
-      else hi = mid;
+      while (L < R) {
+        const int mid = (L + R) >> 1;
+        if (c.cache_key[mid] > key) L = mid + 1;
+        else R = mid;
+      }
+      if (R < cache_hi && c.cache_key[R] == key) found = R;
     }
     if (found >= 0) {
       c.ln[dst] = c.cache_ln[found];
@@ -806,7 +836,7 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
     HIPCHK(hipMemcpy(c.tab_off, tab_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
     TRY(dalloc(b, &c.bucket_cnt, tb_total + 1));
     TRY(dalloc(b, &c.bucket_start, tb_total + 1));
-    TRY(dalloc(b, &c.bucket_fill, tb_total));
+    TRY(dalloc(b, &c.body_slot, NB));
     TRY(dalloc(b, &c.grid_rec, 6L * NB));
     TRY(dalloc(b, &c.grid_slots, (size_t)GRID_SLOTS * NB));
     TRY(dalloc(b, &c.large_off, n_worlds + 1));

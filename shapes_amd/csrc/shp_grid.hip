@@ -72,7 +72,7 @@ __global__ void k_grid_assign(Ctx c) {
   c.body_cell[2 * g + 1] = cy;
   int t0 = c.tab_off[w];
   uint32_t mask = (uint32_t)(c.tab_off[w + 1] - t0 - 1);
-  atomicAdd(&c.bucket_cnt[t0 + cell_hash(cx, cy, mask)], 1);
+  c.body_slot[g] = atomicAdd(&c.bucket_cnt[t0 + cell_hash(cx, cy, mask)], 1);
 }
 
 struct GridRec {
@@ -95,7 +95,7 @@ __global__ void k_grid_fill(Ctx c) {
   uint32_t mask = (uint32_t)(c.tab_off[w + 1] - t0 - 1);
   const int cx = c.body_cell[2 * g], cy = c.body_cell[2 * g + 1];
   int b = t0 + cell_hash(cx, cy, mask);
-  int pos = c.bucket_start[b] + atomicAdd(&c.bucket_fill[b], 1);
+  int pos = c.bucket_start[b] + c.body_slot[g];
   double2* p = (double2*)(c.grid_rec + 6L * pos);
   p[0] = make_double2(c.bminx[g], c.bmaxx[g]);
   p[1] = make_double2(c.bminy[g], c.bmaxy[g]);
@@ -278,7 +278,6 @@ hipError_t launch_grid_count(shp_batch* b) {
   Ctx& c = b->ctx;
   hipStream_t s = b->stream;
   k_zero_i32<<<nblk(c.grid_buckets + 1, 256), 256, 0, s>>>(c.bucket_cnt, c.grid_buckets + 1);
-  k_zero_i32<<<nblk(c.grid_buckets, 256), 256, 0, s>>>(c.bucket_fill, c.grid_buckets);
   k_grid_assign<<<nblk(c.NB, 256), 256, 0, s>>>(c);
   size_t tb = b->scan_tmp_bytes;
   hipError_t e = hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, c.bucket_cnt, c.bucket_start,
