@@ -846,14 +846,19 @@ shp_status colored_alloc(shp_batch* b) {
   return SHP_OK;
 }
 
-void launch_colored_solver(shp_batch* b) {
+void launch_colouring(shp_batch* b, hipStream_t s) {
+  const Ctx& c = b->ctx;
+  const int pool = (int)(b->color_lds - (2 * (COLOR_CAP + 3) + 2) * 4);
+  if (b->color_threads >= 512)
+    k_colour<1024><<<c.W, 1024, b->color_lds, s>>>(c, pool);
+  else
+    k_colour<256><<<c.W, 256, b->color_lds, s>>>(c, pool);
+}
+
+void launch_colored_sweeps(shp_batch* b) {
   const Ctx& c = b->ctx;
   const int pool = (int)(b->color_lds - (2 * (COLOR_CAP + 3) + 2) * 4);
   const size_t vel_lds = (size_t)pool;
-  if (b->color_threads >= 512)
-    k_colour<1024><<<c.W, 1024, b->color_lds, b->stream>>>(c, pool);
-  else
-    k_colour<256><<<c.W, 256, b->color_lds, b->stream>>>(c, pool);
   const long tp = c.pair_cap;
   k_pack_manifolds<<<std::max(1L, std::min((tp + 255) / 256, 4096L)), 256, 0, b->stream>>>(c);
   if (b->color_threads == 1024)

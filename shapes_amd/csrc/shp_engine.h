@@ -152,6 +152,10 @@ struct shp_batch {
   std::vector<int32_t> h_body_off, h_vert_off;
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
+  // the coloured solver's colouring runs on side_stream, concurrently with
+  // the contact generation on `stream` (fork and join through the events)
+  hipStream_t side_stream = nullptr;
+  hipEvent_t fork_event = nullptr, join_event = nullptr;
   hipGraphExec_t graph_exec{};
   bool graph_made = false;   // graph_exec holds an instantiated graph
   bool graph_ready = false;  // ... and it matches the current kernel arguments
@@ -181,9 +185,11 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
                        long pair_cap);
 shp_status solver_alloc(shp_batch* b);
 shp_status colored_alloc(shp_batch* b);
-void launch_solver(shp_batch* b);
 void launch_ordered_solver(shp_batch* b);
-void launch_colored_solver(shp_batch* b);
+// coloured solver: colouring (needs pairs and contact counts only), then,
+// after the contacts, manifold packing and the sweeps
+void launch_colouring(shp_batch* b, hipStream_t s);
+void launch_colored_sweeps(shp_batch* b);
 void launch_quality(shp_batch* b);
 hipError_t launch_grid_count(shp_batch* b);
 hipError_t launch_grid_emit(shp_batch* b);

@@ -542,10 +542,24 @@ static shp_status enqueue_step(shp_batch* b) {
   mark(b, 6);
   k_world_ranges<<<blocks(c.W + 1, 256), 256, 0, s>>>(c);
   mark(b, 7);
+  // the colouring needs only the pairs and contact counts: it runs beside
+  // the contact generation and the learned warm start
+  const bool fork = c.solver == SHP_SOLVER_COLORED_F32;
+  if (fork) {
+    HIPCHK(hipEventRecord(b->fork_event, s));
+    HIPCHK(hipStreamWaitEvent(b->side_stream, b->fork_event, 0));
+    launch_colouring(b, b->side_stream);
+    HIPCHK(hipEventRecord(b->join_event, b->side_stream));
+  }
   k_contacts<<<gs_blocks(c.pair_cap), 256, 0, s>>>(c);
   if (c.warm_mode == SHP_WARM_LEARNED && c.mlp_ready) mlp_launch_infer(b);
   mark(b, 8);
-  launch_solver(b);
+  if (fork) {
+    HIPCHK(hipStreamWaitEvent(s, b->join_event, 0));
+    launch_colored_sweeps(b);
+  } else {
+    launch_ordered_solver(b);
+  }
   mark(b, 9);
   launch_quality(b);
   if (b->logging) mlp_launch_log(b);
@@ -576,14 +590,6 @@ static shp_status accumulate_timing(shp_batch* b) {
   return SHP_OK;
 }
 
-void launch_solver(shp_batch* b) {
-  Ctx& c = b->ctx;
-  if (c.solver == SHP_SOLVER_ORDERED_F64) {
-    launch_ordered_solver(b);
-  } else {
-    launch_colored_solver(b);
-  }
-}
 
 extern "C" {
 
@@ -638,6 +644,12 @@ shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, cons
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     delete b;
     return fail(SHP_E_HIP, "hipStreamCreate failed");
+  }
+  if (hipStreamCreateWithFlags(&b->side_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork_event, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join_event, hipEventDisableTiming) != hipSuccess) {
+    shp_batch_destroy(b);
+    return fail(SHP_E_HIP, "stream or event creation failed");
   }
   shp_status st = batch_alloc(b, worlds, n_worlds, NB, NV, pair_cap);
   if (st != SHP_OK) {
@@ -918,6 +930,12 @@ shp_status shp_batch_destroy(shp_batch* b) {
     for (int k = 0; k <= SHP_NUM_PHASES; ++k) (void)hipEventDestroy(b->events[k]);
   mlp_params_free(&b->mlp);
   for (void* p : b->allocs) (void)hipFree(p);
+  if (b->side_stream) {
+    (void)hipStreamSynchronize(b->side_stream);
+    (void)hipStreamDestroy(b->side_stream);
+  }
+  if (b->fork_event) (void)hipEventDestroy(b->fork_event);
+  if (b->join_event) (void)hipEventDestroy(b->join_event);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return SHP_OK;
