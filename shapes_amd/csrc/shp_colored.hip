@@ -359,17 +359,42 @@ __device__ void sweep_colours(const Ctx& c, const PackF& P, int p0, int ncol, bo
   const int tid = threadIdx.x;
   const int phases = (c.iterations + 1) * ncol;
   if constexpr (T > 512) {
-    // 16 waves leave room for one record per lane: loaded where it is used
+    // 16 waves leave room for two records per lane: the lane's next
+    // manifold of a colour loads while the current one solves, and its
+    // first manifold of the next colour loads before the barrier (not with
+    // one colour: that is the manifold just solved, whose lagrangians the
+    // pass changes)
+    auto first_of = [&](int h) {
+      const int l = h % ncol;
+      const bool tail = serial_tail && l == ncol - 1;
+      return tail ? (tid == 0 ? s_off[l] : s_off[l + 1]) : s_off[l] + tid;
+    };
+    ManF m, n;
+    bool loaded = false;
     for (int h = 0; h < phases; ++h) {
       const int l = h % ncol;
       const bool warm = h < ncol;
-      const bool tail = serial_tail && l == ncol - 1;
-      const int i0 = tail ? (tid == 0 ? s_off[l] : s_off[l + 1]) : s_off[l] + tid;
-      const int di = tail ? 1 : T;
-      for (int i = i0; i < s_off[l + 1]; i += di) {
-        ManF m;
-        load_man(c, P, p0 + i, m);
+      const int di = serial_tail && l == ncol - 1 ? 1 : T;
+      const int end = s_off[l + 1];
+      int i = first_of(h);
+      if (i < end && !loaded) load_man(c, P, p0 + i, m);
+      // the lane's next manifold of the colour loads while this one solves
+      while (i < end) {
+        const int inext = i + di;
+        const bool more = inext < end;
+        if (more) load_man(c, P, p0 + inext, n);
         solve_man<VLDS>(P, p0 + i, m, warm, V);
+        if (!more) break;
+        m = n;
+        i = inext;
+      }
+      loaded = false;
+      if (ncol > 1 && h + 1 < phases) {
+        const int j = first_of(h + 1);
+        if (j < s_off[(h + 1) % ncol + 1]) {
+          load_man(c, P, p0 + j, m);
+          loaded = true;
+        }
       }
       __syncthreads();
     }
@@ -522,6 +547,82 @@ __device__ int count_manifolds(const Ctx& c, int p0, int np, int c1, int* s_cnt)
   return *s_cnt;
 }
 
+// The rounds of colour_world_lds with each lane's manifolds (m = lane,
+// lane + T, ...) tracked in registers with their priorities: a round reads
+// only the live manifolds' bodies and the bodies' priorities and colour
+// masks, and a lane's coloured manifolds drop out of its later rounds.  Colours go straight to
+// `colour` (global, per pair); s_top / s_left end as in the LDS rounds.
+#define COLOUR_REG_ITEMS 16
+template <int T>
+__device__ void colour_rounds_regs(int w, int nm, int nb, uint32_t* s_best, uint16_t* s_used,
+                                   const uint32_t* m_ab, const uint16_t* m_q, int32_t* colour,
+                                   int* sm) {
+  int &s_left = sm[0], &s_taken = sm[1], &s_top = sm[2];
+  const int tid = threadIdx.x;
+  // pair index q = (prio32 - 1) & 0x7FFF; the bodies stay in LDS (m_ab)
+  uint32_t pr[COLOUR_REG_ITEMS];
+  uint32_t alive = 0;
+#pragma unroll
+  for (int k = 0; k < COLOUR_REG_ITEMS; ++k) {
+    const int m = tid + k * T;
+    pr[k] = 0;
+    if (m < nm) {
+      pr[k] = prio32(m_q[m], w);
+      alive |= 1u << k;
+    }
+  }
+  int rounds = 0;
+  while (s_left > 0 && rounds < JP_ROUNDS) {
+    for (int i = tid; i < nb; i += T) s_best[i] = 0;
+    if (tid == 0) s_taken = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < COLOUR_REG_ITEMS; ++k) {
+      if (!((alive >> k) & 1u)) continue;
+      const uint32_t abk = m_ab[tid + k * T], a = abk & 0xFFFF, b = abk >> 16;
+      if (a != NO_BODY) atomicMax(&s_best[a], pr[k]);
+      if (b != NO_BODY) atomicMax(&s_best[b], pr[k]);
+    }
+    __syncthreads();
+    // the selected manifolds share no dynamic body: each updates the masks
+    // of its own bodies
+    int taken = 0, top = 0;
+#pragma unroll
+    for (int k = 0; k < COLOUR_REG_ITEMS; ++k) {
+      if (!((alive >> k) & 1u)) continue;
+      const uint32_t abk = m_ab[tid + k * T], a = abk & 0xFFFF, b = abk >> 16;
+      if ((a == NO_BODY || s_best[a] == pr[k]) && (b == NO_BODY || s_best[b] == pr[k])) {
+        const uint32_t used = (a != NO_BODY ? s_used[a] : 0u) | (b != NO_BODY ? s_used[b] : 0u);
+        const int l = greedy_colour(used, rounds);
+        colour[(pr[k] - 1u) & 0x7FFFu] = l;
+        if (l < GREEDY_COLOURS) {
+          if (a != NO_BODY) s_used[a] = (uint16_t)(s_used[a] | (1u << l));
+          if (b != NO_BODY) s_used[b] = (uint16_t)(s_used[b] | (1u << l));
+        }
+        top = max(top, l + 1);
+        alive &= ~(1u << k);
+        ++taken;
+      }
+    }
+    taken = wave_sum(taken);
+    top = wave_max(top);
+    if ((tid & 63) == 0) {
+      atomicAdd(&s_taken, taken);
+      atomicMax(&s_top, top);
+    }
+    __syncthreads();
+    if (tid == 0) s_left -= s_taken;
+    ++rounds;
+    __syncthreads();
+  }
+  // manifolds left after JP_ROUNDS form the serial tail colour s_top
+  const int top = s_top;
+#pragma unroll
+  for (int k = 0; k < COLOUR_REG_ITEMS; ++k)
+    if ((alive >> k) & 1u) colour[(pr[k] - 1u) & 0x7FFFu] = top;
+  __syncthreads();
+}
+
 // Jones-Plassmann colouring of world w's manifolds in LDS.  The world's
 // manifolds are first gathered into a list (any order: the colours depend
 // only on the priorities).  LDS pool: s_best (4 B/body), s_used (2 B/body),
@@ -572,6 +673,13 @@ __device__ void colour_world_lds(const Ctx& c, int w, int p0, int np, int c1, in
   const int nm = s_fill;
   if (tid == 0) s_left = nm;
   __syncthreads();
+  if constexpr (T >= 1024) {
+    if (nm <= COLOUR_REG_ITEMS * T) {
+      colour_rounds_regs<T>(w, nm, nb, s_best, s_used, m_ab, m_q, colour, sm);
+      colour_finish<T>(c, w, p0, np, colour, s_top, s_left > 0, s_off, s_cur);
+      return;
+    }
+  }
   int rounds = 0;
   while (s_left > 0 && rounds < JP_ROUNDS) {
     for (int i = tid; i < nb; i += T) s_best[i] = 0;
