@@ -276,6 +276,62 @@ def run_mode(scene, solver, warm, steps, warmup, graph=True, model=None, broadph
     return elapsed, st
 
 
+def final_positions(scene, solver, warm, iterations, steps, model=None, broadphase="auto",
+                    ppb=0, threads=0, worlds=None):
+    """Positions (x, y) of every body of every world after `steps` steps, and
+    the run's quality."""
+    worlds = worlds if worlds is not None else scene.worlds
+    cfg = scenes.SceneConfig(**{**scene.config.__dict__, "iterations": iterations})
+    b = engine.make_batch(worlds, cfg, solver=solver, warm_start=warm,
+                          broadphase=BROADPHASES[broadphase], max_pairs_per_body=ppb,
+                          solver_threads=threads)
+    if warm == engine.WARM_LEARNED:
+        b.set_mlp(model)
+    b.step_async(steps, use_graph=True)
+    b.sync()
+    st = b.stats()
+    pos = [b.bodies(k)[:, :2].copy() for k in range(len(worlds))]
+    b.close()
+    return pos, quality(st)
+
+
+def divergence(pa, pb, worlds):
+    """Distance between two runs' final positions over the dynamic bodies."""
+    d = np.concatenate([np.hypot(a[:, 0] - b[:, 0], a[:, 1] - b[:, 1])[(w.mass != 0) | (w.inertia != 0)]
+                        for a, b, w in zip(pa, pb, worlds)])
+    return {"max": round(float(d.max()), 6), "rms": round(float(np.sqrt((d ** 2).mean())), 6)}
+
+
+def fp64_reference(scene, steps, runs, bp, ppb, model):
+    """The FP64 drop-in solver (the reference's order, bit-exact to the
+    oracle) at the workload's sweeps from the same initial state: its
+    quality, and the divergence of each measured run's final positions from
+    it.  `chaos_floor` is the divergence of the FP64 run itself when every
+    dynamic body's initial x-velocity is moved by 1e-12: what any change of
+    arithmetic leads to in these scenes."""
+    base_it = scene.config.iterations
+    ref, q_ref = final_positions(scene, engine.SOLVER_ORDERED_F64, engine.WARM_ZERO, base_it,
+                                 steps, broadphase=bp, ppb=ppb)
+    moved = []
+    for w in scene.worlds:
+        m = w.copy()
+        dynm = (m.mass != 0) | (m.inertia != 0)
+        m.vx = m.vx + np.where(dynm, 1e-12, 0.0)
+        moved.append(m)
+    pert, _ = final_positions(scene, engine.SOLVER_ORDERED_F64, engine.WARM_ZERO, base_it, steps,
+                              broadphase=bp, ppb=ppb, worlds=moved)
+    out = {"solver": "ordered", "iterations": base_it, "steps": steps, "quality": q_ref,
+           "chaos_floor": divergence(pert, ref, scene.worlds), "runs": []}
+    for warm, it, threads in runs:
+        pos, q = final_positions(scene, engine.SOLVER_COLORED_F32, warm, it, steps, model=model,
+                                 broadphase=bp, ppb=ppb, threads=threads)
+        out["runs"].append({"warm_start": "learned" if warm == engine.WARM_LEARNED else "zero",
+                            "iterations": it, "divergence": divergence(pos, ref, scene.worlds),
+                            "mean_step_max_penetration": q["mean_step_max_penetration"],
+                            "mean_step_normal_residual": q["mean_step_normal_residual"]})
+    return out
+
+
 def quality(st):
     return {"max_penetration": st["run_max_depth"],
             "mean_step_max_penetration": st["mean_max_depth"],
@@ -362,6 +418,10 @@ def main():
                     help="pair capacity per body (0: library default)")
     ap.add_argument("--worlds", type=int, default=0,
                     help="batch workloads: number of worlds (default: the config's)")
+    ap.add_argument("--fp64-reference", action="store_true",
+                    help="single mode, coloured solver: also run the FP64 drop-in solver and "
+                         "report quality and trajectory divergence against it (headline mode "
+                         "always does)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -416,11 +476,21 @@ def main():
                  "quality_tolerance": QUALITY_TOL,
                  "quality_matched": bool(row.get("matches_plain"))}
         st_last = None
+        scene.config.iterations = plain["iterations"]
+        extra["fp64_reference"] = fp64_reference(
+            scene, args.steps, [(engine.WARM_ZERO, plain["iterations"], threads),
+                                (engine.WARM_LEARNED, it, threads)], bp, ppb, model)
+        scene.config.iterations = it
+        note("FP64 reference runs done")
     else:
         elapsed, st_last = run_mode(scene, solver, warm, args.steps, args.warmup,
                                     graph=not args.no_graph, model=model, broadphase=bp,
                                     ppb=ppb, threads=threads)
         q = quality(st_last)
+        if args.fp64_reference and solver == engine.SOLVER_COLORED_F32:
+            extra["fp64_reference"] = fp64_reference(
+                scene, args.steps, [(warm, scene.config.iterations, threads)], bp, ppb, model)
+            note("FP64 reference runs done")
     value = dyn * args.steps / elapsed
     note(f"timed run: {args.steps} steps in {elapsed:.2f} s")
     prof_steps = args.profile_steps or args.steps
