@@ -5,15 +5,20 @@
 //
 //   k_reset          clear per-step counters
 //   k_broadphase(0)  one wavefront per body: gravity (applyExternal) and the
-//                    count of its pairs (i, j<i)
+//                    count of its pairs (i, j<i); or the grid's binning and
+//                    count pass (shp_grid.hip)
 //   scan             exclusive sum of the counts, in descending body order
 //   k_broadphase(1)  same sweep, ballot/popcount compaction -> pairs in
-//                    descending (i, j) order (G.culledKeys)
+//                    descending (i, j) order (G.culledKeys); or the grid's
+//                    emit pass
 //   k_sat            one lane per pair: SAT + clipping -> <= 2 contacts
 //   scan             exclusive sum of contacts per pair
 //   k_world_ranges   per-world pair and contact ranges
-//   k_contacts       compaction, constraintGen, warm-start join
-//   k_solve_*        warm-start application + Gauss-Seidel sweeps
+//   k_contacts       compaction, constraintGen, warm-start join (beside it,
+//                    on a second stream, the coloured solver's k_colour)
+//   solve            warm-start application + Gauss-Seidel sweeps
+//                    (k_solve_flow, or k_pack_manifolds + k_sweep_colours)
+//   k_quality        solve quality (maxima, run sums)
 //   k_cache_store    this frame's (key, lambda) become next frame's cache
 //   k_integrate      advance + moveShapes + AABBs
 //
