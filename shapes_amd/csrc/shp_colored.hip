@@ -43,6 +43,7 @@ using namespace shp;
 #define GREEDY_COLOURS 16
 #define JP_ROUNDS (COLOR_CAP - GREEDY_COLOURS - 3)
 #define NO_BODY 0xFFFFu
+#define NO_PAIR 0xFFFFFFFFu  // pair_ab of a pair without stored contacts
 #define LDS_MAX_BYTES (160 * 1024 - 512)
 
 // per-contact FP32 fields (stride contact_cap), then the FP64 lagrangians,
@@ -537,10 +538,7 @@ __device__ int count_manifolds(const Ctx& c, int p0, int np, int c1, int* s_cnt)
   if (threadIdx.x == 0) *s_cnt = 0;
   __syncthreads();
   int mine = 0;
-  for (int q = threadIdx.x; q < np; q += T) {
-    const int p = p0 + q;
-    mine += c.ccount[p] > 0 && c.coff[p] < c1;  // contacts stored within capacity
-  }
+  for (int q = threadIdx.x; q < np; q += T) mine += c.pair_ab[p0 + q] != NO_PAIR;
   mine = wave_sum(mine);
   if ((threadIdx.x & 63) == 0) atomicAdd(s_cnt, mine);
   __syncthreads();
@@ -548,33 +546,43 @@ __device__ int count_manifolds(const Ctx& c, int p0, int np, int c1, int* s_cnt)
 }
 
 // The rounds of colour_world_lds with each lane's manifolds (m = lane,
-// lane + T, ...) tracked in registers with their priorities: a round reads
-// only the live manifolds' bodies and the bodies' priorities and colour
-// masks, and a lane's coloured manifolds drop out of its later rounds.  Colours go straight to
-// `colour` (global, per pair); s_top / s_left end as in the LDS rounds.
+// lane + T, ...) tracked in registers with their priorities and colours: a
+// round reads only the live manifolds' bodies and the bodies' priorities
+// and colour masks, a lane's coloured manifolds drop out of its later
+// rounds, and the per-colour counts and manifold slots come from the
+// registers.  The taken count is double-buffered, so a round needs three
+// barriers.  Returns false (colours written to `colour` per pair instead)
+// when the rounds stopped at JP_ROUNDS: the caller then places the serial
+// tail in pair order.
 #define COLOUR_REG_ITEMS 16
 template <int T>
-__device__ void colour_rounds_regs(int w, int nm, int nb, uint32_t* s_best, uint16_t* s_used,
-                                   const uint32_t* m_ab, const uint16_t* m_q, int32_t* colour,
-                                   int* sm) {
-  int &s_left = sm[0], &s_taken = sm[1], &s_top = sm[2];
+__device__ bool colour_regs(const Ctx& c, int w, int p0, int nm, int nb, uint32_t* s_best,
+                            uint16_t* s_used, const uint32_t* m_ab, const uint16_t* m_q,
+                            int32_t* colour, int* s_off, int* s_cur, int* sm) {
+  int* s_tk = sm;  // [2]: taken this round, next round
+  int& s_top = sm[2];
   const int tid = threadIdx.x;
   // pair index q = (prio32 - 1) & 0x7FFF; the bodies stay in LDS (m_ab)
   uint32_t pr[COLOUR_REG_ITEMS];
+  uint32_t col[COLOUR_REG_ITEMS / 4];  // colour of manifold k in byte k % 4
   uint32_t alive = 0;
 #pragma unroll
   for (int k = 0; k < COLOUR_REG_ITEMS; ++k) {
     const int m = tid + k * T;
     pr[k] = 0;
+    if (k % 4 == 0) col[k / 4] = 0;
     if (m < nm) {
       pr[k] = prio32(m_q[m], w);
       alive |= 1u << k;
     }
   }
-  int rounds = 0;
-  while (s_left > 0 && rounds < JP_ROUNDS) {
+  if (tid == 0) {
+    s_tk[0] = 0;
+    s_tk[1] = 0;
+  }
+  int left = nm, rounds = 0;
+  while (left > 0 && rounds < JP_ROUNDS) {
     for (int i = tid; i < nb; i += T) s_best[i] = 0;
-    if (tid == 0) s_taken = 0;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < COLOUR_REG_ITEMS; ++k) {
@@ -594,7 +602,7 @@ __device__ void colour_rounds_regs(int w, int nm, int nb, uint32_t* s_best, uint
       if ((a == NO_BODY || s_best[a] == pr[k]) && (b == NO_BODY || s_best[b] == pr[k])) {
         const uint32_t used = (a != NO_BODY ? s_used[a] : 0u) | (b != NO_BODY ? s_used[b] : 0u);
         const int l = greedy_colour(used, rounds);
-        colour[(pr[k] - 1u) & 0x7FFFu] = l;
+        col[k / 4] |= (uint32_t)l << (8 * (k % 4));
         if (l < GREEDY_COLOURS) {
           if (a != NO_BODY) s_used[a] = (uint16_t)(s_used[a] | (1u << l));
           if (b != NO_BODY) s_used[b] = (uint16_t)(s_used[b] | (1u << l));
@@ -607,20 +615,54 @@ __device__ void colour_rounds_regs(int w, int nm, int nb, uint32_t* s_best, uint
     taken = wave_sum(taken);
     top = wave_max(top);
     if ((tid & 63) == 0) {
-      atomicAdd(&s_taken, taken);
+      atomicAdd(&s_tk[rounds & 1], taken);
       atomicMax(&s_top, top);
     }
+    // every lane read last round's count before this round's first barrier
+    if (tid == 0) s_tk[(rounds + 1) & 1] = 0;
     __syncthreads();
-    if (tid == 0) s_left -= s_taken;
+    left -= s_tk[rounds & 1];
     ++rounds;
-    __syncthreads();
   }
-  // manifolds left after JP_ROUNDS form the serial tail colour s_top
-  const int top = s_top;
+  const int ncol = s_top;
+  auto colour_at = [&](int k) { return (int)((col[k / 4] >> (8 * (k % 4))) & 0xFFu); };
+  if (left > 0) {
+    // serial tail: colours per pair, the tail (colour ncol) for the rest
+#pragma unroll
+    for (int k = 0; k < COLOUR_REG_ITEMS; ++k)
+      if (tid + k * T < nm) colour[(pr[k] - 1u) & 0x7FFFu] = (alive >> k) & 1u ? ncol : colour_at(k);
+    __syncthreads();
+    return false;
+  }
+  // per-colour counts -> offsets, then each manifold's slot in its colour
+  for (int i = tid; i <= ncol + 1; i += T) s_cur[i] = 0;
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < COLOUR_REG_ITEMS; ++k)
-    if ((alive >> k) & 1u) colour[(pr[k] - 1u) & 0x7FFFu] = top;
+    (void)wave_count(s_cur, tid + k * T < nm ? colour_at(k) : -1);
   __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int l = 0; l <= ncol; ++l) {
+      const int h = s_cur[l];
+      s_off[l] = acc;
+      s_cur[l] = acc;
+      acc += h;
+    }
+    s_off[ncol + 1] = acc;
+    atomicMax(&c.counters[CNT_LEVELS], ncol);
+    c.col_info[w] = ncol;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < COLOUR_REG_ITEMS; ++k) {
+    const bool mine = tid + k * T < nm;
+    const int m = wave_count(s_cur, mine ? colour_at(k) : -1);
+    if (mine) c.sched_order[p0 + ((pr[k] - 1u) & 0x7FFFu)] = p0 + m;
+  }
+  int* off = c.col_off + (long)w * (COLOR_CAP + 3);
+  for (int l = tid; l <= ncol; l += T) off[l] = s_off[l];
+  return true;
 }
 
 // Jones-Plassmann colouring of world w's manifolds in LDS.  The world's
@@ -648,11 +690,11 @@ __device__ void colour_world_lds(const Ctx& c, int w, int p0, int np, int c1, in
   __syncthreads();
   for (int q0 = 0; q0 < np; q0 += T) {
     const int q = q0 + tid;
-    bool has = false;
-    if (q < np) {
-      const int p = p0 + q;
-      has = c.ccount[p] > 0 && c.coff[p] < c1;
-      if (!has) colour[q] = -2;
+    const uint32_t ab = q < np ? c.pair_ab[p0 + q] : NO_PAIR;
+    const bool has = ab != NO_PAIR;
+    if (q < np && !has) {
+      colour[q] = -2;
+      c.sched_order[p0 + q] = -1;
     }
     const unsigned long long bal = __ballot(has);
     int base = 0;
@@ -660,11 +702,7 @@ __device__ void colour_world_lds(const Ctx& c, int w, int p0, int np, int c1, in
     base = __shfl(base, 0);
     if (has) {
       const int m = base + (int)__popcll(bal & ((1ull << (tid & 63)) - 1ull));
-      const int p = p0 + q;
-      const int ga = c.pa[p], gb = c.pb[p];
-      const uint32_t a = c.is_static[ga] ? NO_BODY : (uint32_t)(ga - b0);
-      const uint32_t b = c.is_static[gb] ? NO_BODY : (uint32_t)(gb - b0);
-      m_ab[m] = a | (b << 16);
+      m_ab[m] = ab;
       m_q[m] = (uint16_t)q;
       m_col[m] = 0xFFu;
     }
@@ -675,8 +713,9 @@ __device__ void colour_world_lds(const Ctx& c, int w, int p0, int np, int c1, in
   __syncthreads();
   if constexpr (T >= 1024) {
     if (nm <= COLOUR_REG_ITEMS * T) {
-      colour_rounds_regs<T>(w, nm, nb, s_best, s_used, m_ab, m_q, colour, sm);
-      colour_finish<T>(c, w, p0, np, colour, s_top, s_left > 0, s_off, s_cur);
+      if (!colour_regs<T>(c, w, p0, nm, nb, s_best, s_used, m_ab, m_q, colour, s_off, s_cur,
+                          sm))
+        colour_finish<T>(c, w, p0, np, colour, s_top, true, s_off, s_cur);
       return;
     }
   }
@@ -844,6 +883,26 @@ __global__ __launch_bounds__(T) void k_colour(Ctx c, int pool_bytes) {
   tmark(c, 1);
 }
 
+// Every pair of the batch: its world-local bodies (NO_BODY for a static
+// one) packed as a | b << 16, or NO_PAIR when it has no stored contacts
+// (none, or beyond the contact capacity).  The colouring of worlds below
+// 65535 bodies reads these words.
+__global__ void k_pair_prep(Ctx c) {
+  const int tp = c.wpair_off[c.W];
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < tp; p += gridDim.x * blockDim.x) {
+    const int ga = c.pa[p], gb = c.pb[p];
+    const int w = c.body_world[ga];
+    uint32_t v = NO_PAIR;
+    if (c.ccount[p] > 0 && c.coff[p] < c.wcont_off[w + 1]) {
+      const int b0 = c.body_off[w];
+      const uint32_t a = c.is_static[ga] ? NO_BODY : (uint32_t)(ga - b0) & 0xFFFFu;
+      const uint32_t b = c.is_static[gb] ? NO_BODY : (uint32_t)(gb - b0) & 0xFFFFu;
+      v = a | (b << 16);
+    }
+    c.pair_ab[p] = v;
+  }
+}
+
 // Every coloured pair's <= 2 contacts become its manifold record, over the
 // whole batch at once.
 __global__ void k_pack_manifolds(Ctx c) {
@@ -917,11 +976,11 @@ __global__ __launch_bounds__(T) void k_sweep_colours(Ctx c, int pool_bytes) {
 shp_status colored_alloc(shp_batch* b) {
   Ctx& c = b->ctx;
   static_assert((CF_LAMBDA + 4) * 2 + MF_FIELDS <= 2 * 29 * 2, "records exceed the packed buffer");
-  void** bufs[] = {(void**)&c.pk_k, (void**)&c.col_best, (void**)&c.col_used, (void**)&c.col_off,
-                   (void**)&c.col_info};
-  size_t sizes[] = {(size_t)c.contact_cap * 4, (size_t)c.NB * 8, (size_t)c.NB * 4,
-                    (size_t)c.W * (COLOR_CAP + 3) * 4, (size_t)c.W * 4};
-  for (int k = 0; k < 5; ++k) {
+  void** bufs[] = {(void**)&c.pk_k,    (void**)&c.col_best, (void**)&c.col_used,
+                   (void**)&c.col_off, (void**)&c.col_info, (void**)&c.pair_ab};
+  size_t sizes[] = {(size_t)c.contact_cap * 4, (size_t)c.NB * 8,  (size_t)c.NB * 4,
+                    (size_t)c.W * (COLOR_CAP + 3) * 4, (size_t)c.W * 4, (size_t)c.pair_cap * 4};
+  for (int k = 0; k < 6; ++k) {
     if (hipMalloc(bufs[k], sizes[k] ? sizes[k] : 16) != hipSuccess) return SHP_E_OOM;
     b->allocs.push_back(*bufs[k]);
   }
@@ -957,6 +1016,8 @@ shp_status colored_alloc(shp_batch* b) {
 void launch_colouring(shp_batch* b, hipStream_t s) {
   const Ctx& c = b->ctx;
   const int pool = (int)(b->color_lds - (2 * (COLOR_CAP + 3) + 2) * 4);
+  const long tp = c.pair_cap;
+  k_pair_prep<<<std::max(1L, std::min((tp + 255) / 256, 4096L)), 256, 0, s>>>(c);
   if (b->color_threads >= 512)
     k_colour<1024><<<c.W, 1024, b->color_lds, s>>>(c, pool);
   else

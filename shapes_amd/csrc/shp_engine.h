@@ -105,6 +105,7 @@ struct Ctx {
   uint8_t* pk_f;  // flags: 1 a static, 2 b static, 4 warm, 8 pair continues
   unsigned long long* col_best;  // colouring priorities per body (large worlds)
   uint32_t* col_used;            // colours taken at each body (large worlds)
+  uint32_t* pair_ab;             // per pair: world-local bodies for the colouring
   int32_t* col_off;   // per world: COLOR_CAP + 3 colour offsets (manifolds)
   int32_t* col_info;  // per world: colours | serial tail << 16 (0: nothing to solve)
   // solver sub-phase timestamps of world 0 (wall_clock64), timing mode only
