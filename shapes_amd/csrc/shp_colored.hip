@@ -546,11 +546,10 @@ __device__ int count_manifolds(const Ctx& c, int p0, int np, int c1, int* s_cnt)
 }
 
 // The rounds of colour_world_lds with each lane's manifolds (m = lane,
-// lane + T, ...) tracked in registers with their priorities and colours: a
-// round reads only the live manifolds' bodies and the bodies' priorities
-// and colour masks, a lane's coloured manifolds drop out of its later
-// rounds, and the per-colour counts and manifold slots come from the
-// registers.  The taken count is double-buffered, so a round needs three
+// lane + T, ...) held in registers with their bodies, priorities and
+// colours: a round reads only the bodies' priorities and colour masks, a
+// lane's coloured manifolds drop out of its later rounds, and the
+// per-colour counts and manifold slots come from the registers.  The taken count is double-buffered, so a round needs three
 // barriers.  Returns false (colours written to `colour` per pair instead)
 // when the rounds stopped at JP_ROUNDS: the caller then places the serial
 // tail in pair order.
@@ -562,17 +561,19 @@ __device__ bool colour_regs(const Ctx& c, int w, int p0, int nm, int nb, uint32_
   int* s_tk = sm;  // [2]: taken this round, next round
   int& s_top = sm[2];
   const int tid = threadIdx.x;
-  // pair index q = (prio32 - 1) & 0x7FFF; the bodies stay in LDS (m_ab)
-  uint32_t pr[COLOUR_REG_ITEMS];
+  // pair index q = (prio32 - 1) & 0x7FFF
+  uint32_t pr[COLOUR_REG_ITEMS], ab[COLOUR_REG_ITEMS];
   uint32_t col[COLOUR_REG_ITEMS / 4];  // colour of manifold k in byte k % 4
   uint32_t alive = 0;
 #pragma unroll
   for (int k = 0; k < COLOUR_REG_ITEMS; ++k) {
     const int m = tid + k * T;
     pr[k] = 0;
+    ab[k] = 0;
     if (k % 4 == 0) col[k / 4] = 0;
     if (m < nm) {
       pr[k] = prio32(m_q[m], w);
+      ab[k] = m_ab[m];
       alive |= 1u << k;
     }
   }
@@ -587,7 +588,7 @@ __device__ bool colour_regs(const Ctx& c, int w, int p0, int nm, int nb, uint32_
 #pragma unroll
     for (int k = 0; k < COLOUR_REG_ITEMS; ++k) {
       if (!((alive >> k) & 1u)) continue;
-      const uint32_t abk = m_ab[tid + k * T], a = abk & 0xFFFF, b = abk >> 16;
+      const uint32_t a = ab[k] & 0xFFFF, b = ab[k] >> 16;
       if (a != NO_BODY) atomicMax(&s_best[a], pr[k]);
       if (b != NO_BODY) atomicMax(&s_best[b], pr[k]);
     }
@@ -598,7 +599,7 @@ __device__ bool colour_regs(const Ctx& c, int w, int p0, int nm, int nb, uint32_
 #pragma unroll
     for (int k = 0; k < COLOUR_REG_ITEMS; ++k) {
       if (!((alive >> k) & 1u)) continue;
-      const uint32_t abk = m_ab[tid + k * T], a = abk & 0xFFFF, b = abk >> 16;
+      const uint32_t a = ab[k] & 0xFFFF, b = ab[k] >> 16;
       if ((a == NO_BODY || s_best[a] == pr[k]) && (b == NO_BODY || s_best[b] == pr[k])) {
         const uint32_t used = (a != NO_BODY ? s_used[a] : 0u) | (b != NO_BODY ? s_used[b] : 0u);
         const int l = greedy_colour(used, rounds);
@@ -1113,22 +1114,8 @@ __global__ __launch_bounds__(QUALITY_THREADS) void k_quality(Ctx c) {
   }
 }
 
-// Run sums of the step's maxima (one thread, after k_quality): their
-// means over the run are less sensitive to a single impact than the run
-// maxima.
-__global__ void k_quality_sum(Ctx c) {
-  if (threadIdx.x != 0) return;
-  for (int k = 0; k < 3; ++k) {
-    double m = __longlong_as_double((long long)c.quality[Q_DEPTH + k]);
-    double s = __longlong_as_double((long long)c.quality[Q_SUM_DEPTH + k]);
-    c.quality[Q_SUM_DEPTH + k] = (unsigned long long)__double_as_longlong(s + m);
-  }
-  c.quality[Q_STEPS] += 1;
-}
-
 void launch_quality(shp_batch* b) {
   long n = b->ctx.contact_cap;
   int grid = (int)std::max<long>(1, std::min<long>((n + 255) / 256, 2048));
   k_quality<<<grid, QUALITY_THREADS, 0, b->stream>>>(b->ctx);
-  k_quality_sum<<<1, 64, 0, b->stream>>>(b->ctx);
 }

@@ -71,9 +71,12 @@ __device__ __forceinline__ HullRef hull_of(const Ctx& c, int g) {
 
 // ---------------------------------------------------------------- kernels
 
+// Per-step counters and quality maxima, and the grid's bucket counts.
 __global__ void k_reset(Ctx c) {
-  if (threadIdx.x < 8) c.counters[threadIdx.x] = 0;
-  if (threadIdx.x < 3) c.quality[threadIdx.x] = 0ull;
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < 8) c.counters[g] = 0;
+  if (g < 3) c.quality[g] = 0ull;
+  if (c.use_grid && g <= c.grid_buckets) c.bucket_cnt[g] = 0;
 }
 
 // listToHull in local space (ConvexHull.hs:151-167): unit edge normals and
@@ -450,6 +453,17 @@ __global__ void k_contacts(Ctx c) {
 // This frame's lagrangians become the next frame's EngineCache
 // (Engine/Main.hs:32, wrapInitializer :61-69).
 __global__ void k_cache_store(Ctx c) {
+  // the step's quality maxima are final (k_quality ran before): add them to
+  // the run sums, whose means over the run are less sensitive to a single
+  // impact than the run maxima
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int k = 0; k < 3; ++k) {
+      const double m = __longlong_as_double((long long)c.quality[Q_DEPTH + k]);
+      const double s = __longlong_as_double((long long)c.quality[Q_SUM_DEPTH + k]);
+      c.quality[Q_SUM_DEPTH + k] = (unsigned long long)__double_as_longlong(s + m);
+    }
+    c.quality[Q_STEPS] += 1;
+  }
   int stride = gridDim.x * blockDim.x;
   int tc = c.wcont_off[c.W];
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= c.W || k < tc; k += stride) {
@@ -520,7 +534,8 @@ static shp_status enqueue_step(shp_batch* b) {
   Ctx& c = b->ctx;
   hipStream_t s = b->stream;
   mark(b, 0);
-  k_reset<<<1, 64, 0, s>>>(c);
+  k_reset<<<blocks(std::max<long>(64, c.use_grid ? (long)c.grid_buckets + 1 : 0), 256), 256, 0,
+            s>>>(c);
   mark(b, 1);
   if (c.use_grid)
     HIPCHK(launch_grid_count(b));

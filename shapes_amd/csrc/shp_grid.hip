@@ -267,17 +267,11 @@ static void launch_grid_pairs(shp_batch* b, int fill) {
     k_grid_pairs<1><<<nblk(c.NB, 256), 256, 0, b->stream>>>(c, fill);
 }
 
-__global__ void k_zero_i32(int32_t* p, long n) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
 // Grid broadphase, first half: binning and per-body pair counts into
 // cnt_rank (the caller then scans cnt_rank into off_rank).
 hipError_t launch_grid_count(shp_batch* b) {
   Ctx& c = b->ctx;
   hipStream_t s = b->stream;
-  k_zero_i32<<<nblk(c.grid_buckets + 1, 256), 256, 0, s>>>(c.bucket_cnt, c.grid_buckets + 1);
   k_grid_assign<<<nblk(c.NB, 256), 256, 0, s>>>(c);
   size_t tb = b->scan_tmp_bytes;
   hipError_t e = hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, c.bucket_cnt, c.bucket_start,
