@@ -61,9 +61,11 @@ def make_scene(name: str, worlds: int = 0):
 
 # --------------------------------------------------------------- roofline
 
-# the kernels a phase launches (its HIP-event time covers them all)
+# the kernels of a phase (its HIP-event time covers them; the coloured
+# solve's k_pair_prep and k_colour run on a second stream from the end of
+# the world ranges, and the phase waits for them)
 PHASE_KERNELS = {
-    ("solve", "colored"): ["k_colour", "k_pack_manifolds", "k_sweep_colours"],
+    ("solve", "colored"): ["k_pair_prep", "k_colour", "k_pack_manifolds", "k_sweep_colours"],
     ("solve", "ordered"): ["k_solve_flow"],
     ("sat", None): ["k_sat"],
     ("contacts", None): ["k_contacts"],
