@@ -456,6 +456,7 @@ __device__ void colour_writeback(const Ctx& c, const PackF& P, int p0, int nman,
     for (int j = 0; j < nk; ++j) {
       const long s = 2L * M + j;
       const int k = c.pk_k[s];
+      if (k < 0) continue;  // a manifold whose contacts were not stored
       c.ln[k] = P.lam[s];
       c.lt[k] = P.lam[P.CC + s];
     }
@@ -534,7 +535,7 @@ __device__ void colour_finish(const Ctx& c, int w, int p0, int np, const CT* col
 // Number of manifolds (pairs with stored contacts) of a world, summed over
 // the workgroup; every thread gets it.
 template <int T>
-__device__ int count_manifolds(const Ctx& c, int p0, int np, int c1, int* s_cnt) {
+__device__ int count_manifolds(const Ctx& c, int p0, int np, int* s_cnt) {
   if (threadIdx.x == 0) *s_cnt = 0;
   __syncthreads();
   int mine = 0;
@@ -673,7 +674,7 @@ __device__ bool colour_regs(const Ctx& c, int w, int p0, int nm, int nb, uint32_
 // 0xFF uncoloured), room for `cap` manifolds.  Colours go to sched_level per
 // pair (-2 for pairs without contacts).
 template <int T>
-__device__ void colour_world_lds(const Ctx& c, int w, int p0, int np, int c1, int b0, int nb,
+__device__ void colour_world_lds(const Ctx& c, int w, int p0, int np, int b0, int nb,
                                  int cap, unsigned char* pool, int* s_off, int* s_cur, int* sm) {
   int &s_left = sm[0], &s_taken = sm[1], &s_top = sm[2], &s_fill = sm[3];
   const int tid = threadIdx.x;
@@ -774,7 +775,7 @@ __device__ void colour_world_lds(const Ctx& c, int w, int p0, int np, int c1, in
 // 64-bit priorities and colour masks per body (col_best, col_used) and
 // colours per pair (sched_level) in global memory (L2-resident).
 template <int T>
-__device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int c1, int b0, int nb,
+__device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int b0, int nb,
                                     int* s_off, int* s_cur, int* sm) {
   int &s_left = sm[0], &s_taken = sm[1], &s_top = sm[2];
   const int tid = threadIdx.x;
@@ -790,7 +791,7 @@ __device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int c1,
   int mine = 0;
   for (int q = tid; q < np; q += T) {
     const int p = p0 + q;
-    const bool has = c.ccount[p] > 0 && c.coff[p] < c1;
+    const bool has = c.ccount[p] > 0;
     colour[q] = has ? -1 : -2;
     mine += has;
   }
@@ -860,12 +861,12 @@ __global__ __launch_bounds__(T) void k_colour(Ctx c, int pool_bytes) {
   unsigned char* pool = (unsigned char*)(lds + 2 * (COLOR_CAP + 3) + 2);
   __shared__ int sm[4];
   const int w = c.world_order[blockIdx.x], tid = threadIdx.x;
-  const int c0 = c.wcont_off[w], c1 = c.wcont_off[w + 1];
-  const int p0 = c.wpair_off[w], p1 = c.wpair_off[w + 1];
-  const int np = p1 - p0;
+  // the world's pairs (k_world_ranges computes the same offsets beside
+  // this kernel)
   const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
-  if (c1 <= c0) {  // nothing to solve: no manifold slots for the packer
-    for (int q = tid; q < np; q += T) c.sched_order[p0 + q] = -1;
+  const int p0 = min(c.off_rank[b0], c.pair_cap), p1 = min(c.off_rank[b0 + nb], c.pair_cap);
+  const int np = p1 - p0;
+  if (np == 0) {  // nothing to solve
     if (tid == 0) c.col_info[w] = 0;
     return;
   }
@@ -874,27 +875,29 @@ __global__ __launch_bounds__(T) void k_colour(Ctx c, int pool_bytes) {
   int cap = np;
   bool in_lds = colour_fits_lds(nb, np, np, pool_bytes);
   if (!in_lds) {
-    cap = count_manifolds<T>(c, p0, np, c1, &sm[0]);
+    cap = count_manifolds<T>(c, p0, np, &sm[0]);
     in_lds = colour_fits_lds(nb, np, cap, pool_bytes);
   }
   if (in_lds)
-    colour_world_lds<T>(c, w, p0, np, c1, b0, nb, cap, pool, s_off, s_cur, sm);
+    colour_world_lds<T>(c, w, p0, np, b0, nb, cap, pool, s_off, s_cur, sm);
   else
-    colour_world_global<T>(c, w, p0, np, c1, b0, nb, s_off, s_cur, sm);
+    colour_world_global<T>(c, w, p0, np, b0, nb, s_off, s_cur, sm);
   tmark(c, 1);
 }
 
 // Every pair of the batch: its world-local bodies (NO_BODY for a static
-// one) packed as a | b << 16, or NO_PAIR when it has no stored contacts
-// (none, or beyond the contact capacity).  The colouring of worlds below
-// 65535 bodies reads these words.
+// one) packed as a | b << 16, or NO_PAIR when it has no contacts.  The
+// colouring of worlds below 65535 bodies reads these words.  It runs right
+// after the narrowphase, beside the contact scan: manifolds whose contacts
+// fall beyond the contact capacity are coloured too and packed as records
+// that change nothing (the step reports the overflow).
 __global__ void k_pair_prep(Ctx c) {
-  const int tp = c.wpair_off[c.W];
+  const int tp = min(c.off_rank[c.NB], c.pair_cap);
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < tp; p += gridDim.x * blockDim.x) {
     const int ga = c.pa[p], gb = c.pb[p];
     const int w = c.body_world[ga];
     uint32_t v = NO_PAIR;
-    if (c.ccount[p] > 0 && c.coff[p] < c.wcont_off[w + 1]) {
+    if (c.ccount[p] > 0) {
       const int b0 = c.body_off[w];
       const uint32_t a = c.is_static[ga] ? NO_BODY : (uint32_t)(ga - b0) & 0xFFFFu;
       const uint32_t b = c.is_static[gb] ? NO_BODY : (uint32_t)(gb - b0) & 0xFFFFu;
@@ -902,6 +905,22 @@ __global__ void k_pair_prep(Ctx c) {
     }
     c.pair_ab[p] = v;
   }
+}
+
+// A manifold whose contacts lie beyond the contact capacity (the step then
+// reports the overflow): a record that touches no velocity and no contact.
+__device__ __forceinline__ void pack_null(const Ctx& c, const PackF& P, int p, int M) {
+  const long PC = P.PC, CC = P.CC, s = 2L * M;
+#pragma unroll
+  for (int q = 0; q < MF_FIELDS; ++q) P.mf[q * PC + M] = 0.0f;
+  c.pk_g[M] = c.pa[p];
+  c.pk_g[PC + M] = c.pb[p];
+  c.pk_f[M] = (uint8_t)(MFL_A_STATIC | MFL_B_STATIC);
+#pragma unroll
+  for (int q = 0; q < CF_FIELDS; ++q) P.f[q * CC + s] = 0.0f;
+  P.lam[s] = 0.0;
+  P.lam[CC + s] = 0.0;
+  c.pk_k[s] = -1;
 }
 
 // Every coloured pair's <= 2 contacts become its manifold record, over the
@@ -913,6 +932,10 @@ __global__ void k_pack_manifolds(Ctx c) {
     const int M = c.sched_order[p];
     if (M < 0) continue;
     const int k0 = c.coff[p];
+    if (k0 >= c.contact_cap) {
+      pack_null(c, P, p, M);
+      continue;
+    }
     int k1 = k0 + c.ccount[p];
     if (k1 > c.contact_cap) k1 = c.contact_cap;  // contacts beyond the capacity were not stored
     pack_man(c, P, k0, k1, M);

@@ -551,6 +551,15 @@ static shp_status enqueue_step(shp_batch* b) {
     k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 1);
   mark(b, 4);
   k_sat<<<gs_blocks(c.pair_cap), 256, 0, s>>>(c);
+  // the colouring needs only the pairs and which of them have contacts: it
+  // runs beside the contact scan, contact generation and learned warm start
+  const bool fork = c.solver == SHP_SOLVER_COLORED_F32;
+  if (fork) {
+    HIPCHK(hipEventRecord(b->fork_event, s));
+    HIPCHK(hipStreamWaitEvent(b->side_stream, b->fork_event, 0));
+    launch_colouring(b, b->side_stream);
+    HIPCHK(hipEventRecord(b->join_event, b->side_stream));
+  }
   mark(b, 5);
   tb = b->scan_tmp_bytes;
   {
@@ -562,15 +571,6 @@ static shp_status enqueue_step(shp_batch* b) {
   mark(b, 6);
   k_world_ranges<<<blocks(c.W + 1, 256), 256, 0, s>>>(c);
   mark(b, 7);
-  // the colouring needs only the pairs and contact counts: it runs beside
-  // the contact generation and the learned warm start
-  const bool fork = c.solver == SHP_SOLVER_COLORED_F32;
-  if (fork) {
-    HIPCHK(hipEventRecord(b->fork_event, s));
-    HIPCHK(hipStreamWaitEvent(b->side_stream, b->fork_event, 0));
-    launch_colouring(b, b->side_stream);
-    HIPCHK(hipEventRecord(b->join_event, b->side_stream));
-  }
   k_contacts<<<gs_blocks(c.pair_cap), 256, 0, s>>>(c);
   if (c.warm_mode == SHP_WARM_LEARNED && c.mlp_ready) mlp_launch_infer(b);
   mark(b, 8);
