@@ -683,12 +683,17 @@ int colour_pairs(orc_engine* e, std::vector<int>& colour, bool* tail) {
   return top + (*tail ? 1 : 0);
 }
 
+// The coloured FP32 solver's arithmetic (shp_colored.hip, solve_man):
+// a . b as a product and five fused multiply-adds in index order; v += l*w
+// fused per entry.  fmaf rounds once, on the CPU as on the GPU.
 inline float dot6f(const float* a, const float* b) {
-  return ((((a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]) + a[3] * b[3]) + a[4] * b[4]) + a[5] * b[5];
+  float d = a[0] * b[0];
+  for (int q = 1; q < 6; ++q) d = fmaf(a[q], b[q], d);
+  return d;
 }
 
-inline void applyf(float l, const float* j, const float* im, float* v) {
-  for (int q = 0; q < 6; ++q) v[q] = v[q] + ((j[q] * l) * im[q]);
+inline void applyw(float l, const float* w, float* v) {
+  for (int q = 0; q < 6; ++q) v[q] = fmaf(l, w[q], v[q]);
 }
 
 // Pass `pass` (0: warm start) over the contacts [k0, k1) of one manifold
@@ -700,8 +705,11 @@ void solve_manifold_f32(orc_engine* e, std::vector<Contact>& cs, int k0, int k1,
   const Phys& B = e->objs[ib];
   const Material& ma = e->mats[ia];
   const Material& mb = e->mats[ib];
-  const float im[6] = {(float)A.iml, (float)A.iml, (float)A.imr,
-                       (float)B.iml, (float)B.iml, (float)B.imr};
+  // the row normal of the manifold's first contact times the inverse
+  // masses, formed in FP64 and rounded once
+  const double ux = cs[k0].cc.jn[3], uy = cs[k0].cc.jn[4];
+  const float ax = (float)(A.iml * ux), ay = (float)(A.iml * uy);
+  const float bx = (float)(B.iml * ux), by = (float)(B.iml * uy);
   const float mu = (float)((ma.mu + mb.mu) / 2.0);
   const float eb = (float)hs_min(ma.bounce, mb.bounce);
   float v[6] = {vf[3 * ia], vf[3 * ia + 1], vf[3 * ia + 2], vf[3 * ib], vf[3 * ib + 1], vf[3 * ib + 2]};
@@ -713,34 +721,38 @@ void solve_manifold_f32(orc_engine* e, std::vector<Contact>& cs, int k0, int k1,
       jn[q] = (float)c.cc.jn[q];
       jt[q] = (float)c.cc.jt[q];
     }
+    // the rows times the inverse masses: linear parts from the manifold's
+    // products (the friction row's direction is clockwise of the normal),
+    // angular parts rounded once from FP64
+    const float wn[6] = {-ax, -ay, (float)(A.imr * c.cc.jn[2]), bx, by, (float)(B.imr * c.cc.jn[5])};
+    const float wt[6] = {-ay, ax, (float)(A.imr * c.cc.jt[2]), by, -bx, (float)(B.imr * c.cc.jt[5])};
     if (pass == 0) {
       if (!c.warm) continue;
-      applyf((float)c.ln, jn, im, v);
-      applyf((float)c.lt, jt, im, v);
+      applyw((float)c.ln, wn, v);
+      applyw((float)c.lt, wt, v);
       changed = true;
       continue;
     }
     double imd[6];
     im6(A, B, imd);
-    const float mcn = (float)eff_mass(c.cc.jn, imd), mct = (float)eff_mass(c.cc.jt, imd);
-    const float rax = (float)c.cc.ra.x, ray = (float)c.cc.ra.y;
-    const float rbx = (float)c.cc.rb.x, rby = (float)c.cc.rb.y;
-    const float nrx = (float)c.cc.nr.x, nry = (float)c.cc.nr.y;
-    float cx = ((-v[0] + v[2] * ray) + v[3]) - v[5] * rby;  // bounceB closing velocity
-    float cy = ((-v[1] - v[2] * rax) + v[4]) + v[5] * rbx;
-    float bb = eb * (cx * nrx + cy * nry);
+    // reciprocal effective masses, rounded once from FP64
+    const float icn = (float)(1.0 / eff_mass(c.cc.jn, imd));
+    const float ict = (float)(1.0 / eff_mass(c.cc.jt, imd));
+    // the bounce term's closing velocity (bounceB) is the normal row times
+    // the velocities
+    const float dn = dot6f(jn, v), dt = dot6f(jt, v);
+    float bb = eb * dn;
     bb = 0.0f <= bb ? 0.0f : bb;
-    float bn = (float)c.cc.bn + bb;
-    float new_n = -(dot6f(jn, v) + bn) / mcn;
-    float new_t = -dot6f(jt, v) / mct;
+    float new_n = -(dn + ((float)c.cc.bn + bb)) * icn;
+    float new_t = -dt * ict;
     double app_n = hs_max((double)new_n, -c.ln);
     double acc_n = c.ln + app_n;
     double thr = acc_n * (double)mu;
     double acc_t = c.lt + (double)new_t;
     double accp = acc_t > thr ? thr : (acc_t < -thr ? -thr : acc_t);
     double app_t = accp - c.lt;
-    applyf((float)app_n, jn, im, v);
-    applyf((float)app_t, jt, im, v);
+    applyw((float)app_n, wn, v);
+    applyw((float)app_t, wt, v);
     c.ln = acc_n;
     c.lt = accp;
     changed = true;

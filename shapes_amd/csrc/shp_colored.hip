@@ -48,23 +48,30 @@ using namespace shp;
 
 // per-contact FP32 fields (stride contact_cap), then the FP64 lagrangians,
 // then the per-manifold FP32 fields (stride pair_cap); manifold bodies in
-// pk_g (2 x pair_cap), flags in pk_f, the contact of a slot in pk_k
+// pk_g (2 x pair_cap), flags in pk_f, the contact of a slot in pk_k.
+// Manifold M's contacts sit in slots 2M and 2M + 1, so a field of both
+// loads as one 8-byte pair.
+//   JN2, JN5 / JT2, JT5: the angular entries of the normal / friction rows
+//   ICN, ICT: reciprocal effective masses
+//   WN2, WN5 / WT2, WT5: angular entries times the inverse inertias
+//   AX, AY / BX, BY: the normal times the inverse masses of a / b
+// Products and reciprocals are formed in FP64 and rounded once.
 enum {
   CF_JN2 = 0,
   CF_JN5,
   CF_JT2,
   CF_JT5,
   CF_BN,
-  CF_RAX,
-  CF_RAY,
-  CF_RBX,
-  CF_RBY,
-  CF_MCN,
-  CF_MCT,
+  CF_ICN,
+  CF_ICT,
+  CF_WN2,
+  CF_WN5,
+  CF_WT2,
+  CF_WT5,
   CF_FIELDS,
-  CF_LAMBDA = 12  // float offset of the lambdas (x contact_cap), 8-byte aligned
+  CF_LAMBDA = 12  // float offset of the lambdas (x contact_cap), 16-byte aligned
 };
-enum { MF_UX = 0, MF_UY, MF_MU, MF_E, MF_IMLA, MF_IMRA, MF_IMLB, MF_IMRB, MF_FIELDS };
+enum { MF_UX = 0, MF_UY, MF_MU, MF_E, MF_AX, MF_AY, MF_BX, MF_BY, MF_FIELDS };
 // manifold flags: 1 a static, 2 b static, 4 first contact warm, 8 second
 // contact present, 16 second contact warm
 enum { MFL_A_STATIC = 1, MFL_B_STATIC = 2, MFL_WARM0 = 4, MFL_TWO = 8, MFL_WARM1 = 16 };
@@ -78,8 +85,12 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
   return x;
 }
 
+// a . b as a product and five fused multiply-adds, in index order
 __device__ __forceinline__ float dot6f(const float* a, const float* b) {
-  return ((((a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]) + a[3] * b[3]) + a[4] * b[4]) + a[5] * b[5];
+  float d = a[0] * b[0];
+#pragma unroll
+  for (int q = 1; q < 6; ++q) d = fmaf(a[q], b[q], d);
+  return d;
 }
 
 struct PackF {
@@ -100,49 +111,83 @@ __device__ __forceinline__ PackF packf(const Ctx& c) {
 }
 
 struct ConF {
-  float jn2, jn5, jt2, jt5, bn, rax, ray, rbx, rby, mcn, mct;
+  float jn2, jn5, jt2, jt5, bn, icn, ict, wn2, wn5, wt2, wt5;
   double ln, lt;
 };
 
 struct ManF {
-  float ux, uy, mu, e, ima, ira, imb, irb;
+  float ux, uy, mu, e, ax, ay, bx, by;
   int ga, gb, flags;
   ConF k[2];
 };
 
-// Manifold M (world offset included) and its contact slots 2M, 2M + 1.
-__device__ __forceinline__ void load_man(const Ctx& c, const PackF& P, int M, ManF& m) {
-  const long PC = P.PC, CC = P.CC;
-  m.ux = P.mf[MF_UX * PC + M];
-  m.uy = P.mf[MF_UY * PC + M];
-  m.mu = P.mf[MF_MU * PC + M];
-  m.e = P.mf[MF_E * PC + M];
-  m.ima = P.mf[MF_IMLA * PC + M];
-  m.ira = P.mf[MF_IMRA * PC + M];
-  m.imb = P.mf[MF_IMLB * PC + M];
-  m.irb = P.mf[MF_IMRB * PC + M];
-  m.ga = c.pk_g[M];
-  m.gb = c.pk_g[PC + M];
-  m.flags = c.pk_f[M];
-  const int nk = (m.flags & MFL_TWO) ? 2 : 1;
+// The records of one world: every pointer is offset to the world's first
+// manifold, so a manifold's fields load from a workgroup-uniform base and
+// one 32-bit byte offset per lane (the worlds' manifold counts stay below
+// 2^27, checked at batch creation).
+struct RecW {
+  const float* f;     // field q of slot 2m + j at f[q * CC + 2m + j]
+  double* lam;        // normal lambda of slot 2m + j at lam[2m + j], friction at + CC
+  const float* mf;    // field q of manifold m at mf[q * PC + m]
+  const int* g;       // body a of manifold m at g[m], body b at g[PC + m]
+  const uint8_t* fl;  // flags of manifold m
+  long CC, PC;
+};
+
+__device__ __forceinline__ RecW recw(const Ctx& c, const PackF& P, int p0) {
+  RecW r;
+  r.f = P.f + 2L * p0;
+  r.lam = P.lam + 2L * p0;
+  r.mf = P.mf + p0;
+  r.g = c.pk_g + p0;
+  r.fl = c.pk_f + p0;
+  r.CC = P.CC;
+  r.PC = P.PC;
+  return r;
+}
+
+template <class V>
+__device__ __forceinline__ V ldo(const void* base, uint32_t byte_off) {
+  return *(const V*)((const char*)base + byte_off);
+}
+
+// Manifold m of the world (local index) and its contact slots 2m, 2m + 1.
+// Both slots load, used or not: slot 2m + 1 of a one-contact manifold
+// holds whatever an earlier step left there and is never solved.
+__device__ __forceinline__ void load_man(const RecW& R, int m_loc, ManF& m) {
+  const uint32_t o4 = (uint32_t)m_loc * 4u, o8 = (uint32_t)m_loc * 8u, o16 = (uint32_t)m_loc * 16u;
+  const long PC = R.PC, CC = R.CC;
+  m.ux = ldo<float>(R.mf + MF_UX * PC, o4);
+  m.uy = ldo<float>(R.mf + MF_UY * PC, o4);
+  m.mu = ldo<float>(R.mf + MF_MU * PC, o4);
+  m.e = ldo<float>(R.mf + MF_E * PC, o4);
+  m.ax = ldo<float>(R.mf + MF_AX * PC, o4);
+  m.ay = ldo<float>(R.mf + MF_AY * PC, o4);
+  m.bx = ldo<float>(R.mf + MF_BX * PC, o4);
+  m.by = ldo<float>(R.mf + MF_BY * PC, o4);
+  m.ga = ldo<int>(R.g, o4);
+  m.gb = ldo<int>(R.g + PC, o4);
+  m.flags = ldo<uint8_t>(R.fl, (uint32_t)m_loc);
+  float2 x[CF_FIELDS];
+#pragma unroll
+  for (int q = 0; q < CF_FIELDS; ++q) x[q] = ldo<float2>(R.f + q * CC, o8);
+  const double2 ln = ldo<double2>(R.lam, o16), lt = ldo<double2>(R.lam + CC, o16);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
-    if (j >= nk) break;
-    const long s = 2L * M + j;
     ConF& k = m.k[j];
-    k.jn2 = P.f[CF_JN2 * CC + s];
-    k.jn5 = P.f[CF_JN5 * CC + s];
-    k.jt2 = P.f[CF_JT2 * CC + s];
-    k.jt5 = P.f[CF_JT5 * CC + s];
-    k.bn = P.f[CF_BN * CC + s];
-    k.rax = P.f[CF_RAX * CC + s];
-    k.ray = P.f[CF_RAY * CC + s];
-    k.rbx = P.f[CF_RBX * CC + s];
-    k.rby = P.f[CF_RBY * CC + s];
-    k.mcn = P.f[CF_MCN * CC + s];
-    k.mct = P.f[CF_MCT * CC + s];
-    k.ln = P.lam[s];
-    k.lt = P.lam[CC + s];
+    k.jn2 = j ? x[CF_JN2].y : x[CF_JN2].x;
+    k.jn5 = j ? x[CF_JN5].y : x[CF_JN5].x;
+    k.jt2 = j ? x[CF_JT2].y : x[CF_JT2].x;
+    k.jt5 = j ? x[CF_JT5].y : x[CF_JT5].x;
+    k.bn = j ? x[CF_BN].y : x[CF_BN].x;
+    k.icn = j ? x[CF_ICN].y : x[CF_ICN].x;
+    k.ict = j ? x[CF_ICT].y : x[CF_ICT].x;
+    k.wn2 = j ? x[CF_WN2].y : x[CF_WN2].x;
+    k.wn5 = j ? x[CF_WN5].y : x[CF_WN5].x;
+    k.wt2 = j ? x[CF_WT2].y : x[CF_WT2].x;
+    k.wt5 = j ? x[CF_WT5].y : x[CF_WT5].x;
+    k.ln = j ? ln.y : ln.x;
+    k.lt = j ? lt.y : lt.x;
   }
 }
 
@@ -151,15 +196,17 @@ __device__ __forceinline__ void load_man(const Ctx& c, const PackF& P, int M, Ma
 __device__ __forceinline__ void pack_man(const Ctx& c, const PackF& P, int k0, int k1, int M) {
   const long PC = P.PC, CC = P.CC;
   const int ga = c.ga[k0], gb = c.gb[k0];
+  const double ux = c.jn[3 * CC + k0], uy = c.jn[4 * CC + k0];  // the row normal (flip applied)
+  const double ima = c.iml[ga], ira = c.imr[ga], imb = c.iml[gb], irb = c.imr[gb];
   float mf[MF_FIELDS];
-  mf[MF_UX] = (float)c.jn[3 * CC + k0];  // the row normal (flip applied)
-  mf[MF_UY] = (float)c.jn[4 * CC + k0];
+  mf[MF_UX] = (float)ux;
+  mf[MF_UY] = (float)uy;
   mf[MF_MU] = (float)((c.mu[ga] + c.mu[gb]) / 2.0);
   mf[MF_E] = (float)hs_min(c.e[ga], c.e[gb]);
-  mf[MF_IMLA] = (float)c.iml[ga];
-  mf[MF_IMRA] = (float)c.imr[ga];
-  mf[MF_IMLB] = (float)c.iml[gb];
-  mf[MF_IMRB] = (float)c.imr[gb];
+  mf[MF_AX] = (float)(ima * ux);
+  mf[MF_AY] = (float)(ima * uy);
+  mf[MF_BX] = (float)(imb * ux);
+  mf[MF_BY] = (float)(imb * uy);
   const bool two = k1 - k0 > 1;
   int flags = (c.is_static[ga] ? MFL_A_STATIC : 0) | (c.is_static[gb] ? MFL_B_STATIC : 0) |
               (c.warm[k0] ? MFL_WARM0 : 0) | (two ? MFL_TWO : 0);
@@ -169,17 +216,19 @@ __device__ __forceinline__ void pack_man(const Ctx& c, const PackF& P, int k0, i
   for (int j = 0; j < 2; ++j) {
     if (j == 1 && !two) break;
     const int k = k0 + j;
-    cf[j][CF_JN2] = (float)c.jn[2 * CC + k];
-    cf[j][CF_JN5] = (float)c.jn[5 * CC + k];
-    cf[j][CF_JT2] = (float)c.jt[2 * CC + k];
-    cf[j][CF_JT5] = (float)c.jt[5 * CC + k];
+    const double jn2 = c.jn[2 * CC + k], jn5 = c.jn[5 * CC + k];
+    const double jt2 = c.jt[2 * CC + k], jt5 = c.jt[5 * CC + k];
+    cf[j][CF_JN2] = (float)jn2;
+    cf[j][CF_JN5] = (float)jn5;
+    cf[j][CF_JT2] = (float)jt2;
+    cf[j][CF_JT5] = (float)jt5;
     cf[j][CF_BN] = (float)c.bn[k];
-    cf[j][CF_RAX] = (float)c.rax[k];
-    cf[j][CF_RAY] = (float)c.ray[k];
-    cf[j][CF_RBX] = (float)c.rbx[k];
-    cf[j][CF_RBY] = (float)c.rby[k];
-    cf[j][CF_MCN] = (float)c.mcn[k];
-    cf[j][CF_MCT] = (float)c.mct[k];
+    cf[j][CF_ICN] = (float)(1.0 / c.mcn[k]);
+    cf[j][CF_ICT] = (float)(1.0 / c.mct[k]);
+    cf[j][CF_WN2] = (float)(ira * jn2);
+    cf[j][CF_WN5] = (float)(irb * jn5);
+    cf[j][CF_WT2] = (float)(ira * jt2);
+    cf[j][CF_WT5] = (float)(irb * jt5);
     lam[j][0] = c.ln[k];
     lam[j][1] = c.lt[k];
     if (j == 1 && c.warm[k]) flags |= MFL_WARM1;
@@ -201,9 +250,10 @@ __device__ __forceinline__ void pack_man(const Ctx& c, const PackF& P, int k0, i
   }
 }
 
-__device__ __forceinline__ void applyf(float l, const float* j, const float* im, float* v) {
+// v += l * w, fused per entry (w: a row times the inverse masses)
+__device__ __forceinline__ void applyw(float l, const float* w, float* v) {
 #pragma unroll
-  for (int q = 0; q < 6; ++q) v[q] = v[q] + ((j[q] * l) * im[q]);
+  for (int q = 0; q < 6; ++q) v[q] = fmaf(l, w[q], v[q]);
 }
 
 // FP32 velocities: LDS (3 floats per local body) or fvx/fvy/fw.
@@ -240,12 +290,15 @@ struct VelF {
 
 // One manifold: warm start (pass 0: useCache's applySln) or one sweep
 // (improveContactSln with FP32 velocities/rows and FP64 accumulators) of
-// its contacts in key order.  M: manifold index for the lagrangian stores.
+// its contacts in key order.  m_loc: the world-local manifold index for the
+// lagrangian stores.  The FP32 arithmetic is this solver's own: the bounce
+// term's closing velocity is the normal row times the velocities (the same
+// quantity as bounceB's, Solvers/Contact.hs), the effective masses enter as
+// reciprocals, and products with velocities are fused.
 template <bool VLDS>
-__device__ __forceinline__ void solve_man(const PackF& P, int M, ManF& m, bool warm,
+__device__ __forceinline__ void solve_man(const RecW& R, int m_loc, ManF& m, bool warm,
                                           const VelF<VLDS>& V) {
-  float v[6], im[6] = {m.ima, m.ima, m.ira, m.imb, m.imb, m.irb};
-  const float tx = m.uy, ty = -m.ux;  // clockwise(u)
+  float v[6];
   V.load(m.ga, v);
   V.load(m.gb, v + 3);
   const int nk = (m.flags & MFL_TWO) ? 2 : 1;
@@ -254,38 +307,42 @@ __device__ __forceinline__ void solve_man(const PackF& P, int M, ManF& m, bool w
   for (int j = 0; j < 2; ++j) {
     if (j >= nk) break;
     ConF& k = m.k[j];
+    // rows (the friction row's direction is clockwise(u)) and the rows
+    // times the inverse masses
     const float jn[6] = {-m.ux, -m.uy, k.jn2, m.ux, m.uy, k.jn5};
-    const float jt[6] = {-tx, -ty, k.jt2, tx, ty, k.jt5};
+    const float jt[6] = {-m.uy, m.ux, k.jt2, m.uy, -m.ux, k.jt5};
+    const float wn[6] = {-m.ax, -m.ay, k.wn2, m.bx, m.by, k.wn5};
+    const float wt[6] = {-m.ay, m.ax, k.wt2, m.by, -m.bx, k.wt5};
     if (warm) {
       if (!(m.flags & (j == 0 ? MFL_WARM0 : MFL_WARM1))) continue;
-      applyf((float)k.ln, jn, im, v);
-      applyf((float)k.lt, jt, im, v);
+      applyw((float)k.ln, wn, v);
+      applyw((float)k.lt, wt, v);
       changed = true;
       continue;
     }
-    float cx = ((-v[0] + v[2] * k.ray) + v[3]) - v[5] * k.rby;  // bounceB closing velocity
-    float cy = ((-v[1] - v[2] * k.rax) + v[4]) + v[5] * k.rbx;
-    float bb = m.e * (cx * m.ux + cy * m.uy);
+    const float dn = dot6f(jn, v), dt = dot6f(jt, v);
+    float bb = m.e * dn;  // bounce term
     bb = 0.0f <= bb ? 0.0f : bb;
-    float bn = k.bn + bb;
-    float new_n = -(dot6f(jn, v) + bn) / k.mcn;
-    float new_t = -dot6f(jt, v) / k.mct;
+    const float new_n = -(dn + (k.bn + bb)) * k.icn;
+    const float new_t = -dt * k.ict;
     double app_n = hs_max((double)new_n, -k.ln);
     double acc_n = k.ln + app_n;
     double thr = acc_n * (double)m.mu;
     double acc_t = k.lt + (double)new_t;
     double accp = acc_t > thr ? thr : (acc_t < -thr ? -thr : acc_t);
     double app_t = accp - k.lt;
-    applyf((float)app_n, jn, im, v);
-    applyf((float)app_t, jt, im, v);
+    applyw((float)app_n, wn, v);
+    applyw((float)app_t, wt, v);
     k.ln = acc_n;
     k.lt = accp;
-    const long s = 2L * M + j;
-    P.lam[s] = acc_n;
-    P.lam[P.CC + s] = accp;
     changed = true;
   }
   if (!changed) return;
+  if (!warm) {  // both slots' lagrangians, as one 16-byte store each
+    const uint32_t o16 = (uint32_t)m_loc * 16u;
+    *(double2*)((char*)R.lam + o16) = make_double2(m.k[0].ln, m.k[1].ln);
+    *(double2*)((char*)(R.lam + R.CC) + o16) = make_double2(m.k[0].lt, m.k[1].lt);
+  }
   if (!(m.flags & MFL_A_STATIC)) V.store(m.ga, v);
   if (!(m.flags & MFL_B_STATIC)) V.store(m.gb, v + 3);
 }
@@ -355,7 +412,7 @@ __device__ void assign_slots(int p0, int np, const CT* colour, int* scur, int ta
 // thread t takes manifolds s_off[l] + t, + T, ... of colour l (the last
 // colour serially by thread 0 when colouring stopped after JP_ROUNDS).
 template <bool VLDS, int T>
-__device__ void sweep_colours(const Ctx& c, const PackF& P, int p0, int ncol, bool serial_tail,
+__device__ void sweep_colours(const Ctx& c, const RecW& R, int ncol, bool serial_tail,
                               const int* s_off, const VelF<VLDS>& V) {
   const int tid = threadIdx.x;
   const int phases = (c.iterations + 1) * ncol;
@@ -378,13 +435,13 @@ __device__ void sweep_colours(const Ctx& c, const PackF& P, int p0, int ncol, bo
       const int di = serial_tail && l == ncol - 1 ? 1 : T;
       const int end = s_off[l + 1];
       int i = first_of(h);
-      if (i < end && !loaded) load_man(c, P, p0 + i, m);
+      if (i < end && !loaded) load_man(R, i, m);
       // the lane's next manifold of the colour loads while this one solves
       while (i < end) {
         const int inext = i + di;
         const bool more = inext < end;
-        if (more) load_man(c, P, p0 + inext, n);
-        solve_man<VLDS>(P, p0 + i, m, warm, V);
+        if (more) load_man(R, inext, n);
+        solve_man<VLDS>(R, i, m, warm, V);
         if (!more) break;
         m = n;
         i = inext;
@@ -393,7 +450,7 @@ __device__ void sweep_colours(const Ctx& c, const PackF& P, int p0, int ncol, bo
       if (ncol > 1 && h + 1 < phases) {
         const int j = first_of(h + 1);
         if (j < s_off[(h + 1) % ncol + 1]) {
-          load_man(c, P, p0 + j, m);
+          load_man(R, j, m);
           loaded = true;
         }
       }
@@ -413,26 +470,26 @@ __device__ void sweep_colours(const Ctx& c, const PackF& P, int p0, int ncol, bo
   ManF cur, nxt;
   int Mc = 0, Mn = 0;
   bool has = phases > 0 && first(0, Mc);
-  if (has) load_man(c, P, p0 + Mc, cur);
+  if (has) load_man(R, Mc, cur);
   for (int h = 0; h < phases; ++h) {
     const int l = h % ncol;
     const bool warm = h < ncol;
     const bool hasn = h + 1 < phases && first(h + 1, Mn);
-    if (hasn && prefetch) load_man(c, P, p0 + Mn, nxt);
+    if (hasn && prefetch) load_man(R, Mn, nxt);
     if (serial_tail && l == ncol - 1) {
       if (tid == 0) {
         for (int i = s_off[l]; i < s_off[l + 1]; ++i) {
           ManF m;
-          load_man(c, P, p0 + i, m);
-          solve_man<VLDS>(P, p0 + i, m, warm, V);
+          load_man(R, i, m);
+          solve_man<VLDS>(R, i, m, warm, V);
         }
       }
     } else {
-      if (has) solve_man<VLDS>(P, p0 + Mc, cur, warm, V);
+      if (has) solve_man<VLDS>(R, Mc, cur, warm, V);
       for (int i = Mc + T; i < s_off[l + 1]; i += T) {
         ManF m;
-        load_man(c, P, p0 + i, m);
-        solve_man<VLDS>(P, p0 + i, m, warm, V);
+        load_man(R, i, m);
+        solve_man<VLDS>(R, i, m, warm, V);
       }
     }
     __syncthreads();
@@ -441,7 +498,7 @@ __device__ void sweep_colours(const Ctx& c, const PackF& P, int p0, int ncol, bo
     if (prefetch)
       cur = nxt;
     else if (has)
-      load_man(c, P, p0 + Mc, cur);
+      load_man(R, Mc, cur);
   }
 }
 
@@ -972,7 +1029,7 @@ __device__ void sweep_world(const Ctx& c, int w, int ncol, bool serial_tail, flo
   tmark(c, 2);
   tmark(c, 3);
   const PackF P = packf(c);
-  sweep_colours<VLDS, T>(c, P, p0, ncol, serial_tail, s_off, V);
+  sweep_colours<VLDS, T>(c, recw(c, P, p0), ncol, serial_tail, s_off, V);
   tmark(c, 4);
   colour_writeback<VLDS, T>(c, P, p0, s_off[ncol], b0, nb, V);
   __syncthreads();

@@ -651,6 +651,9 @@ shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, cons
   int ppb = cfg->max_pairs_per_body > 0 ? cfg->max_pairs_per_body : 8;
   long pair_cap = NB * (long)ppb + 1024;
   if (pair_cap >= (1L << 30)) return fail(SHP_E_UNSUPPORTED, "pair capacity too large");
+  // the coloured sweeps address a world's records with 32-bit byte offsets
+  if (cfg->solver == SHP_SOLVER_COLORED_F32 && pair_cap >= (1L << 28))
+    return fail(SHP_E_UNSUPPORTED, "pair capacity too large for the coloured solver (2^28)");
 
   shp_batch* b = new shp_batch();
   b->cfg = *cfg;
