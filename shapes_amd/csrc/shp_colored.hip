@@ -48,7 +48,7 @@ using namespace shp;
 
 // per-contact FP32 fields (stride contact_cap), then the FP64 lagrangians,
 // then the per-manifold FP32 fields (stride pair_cap); manifold bodies in
-// pk_g (2 x pair_cap), flags in pk_f, the contact of a slot in pk_k.
+// pk_g (2 x pair_cap), flags in pk_f.
 // Manifold M's contacts sit in slots 2M and 2M + 1, so a field of both
 // loads as one 8-byte pair.
 //   JN2, JN5 / JT2, JT5: the angular entries of the normal / friction rows
@@ -246,7 +246,6 @@ __device__ __forceinline__ void pack_man(const Ctx& c, const PackF& P, int k0, i
     for (int q = 0; q < CF_FIELDS; ++q) P.f[q * CC + s] = cf[j][q];
     P.lam[s] = lam[j][0];
     P.lam[CC + s] = lam[j][1];
-    c.pk_k[s] = k0 + j;
   }
 }
 
@@ -499,33 +498,6 @@ __device__ void sweep_colours(const Ctx& c, const RecW& R, int ncol, bool serial
       cur = nxt;
     else if (has)
       load_man(R, Mc, cur);
-  }
-}
-
-// Lagrangians back to contact order; the FP32 velocity change into FP64.
-template <bool VLDS, int T>
-__device__ void colour_writeback(const Ctx& c, const PackF& P, int p0, int nman, int b0, int nb,
-                                 const VelF<VLDS>& V) {
-  const int tid = threadIdx.x;
-  for (int i = tid; i < nman; i += T) {
-    const int M = p0 + i;
-    const int nk = (c.pk_f[M] & MFL_TWO) ? 2 : 1;
-    for (int j = 0; j < nk; ++j) {
-      const long s = 2L * M + j;
-      const int k = c.pk_k[s];
-      if (k < 0) continue;  // a manifold whose contacts were not stored
-      c.ln[k] = P.lam[s];
-      c.lt[k] = P.lam[P.CC + s];
-    }
-  }
-  for (int i = tid; i < nb; i += T) {
-    const int g = b0 + i;
-    if (c.is_static[g]) continue;
-    float v[3];
-    V.load(g, v);
-    c.vx[g] = c.vx[g] + (double)(v[0] - (float)c.vx[g]);
-    c.vy[g] = c.vy[g] + (double)(v[1] - (float)c.vy[g]);
-    c.w[g] = c.w[g] + (double)(v[2] - (float)c.w[g]);
   }
 }
 
@@ -977,7 +949,6 @@ __device__ __forceinline__ void pack_null(const Ctx& c, const PackF& P, int p, i
   for (int q = 0; q < CF_FIELDS; ++q) P.f[q * CC + s] = 0.0f;
   P.lam[s] = 0.0;
   P.lam[CC + s] = 0.0;
-  c.pk_k[s] = -1;
 }
 
 // Every coloured pair's <= 2 contacts become its manifold record, over the
@@ -1013,17 +984,21 @@ __device__ void sweep_world(const Ctx& c, int w, int ncol, bool serial_tail, flo
   V.lds = vlds;
   V.c = &c;
   V.b0 = b0;
-  for (int i = tid; i < nb; i += T) {
-    const float vx = (float)c.vx[b0 + i], vy = (float)c.vy[b0 + i], vw = (float)c.w[b0 + i];
-    if (VLDS) {
-      V.lds[3 * i] = vx;
-      V.lds[3 * i + 1] = vy;
-      V.lds[3 * i + 2] = vw;
-    } else {
-      c.fvx[b0 + i] = vx;
-      c.fvy[b0 + i] = vy;
-      c.fw[b0 + i] = vw;
+  // FP32 velocities; a lane has the loads of four bodies in flight
+  for (int i0 = tid; i0 < nb; i0 += 4 * T) {
+    float x[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int g = b0 + i0 + u * T;
+      if (i0 + u * T < nb) {
+        x[u][0] = (float)c.vx[g];
+        x[u][1] = (float)c.vy[g];
+        x[u][2] = (float)c.w[g];
+      }
     }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (i0 + u * T < nb) V.store(b0 + i0 + u * T, x[u]);
   }
   __syncthreads();
   tmark(c, 2);
@@ -1031,8 +1006,17 @@ __device__ void sweep_world(const Ctx& c, int w, int ncol, bool serial_tail, flo
   const PackF P = packf(c);
   sweep_colours<VLDS, T>(c, recw(c, P, p0), ncol, serial_tail, s_off, V);
   tmark(c, 4);
-  colour_writeback<VLDS, T>(c, P, p0, s_off[ncol], b0, nb, V);
-  __syncthreads();
+  // (every phase ends with a barrier) the final FP32 velocities to global
+  // memory, for k_colour_unpack
+  if (VLDS) {
+    for (int i = tid; i < nb; i += T) {
+      float v[3];
+      V.load(b0 + i, v);
+      c.fvx[b0 + i] = v[0];
+      c.fvy[b0 + i] = v[1];
+      c.fw[b0 + i] = v[2];
+    }
+  }
   tmark(c, 5);
 }
 
@@ -1054,14 +1038,45 @@ __global__ __launch_bounds__(T) void k_sweep_colours(Ctx c, int pool_bytes) {
     sweep_world<false, T>(c, w, ncol, serial_tail, nullptr, s_off);
 }
 
+// After the sweeps, over the whole batch: the lagrangians back to contact
+// order, and each swept world's FP32 velocity change into the FP64
+// velocities (a body whose world had nothing to solve keeps its own).
+__global__ void k_colour_unpack(Ctx c) {
+  const long tp = c.wpair_off[c.W];
+  const long n = tp > c.NB ? tp : (long)c.NB;
+  const PackF P = packf(c);
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    if (t < tp) {
+      const int M = c.sched_order[t];
+      if (M >= 0) {
+        const int k0 = c.coff[t];
+        const int nk = min(c.ccount[t], c.contact_cap - k0);  // contacts that were stored
+        for (int j = 0; j < nk; ++j) {
+          c.ln[k0 + j] = P.lam[2L * M + j];
+          c.lt[k0 + j] = P.lam[P.CC + 2L * M + j];
+        }
+      }
+    }
+    if (t < c.NB) {
+      const int g = (int)t;
+      if (!c.is_static[g] && (c.col_info[c.body_world[g]] & 0xFFFF) != 0) {
+        c.vx[g] = c.vx[g] + (double)(c.fvx[g] - (float)c.vx[g]);
+        c.vy[g] = c.vy[g] + (double)(c.fvy[g] - (float)c.vy[g]);
+        c.w[g] = c.w[g] + (double)(c.fw[g] - (float)c.w[g]);
+      }
+    }
+  }
+}
+
 shp_status colored_alloc(shp_batch* b) {
   Ctx& c = b->ctx;
   static_assert((CF_LAMBDA + 4) * 2 + MF_FIELDS <= 2 * 29 * 2, "records exceed the packed buffer");
-  void** bufs[] = {(void**)&c.pk_k,    (void**)&c.col_best, (void**)&c.col_used,
-                   (void**)&c.col_off, (void**)&c.col_info, (void**)&c.pair_ab};
-  size_t sizes[] = {(size_t)c.contact_cap * 4, (size_t)c.NB * 8,  (size_t)c.NB * 4,
-                    (size_t)c.W * (COLOR_CAP + 3) * 4, (size_t)c.W * 4, (size_t)c.pair_cap * 4};
-  for (int k = 0; k < 6; ++k) {
+  void** bufs[] = {(void**)&c.col_best, (void**)&c.col_used, (void**)&c.col_off,
+                   (void**)&c.col_info, (void**)&c.pair_ab};
+  size_t sizes[] = {(size_t)c.NB * 8, (size_t)c.NB * 4, (size_t)c.W * (COLOR_CAP + 3) * 4,
+                    (size_t)c.W * 4, (size_t)c.pair_cap * 4};
+  for (int k = 0; k < 5; ++k) {
     if (hipMalloc(bufs[k], sizes[k] ? sizes[k] : 16) != hipSuccess) return SHP_E_OOM;
     b->allocs.push_back(*bufs[k]);
   }
@@ -1117,6 +1132,8 @@ void launch_colored_sweeps(shp_batch* b) {
     k_sweep_colours<512><<<c.W, 512, vel_lds, b->stream>>>(c, pool);
   else
     k_sweep_colours<256><<<c.W, 256, vel_lds, b->stream>>>(c, pool);
+  const long n = std::max<long>(tp, c.NB);
+  k_colour_unpack<<<std::max(1L, std::min((n + 255) / 256, 4096L)), 256, 0, b->stream>>>(c);
 }
 
 // ---- solve quality -----------------------------------------------------------

@@ -101,7 +101,6 @@ struct Ctx {
   // colour order for the coloured one)
   double* pk;
   int32_t* pk_g;  // 2 x contact_cap: bodies a, b
-  int32_t* pk_k;  // contact index of a slot (coloured)
   uint8_t* pk_f;  // flags: 1 a static, 2 b static, 4 warm, 8 pair continues
   unsigned long long* col_best;  // colouring priorities per body (large worlds)
   uint32_t* col_used;            // colours taken at each body (large worlds)
