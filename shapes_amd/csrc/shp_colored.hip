@@ -421,19 +421,24 @@ __device__ void sweep_colours(const Ctx& c, const RecW& R, int ncol, bool serial
     // first manifold of the next colour loads before the barrier (not with
     // one colour: that is the manifold just solved, whose lagrangians the
     // pass changes)
-    auto first_of = [&](int h) {
-      const int l = h % ncol;
+    // the colour bounds of this phase (cs, ce) and the next (ns, ne) stay
+    // in registers; those of the phase after next are read from LDS early
+    // in the phase, so no LDS read sits between a barrier and the work
+    auto first_of = [&](int l, int b0_, int b1_) {
       const bool tail = serial_tail && l == ncol - 1;
-      return tail ? (tid == 0 ? s_off[l] : s_off[l + 1]) : s_off[l] + tid;
+      return tail ? (tid == 0 ? b0_ : b1_) : b0_ + tid;
     };
+    int cs = s_off[0], ce = s_off[1];
+    int ns = s_off[1 % ncol], ne = s_off[1 % ncol + 1];
     ManF m, n;
     bool loaded = false;
     for (int h = 0; h < phases; ++h) {
-      const int l = h % ncol;
+      const int l = h % ncol, l1 = (h + 1) % ncol, l2 = (h + 2) % ncol;
+      const int s2 = s_off[l2], e2 = s_off[l2 + 1];
       const bool warm = h < ncol;
       const int di = serial_tail && l == ncol - 1 ? 1 : T;
-      const int end = s_off[l + 1];
-      int i = first_of(h);
+      const int end = ce;
+      int i = first_of(l, cs, ce);
       if (i < end && !loaded) load_man(R, i, m);
       // the lane's next manifold of the colour loads while this one solves
       while (i < end) {
@@ -447,13 +452,17 @@ __device__ void sweep_colours(const Ctx& c, const RecW& R, int ncol, bool serial
       }
       loaded = false;
       if (ncol > 1 && h + 1 < phases) {
-        const int j = first_of(h + 1);
-        if (j < s_off[(h + 1) % ncol + 1]) {
+        const int j = first_of(l1, ns, ne);
+        if (j < ne) {
           load_man(R, j, m);
           loaded = true;
         }
       }
       __syncthreads();
+      cs = ns;
+      ce = ne;
+      ns = s2;
+      ne = e2;
     }
     return;
   }
