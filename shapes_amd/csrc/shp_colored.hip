@@ -634,7 +634,7 @@ __device__ bool colour_regs(const Ctx& c, int w, int p0, int nm, int nb, uint32_
     __syncthreads();
     // the selected manifolds share no dynamic body: each updates the masks
     // of its own bodies
-    int taken = 0, top = 0;
+    int taken = 0;
 #pragma unroll
     for (int k = 0; k < COLOUR_REG_ITEMS; ++k) {
       if (!((alive >> k) & 1u)) continue;
@@ -647,25 +647,29 @@ __device__ bool colour_regs(const Ctx& c, int w, int p0, int nm, int nb, uint32_
           if (a != NO_BODY) s_used[a] = (uint16_t)(s_used[a] | (1u << l));
           if (b != NO_BODY) s_used[b] = (uint16_t)(s_used[b] | (1u << l));
         }
-        top = max(top, l + 1);
         alive &= ~(1u << k);
         ++taken;
       }
     }
     taken = wave_sum(taken);
-    top = wave_max(top);
-    if ((tid & 63) == 0) {
-      atomicAdd(&s_tk[rounds & 1], taken);
-      atomicMax(&s_top, top);
-    }
+    if ((tid & 63) == 0) atomicAdd(&s_tk[rounds & 1], taken);
     // every lane read last round's count before this round's first barrier
     if (tid == 0) s_tk[(rounds + 1) & 1] = 0;
     __syncthreads();
     left -= s_tk[rounds & 1];
     ++rounds;
   }
-  const int ncol = s_top;
   auto colour_at = [&](int k) { return (int)((col[k / 4] >> (8 * (k % 4))) & 0xFFu); };
+  {  // the colour count: the highest colour of a coloured manifold, plus one
+    int top = 0;
+#pragma unroll
+    for (int k = 0; k < COLOUR_REG_ITEMS; ++k)
+      if (tid + k * T < nm && !((alive >> k) & 1u)) top = max(top, colour_at(k) + 1);
+    top = wave_max(top);
+    if ((tid & 63) == 0) atomicMax(&s_top, top);
+    __syncthreads();
+  }
+  const int ncol = s_top;
   if (left > 0) {
     // serial tail: colours per pair, the tail (colour ncol) for the rest
 #pragma unroll
@@ -675,11 +679,13 @@ __device__ bool colour_regs(const Ctx& c, int w, int p0, int nm, int nb, uint32_
     return false;
   }
   // per-colour counts -> offsets, then each manifold's slot in its colour
+  // (LDS atomics; the order inside a colour is free, since its manifolds
+  // share no dynamic body)
   for (int i = tid; i <= ncol + 1; i += T) s_cur[i] = 0;
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < COLOUR_REG_ITEMS; ++k)
-    (void)wave_count(s_cur, tid + k * T < nm ? colour_at(k) : -1);
+    if (tid + k * T < nm) atomicAdd(&s_cur[colour_at(k)], 1);
   __syncthreads();
   if (tid == 0) {
     int acc = 0;
@@ -695,11 +701,11 @@ __device__ bool colour_regs(const Ctx& c, int w, int p0, int nm, int nb, uint32_
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < COLOUR_REG_ITEMS; ++k) {
-    const bool mine = tid + k * T < nm;
-    const int m = wave_count(s_cur, mine ? colour_at(k) : -1);
-    if (mine) c.sched_order[p0 + ((pr[k] - 1u) & 0x7FFFu)] = p0 + m;
-  }
+  for (int k = 0; k < COLOUR_REG_ITEMS; ++k)
+    if (tid + k * T < nm) {
+      const int m = atomicAdd(&s_cur[colour_at(k)], 1);
+      c.sched_order[p0 + ((pr[k] - 1u) & 0x7FFFu)] = p0 + m;
+    }
   int* off = c.col_off + (long)w * (COLOR_CAP + 3);
   for (int l = tid; l <= ncol; l += T) off[l] = s_off[l];
   return true;
@@ -728,23 +734,40 @@ __device__ void colour_world_lds(const Ctx& c, int w, int p0, int np, int b0, in
   }
   for (int i = tid; i < nb; i += T) s_used[i] = 0;
   __syncthreads();
-  for (int q0 = 0; q0 < np; q0 += T) {
-    const int q = q0 + tid;
-    const uint32_t ab = q < np ? c.pair_ab[p0 + q] : NO_PAIR;
-    const bool has = ab != NO_PAIR;
-    if (q < np && !has) {
-      colour[q] = -2;
-      c.sched_order[p0 + q] = -1;
+  // the manifold list: four pairs per lane in flight, one atomic per
+  // wavefront per batch
+  constexpr int LB = 4;
+  const unsigned long long below = (1ull << (tid & 63)) - 1ull;
+  for (int q0 = 0; q0 < np; q0 += LB * T) {
+    uint32_t abv[LB];
+#pragma unroll
+    for (int u = 0; u < LB; ++u) {
+      const int q = q0 + u * T + tid;
+      abv[u] = q < np ? c.pair_ab[p0 + q] : NO_PAIR;
     }
-    const unsigned long long bal = __ballot(has);
+    unsigned long long bal[LB];
+    int tot = 0;
+#pragma unroll
+    for (int u = 0; u < LB; ++u) {
+      bal[u] = __ballot(abv[u] != NO_PAIR);
+      tot += (int)__popcll(bal[u]);
+    }
     int base = 0;
-    if ((tid & 63) == 0 && bal) base = atomicAdd(&s_fill, (int)__popcll(bal));
+    if ((tid & 63) == 0 && tot) base = atomicAdd(&s_fill, tot);
     base = __shfl(base, 0);
-    if (has) {
-      const int m = base + (int)__popcll(bal & ((1ull << (tid & 63)) - 1ull));
-      m_ab[m] = ab;
-      m_q[m] = (uint16_t)q;
-      m_col[m] = 0xFFu;
+#pragma unroll
+    for (int u = 0; u < LB; ++u) {
+      const int q = q0 + u * T + tid;
+      if (abv[u] != NO_PAIR) {
+        const int m = base + (int)__popcll(bal[u] & below);
+        m_ab[m] = abv[u];
+        m_q[m] = (uint16_t)q;
+        m_col[m] = 0xFFu;
+      } else if (q < np) {
+        colour[q] = -2;
+        c.sched_order[p0 + q] = -1;
+      }
+      base += (int)__popcll(bal[u]);
     }
   }
   __syncthreads();
