@@ -192,5 +192,6 @@ void launch_colouring(shp_batch* b, hipStream_t s);
 void launch_colored_sweeps(shp_batch* b);
 void launch_quality(shp_batch* b);
 hipError_t launch_grid_count(shp_batch* b);
+hipError_t scan_grid_buckets(shp_batch* b);
 hipError_t launch_grid_emit(shp_batch* b);
 size_t grid_scan_bytes(shp_batch* b);

@@ -530,6 +530,8 @@ static inline void mark(shp_batch* b, int k) {
   if (b->timing) (void)hipEventRecord(b->events[k], b->stream);
 }
 
+static hipError_t exclusive_sum(shp_batch* b, const int* in, int* out, long n, hipStream_t s);
+
 static shp_status enqueue_step(shp_batch* b) {
   Ctx& c = b->ctx;
   hipStream_t s = b->stream;
@@ -542,8 +544,7 @@ static shp_status enqueue_step(shp_batch* b) {
   else
     k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 0);
   mark(b, 2);
-  size_t tb = b->scan_tmp_bytes;
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, c.cnt_rank, c.off_rank, c.NB + 1, s));
+  HIPCHK(exclusive_sum(b, c.cnt_rank, c.off_rank, (long)c.NB + 1, s));
   mark(b, 3);
   if (c.use_grid)
     HIPCHK(launch_grid_emit(b));
@@ -561,7 +562,7 @@ static shp_status enqueue_step(shp_batch* b) {
     HIPCHK(hipEventRecord(b->join_event, b->side_stream));
   }
   mark(b, 5);
-  tb = b->scan_tmp_bytes;
+  size_t tb = b->scan_tmp_bytes;
   {
     hipcub::CountingInputIterator<int> idx(0);
     hipcub::TransformInputIterator<int, PairCount, hipcub::CountingInputIterator<int>> cnt(
@@ -586,10 +587,116 @@ static shp_status enqueue_step(shp_batch* b) {
   mark(b, 10);
   k_cache_store<<<gs_blocks(std::max(c.contact_cap, c.W + 1)), 256, 0, s>>>(c);
   mark(b, 11);
+  // (on the second stream beside the quality and cache store, measured
+  // slower: the fork and join cost more than the kernel)
   k_integrate<<<blocks(c.NB, INTEGRATE_THREADS), INTEGRATE_THREADS, 0, s>>>(c, 1);
   mark(b, 12);
   HIPCHK(hipGetLastError());
   return SHP_OK;
+}
+
+// ---- exclusive sums of up to 16384 counts in one workgroup ---------------
+// Chunk u = entries [4096u, 4096u + 4096): thread t loads entries 4t..4t+3
+// of every chunk as int4 (coalesced, all loads in flight at once), the
+// wavefronts scan their thread sums, wavefront 0 scans the (chunk,
+// wavefront) totals in chunk-major order, and every thread writes its
+// entries.  One launch against the library scan's two (state set-up and
+// scan), for the per-body pair counts and grid bucket counts of batches up
+// to 16383 bodies / buckets.
+template <int V>
+__global__ __launch_bounds__(1024) void k_scan_small(const int* __restrict__ in, int* out, int n) {
+  constexpr int P = V * 16, E = (P + 63) / 64;
+  __shared__ int s_part[P];  // (chunk, wavefront) totals, then their exclusive sums
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int4 x[V];
+  int sum[V], incl[V];
+#pragma unroll
+  for (int u = 0; u < V; ++u) {
+    const int i = u * 4096 + 4 * tid;
+    if (i + 3 < n) {
+      x[u] = *(const int4*)(in + i);
+    } else {
+      x[u].x = i < n ? in[i] : 0;
+      x[u].y = i + 1 < n ? in[i + 1] : 0;
+      x[u].z = i + 2 < n ? in[i + 2] : 0;
+      x[u].w = 0;
+    }
+    sum[u] = (x[u].x + x[u].y) + (x[u].z + x[u].w);
+    incl[u] = sum[u];
+  }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      const int y = __shfl_up(incl[u], o);
+      if (lane >= o) incl[u] += y;
+    }
+  }
+  if (lane == 63) {
+#pragma unroll
+    for (int u = 0; u < V; ++u) s_part[u * 16 + wv] = incl[u];
+  }
+  __syncthreads();
+  if (wv == 0) {
+    int v[E], t = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = lane * E + e;
+      v[e] = k < P ? s_part[k] : 0;
+      t += v[e];
+    }
+    int sc = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(sc, o);
+      if (lane >= o) sc += y;
+    }
+    int ex = sc - t;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = lane * E + e;
+      if (k < P) s_part[k] = ex;
+      ex += v[e];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < V; ++u) {
+    const int i = u * 4096 + 4 * tid;
+    int4 e;
+    e.x = s_part[u * 16 + wv] + incl[u] - sum[u];
+    e.y = e.x + x[u].x;
+    e.z = e.y + x[u].y;
+    e.w = e.z + x[u].z;
+    if (i + 3 < n) {
+      *(int4*)(out + i) = e;
+    } else {
+      if (i < n) out[i] = e.x;
+      if (i + 1 < n) out[i + 1] = e.y;
+      if (i + 2 < n) out[i + 2] = e.z;
+    }
+  }
+}
+
+// out[i] = in[0] + ... + in[i-1] for i < n, on stream s: one workgroup for
+// n <= 16384 (int4-aligned arrays), the library scan beyond.
+static hipError_t exclusive_sum(shp_batch* b, const int* in, int* out, long n, hipStream_t s) {
+  if (n <= 4096) {
+    k_scan_small<1><<<1, 1024, 0, s>>>(in, out, (int)n);
+  } else if (n <= 8192) {
+    k_scan_small<2><<<1, 1024, 0, s>>>(in, out, (int)n);
+  } else if (n <= 16384) {
+    k_scan_small<4><<<1, 1024, 0, s>>>(in, out, (int)n);
+  } else {  // (measured: from 8 chunks on, the wavefront scans cost more than the library's launch)
+    size_t tb = b->scan_tmp_bytes;
+    return hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, in, out, (int)n, s);
+  }
+  return hipGetLastError();
+}
+
+hipError_t scan_grid_buckets(shp_batch* b) {
+  return exclusive_sum(b, b->ctx.bucket_cnt, b->ctx.bucket_start, (long)b->ctx.grid_buckets + 1,
+                       b->stream);
 }
 
 static shp_status accumulate_timing(shp_batch* b) {

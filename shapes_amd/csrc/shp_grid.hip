@@ -273,9 +273,7 @@ hipError_t launch_grid_count(shp_batch* b) {
   Ctx& c = b->ctx;
   hipStream_t s = b->stream;
   k_grid_assign<<<nblk(c.NB, 256), 256, 0, s>>>(c);
-  size_t tb = b->scan_tmp_bytes;
-  hipError_t e = hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, c.bucket_cnt, c.bucket_start,
-                                                  c.grid_buckets + 1, s);
+  hipError_t e = scan_grid_buckets(b);
   if (e != hipSuccess) return e;
   k_grid_fill<<<nblk(c.NB, 256), 256, 0, s>>>(c);
   launch_grid_pairs(b, 0);
