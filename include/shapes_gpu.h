@@ -86,7 +86,10 @@ typedef struct shp_config {
   int32_t iterations;
   int32_t solver;             /* SHP_SOLVER_* */
   int32_t warm_start;         /* SHP_WARM_* */
-  int32_t max_pairs_per_body; /* capacity; 0 = default (8) */
+  int32_t max_pairs_per_body; /* capacity; 0 = default (8).  Pair capacity
+                                 = bodies * this + 1024, below 2^30, and
+                                 below 2^28 with the coloured solver
+                                 (SHP_E_UNSUPPORTED otherwise) */
   int32_t broadphase;         /* SHP_BROADPHASE_* */
   int32_t solver_threads;     /* lanes per world in the solve: 0 = by world
                                  size, or 256 or 512, or 1024 for the coloured

@@ -74,3 +74,9 @@ def test_create_rejects_bad_arguments():
     b.add([(0, 0), (1, 0)], (0, 0), mass=1, inertia=1)
     d2, keep2 = _desc(b.build())
     assert L.shp_batch_create(C.byref(d2), 1, C.byref(cfg), C.byref(h)) == 1
+    # the coloured solver addresses a world's records with 32-bit offsets:
+    # pair capacities from 2^28 are refused before any device work
+    ppb = (1 << 28) // w.n + 1
+    big = engine.ConfigC(1 / 60, 0.01, 0.02, 0, -9.8, 4, engine.SOLVER_COLORED_F32, 0, ppb, 0)
+    assert L.shp_batch_create(C.byref(d), 1, C.byref(big), C.byref(h)) == 4
+    assert b"coloured" in L.shp_last_error()
