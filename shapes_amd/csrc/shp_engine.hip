@@ -595,14 +595,14 @@ static shp_status enqueue_step(shp_batch* b) {
   return SHP_OK;
 }
 
-// ---- exclusive sums of up to 16384 counts in one workgroup ---------------
+// ---- exclusive sums of up to 20480 counts in one workgroup ---------------
 // Chunk u = entries [4096u, 4096u + 4096): thread t loads entries 4t..4t+3
 // of every chunk as int4 (coalesced, all loads in flight at once), the
 // wavefronts scan their thread sums, wavefront 0 scans the (chunk,
 // wavefront) totals in chunk-major order, and every thread writes its
 // entries.  One launch against the library scan's two (state set-up and
 // scan), for the per-body pair counts and grid bucket counts of batches up
-// to 16383 bodies / buckets.
+// to 20479 bodies / buckets.
 template <int V>
 __global__ __launch_bounds__(1024) void k_scan_small(const int* __restrict__ in, int* out, int n) {
   constexpr int P = V * 16, E = (P + 63) / 64;
@@ -679,7 +679,7 @@ __global__ __launch_bounds__(1024) void k_scan_small(const int* __restrict__ in,
 }
 
 // out[i] = in[0] + ... + in[i-1] for i < n, on stream s: one workgroup for
-// n <= 16384 (int4-aligned arrays), the library scan beyond.
+// n <= 20480 (int4-aligned arrays), the library scan beyond.
 static hipError_t exclusive_sum(shp_batch* b, const int* in, int* out, long n, hipStream_t s) {
   if (n <= 4096) {
     k_scan_small<1><<<1, 1024, 0, s>>>(in, out, (int)n);
@@ -687,6 +687,8 @@ static hipError_t exclusive_sum(shp_batch* b, const int* in, int* out, long n, h
     k_scan_small<2><<<1, 1024, 0, s>>>(in, out, (int)n);
   } else if (n <= 16384) {
     k_scan_small<4><<<1, 1024, 0, s>>>(in, out, (int)n);
+  } else if (n <= 20480) {  // e.g. a power-of-two bucket table + 1
+    k_scan_small<5><<<1, 1024, 0, s>>>(in, out, (int)n);
   } else {  // (measured: from 8 chunks on, the wavefront scans cost more than the library's launch)
     size_t tb = b->scan_tmp_bytes;
     return hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, in, out, (int)n, s);
@@ -962,7 +964,7 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
         }
       int nw = body_off[w + 1] - body_off[w];
       long t = 64;
-      while (t < 2L * nw) t <<= 1;
+      while (t < nw) t <<= 1;
       tab_off[w] = (int32_t)tb_total;
       tb_total += t;
     }
