@@ -62,11 +62,10 @@ def make_scene(name: str, worlds: int = 0):
 # --------------------------------------------------------------- roofline
 
 # the kernels of a phase (its HIP-event time covers them; the coloured
-# solve's k_pair_prep and k_colour run on a second stream from the end of
-# the world ranges, and the phase waits for them)
+# solve's k_colour runs on a second stream from the end of k_sat, and the
+# phase waits for it)
 PHASE_KERNELS = {
-    ("solve", "colored"): ["k_pair_prep", "k_colour", "k_pack_manifolds", "k_sweep_colours",
-                           "k_colour_unpack"],
+    ("solve", "colored"): ["k_colour", "k_pack_manifolds", "k_sweep_colours", "k_colour_unpack"],
     ("solve", "ordered"): ["k_solve_flow"],
     ("sat", None): ["k_sat"],
     ("contacts", None): ["k_contacts"],

@@ -42,8 +42,6 @@ using namespace shp;
 // the serial tail below COLOR_CAP (and below the uint8 sentinel 0xFF)
 #define GREEDY_COLOURS 16
 #define JP_ROUNDS (COLOR_CAP - GREEDY_COLOURS - 3)
-#define NO_BODY 0xFFFFu
-#define NO_PAIR 0xFFFFFFFFu  // pair_ab of a pair without stored contacts
 #define LDS_MAX_BYTES (160 * 1024 - 512)
 
 // per-contact FP32 fields (stride contact_cap), then the FP64 lagrangians,
@@ -946,28 +944,6 @@ __global__ __launch_bounds__(T) void k_colour(Ctx c, int pool_bytes) {
   tmark(c, 1);
 }
 
-// Every pair of the batch: its world-local bodies (NO_BODY for a static
-// one) packed as a | b << 16, or NO_PAIR when it has no contacts.  The
-// colouring of worlds below 65535 bodies reads these words.  It runs right
-// after the narrowphase, beside the contact scan: manifolds whose contacts
-// fall beyond the contact capacity are coloured too and packed as records
-// that change nothing (the step reports the overflow).
-__global__ void k_pair_prep(Ctx c) {
-  const int tp = min(c.off_rank[c.NB], c.pair_cap);
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < tp; p += gridDim.x * blockDim.x) {
-    const int ga = c.pa[p], gb = c.pb[p];
-    const int w = c.body_world[ga];
-    uint32_t v = NO_PAIR;
-    if (c.ccount[p] > 0) {
-      const int b0 = c.body_off[w];
-      const uint32_t a = c.is_static[ga] ? NO_BODY : (uint32_t)(ga - b0) & 0xFFFFu;
-      const uint32_t b = c.is_static[gb] ? NO_BODY : (uint32_t)(gb - b0) & 0xFFFFu;
-      v = a | (b << 16);
-    }
-    c.pair_ab[p] = v;
-  }
-}
-
 // A manifold whose contacts lie beyond the contact capacity (the step then
 // reports the overflow): a record that touches no velocity and no contact.
 __device__ __forceinline__ void pack_null(const Ctx& c, const PackF& P, int p, int M) {
@@ -1144,8 +1120,6 @@ shp_status colored_alloc(shp_batch* b) {
 void launch_colouring(shp_batch* b, hipStream_t s) {
   const Ctx& c = b->ctx;
   const int pool = (int)(b->color_lds - (2 * (COLOR_CAP + 3) + 2) * 4);
-  const long tp = c.pair_cap;
-  k_pair_prep<<<std::max(1L, std::min((tp + 255) / 256, 4096L)), 256, 0, s>>>(c);
   if (b->color_threads >= 512)
     k_colour<1024><<<c.W, 1024, b->color_lds, s>>>(c, pool);
   else

@@ -16,6 +16,11 @@
 #include "../../include/shapes_gpu.h"
 #include "shp_mlp.h"
 
+// pair_ab words (coloured solver): world-local body ids a | b << 16, NO_BODY
+// for a static body, NO_PAIR for a pair without contacts
+#define NO_BODY 0xFFFFu
+#define NO_PAIR 0xFFFFFFFFu
+
 enum { CNT_FLAGS = 0, CNT_WARM = 1, CNT_LEARNED = 2, CNT_LEVELS = 3, CNT_MISS = 4 };
 // grid broadphase: partners per body kept by the count pass for the emit
 // pass (bodies with more are visited again)
@@ -104,7 +109,7 @@ struct Ctx {
   uint8_t* pk_f;  // flags: 1 a static, 2 b static, 4 warm, 8 pair continues
   unsigned long long* col_best;  // colouring priorities per body (large worlds)
   uint32_t* col_used;            // colours taken at each body (large worlds)
-  uint32_t* pair_ab;             // per pair: world-local bodies for the colouring
+  uint32_t* pair_ab;             // per pair: world-local bodies for the colouring (k_sat)
   int32_t* col_off;   // per world: COLOR_CAP + 3 colour offsets (manifolds)
   int32_t* col_info;  // per world: colours | serial tail << 16 (0: nothing to solve)
   // solver sub-phase timestamps of world 0 (wall_clock64), timing mode only

@@ -269,6 +269,17 @@ __device__ __forceinline__ void sat_pair(const Ctx& c, int p) {
     c.s_depth[s] = gm[k].depth;
   }
   c.ccount[p] = n;
+  if (c.solver == SHP_SOLVER_COLORED_F32) {  // the pair's word for the colouring
+    const int ga = c.pa[p], gb = c.pb[p];
+    uint32_t v = NO_PAIR;
+    if (n > 0) {
+      const int b0 = c.body_off[c.body_world[ga]];
+      const uint32_t a = c.is_static[ga] ? NO_BODY : (uint32_t)(ga - b0) & 0xFFFFu;
+      const uint32_t b = c.is_static[gb] ? NO_BODY : (uint32_t)(gb - b0) & 0xFFFFu;
+      v = a | (b << 16);
+    }
+    c.pair_ab[p] = v;
+  }
 }
 
 // grid-stride over this step's pairs (the launch is sized for the capacity)
