@@ -69,7 +69,9 @@ PHASE_KERNELS = {
     ("solve", "ordered"): ["k_solve_flow"],
     ("sat", None): ["k_sat"],
     ("contacts", None): ["k_contacts"],
-    ("integrate", None): ["k_integrate"],
+    # advance and the solve quality share one launch (in the quality phase)
+    ("quality", None): ["k_finish"],
+    ("integrate", None): ["k_finish"],
     ("broadphase_count", None): ["k_grid_assign", "k_grid_fill", "k_grid_pairs", "k_broadphase"],
     ("broadphase_fill", None): ["k_grid_pairs", "k_broadphase"],
 }

@@ -1142,83 +1142,4 @@ void launch_colored_sweeps(shp_batch* b) {
   k_colour_unpack<<<std::max(1L, std::min((n + 255) / 256, 4096L)), 256, 0, b->stream>>>(c);
 }
 
-// ---- solve quality -----------------------------------------------------------
 
-__device__ __forceinline__ unsigned long long nonneg_bits(double x) {
-  if (!(x >= 0.0)) x = x != x ? 1.0e300 : 0.0;  // NaN counts as huge
-  return (unsigned long long)__double_as_longlong(x);
-}
-
-__device__ __forceinline__ unsigned long long wave_max(unsigned long long x) {
-  for (int off = 32; off > 0; off >>= 1) {
-    unsigned long long o = __shfl_xor(x, off);
-    x = o > x ? o : x;
-  }
-  return x;
-}
-
-// Per contact after the last sweep (FP64 state): penetration depth, normal
-// complementarity residual |min(lambda_n, Jn.v + b)| and friction residual
-// |lambda_t - clamp(lambda_t - Jt.v / mt, +-mu*lambda_n)|; maxima of the step
-// and of the run.  Non-negative doubles order like their bit patterns, so the
-// maxima are integer maxima, reduced per workgroup before one atomic each.
-#define QUALITY_THREADS 256
-__global__ __launch_bounds__(QUALITY_THREADS) void k_quality(Ctx c) {
-  int tc = c.wcont_off[c.W];
-  unsigned long long qd = 0, qn = 0, qt = 0;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < tc; k += gridDim.x * blockDim.x) {
-    int ga = c.ga[k], gb = c.gb[k];
-    double v[6], jn[6], jt[6];
-    load_vel(c, ga, gb, v);
-    load_rows(c, k, jn, jt);
-    d2 ra = mk(c.rax[k], c.ray[k]), rb = mk(c.rbx[k], c.rby[k]);
-    d2 nr = c.flip[k] ? mk(-c.cnx[k], -c.cny[k]) : mk(c.cnx[k], c.cny[k]);
-    d2 closing = add(add(add(neg(mk(v[0], v[1])), zcross(-v[2], ra)), mk(v[3], v[4])), zcross(v[5], rb));
-    double e = hs_min(c.e[ga], c.e[gb]);
-    double bn = c.bn[k] + hs_min(0.0, e * dot(closing, nr));
-    double cn = dot6(jn, v) + bn;
-    double ln = c.ln[k], lt = c.lt[k];
-    double rn = fabs(hs_min(ln, cn));
-    double mu = (c.mu[ga] + c.mu[gb]) / 2.0;
-    double thr = ln * mu;
-    double t = lt - dot6(jt, v) / c.mct[k];
-    double tcl = t > thr ? thr : (t < -thr ? -thr : t);
-    double rt = fabs(lt - tcl);
-    unsigned long long a = nonneg_bits(c.depth[k]), b = nonneg_bits(rn), d = nonneg_bits(rt);
-    qd = a > qd ? a : qd;
-    qn = b > qn ? b : qn;
-    qt = d > qt ? d : qt;
-  }
-  // wavefront, then workgroup maxima: one set of atomics per workgroup
-  __shared__ unsigned long long s_q[3][QUALITY_THREADS / 64];
-  const int wid = threadIdx.x >> 6;
-  qd = wave_max(qd);
-  qn = wave_max(qn);
-  qt = wave_max(qt);
-  if ((threadIdx.x & 63) == 0) {
-    s_q[0][wid] = qd;
-    s_q[1][wid] = qn;
-    s_q[2][wid] = qt;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  for (int q = 1; q < QUALITY_THREADS / 64; ++q) {
-    qd = s_q[0][q] > qd ? s_q[0][q] : qd;
-    qn = s_q[1][q] > qn ? s_q[1][q] : qn;
-    qt = s_q[2][q] > qt ? s_q[2][q] : qt;
-  }
-  if (qd | qn | qt) {
-    atomicMax(&c.quality[Q_DEPTH], qd);
-    atomicMax(&c.quality[Q_NRES], qn);
-    atomicMax(&c.quality[Q_FRES], qt);
-    atomicMax(&c.quality[Q_RUN_DEPTH], qd);
-    atomicMax(&c.quality[Q_RUN_NRES], qn);
-    atomicMax(&c.quality[Q_RUN_FRES], qt);
-  }
-}
-
-void launch_quality(shp_batch* b) {
-  long n = b->ctx.contact_cap;
-  int grid = (int)std::max<long>(1, std::min<long>((n + 255) / 256, 2048));
-  k_quality<<<grid, QUALITY_THREADS, 0, b->stream>>>(b->ctx);
-}

@@ -195,7 +195,6 @@ void launch_ordered_solver(shp_batch* b);
 // after the contacts, manifold packing and the sweeps
 void launch_colouring(shp_batch* b, hipStream_t s);
 void launch_colored_sweeps(shp_batch* b);
-void launch_quality(shp_batch* b);
 hipError_t launch_grid_count(shp_batch* b);
 hipError_t scan_grid_buckets(shp_batch* b);
 hipError_t launch_grid_emit(shp_batch* b);

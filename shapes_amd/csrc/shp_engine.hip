@@ -17,10 +17,12 @@
 //   k_contacts       compaction, constraintGen, warm-start join (beside it,
 //                    on a second stream, the coloured solver's k_colour)
 //   solve            warm-start application + Gauss-Seidel sweeps
-//                    (k_solve_flow, or k_pack_manifolds + k_sweep_colours)
-//   k_quality        solve quality (maxima, run sums)
-//   k_cache_store    this frame's (key, lambda) become next frame's cache
-//   k_integrate      advance + moveShapes + AABBs
+//                    (k_solve_flow, or k_pack_manifolds + k_sweep_colours +
+//                    k_colour_unpack)
+//   k_finish         advance + moveShapes + AABBs (k_integrate's tiles) and
+//                    the solve quality maxima, in one launch
+//   k_cache_store    this frame's (key, lambda) become next frame's cache,
+//                    quality run sums
 //
 // Worlds are independent: every kernel handles all worlds of the batch at
 // once, and the ordered solver runs one workgroup per world.
@@ -117,13 +119,14 @@ __global__ void k_init_hulls(Ctx c) {
 constexpr int INTEGRATE_THREADS = 256;
 constexpr int INTEGRATE_VERTS = INTEGRATE_THREADS * SHP_MAXV;
 
-__global__ __launch_bounds__(INTEGRATE_THREADS) void k_integrate(Ctx c, int advance) {
+// The bodies [vb * INTEGRATE_THREADS, ...) of the batch (one workgroup).
+__device__ __forceinline__ void integrate_tile(const Ctx& c, int advance, int vb) {
   __shared__ double s_m[6][INTEGRATE_THREADS];
   __shared__ double s_wx[INTEGRATE_VERTS], s_wy[INTEGRATE_VERTS];
   __shared__ uint8_t s_body[INTEGRATE_VERTS];
   __shared__ int s_voff[INTEGRATE_THREADS + 1];
   const int t = threadIdx.x;
-  const int g0 = blockIdx.x * INTEGRATE_THREADS;
+  const int g0 = vb * INTEGRATE_THREADS;
   const int nbk = min(INTEGRATE_THREADS, c.NB - g0);
   const int g = g0 + t;
   const double one = 1.0, zer = 0.0;
@@ -192,6 +195,97 @@ __global__ __launch_bounds__(INTEGRATE_THREADS) void k_integrate(Ctx c, int adva
     c.bminy[g] = bminy;
     c.bmaxy[g] = bmaxy;
   }
+}
+
+// Per contact after the solve (FP64 state): penetration depth, normal
+// complementarity residual |min(lambda_n, Jn.v + b)| and friction residual
+// |lambda_t - clamp(lambda_t - Jt.v / mt, +-mu*lambda_n)|; maxima of the step
+// and of the run.  Non-negative doubles order like their bit patterns, so the
+// maxima are integer maxima, reduced per workgroup before one atomic each.
+// Workgroup vb of nvb (grid-stride over the step's contacts).
+#define QUALITY_THREADS 256
+
+__device__ __forceinline__ unsigned long long nonneg_bits(double x) {
+  if (!(x >= 0.0)) x = x != x ? 1.0e300 : 0.0;  // NaN counts as huge
+  return (unsigned long long)__double_as_longlong(x);
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(x, off);
+    x = o > x ? o : x;
+  }
+  return x;
+}
+
+__device__ __forceinline__ void quality_tile(const Ctx& c, int vb, int nvb) {
+  int tc = c.wcont_off[c.W];
+  unsigned long long qd = 0, qn = 0, qt = 0;
+  for (int k = vb * QUALITY_THREADS + threadIdx.x; k < tc; k += nvb * QUALITY_THREADS) {
+    int ga = c.ga[k], gb = c.gb[k];
+    double v[6], jn[6], jt[6];
+    load_vel(c, ga, gb, v);
+    load_rows(c, k, jn, jt);
+    d2 ra = mk(c.rax[k], c.ray[k]), rb = mk(c.rbx[k], c.rby[k]);
+    d2 nr = c.flip[k] ? mk(-c.cnx[k], -c.cny[k]) : mk(c.cnx[k], c.cny[k]);
+    d2 closing = add(add(add(neg(mk(v[0], v[1])), zcross(-v[2], ra)), mk(v[3], v[4])), zcross(v[5], rb));
+    double e = hs_min(c.e[ga], c.e[gb]);
+    double bn = c.bn[k] + hs_min(0.0, e * dot(closing, nr));
+    double cn = dot6(jn, v) + bn;
+    double ln = c.ln[k], lt = c.lt[k];
+    double rn = fabs(hs_min(ln, cn));
+    double mu = (c.mu[ga] + c.mu[gb]) / 2.0;
+    double thr = ln * mu;
+    double t = lt - dot6(jt, v) / c.mct[k];
+    double tcl = t > thr ? thr : (t < -thr ? -thr : t);
+    double rt = fabs(lt - tcl);
+    unsigned long long a = nonneg_bits(c.depth[k]), b = nonneg_bits(rn), d = nonneg_bits(rt);
+    qd = a > qd ? a : qd;
+    qn = b > qn ? b : qn;
+    qt = d > qt ? d : qt;
+  }
+  // wavefront, then workgroup maxima: one set of atomics per workgroup
+  __shared__ unsigned long long s_q[3][QUALITY_THREADS / 64];
+  const int wid = threadIdx.x >> 6;
+  qd = wave_max_u64(qd);
+  qn = wave_max_u64(qn);
+  qt = wave_max_u64(qt);
+  if ((threadIdx.x & 63) == 0) {
+    s_q[0][wid] = qd;
+    s_q[1][wid] = qn;
+    s_q[2][wid] = qt;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int q = 1; q < QUALITY_THREADS / 64; ++q) {
+    qd = s_q[0][q] > qd ? s_q[0][q] : qd;
+    qn = s_q[1][q] > qn ? s_q[1][q] : qn;
+    qt = s_q[2][q] > qt ? s_q[2][q] : qt;
+  }
+  if (qd | qn | qt) {
+    atomicMax(&c.quality[Q_DEPTH], qd);
+    atomicMax(&c.quality[Q_NRES], qn);
+    atomicMax(&c.quality[Q_FRES], qt);
+    atomicMax(&c.quality[Q_RUN_DEPTH], qd);
+    atomicMax(&c.quality[Q_RUN_NRES], qn);
+    atomicMax(&c.quality[Q_RUN_FRES], qt);
+  }
+}
+
+__global__ __launch_bounds__(INTEGRATE_THREADS) void k_integrate(Ctx c, int advance) {
+  integrate_tile(c, advance, blockIdx.x);
+}
+
+// The end of a step in one launch: workgroups [0, ni) advance the bodies
+// (k_integrate's tiles, first so that the longer chain starts first), the
+// rest compute the solve quality.  The two read the solved velocities and
+// write disjoint data (positions, transforms and AABBs; quality maxima).
+static_assert(QUALITY_THREADS == INTEGRATE_THREADS, "one workgroup size");
+__global__ __launch_bounds__(INTEGRATE_THREADS) void k_finish(Ctx c, int ni) {
+  if ((int)blockIdx.x < ni)
+    integrate_tile(c, 1, blockIdx.x);
+  else
+    quality_tile(c, blockIdx.x - ni, gridDim.x - ni);
 }
 
 // Broadphase: one wavefront per body i scans j = i-1 .. 0 of its world in
@@ -464,7 +558,7 @@ __global__ void k_contacts(Ctx c) {
 // This frame's lagrangians become the next frame's EngineCache
 // (Engine/Main.hs:32, wrapInitializer :61-69).
 __global__ void k_cache_store(Ctx c) {
-  // the step's quality maxima are final (k_quality ran before): add them to
+  // the step's quality maxima are final (k_finish ran before): add them to
   // the run sums, whose means over the run are less sensitive to a single
   // impact than the run maxima
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -593,14 +687,16 @@ static shp_status enqueue_step(shp_batch* b) {
     launch_ordered_solver(b);
   }
   mark(b, 9);
-  launch_quality(b);
   if (b->logging) mlp_launch_log(b);
+  {  // advance + moveShapes beside the solve quality, one launch
+    const int ni = blocks(c.NB, INTEGRATE_THREADS);
+    const int nq = (int)std::max<long>(1, std::min<long>((c.contact_cap + 255) / 256, 2048));
+    k_finish<<<ni + nq, INTEGRATE_THREADS, 0, s>>>(c, ni);
+  }
   mark(b, 10);
+  // the run sums read the quality maxima
   k_cache_store<<<gs_blocks(std::max(c.contact_cap, c.W + 1)), 256, 0, s>>>(c);
   mark(b, 11);
-  // (on the second stream beside the quality and cache store, measured
-  // slower: the fork and join cost more than the kernel)
-  k_integrate<<<blocks(c.NB, INTEGRATE_THREADS), INTEGRATE_THREADS, 0, s>>>(c, 1);
   mark(b, 12);
   HIPCHK(hipGetLastError());
   return SHP_OK;
