@@ -1,26 +1,38 @@
 """Benchmark of the contact-physics step on one MI355X.
 
-Default workload: one large world of 8192 random convex 3-8-gons dropped
-onto a 200-wide static floor, dt = 1/60, 600 steps (scenes.polygon_pile).
-A "step" is one updateWorld of that world; body-steps/s counts dynamic
-bodies.  Each timed region replays K steps from the initial state (as HIP
-graphs) after W warm-up steps on a separate instance of the same scene.
+Headline (no flags): body-steps/s at fixed penetration/residual, batched +
+learned warm start against the plain GPU solver, on the combined-at-scale
+workload: 1024 independent worlds of 2k-32k random 3-8-gons stacked in
+piles up to 60 layers (11.3 M bodies), coloured FP32 solver, dt 1/120, 4
+sweeps (scenes.pile_batch).  A "step" is one updateWorld of every world of
+the batch; body-steps/s counts dynamic bodies.
 
-Default mode ("headline"): body-steps/s at fixed penetration/residual.  The
-plain GPU solver (coloured, zero start for new contacts, the workload's 8
-sweeps) sets the quality: the run means of the per-step maximum
-penetration and normal residual.  The learned warm start then runs at 8,
-7, 6, ... sweeps while both means stay within 2% of the plain run's; the
-value is the body-steps/s of the fewest sweeps that pass (at least the
-workload's sweep count when none below it does), next to the plain run's.
+Every run creates a batch from the initial state, steps it through an
+untimed pre-roll (`--warmup` plus the workload's settling steps, half of its
+configured run) so that the timed window sits in the settled regime
+whatever `--steps` is, then times exactly `--steps` graph-replayed steps
+with inputs resident in HBM.  Quality is compared over the whole run
+(pre-roll and window): the run means of the per-step maximum penetration
+and normal residual.
 
-"--mode single" times one configuration (--solver, --warm, --iterations).
+1. autotune: broadphase x solver lanes per world, timed in the settled
+   regime (plain solver);
+2. plain GPU: zero start at the workload's sweeps;
+3. learned warm start at the same and then fewer sweeps while its quality
+   stays within 2% of the plain run's: `value` is the body-steps/s of the
+   fewest sweeps that match, or the plain run's when none does
+   (`headline.value_is` says which);
+4. the FP64 drop-in solver over the same steps: quality and the divergence
+   of each run's final positions, beside the FP64 run's own chaos floor;
+5. the learned warm start on the 8192-polygon world at 2 and 4 sweeps
+   against plain at 8 (`c4_learned_warm_start`);
+6. an instrumented replay (eager launches, HIP events on the stream each
+   kernel runs on) of the plain configuration after its window: per-phase
+   times and the roofline of the dominant kernel;
+7. the CPU oracle on this host over the same step window on a sample of
+   the worlds (`cpu_baseline`).
 
-Prints ONE JSON line: the value, the roofline of the dominant kernel
-(algorithmic bytes per launch / its average duration, measured with HIP
-events on the batch's stream in an instrumented replay of the same steps),
-the solve quality, and the CPU oracle timed on this host on a bounded
-sample of the same workload.
+Prints ONE JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -39,301 +51,33 @@ from shapes_amd import engine, scenes  # noqa: E402
 
 METRIC = "body-steps/s at fixed penetration/residual: batched+learned vs plain GPU"
 HBM_PEAK_GBS = 8000.0
+QUALITY_TOL = 0.02
 
 SOLVERS = {"ordered": engine.SOLVER_ORDERED_F64, "colored": engine.SOLVER_COLORED_F32}
 DTYPES = {"ordered": "f64", "colored": "f32"}
-DTYPES_BYTES = {"ordered": 8, "colored": 4}
-
-
-def make_scene(name: str, worlds: int = 0):
-    if name == "polygon_pile":
-        return scenes.polygon_pile()
-    if name == "world_batch":
-        return scenes.world_batch_fast(n_worlds=worlds or 4096)
-    if name == "pile_batch":
-        return scenes.pile_batch(n_worlds=worlds or 1024)
-    if name == "pyramid":
-        return scenes.pyramid()
-    if name == "reference_stacks":
-        return scenes.reference_stacks()
-    raise SystemExit(f"unknown workload {name}")
-
-
-# --------------------------------------------------------------- roofline
-
-# the kernels of a phase (its HIP-event time covers them; the coloured
-# solve's k_colour runs on a second stream from the end of k_sat, and the
-# phase waits for it)
-PHASE_KERNELS = {
-    ("solve", "colored"): ["k_colour", "k_pack_manifolds", "k_sweep_colours", "k_colour_unpack"],
-    ("solve", "ordered"): ["k_solve_flow"],
-    ("sat", None): ["k_sat"],
-    ("contacts", None): ["k_contacts"],
-    # advance and the solve quality share one launch (in the quality phase)
-    ("quality", None): ["k_finish"],
-    ("integrate", None): ["k_finish"],
-    ("broadphase_count", None): ["k_grid_assign", "k_grid_fill", "k_grid_pairs", "k_broadphase"],
-    ("broadphase_fill", None): ["k_grid_pairs", "k_broadphase"],
-}
-
-
-def kernel_bytes(phase: str, st: dict, nb: int, nv: int, word: int) -> float:
-    """Algorithmic (compulsory) HBM bytes of one launch of a phase, from
-    the step's counts: nb bodies, nv hull vertices, st['pairs'],
-    st['contacts'] and the solver's passes.  FP64 words are 8 bytes."""
-    P, C = st["pairs"], st["contacts"]
-    passes = st["passes"]
-    if phase == "integrate":
-        # read pos, theta, vel, omega (6W) + write pos, theta (3W) + vert
-        # offsets; read local verts (2W/vertex), write world verts and
-        # normals (4W/vertex) and the AABB (4W)
-        return nb * (9 * word + 4 + 4 * word) + nv * 6 * word
-    if phase in ("broadphase_count", "broadphase_fill"):
-        out = 4 * nb if phase == "broadphase_count" else 8 * P
-        return nb * (4 * word + 1 + 8) + out
-    if phase == "sat":
-        # pair ids, both hulls' verts + normals + extents, <= 2 slots out
-        return P * (8 + 2 * 8) + 2 * (nv / max(nb, 1)) * P * (4 * word + 2) + C * 50 + 4 * P
-    if phase == "contacts":
-        # slot in, body positions and inverse masses, cache lookup; out:
-        # key, bodies, flip, geometry, two 6-rows, bias, radii, masses, lambdas
-        return C * (50 + 2 * 4 * word + 24) + C * (8 + 8 + 1 + 5 * word + 12 * word +
-                                                   word + 4 * word + 2 * word + 2 * 8 + 1)
-    if phase == "solve":
-        # compulsory traffic of one launch: per contact its bodies and warm
-        # flag, the normal, the four cross terms, bias, lever arms, two
-        # effective masses and two lambdas read (15 FP64 words) and two
-        # lambdas written; per body the velocity read and written and the
-        # inverse masses and materials read.  Re-reads of the same records
-        # and velocities across sweeps are served by LDS and L2.
-        return C * (9 + 15 * 8 + 2 * 8) + nb * (6 * 8 + 4 * 8 + 1)
-    return 0.0
-
-
-def instrumented_run(scene, solver, steps, broadphase="auto", ppb=0, warm=engine.WARM_ZERO,
-                     model=None, threads=0):
-    w = scene.worlds
-    b = engine.make_batch(w, scene.config, solver=solver, broadphase=BROADPHASES[broadphase],
-                          max_pairs_per_body=ppb, warm_start=warm, solver_threads=threads)
-    if warm == engine.WARM_LEARNED:
-        b.set_mlp(model)
-    b.set_timing(True)
-    tot = {"pairs": 0, "contacts": 0}
-    for _ in range(steps):
-        b.step(1)
-        s = b.stats()
-        tot["pairs"] += s["pairs"]
-        tot["contacts"] += s["contacts"]
-    t = b.timing()
-    t["solve_profile_us"] = b.solve_profile()
-    b.close()
-    n = max(t["steps"], 1)
-    return t, {k: v / n for k, v in tot.items()}
-
-
-def step_bytes(nb: int, nv: int, contacts: float, word: int) -> float:
-    """Compulsory HBM bytes of one whole step: every body's state and
-    materials read, pos/theta/vel/omega written ((10 + 6) words), its local
-    hull read (2 words per vertex), and the warm-start cache read and
-    written (packed key + two FP64 lambdas = 24 B each way per contact)."""
-    return nb * 16 * word + nv * 2 * word + contacts * 48
-
-
-def measured_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of `kernel` on `workload` measured with
-    rocprofv3 FETCH_SIZE and WRITE_SIZE counters (each scaled by the same
-    counter on a copy of known size), from profiles/pmc_<kernel>_<workload>.md;
-    None when no such measurement exists."""
-    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.md")
-    if not os.path.exists(path):
-        return None
-    with open(path) as fh:
-        for line in fh:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
-    return None
-
-
-def roofline(scene, solver_name, steps, broadphase="auto", ppb=0, warm=engine.WARM_ZERO,
-             model=None, threads=0):
-    t, st = instrumented_run(scene, SOLVERS[solver_name], steps, broadphase, ppb, warm, model,
-                             threads)
-    nb = scene.n_bodies
-    nv = sum(int(w.hull_off[-1]) for w in scene.worlds)
-    word = 8 if solver_name == "ordered" else 4
-    st["passes"] = scene.config.iterations + 1
-    phases = [p for p in engine.PHASES if p not in ("reset",)]
-    dom = max(phases, key=lambda p: t[p])
-    ms = t[dom]
-    by = kernel_bytes(dom, st, nb, nv, 8 if dom == "solve" else word)
-    achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    kname = f"solve_{solver_name}" if dom == "solve" else dom
-    return {
-        "kernel": dom,
-        "kernels": PHASE_KERNELS.get((dom, solver_name), PHASE_KERNELS.get((dom, None), [])),
-        "bound": "hbm",
-        "achieved": round(achieved, 3),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 6),
-        "traffic": measured_traffic(kname, scene.name),
-        "algorithmic_bytes_per_launch": round(by),
-        "avg_launch_ms": round(ms, 5),
-    }, {**{p: round(t[p], 5) for p in phases}, "solve_profile_us": t["solve_profile_us"]}, st
-
-
-def copy_calibration(nbytes=1 << 30, reps=3):
-    """A plain device-to-device copy of a known size: what FETCH_SIZE and
-    WRITE_SIZE read per dispatch for it gives their scale to bytes."""
-    import torch
-    x = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-    x.fill_(1)
-    torch.cuda.synchronize()
-    for _ in range(reps):
-        y = x.clone()
-        del y
-    torch.cuda.synchronize()
-    print(f"copied {nbytes} bytes x{reps}", flush=True)
-
-
-# ------------------------------------------------------------ CPU baseline
-
-def cpu_baseline(scene, sample_steps, max_worlds=64):
-    """The C++ FP64 oracle on this host: one world on one thread, or for a
-    batch the first `max_worlds` worlds over the host's cores (at most 16)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-
-    oracle.build()
-    worlds = scene.worlds[:max_worlds]
-    threads = 1 if len(worlds) == 1 else max(1, min(16, os.cpu_count() or 1, len(worlds)))
-    engines = [oracle.OracleEngine(w, scene.config) for w in worlds]
-    secs = oracle.run_engines(engines, sample_steps, threads=threads)
-    for e in engines:
-        e.close()
-    dyn = sum(w.n_dynamic for w in worlds)
-    what = (f"{scene.name} ({dyn} dynamic bodies)" if len(worlds) == 1 else
-            f"worlds 0-{len(worlds) - 1} of {scene.name} ({dyn} dynamic bodies)")
-    return {
-        "value": round(dyn * sample_steps / secs, 1),
-        "unit": "body-steps/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"first {sample_steps} of {scene.config.steps} steps of {what}, "
-                  f"C++ FP64 oracle, {threads} thread(s), {secs:.1f} s",
-    }
-
-
 BROADPHASES = {"auto": engine.BROADPHASE_AUTO, "scan": engine.BROADPHASE_SCAN,
                "grid": engine.BROADPHASE_GRID}
 
-
-def autotune(scene, solver, ppb, steps=20, warm_steps=10):
-    """Pick the schedule by measurement: broadphase (wavefront scan or
-    uniform grid) x lanes per world in the solve (256 or 512, and 1024 for
-    the coloured solver).  Each candidate steps the workload from its
-    initial state (warm_steps untimed, then `steps` timed graph replays);
-    the fastest wins."""
-    times = {}
-    lanes = (256, 512, 1024) if solver == engine.SOLVER_COLORED_F32 else (256, 512)
-    for name in ("scan", "grid"):
-        for threads in lanes:
-            b = engine.make_batch(scene.worlds, scene.config, solver=solver,
-                                  broadphase=BROADPHASES[name], max_pairs_per_body=ppb,
-                                  solver_threads=threads)
-            b.step_async(warm_steps, use_graph=True)
-            b.sync()
-            t0 = time.perf_counter()
-            b.step_async(steps, use_graph=True)
-            b.sync()
-            times[(name, threads)] = (time.perf_counter() - t0) / steps * 1e3
-            b.close()
-    best = min(times, key=times.get)
-    return best, {f"{k[0]}/{k[1]}": round(v, 4) for k, v in times.items()}
+# workload -> (scene factory, settling pre-roll, pair capacity per body,
+# autotune candidates, description)
+WORKLOADS = {
+    "pile_batch": (lambda w: scenes.pile_batch(n_worlds=w or 1024), 500, 4,
+                   [("grid", 512), ("grid", 1024)],
+                   "worlds of 2k-32k random 3-8-gons stacked in piles up to 60 layers"),
+    "world_batch": (lambda w: scenes.world_batch_fast(n_worlds=w or 4096), 150, 4,
+                    [(b, t) for b in ("grid", "scan") for t in (256, 512, 1024)],
+                    "worlds of 8-1024 bodies (squares and 3-8-gons), each on its own floor"),
+    "polygon_pile": (lambda w: scenes.polygon_pile(), 300, 0,
+                     [(b, t) for b in ("grid", "scan") for t in (256, 512, 1024)],
+                     "one world of 8192 random 3-8-gons dropped onto a 200-wide floor"),
+    "pyramid": (lambda w: scenes.pyramid(), 150, 0, [("scan", 256)],
+                "pyramid of 55 unit boxes on a static floor"),
+}
 
 
-def run_mode(scene, solver, warm, steps, warmup, graph=True, model=None, broadphase="auto",
-             ppb=0, threads=0):
-    """Warm up on a separate instance, then time `steps` steps from the
-    initial state; returns (seconds, run stats)."""
-    def make():
-        b = engine.make_batch(scene.worlds, scene.config, solver=solver, warm_start=warm,
-                              broadphase=BROADPHASES[broadphase], max_pairs_per_body=ppb,
-                              solver_threads=threads)
-        if warm == engine.WARM_LEARNED:
-            b.set_mlp(model)
-        return b
-
-    if warmup > 0:
-        wb = make()
-        wb.step_async(warmup, use_graph=graph)
-        wb.sync()
-        wb.close()
-    b = make()
-    b.sync()
-    t0 = time.perf_counter()
-    b.step_async(steps, use_graph=graph)
-    b.sync()
-    elapsed = time.perf_counter() - t0
-    st = b.stats()
-    b.close()
-    return elapsed, st
-
-
-def final_positions(scene, solver, warm, iterations, steps, model=None, broadphase="auto",
-                    ppb=0, threads=0, worlds=None):
-    """Positions (x, y) of every body of every world after `steps` steps, and
-    the run's quality."""
-    worlds = worlds if worlds is not None else scene.worlds
-    cfg = scenes.SceneConfig(**{**scene.config.__dict__, "iterations": iterations})
-    b = engine.make_batch(worlds, cfg, solver=solver, warm_start=warm,
-                          broadphase=BROADPHASES[broadphase], max_pairs_per_body=ppb,
-                          solver_threads=threads)
-    if warm == engine.WARM_LEARNED:
-        b.set_mlp(model)
-    b.step_async(steps, use_graph=True)
-    b.sync()
-    st = b.stats()
-    pos = [b.bodies(k)[:, :2].copy() for k in range(len(worlds))]
-    b.close()
-    return pos, quality(st)
-
-
-def divergence(pa, pb, worlds):
-    """Distance between two runs' final positions over the dynamic bodies."""
-    d = np.concatenate([np.hypot(a[:, 0] - b[:, 0], a[:, 1] - b[:, 1])[(w.mass != 0) | (w.inertia != 0)]
-                        for a, b, w in zip(pa, pb, worlds)])
-    return {"max": round(float(d.max()), 6), "rms": round(float(np.sqrt((d ** 2).mean())), 6)}
-
-
-def fp64_reference(scene, steps, runs, bp, ppb, model):
-    """The FP64 drop-in solver (the reference's order, bit-exact to the
-    oracle) at the workload's sweeps from the same initial state: its
-    quality, and the divergence of each measured run's final positions from
-    it.  `chaos_floor` is the divergence of the FP64 run itself when every
-    dynamic body's initial x-velocity is moved by 1e-12: what any change of
-    arithmetic leads to in these scenes."""
-    base_it = scene.config.iterations
-    ref, q_ref = final_positions(scene, engine.SOLVER_ORDERED_F64, engine.WARM_ZERO, base_it,
-                                 steps, broadphase=bp, ppb=ppb)
-    moved = []
-    for w in scene.worlds:
-        m = w.copy()
-        dynm = (m.mass != 0) | (m.inertia != 0)
-        m.vx = m.vx + np.where(dynm, 1e-12, 0.0)
-        moved.append(m)
-    pert, _ = final_positions(scene, engine.SOLVER_ORDERED_F64, engine.WARM_ZERO, base_it, steps,
-                              broadphase=bp, ppb=ppb, worlds=moved)
-    out = {"solver": "ordered", "iterations": base_it, "steps": steps, "quality": q_ref,
-           "chaos_floor": divergence(pert, ref, scene.worlds), "runs": []}
-    for warm, it, threads in runs:
-        pos, q = final_positions(scene, engine.SOLVER_COLORED_F32, warm, it, steps, model=model,
-                                 broadphase=bp, ppb=ppb, threads=threads)
-        out["runs"].append({"warm_start": "learned" if warm == engine.WARM_LEARNED else "zero",
-                            "iterations": it, "divergence": divergence(pos, ref, scene.worlds),
-                            "mean_step_max_penetration": q["mean_step_max_penetration"],
-                            "mean_step_normal_residual": q["mean_step_normal_residual"]})
-    return out
+def note(msg: str) -> None:
+    """Progress of a long run on stderr (stdout carries the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def quality(st):
@@ -345,14 +89,6 @@ def quality(st):
             "friction_residual": st["run_friction_residual"]}
 
 
-def note(msg: str) -> None:
-    """Progress of a long run on stderr (stdout carries the JSON line)."""
-    print(f"bench: {msg}", file=sys.stderr, flush=True)
-
-
-QUALITY_TOL = 0.02
-
-
 def matches(q, ref, tol=QUALITY_TOL):
     """Run means of the per-step maximum penetration and normal residual
     within `tol` of the reference run's."""
@@ -360,108 +96,334 @@ def matches(q, ref, tol=QUALITY_TOL):
             q["mean_step_normal_residual"] <= ref["mean_step_normal_residual"] * (1 + tol))
 
 
-# quality is compared over the headline world and these further seeds of
-# the same generator (not the warm-start model's training seeds): a single
-# chaotic run's means move by a few percent with any change of arithmetic
-QUALITY_SEEDS = (1003, 1004)
-
-
-def quality_over_seeds(scene, extra, st, warm, model, bp, ppb, threads, steps):
-    """The run means of the headline world's run `st` and of untimed runs of
-    the `extra` worlds in the same configuration, averaged."""
-    rows = [quality(st)]
-    for sc in extra:
-        sc.config.iterations = scene.config.iterations
-        _, st_x = run_mode(sc, engine.SOLVER_COLORED_F32, warm, steps, 0, graph=True,
-                           model=model, broadphase=bp, ppb=ppb, threads=threads)
-        rows.append(quality(st_x))
-    keys = ("mean_step_max_penetration", "mean_step_normal_residual",
-            "mean_step_friction_residual")
-    out = {k: float(np.mean([r[k] for r in rows])) for k in keys}
-    out["per_world"] = [{k: r[k] for k in keys[:2]} for r in rows]
+def run(scene, solver, warm, iterations, preroll, steps, sched, ppb, model=None,
+        positions=False, instrument=0):
+    """One batch from the initial state: `preroll` untimed graph-replayed
+    steps, then exactly `steps` timed ones; optionally `instrument` eager
+    steps with HIP-event timing after them.  Returns a dict with the timed
+    seconds, the run's stats and quality, final positions (at the end of
+    the timed window) and the instrumented phase times."""
+    cfg = scenes.SceneConfig(**{**scene.config.__dict__, "iterations": iterations})
+    b = engine.make_batch(scene.worlds, cfg, solver=solver, warm_start=warm,
+                          broadphase=BROADPHASES[sched[0]], max_pairs_per_body=ppb,
+                          solver_threads=sched[1])
+    try:
+        if warm == engine.WARM_LEARNED:
+            b.set_mlp(model)
+        if preroll:
+            b.step_async(preroll, use_graph=True)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step_async(steps, use_graph=True)
+        b.sync()
+        elapsed = time.perf_counter() - t0
+        st = b.stats()
+        out = {"seconds": elapsed, "stats": st, "quality": quality(st)}
+        if positions:
+            out["positions"] = [b.bodies(k)[:, :2].copy() for k in range(len(scene.worlds))]
+        if instrument:
+            b.set_timing(True)
+            tot = {"pairs": 0, "contacts": 0}
+            for _ in range(instrument):
+                b.step(1)
+                s = b.stats()
+                tot["pairs"] += s["pairs"]
+                tot["contacts"] += s["contacts"]
+            t = b.timing()
+            out["phases_ms"] = {p: t[p] for p in engine.PHASES}
+            out["instrumented"] = {k: v / instrument for k, v in tot.items()}
+            out["instrumented"]["steps"] = instrument
+    finally:
+        b.close()
     return out
 
 
-def headline(scene, args, model, bp, ppb, dyn, threads):
+def autotune(scene, ppb, candidates, preroll, steps=30):
+    """Pick broadphase x solver lanes per world by timing `steps` graph
+    replays of the plain coloured solver after `preroll` untimed steps."""
+    times = {}
+    for sched in candidates:
+        r = run(scene, engine.SOLVER_COLORED_F32, engine.WARM_ZERO, scene.config.iterations,
+                preroll, steps, sched, ppb)
+        times[sched] = r["seconds"] / steps * 1e3
+        note(f"autotune {sched[0]}/{sched[1]}: {times[sched]:.3f} ms/step")
+    best = min(times, key=times.get)
+    return best, {f"{k[0]}/{k[1]}": round(v, 4) for k, v in times.items()}
+
+
+def divergence(pa, pb, worlds):
+    """Distance between two runs' final positions over the dynamic bodies."""
+    d = np.concatenate([np.hypot(a[:, 0] - b[:, 0], a[:, 1] - b[:, 1])[(w.mass != 0) | (w.inertia != 0)]
+                        for a, b, w in zip(pa, pb, worlds)])
+    return {"max": round(float(d.max()), 6), "rms": round(float(np.sqrt((d ** 2).mean())), 6)}
+
+
+# --------------------------------------------------------------- roofline
+
+def kernel_bytes(phase, nb, nv, st, word):
+    """Algorithmic (compulsory) HBM bytes of one launch of a phase's kernel,
+    from the step's counts: nb bodies, nv hull vertices, st['pairs'],
+    st['contacts'] (manifolds ~ contacts / 1.6 are bounded by pairs)."""
+    P, C = st["pairs"], st["contacts"]
+    M = min(P, C)  # manifolds: pairs with contacts
+    if phase == "sweeps":
+        # per manifold its FP32 record (normal, materials, normal x inverse
+        # masses: 32 B; bodies 8 B; flags 1 B; two contact slots of 11 FP32
+        # fields and two FP64 lagrangians: 120 B) read and the lagrangians
+        # written (32 B); per body the FP64 velocity read (24 B) and the FP32
+        # velocity written (12 B)
+        return M * (41 + 120 + 32) + nb * (24 + 12)
+    if phase == "colouring":
+        # per pair its body word and contact count read, colour and slot
+        # written (16 B); per body its colour mask and priority (12 B)
+        return P * 16 + nb * 12
+    if phase == "finish":
+        # advance + transform: state read (6W) and pose written (3W), local
+        # vertices read (2W), world vertices and normals written (4W), AABB
+        # written (4W); quality: per contact bodies, velocities, rows,
+        # radii, normal, bias, lagrangians, masses, materials (~273 B)
+        return nb * (13 * word + 4) + nv * 6 * word + C * 273
+    if phase == "sat":
+        return P * (8 + 2 * (nv / max(nb, 1)) * (4 * 8 + 2) + 8) + C * 43
+    if phase == "contacts":
+        return C * (43 + 2 * 40 + 24) + C * 226
+    if phase == "solve_setup":  # manifold packing: contact rows in, records out
+        return C * (8 + 8 * 8 + 6 * 8 + 1) + M * (41 + 120)
+    if phase == "solve_writeback":  # lagrangians back, FP32 -> FP64 velocities
+        return C * (16 + 16) + nb * (12 + 48)
+    if phase == "broadphase_fill":
+        return nb * (48 + 32 + 8) + P * 8
+    if phase == "infer":
+        return C * 60
+    if phase == "cache_store":
+        return C * 48
+    return 0.0
+
+
+def roofline(instr, scene, word):
+    """The dominant kernel of the instrumented replay (the phase with the
+    largest average HIP-event time among those holding a single kernel):
+    algorithmic bytes per launch / its average launch duration."""
+    ph = instr["phases_ms"]
+    single = ["sat", "contacts", "infer", "solve_setup", "sweeps", "solve_writeback", "finish",
+              "cache_store", "colouring", "broadphase_fill"]
+    dom = max(single, key=lambda p: ph[p])
+    ms = ph[dom]
+    nb = scene.n_bodies
+    nv = sum(int(w.hull_off[-1]) for w in scene.worlds)
+    by = kernel_bytes(dom, nb, nv, instr["instrumented"], word)
+    achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {
+        "bound": "hbm",
+        "kernel": engine.PHASE_KERNELS[dom][0] if len(engine.PHASE_KERNELS[dom]) == 1 else dom,
+        "phase": dom,
+        "achieved": round(achieved, 3),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 6),
+        "traffic": None,
+        "algorithmic_bytes_per_launch": round(by),
+        "avg_launch_us": round(ms * 1e3, 2),
+        "launches_timed": instr["instrumented"]["steps"],
+    }
+
+
+def step_bytes(nb, nv, contacts, word):
+    """Compulsory HBM bytes of one whole step: every body's state and
+    materials read, pos/theta/vel/omega written ((10 + 6) words), its local
+    hull read (2 words per vertex), and the warm-start cache read and
+    written (packed key + two FP64 lambdas = 24 B each way per contact)."""
+    return nb * 16 * word + nv * 2 * word + contacts * 48
+
+
+# ------------------------------------------------------------ CPU baseline
+
+def cpu_sample(scene, max_worlds=16, max_bodies=8000):
+    """World indices of the CPU sample: one world, or up to `max_worlds`
+    worlds spread over the batch's size ranks among those of at most
+    `max_bodies` bodies (the oracle steps each world on one thread)."""
+    if len(scene.worlds) == 1:
+        return [0]
+    sizes = np.array([w.n for w in scene.worlds])
+    cand = np.nonzero(sizes <= max_bodies)[0]
+    cand = cand[np.argsort(sizes[cand], kind="stable")]
+    pick = np.unique(np.linspace(0, len(cand) - 1, min(max_worlds, len(cand))).astype(int))
+    return sorted(cand[pick].tolist())
+
+
+def cpu_baseline(scene, preroll, steps):
+    """The C++ FP64 oracle on this host over the same step window as the GPU
+    run: the sampled worlds step `preroll` untimed steps (all threads), then
+    `steps` timed steps on one thread and on all threads (up to 16)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    oracle.build()
+    idx = cpu_sample(scene)
+    worlds = [scene.worlds[k] for k in idx]
+    threads = max(1, min(16, os.cpu_count() or 1, len(worlds)))
+    engines = [oracle.OracleEngine(w, scene.config, world_index=k) for w, k in zip(worlds, idx)]
+    try:
+        pre = oracle.run_engines(engines, preroll, threads=threads) if preroll else 0.0
+        one = oracle.run_engines(engines, steps, threads=1)
+        many = oracle.run_engines(engines, steps, threads=threads) if threads > 1 else one
+    finally:
+        for e in engines:
+            e.close()
+    dyn = sum(w.n_dynamic for w in worlds)
+    what = (f"{scene.name} ({dyn} dynamic bodies)" if len(worlds) == 1 else
+            f"{len(worlds)} of the {len(scene.worlds)} worlds of {scene.name} ({dyn} dynamic "
+            f"bodies; indices {idx})")
+    return {
+        "value": round(dyn * steps / many, 1),
+        "unit": "body-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"steps {preroll}-{preroll + steps} (after an untimed pre-roll of {preroll} steps, "
+                  f"{pre:.1f} s) of {what}, C++ FP64 oracle with the reference's grid "
+                  f"broadphase, {threads} thread(s): {many:.2f} s; the same window on 1 thread: "
+                  f"{one:.2f} s",
+        "single_thread_value": round(dyn * steps / one, 1),
+    }
+
+
+# ------------------------------------------------------------------ parts
+
+def headline(scene, args, model, sched, ppb, dyn, preroll):
     """Plain coloured solver at the workload's sweeps, then the learned
-    warm start at decreasing sweeps while its quality (averaged over the
-    headline world and QUALITY_SEEDS) matches."""
-    base_it = scene.config.iterations
-    graph = not args.no_graph
-    extra = [scenes.polygon_pile(seed=sd) for sd in QUALITY_SEEDS] \
-        if args.workload == "polygon_pile" else []
-    el, st = run_mode(scene, engine.SOLVER_COLORED_F32, engine.WARM_ZERO, args.steps,
-                      args.warmup, graph=graph, broadphase=bp, ppb=ppb, threads=threads)
-    plain = {"warm_start": "zero", "iterations": base_it,
-             "body_steps_per_s": round(dyn * args.steps / el, 1),
-             "ms_per_step": round(1e3 * el / args.steps, 4), **quality(st),
-             "quality_over_seeds": quality_over_seeds(scene, extra, st, engine.WARM_ZERO, None,
-                                                      bp, ppb, threads, args.steps)}
-    note(f"plain coloured at {base_it} sweeps: {plain['body_steps_per_s']:.0f} body-steps/s")
-    tried = []
-    chosen = None
-    for it in range(base_it, 0, -1):
-        scene.config.iterations = it
-        el_l, st_l = run_mode(scene, engine.SOLVER_COLORED_F32, engine.WARM_LEARNED, args.steps,
-                              args.warmup, graph=graph, model=model, broadphase=bp, ppb=ppb,
-                              threads=threads)
-        row = {"warm_start": "learned", "iterations": it,
-               "body_steps_per_s": round(dyn * args.steps / el_l, 1),
-               "ms_per_step": round(1e3 * el_l / args.steps, 4), **quality(st_l),
-               "quality_over_seeds": quality_over_seeds(scene, extra, st_l, engine.WARM_LEARNED,
-                                                        model, bp, ppb, threads, args.steps)}
-        row["matches_plain"] = matches(row["quality_over_seeds"], plain["quality_over_seeds"])
-        tried.append(row)
-        note(f"learned at {it} sweeps: {row['body_steps_per_s']:.0f} body-steps/s, "
-             f"matches plain: {row['matches_plain']}")
-        if row["matches_plain"]:
-            chosen = (it, el_l, st_l, row)
-        elif it < base_it:
-            break
-    if chosen is None:  # even the full sweep count does not match: report it
-        row = next(r for r in tried if r["iterations"] == base_it)
-        chosen = (base_it, dyn * args.steps / row["body_steps_per_s"], None, row)
-    scene.config.iterations = chosen[0]
-    return plain, tried, chosen
+    warm start at the same and fewer sweeps while its quality matches."""
+    base = scene.config.iterations
+    col = engine.SOLVER_COLORED_F32
+    r = run(scene, col, engine.WARM_ZERO, base, preroll, args.steps, sched, ppb,
+            positions=True, instrument=args.profile_steps)
+    plain = {"warm_start": "zero", "iterations": base,
+             "body_steps_per_s": round(dyn * args.steps / r["seconds"], 1),
+             "ms_per_step": round(1e3 * r["seconds"] / args.steps, 4), **r["quality"]}
+    note(f"plain coloured at {base} sweeps: {plain['ms_per_step']} ms/step")
+    runs = {("zero", base): r}
+    tried, chosen = [], None
+    if model is not None:
+        for it in range(base, 0, -1):
+            try:
+                rl = run(scene, col, engine.WARM_LEARNED, it, preroll, args.steps, sched, ppb,
+                         model=model, positions=True)
+            except engine.ShapesError as e:
+                tried.append({"warm_start": "learned", "iterations": it, "error": str(e)})
+                note(f"learned at {it} sweeps: {e}")
+                break
+            row = {"warm_start": "learned", "iterations": it,
+                   "body_steps_per_s": round(dyn * args.steps / rl["seconds"], 1),
+                   "ms_per_step": round(1e3 * rl["seconds"] / args.steps, 4), **rl["quality"],
+                   "learned_starts_last_step": rl["stats"]["learned_starts"],
+                   "misses_last_step": rl["stats"]["misses"]}
+            row["matches_plain"] = matches(row, plain)
+            tried.append(row)
+            runs[("learned", it)] = rl
+            note(f"learned at {it} sweeps: {row['ms_per_step']} ms/step, matches plain: "
+                 f"{row['matches_plain']}")
+            if row["matches_plain"]:
+                chosen = row
+            elif it < base:
+                break
+    return plain, tried, chosen, runs
+
+
+def fp64_reference(scene, args, sched, ppb, preroll, runs):
+    """The FP64 drop-in solver (the reference's order, bit-exact to the
+    oracle) over the same steps: its quality, and the divergence of each
+    run's final positions from it; `chaos_floor` is the FP64 run's own
+    divergence when every dynamic body's initial x-velocity moves by 1e-12."""
+    base = scene.config.iterations
+    total = preroll + args.steps
+    ordered = engine.SOLVER_ORDERED_F64
+    ref = run(scene, ordered, engine.WARM_ZERO, base, 0, total, sched, ppb, positions=True)
+    moved = []
+    for w in scene.worlds:
+        m = w.copy()
+        m.vx = m.vx + np.where((m.mass != 0) | (m.inertia != 0), 1e-12, 0.0)
+        moved.append(m)
+    msc = scenes.Scene(moved, scene.config, name=scene.name)
+    pert = run(msc, ordered, engine.WARM_ZERO, base, 0, total, sched, ppb, positions=True)
+    out = {"solver": "ordered", "iterations": base, "steps": total,
+           "ms_per_step": round(1e3 * ref["seconds"] / total, 4),
+           "quality": ref["quality"],
+           "chaos_floor": divergence(pert["positions"], ref["positions"], scene.worlds),
+           "runs": []}
+    for (warm, it), r in runs.items():
+        out["runs"].append({"warm_start": warm, "iterations": it,
+                            "divergence": divergence(r["positions"], ref["positions"],
+                                                     scene.worlds),
+                            "mean_step_max_penetration": r["quality"]["mean_step_max_penetration"],
+                            "mean_step_normal_residual": r["quality"]["mean_step_normal_residual"]})
+    return out
+
+
+def c4_evaluation(model, seeds=(1002, 1003, 1004)):
+    """The learned warm start as the C4 configuration states it: the
+    8192-polygon world (dt 1/60, 600 steps) at 2 and 4 sweeps against the
+    plain solver at 8, quality averaged over three seeds of the generator
+    (not the model's training seeds), throughput of the first."""
+    rows = []
+    for warm, it in ((engine.WARM_ZERO, 8), (engine.WARM_LEARNED, 4), (engine.WARM_LEARNED, 2),
+                     (engine.WARM_ZERO, 4), (engine.WARM_ZERO, 2)):
+        qs, secs = [], None
+        for sd in seeds:
+            sc = scenes.polygon_pile(seed=sd)
+            r = run(sc, engine.SOLVER_COLORED_F32, warm, it, 0, sc.config.steps, ("grid", 1024),
+                    0, model=model)
+            qs.append(r["quality"])
+            if secs is None:
+                secs = r["seconds"]
+        keys = ("mean_step_max_penetration", "mean_step_normal_residual",
+                "mean_step_friction_residual")
+        rows.append({"warm_start": "learned" if warm == engine.WARM_LEARNED else "zero",
+                     "iterations": it, **{k: float(np.mean([q[k] for q in qs])) for k in keys},
+                     "ms_per_step": round(1e3 * secs / 600, 4)})
+    ref = rows[0]
+    for r in rows[1:]:
+        r["matches_plain_8"] = matches(r, ref)
+    return {"workload": "polygon_pile8192, 600 steps, seeds " + ", ".join(map(str, seeds)),
+            "rows": rows}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="polygon_pile")
+    ap.add_argument("--steps", type=int, default=100, help="timed steps (K)")
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed steps before the settling pre-roll (W)")
+    ap.add_argument("--workload", default="pile_batch", choices=list(WORKLOADS))
+    ap.add_argument("--worlds", type=int, default=0, help="batch workloads: number of worlds")
     ap.add_argument("--mode", default="headline", choices=["headline", "single"],
                     help="headline: learned vs plain at fixed quality; single: one configuration")
     ap.add_argument("--solver", default="colored", choices=list(SOLVERS))
     ap.add_argument("--warm", default="zero", choices=["zero", "learned"])
-    ap.add_argument("--iterations", type=int, default=0,
-                    help="solver sweeps (default: the workload's)")
+    ap.add_argument("--iterations", type=int, default=0, help="sweeps (default: the workload's)")
+    ap.add_argument("--preroll", type=int, default=-1,
+                    help="settling steps before the window (default: half the workload's run)")
     ap.add_argument("--model", default=os.path.join(ROOT, "shapes_amd", "warmstart_mlp.npy"))
-    ap.add_argument("--cpu-sample-steps", type=int, default=300)
-    ap.add_argument("--profile-steps", type=int, default=0,
-                    help="steps of the instrumented replay (default: --steps)")
+    ap.add_argument("--profile-steps", type=int, default=20,
+                    help="instrumented eager steps after the plain run's window")
+    ap.add_argument("--broadphase", default="tune", choices=["tune"] + list(BROADPHASES))
+    ap.add_argument("--solver-threads", type=int, default=0, choices=[0, 256, 512, 1024])
+    ap.add_argument("--pairs-per-body", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--broadphase", default="tune", choices=["tune"] + list(BROADPHASES),
-                    help="tune: pick broadphase and solver lanes by measurement")
-    ap.add_argument("--solver-threads", type=int, default=0, choices=[0, 256, 512, 1024],
-                    help="lanes per world in the solve when not tuned (0: by world size)")
-    ap.add_argument("--pairs-per-body", type=int, default=0,
-                    help="pair capacity per body (0: library default)")
-    ap.add_argument("--worlds", type=int, default=0,
-                    help="batch workloads: number of worlds (default: the config's)")
-    ap.add_argument("--fp64-reference", action="store_true",
-                    help="single mode, coloured solver: also run the FP64 drop-in solver and "
-                         "report quality and trajectory divergence against it (headline mode "
-                         "always does)")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
 
     if args.copy_calibration:
-        copy_calibration()
+        import torch
+        x = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        x.fill_(1)
+        for _ in range(3):
+            y = x.clone()
+            del y
+        torch.cuda.synchronize()
+        print("copied 1073741824 bytes x3", flush=True)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -471,71 +433,86 @@ def main():
         sys.exit(2)
 
     t_start = time.perf_counter()
-    scene = make_scene(args.workload, args.worlds)
+    make, settle, ppb_default, candidates, desc = WORKLOADS[args.workload]
+    scene = make(args.worlds)
     note(f"scene {scene.name}: {scene.n_bodies} bodies in {len(scene.worlds)} world(s), "
          f"built in {time.perf_counter() - t_start:.1f} s")
     if args.iterations:
         scene.config.iterations = args.iterations
+    preroll = args.warmup + (settle if args.preroll < 0 else args.preroll)
+    ppb = ppb_default if args.pairs_per_body < 0 else args.pairs_per_body
+    dyn = scene.n_dynamic
     headline_mode = args.mode == "headline"
     solver_name = "colored" if headline_mode else args.solver
-    solver = SOLVERS[solver_name]
-    warm = engine.WARM_LEARNED if (headline_mode or args.warm == "learned") else engine.WARM_ZERO
     model = None
-    if warm == engine.WARM_LEARNED:
+    if headline_mode or args.warm == "learned":
         if not os.path.exists(args.model):
             raise SystemExit(f"bench.py: no model at {args.model}")
         model = np.load(args.model)
-    dyn = scene.n_dynamic
-    ppb = args.pairs_per_body
+
     tuning = None
-    bp = args.broadphase
-    threads = args.solver_threads
-    if bp == "tune":
-        # time each candidate in the middle of the run, where the contacts
-        # of a settled scene set the cost, not on the first falling steps
-        (bp, threads), tuning = autotune(scene, solver, ppb, steps=30,
-                                         warm_steps=min(args.steps // 2, 300))
-        note(f"autotuned schedule: {bp}, {threads} lanes {tuning}")
+    if args.broadphase == "tune":
+        sched, tuning = autotune(scene, ppb, candidates, min(preroll, 200))
+    else:
+        sched = (args.broadphase, args.solver_threads)
+    note(f"schedule: {sched[0]}, {sched[1]} lanes")
 
     extra = {}
     if headline_mode:
-        plain, tried, chosen = headline(scene, args, model, bp, ppb, dyn, threads)
-        it, elapsed, _, row = chosen
-        q = {k: v for k, v in row.items()
-             if k not in ("warm_start", "iterations", "body_steps_per_s", "ms_per_step",
-                          "matches_plain", "quality_over_seeds")}
-        extra = {"plain_gpu": plain, "learned_runs": tried,
-                 "vs_plain_gpu": round(row["body_steps_per_s"] / plain["body_steps_per_s"], 4),
-                 "quality_tolerance": QUALITY_TOL,
-                 "quality_compared_over": ("the timed world and polygon piles with seeds "
-                                           + ", ".join(map(str, QUALITY_SEEDS))
-                                           if args.workload == "polygon_pile" else "the timed world"),
-                 "quality_matched": bool(row.get("matches_plain"))}
-        st_last = None
-        scene.config.iterations = plain["iterations"]
-        extra["fp64_reference"] = fp64_reference(
-            scene, args.steps, [(engine.WARM_ZERO, plain["iterations"], threads),
-                                (engine.WARM_LEARNED, it, threads)], bp, ppb, model)
-        scene.config.iterations = it
-        note("FP64 reference runs done")
-    else:
-        elapsed, st_last = run_mode(scene, solver, warm, args.steps, args.warmup,
-                                    graph=not args.no_graph, model=model, broadphase=bp,
-                                    ppb=ppb, threads=threads)
-        q = quality(st_last)
-        if args.fp64_reference and solver == engine.SOLVER_COLORED_F32:
-            extra["fp64_reference"] = fp64_reference(
-                scene, args.steps, [(warm, scene.config.iterations, threads)], bp, ppb, model)
+        plain, tried, chosen, runs = headline(scene, args, model, sched, ppb, dyn, preroll)
+        r_plain = runs[("zero", scene.config.iterations)]
+        if chosen is not None:
+            value = chosen["body_steps_per_s"]
+            ms = chosen["ms_per_step"]
+            it = chosen["iterations"]
+            warm_name = "learned"
+            value_is = f"learned warm start at {it} sweeps (quality within 2% of plain)"
+            q = {k: chosen[k] for k in plain if k.startswith(("max_", "mean_", "normal_",
+                                                              "friction_"))}
+        else:
+            value = plain["body_steps_per_s"]
+            ms = plain["ms_per_step"]
+            it = plain["iterations"]
+            warm_name = "zero"
+            value_is = ("plain GPU solver: no learned configuration matched its quality"
+                        if model is not None else "plain GPU solver")
+            q = r_plain["quality"]
+        extra["headline"] = {
+            "value_is": value_is,
+            "quality_matched": chosen is not None,
+            "quality_tolerance": QUALITY_TOL,
+            "plain_gpu": plain,
+            "learned_runs": tried,
+            "vs_plain_gpu": round(value / plain["body_steps_per_s"], 4),
+            "sweeps_plain_over_learned": (round(plain["iterations"] / chosen["iterations"], 3)
+                                          if chosen else None),
+        }
+        instr = r_plain
+        if not args.no_fp64:
+            extra["fp64_reference"] = fp64_reference(scene, args, sched, ppb, preroll, runs)
             note("FP64 reference runs done")
-    value = dyn * args.steps / elapsed
-    note(f"timed run: {args.steps} steps in {elapsed:.2f} s")
-    prof_steps = args.profile_steps or args.steps
-    roof, phases, avg = roofline(scene, solver_name, prof_steps, bp, ppb, warm, model, threads)
-    note(f"instrumented replay: {prof_steps} steps")
-    nv = sum(int(w.hull_off[-1]) for w in scene.worlds)
-    sb = step_bytes(scene.n_bodies, nv, avg["contacts"], DTYPES_BYTES[solver_name])
-    step_gbs = sb / (elapsed / args.steps) / 1e9
+        if not args.no_c4 and model is not None:
+            extra["c4_learned_warm_start"] = c4_evaluation(model)
+            note("learned warm start on the 8192-polygon world done")
+    else:
+        warm = engine.WARM_LEARNED if args.warm == "learned" else engine.WARM_ZERO
+        r = run(scene, SOLVERS[solver_name], warm, scene.config.iterations, preroll, args.steps,
+                sched, ppb, model=model, instrument=args.profile_steps)
+        value = dyn * args.steps / r["seconds"]
+        ms = 1e3 * r["seconds"] / args.steps
+        it = scene.config.iterations
+        warm_name = args.warm
+        q = r["quality"]
+        instr = r
+        st = r["stats"]
+        extra["last_step"] = {k: st[k] for k in ("pairs", "contacts", "warm_hits", "misses",
+                                                 "learned_starts", "max_levels")}
 
+    word = 4 if solver_name == "colored" else 8
+    nv = sum(int(w.hull_off[-1]) for w in scene.worlds)
+    roof = roofline(instr, scene, word)
+    sb = step_bytes(scene.n_bodies, nv, instr["instrumented"]["contacts"], word)
+    step_gbs = sb / (ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -543,25 +520,28 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "ms_per_step": round(ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": DTYPES[solver_name],
         "data": "synthetic",
         "config": {
-            "workload": scene.name,
+            "workload": f"{scene.name}: {len(scene.worlds)} {desc}" if len(scene.worlds) > 1
+            else f"{scene.name}: {desc}",
             "worlds": len(scene.worlds),
             "bodies": scene.n_bodies,
             "dynamic_bodies": dyn,
-            "iterations": scene.config.iterations,
+            "iterations": it,
             "dt": scene.config.dt,
             "solver": solver_name,
-            "warm_start": "learned" if warm == engine.WARM_LEARNED else "zero",
+            "warm_start": warm_name,
             "mode": args.mode,
-            "graph": not args.no_graph,
-            "broadphase": bp,
-            "solver_threads": threads,
+            "graph": True,
+            "timed_window": [preroll, preroll + args.steps],
+            "broadphase": sched[0],
+            "solver_threads": sched[1],
+            "pairs_per_body": ppb,
             "autotune_ms_per_step": tuning,
         },
         "quality": q,
@@ -570,18 +550,14 @@ def main():
         "step_roofline": {"bound": "hbm", "algorithmic_bytes_per_step": round(sb),
                           "achieved": round(step_gbs, 3), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 6)},
-        "phase_ms": phases,
-        "avg_pairs": round(avg["pairs"], 1),
-        "avg_contacts": round(avg["contacts"], 1),
+        "phase_us": {p: round(v * 1e3, 2) for p, v in instr["phases_ms"].items()},
+        "avg_pairs": round(instr["instrumented"]["pairs"], 1),
+        "avg_contacts": round(instr["instrumented"]["contacts"], 1),
     }
-    if st_last is not None:
-        out["last_step"] = {"pairs": st_last["pairs"], "contacts": st_last["contacts"],
-                            "warm_hits": st_last["warm_hits"], "misses": st_last["misses"],
-                            "learned_starts": st_last["learned_starts"],
-                            "max_levels": st_last["max_levels"]}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_sample_steps)
+        out["cpu_baseline"] = cpu_baseline(scene, preroll, min(args.steps, 20))
         note("cpu baseline done")
+    note(f"total {time.perf_counter() - t_start:.0f} s")
     print(json.dumps(out), flush=True)
 
 

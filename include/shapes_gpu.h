@@ -231,11 +231,16 @@ shp_status shp_mlp_train(shp_mlp* m, const int32_t* indices, int32_t steps,
 /* ---- per-phase timing (no counterpart in the reference, whose only
  * timing is the criterion benchmark, shapes/bench/Main.hs:19-26) --------- */
 
-/* Phases of one step, in launch order: 0 reset, 1 broadphase count (with
- * gravity), 2 pair scan, 3 broadphase fill, 4 SAT, 5 contact scan,
- * 6 world ranges, 7 contacts (constraintGen + warm-start join), 8 solve,
- * 9 quality, 10 cache store, 11 integrate (advance + moveShapes + AABBs). */
-#define SHP_NUM_PHASES 12
+/* Phases of one step on the batch's stream, in launch order: 0 reset,
+ * 1 broadphase count (with gravity), 2 pair scan, 3 broadphase fill, 4 SAT,
+ * 5 contact scan, 6 world ranges, 7 contacts (constraintGen + warm-start
+ * join), 8 learned inference, 9 solve set-up (the coloured solver: the wait
+ * for the colouring and the manifold packing), 10 sweeps (k_sweep_colours or
+ * k_solve_flow), 11 solve write-back (coloured: k_colour_unpack), 12 advance
+ * + moveShapes + AABBs + solve quality (k_finish), 13 cache store; and 14
+ * the coloured solver's colouring (k_colour), timed on the second stream,
+ * where it overlaps phases 5-9. */
+#define SHP_NUM_PHASES 15
 /* Enable (1) or disable (0) HIP-event timing of each phase on the batch's
  * stream; resets the accumulated times.  Timed steps synchronise once per
  * step and are never replayed from a graph. */
@@ -249,6 +254,46 @@ shp_status shp_batch_read_timing(shp_batch* b, double* ms_per_phase,
  * warm-start pass and sweeps, 4 write-back; coloured: 0 colouring, 1 bucketing + packing +
  * velocity staging, 2 warm-start pass, 3 sweeps, 4 write-back. */
 shp_status shp_batch_read_solve_profile(shp_batch* b, double* us5);
+
+/* ---- plan and audit of a batch (no counterpart in the reference) -------
+ *
+ * What shp_batch_create would allocate and launch for these worlds and this
+ * configuration, computed on the host only (no GPU needed): capacities,
+ * the device footprint, and every kernel launch of one step.  Returns what
+ * shp_batch_create would return for the same arguments before touching the
+ * device: SHP_OK, or the validation or audit error (a launch beyond the
+ * hardware's limits - more than 2^31 - 1 workgroups, 2^32 work-items or
+ * 160 KiB of LDS - or an index beyond 32 bits) or a refused configuration.
+ * `out` is filled whenever the worlds are valid, also when refused. */
+typedef struct shp_plan {
+  int64_t bodies;
+  int64_t vertices;
+  int64_t worlds;
+  int64_t max_world_bodies;
+  int64_t pair_capacity;
+  int64_t contact_capacity;
+  int64_t miss_capacity;    /* cache misses the learned warm start can take */
+  int64_t grid_buckets;     /* hash buckets of the grid broadphase (0: scan) */
+  int64_t device_bytes;     /* device memory allocated at creation */
+  int64_t max_workitems;    /* largest grid x workgroup of a step's launches */
+  int64_t max_lds_bytes;    /* largest static + dynamic LDS of a launch */
+  int32_t launches;         /* kernel launches per step */
+  int32_t broadphase;       /* SHP_BROADPHASE_SCAN or SHP_BROADPHASE_GRID */
+  int32_t solver_threads;   /* lanes per world of the solve */
+  int32_t reserved;
+} shp_plan;
+
+typedef struct shp_launch {
+  char name[32];
+  int64_t workgroups;
+  int32_t workgroup_size;
+  int32_t lds_bytes; /* static + dynamic */
+} shp_launch;
+
+/* launches (may be NULL) receives up to launch_cap launch records. */
+shp_status shp_batch_plan(const shp_world_desc* worlds, int32_t n_worlds,
+                          const shp_config* cfg, shp_plan* out,
+                          shp_launch* launches, int32_t launch_cap);
 
 const char* shp_last_error(void);
 const char* shp_version(void);

@@ -24,13 +24,16 @@ _ip = C.POINTER(C.c_int32)
 
 SOLVER_ORDERED = 0   # the reference's sequential FP64 sweep
 SOLVER_COLORED = 1   # restatement of the GPU's coloured FP32 schedule
+BROADPHASE_GRID = 0   # the reference's grid (Grid.hs), the default
+BROADPHASE_BRUTE = 1  # every pair (Aabb.hs culledKeys)
 
 
 class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("baumgarte", C.c_double),
                 ("slop", C.c_double), ("gx", C.c_double), ("gy", C.c_double),
                 ("iterations", C.c_int32), ("trig", C.c_int32),
-                ("solver", C.c_int32), ("world", C.c_int32)]
+                ("solver", C.c_int32), ("world", C.c_int32),
+                ("broadphase", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -100,7 +103,7 @@ class OracleEngine:
     """One world stepped by the oracle (updateWorld semantics)."""
 
     def __init__(self, world, config, trig: int = TRIG_ENGINE, solver: int = SOLVER_ORDERED,
-                 world_index: int = 0):
+                 world_index: int = 0, broadphase: int = BROADPHASE_GRID):
         L = lib()
         keep = []
         args = []
@@ -114,7 +117,7 @@ class OracleEngine:
         hy, hyp = _d(world.hull_y)
         keep += [ho, hx, hy]
         prm = Params(config.dt, config.baumgarte, config.slop, config.gx,
-                     config.gy, config.iterations, trig, solver, world_index)
+                     config.gy, config.iterations, trig, solver, world_index, broadphase)
         self.n = world.n
         self._h = L.orc_engine_create(world.n, *args, hop, hxp, hyp, C.byref(prm))
         self._L = L

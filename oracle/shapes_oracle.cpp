@@ -492,8 +492,72 @@ struct orc_engine {
 
 namespace {
 
-// culledKeys (Grid.hs:74-86 == Aabb.hs:168-183): {(i,j) | i > j, not both
-// static, inclusive AABB overlap} in descending lexicographic order.
+// culledKeys (Aabb.hs:168-183): {(i,j) | i > j, not both static, inclusive
+// AABB overlap} in descending lexicographic order, by testing every pair.
+void culled_keys_brute(const std::vector<Aabb>& boxes, const std::vector<char>& st,
+                       std::vector<std::pair<int, int>>& out) {
+  const int n = (int)boxes.size();
+  out.clear();
+  for (int i = n - 1; i >= 1; --i)
+    for (int j = i - 1; j >= 0; --j)
+      if (!(st[i] && st[j]) && aabb_check(boxes[i], boxes[j]))
+        out.push_back(std::make_pair(i, j));
+}
+
+// axialIndex (Grid.hs:127-129) with GridAxis 20 1 (-10) (Engine/Main.hs:
+// 40-41): floor((v - origin) / unit).  Clamped to +-2^30 so that huge or
+// non-finite values stay defined (the reference's floor :: Double -> Int
+// has no defined value for them; finite worlds never reach the clamp).
+inline int axial_index(double v) {
+  double q = std::floor((v - -10.0) / 1.0);
+  if (!(q == q)) return 0;
+  if (q > 1073741824.0) return 1073741824;
+  if (q < -1073741824.0) return -1073741824;
+  return (int)q;
+}
+
+// toGrid + culledKeys (Grid.hs:67-100,102-141): every AABB is inserted into
+// each flattened cell x + 20 y its index ranges cover (boxIndices, flattened
+// indices alias across rows); each cell's shapes in descending key order
+// give allPairs, pairs of two static shapes are skipped and the rest kept
+// on aabbCheck; the concatenation is sorted descending and deduplicated
+// (uniq).  Cells are gathered here by sorting (cell, key) entries instead
+// of IntMap inserts: the same cells with the same members.
+void culled_keys_grid(const std::vector<Aabb>& boxes, const std::vector<char>& st,
+                      std::vector<std::pair<int, int>>& out) {
+  const int n = (int)boxes.size();
+  std::vector<std::pair<long long, int>> ent;  // (cell, -key): keys descending in a cell
+  for (int k = 0; k < n; ++k) {
+    const Aabb& b = boxes[k];
+    const int x0 = axial_index(b.minx), x1 = axial_index(b.maxx);
+    const int y0 = axial_index(b.miny), y1 = axial_index(b.maxy);
+    for (long long x = x0; x <= x1; ++x)
+      for (long long y = y0; y <= y1; ++y) ent.push_back(std::make_pair(x + y * 20, -k));
+  }
+  std::sort(ent.begin(), ent.end());
+  out.clear();
+  size_t a = 0;
+  while (a < ent.size()) {
+    size_t e = a;
+    while (e < ent.size() && ent[e].first == ent[a].first) ++e;
+    // a shape inserted twice into one flattened cell (aliasing) is one
+    // IntMap entry
+    std::vector<int> keys;
+    for (size_t q = a; q < e; ++q)
+      if (keys.empty() || keys.back() != -ent[q].second) keys.push_back(-ent[q].second);
+    for (size_t p = 0; p < keys.size(); ++p)
+      for (size_t q = p + 1; q < keys.size(); ++q) {
+        const int i = keys[p], j = keys[q];  // i > j
+        if (!(st[i] && st[j]) && aabb_check(boxes[i], boxes[j])) out.push_back(std::make_pair(i, j));
+      }
+    a = e;
+  }
+  std::sort(out.begin(), out.end(), [](const std::pair<int, int>& x, const std::pair<int, int>& y) {
+    return x > y;
+  });
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+}
+
 void culled_keys(orc_engine* e, std::vector<std::pair<int, int>>& out) {
   int n = (int)e->objs.size();
   std::vector<Aabb> boxes(n);
@@ -502,11 +566,10 @@ void culled_keys(orc_engine* e, std::vector<std::pair<int, int>>& out) {
     boxes[i] = hull_aabb(e->hulls[i]);
     st[i] = is_static(e->objs[i]);
   }
-  out.clear();
-  for (int i = n - 1; i >= 1; --i)
-    for (int j = i - 1; j >= 0; --j)
-      if (!(st[i] && st[j]) && aabb_check(boxes[i], boxes[j]))
-        out.push_back(std::make_pair(i, j));
+  if (e->prm.broadphase == ORC_BROADPHASE_BRUTE)
+    culled_keys_brute(boxes, st, out);
+  else
+    culled_keys_grid(boxes, st, out);
 }
 
 // prepareFrame (Solvers/Contact.hs:40-52) + keyedContacts

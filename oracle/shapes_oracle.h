@@ -31,6 +31,11 @@ typedef struct orc_engine orc_engine;
 #define ORC_SOLVER_COLORED 1 /* restatement of the GPU's graph-coloured FP32
                                 Gauss-Seidel (not a reference function) */
 
+/* broadphase algorithms (the same pair list for finite AABBs) */
+#define ORC_BROADPHASE_GRID 0  /* the reference's grid, Grid.hs:67-141 with
+                                  the engine's axes (Engine/Main.hs:40-41) */
+#define ORC_BROADPHASE_BRUTE 1 /* every pair, Aabb.hs:168-183 */
+
 typedef struct orc_params {
   double dt;
   double baumgarte;   /* ContactBehavior.contactBaumgarte */
@@ -40,6 +45,7 @@ typedef struct orc_params {
   int32_t trig;       /* ORC_TRIG_* */
   int32_t solver;     /* ORC_SOLVER_* */
   int32_t world;      /* the world's index in a GPU batch (colouring hash) */
+  int32_t broadphase; /* ORC_BROADPHASE_* */
 } orc_params;
 
 /* One world.  hull_off has n+1 entries (CSR into hull_x/hull_y); vertices

@@ -2,7 +2,8 @@
 
 Device code is compiled without floating-point contraction (the drop-in
 solver must round every multiply and add separately, like the reference's
-SSE2 doubles) and without packed-FP32 instructions.
+SSE2 doubles) and without packed-FP32 instructions.  The sources compile
+in parallel to objects under shapes_amd/_build, which are then linked.
 """
 from __future__ import annotations
 
@@ -12,14 +13,16 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libshapes_gpu.so")
 SOURCES = ["shp_engine.hip", "shp_solver.hip", "shp_colored.hip", "shp_mlp.hip", "shp_grid.hip"]
 HEADERS = ["shp_engine.h", "shp_geom.h", "shp_trig.h", "shp_solve_common.h", "shp_mlp.h",
            os.path.join("..", "..", "include", "shapes_gpu.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+ARCH = ["--offload-arch=gfx950"]
+FLAGS = ARCH + [
+    "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
@@ -28,9 +31,19 @@ FLAGS = [
 
 def build(verbose: bool = False) -> str:
     """Compile every HIP source into libshapes_gpu.so (always rebuilds)."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    os.makedirs(OBJ, exist_ok=True)
+    procs = []
+    for src in SOURCES:
+        obj = os.path.join(OBJ, src.replace(".hip", ".o"))
+        cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, obj, subprocess.Popen(cmd)))
+    failed = [src for src, _, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, f"hipcc {' '.join(failed)}")
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp] + srcs
+    cmd = [HIPCC] + ARCH + ["-shared", "-fPIC", "-o", tmp] + [o for _, o, _ in procs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

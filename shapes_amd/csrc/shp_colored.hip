@@ -35,14 +35,12 @@
 
 using namespace shp;
 
-#define COLOR_CAP 256
 // colours 0..15 are assigned smallest-free from the per-body masks; a
 // manifold whose bodies already use all 16 takes GREEDY_COLOURS + round,
 // so the Jones-Plassmann rounds stop at JP_ROUNDS to keep every colour and
 // the serial tail below COLOR_CAP (and below the uint8 sentinel 0xFF)
 #define GREEDY_COLOURS 16
 #define JP_ROUNDS (COLOR_CAP - GREEDY_COLOURS - 3)
-#define LDS_MAX_BYTES (160 * 1024 - 512)
 
 // per-contact FP32 fields (stride contact_cap), then the FP64 lagrangians,
 // then the per-manifold FP32 fields (stride pair_cap); manifold bodies in
@@ -1077,69 +1075,77 @@ __global__ void k_colour_unpack(Ctx c) {
   }
 }
 
+static_assert(CF_LAMBDA + 4 == 16 && MF_FIELDS == 8, "record layout differs from COLOR_PACK_FLOATS");
+
+// LDS of the colouring: colour offsets and cursors, then a pool for the
+// per-body priorities and colour masks and the manifold list (pairs per
+// world are unknown before a step: budgeted at 8 per body; a world that
+// needs more is coloured in global memory); the sweeps use the pool for
+// the FP32 velocities.  4 waves per world; a large world gets 8, and its
+// colouring 16 (light on registers).
+void plan_colored_solver(StepPlan* P, int solver_threads) {
+  const long nb = P->max_nb;
+  const long np = std::min<long>(32767, 8 * nb + 64);
+  const long need = std::max(6 * nb + 2 + 7 * np, 12 * nb);
+  const size_t fixed = (2 * (COLOR_CAP + 3) + 2) * 4;
+  P->color_lds = std::min<size_t>(fixed + need, LDS_MAX_BYTES);
+  P->color_pool = P->color_lds - fixed;
+  P->color_threads = solver_threads ? solver_threads : (nb > 2048 ? 512 : 256);
+}
+
+// k_colour: sm[4]; k_sweep_colours: s_off[COLOR_CAP + 3]
+size_t colored_static_lds(bool sweep) { return sweep ? lds16((COLOR_CAP + 3) * 4) : lds16(4 * 4); }
+
 shp_status colored_alloc(shp_batch* b) {
-  Ctx& c = b->ctx;
-  static_assert((CF_LAMBDA + 4) * 2 + MF_FIELDS <= 2 * 29 * 2, "records exceed the packed buffer");
-  void** bufs[] = {(void**)&c.col_best, (void**)&c.col_used, (void**)&c.col_off,
-                   (void**)&c.col_info, (void**)&c.pair_ab};
-  size_t sizes[] = {(size_t)c.NB * 8, (size_t)c.NB * 4, (size_t)c.W * (COLOR_CAP + 3) * 4,
-                    (size_t)c.W * 4, (size_t)c.pair_cap * 4};
-  for (int k = 0; k < 5; ++k) {
-    if (hipMalloc(bufs[k], sizes[k] ? sizes[k] : 16) != hipSuccess) return SHP_E_OOM;
-    b->allocs.push_back(*bufs[k]);
-  }
-  long need = 0;
-  int max_nb = 0;
-  for (int w = 0; w < c.W; ++w) {
-    long nb = b->h_body_off[w + 1] - b->h_body_off[w];
-    max_nb = std::max<int>(max_nb, (int)nb);
-    // pairs per world are unknown before a step: budget 8 per body (a world
-    // that needs more this step is coloured in global memory)
-    long np = std::min<long>(32767, 8 * nb + 64);
-    need = std::max(need, std::max(6 * nb + 2 + 7 * np, 12 * nb));
-  }
-  size_t fixed = (2 * (COLOR_CAP + 3) + 2) * 4;
-  b->color_lds = std::min<size_t>(fixed + need, LDS_MAX_BYTES);
-  // 4 waves per world; a large world gets 8
-  b->color_threads = b->cfg.solver_threads ? b->cfg.solver_threads : (max_nb > 2048 ? 512 : 256);
-  const size_t vel_lds = b->color_lds - fixed;
-  // colouring is light on registers: a large world colours with 16 waves
-  const int t = b->color_threads;
+  const StepPlan& P = b->plan;
+  const int t = P.color_threads;
   const void* fns[] = {t >= 512 ? (const void*)k_colour<1024> : (const void*)k_colour<256>,
                        t == 1024  ? (const void*)k_sweep_colours<1024>
                        : t == 512 ? (const void*)k_sweep_colours<512>
                                   : (const void*)k_sweep_colours<256>};
-  const size_t ldsz[] = {b->color_lds, vel_lds};
-  for (int k = 0; k < 2; ++k)
+  const size_t ldsz[] = {P.color_lds, P.color_pool};
+  for (int k = 0; k < 2; ++k) {
     if (hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsz[k]) !=
-        hipSuccess)
+        hipSuccess) {
+      shp_set_error("hipFuncSetAttribute (coloured solver LDS) failed");
       return SHP_E_HIP;
+    }
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, fns[k]) != hipSuccess ||
+        a.sharedSizeBytes > colored_static_lds(k == 1)) {
+      shp_set_error("coloured solver: static LDS above the plan's");
+      return SHP_E_HIP;
+    }
+  }
   return SHP_OK;
 }
 
 void launch_colouring(shp_batch* b, hipStream_t s) {
   const Ctx& c = b->ctx;
-  const int pool = (int)(b->color_lds - (2 * (COLOR_CAP + 3) + 2) * 4);
-  if (b->color_threads >= 512)
-    k_colour<1024><<<c.W, 1024, b->color_lds, s>>>(c, pool);
+  const StepPlan& P = b->plan;
+  const int pool = (int)P.color_pool;
+  if (P.color_threads >= 512)
+    k_colour<1024><<<c.W, 1024, P.color_lds, s>>>(c, pool);
   else
-    k_colour<256><<<c.W, 256, b->color_lds, s>>>(c, pool);
+    k_colour<256><<<c.W, 256, P.color_lds, s>>>(c, pool);
 }
 
-void launch_colored_sweeps(shp_batch* b) {
+void launch_colored_pack(shp_batch* b) {
+  k_pack_manifolds<<<b->plan.g_pack, 256, 0, b->stream>>>(b->ctx);
+}
+
+void launch_colored_sweep(shp_batch* b) {
   const Ctx& c = b->ctx;
-  const int pool = (int)(b->color_lds - (2 * (COLOR_CAP + 3) + 2) * 4);
-  const size_t vel_lds = (size_t)pool;
-  const long tp = c.pair_cap;
-  k_pack_manifolds<<<std::max(1L, std::min((tp + 255) / 256, 4096L)), 256, 0, b->stream>>>(c);
-  if (b->color_threads == 1024)
-    k_sweep_colours<1024><<<c.W, 1024, vel_lds, b->stream>>>(c, pool);
-  else if (b->color_threads == 512)
-    k_sweep_colours<512><<<c.W, 512, vel_lds, b->stream>>>(c, pool);
+  const StepPlan& P = b->plan;
+  const int pool = (int)P.color_pool;
+  if (P.color_threads == 1024)
+    k_sweep_colours<1024><<<c.W, 1024, P.color_pool, b->stream>>>(c, pool);
+  else if (P.color_threads == 512)
+    k_sweep_colours<512><<<c.W, 512, P.color_pool, b->stream>>>(c, pool);
   else
-    k_sweep_colours<256><<<c.W, 256, vel_lds, b->stream>>>(c, pool);
-  const long n = std::max<long>(tp, c.NB);
-  k_colour_unpack<<<std::max(1L, std::min((n + 255) / 256, 4096L)), 256, 0, b->stream>>>(c);
+    k_sweep_colours<256><<<c.W, 256, P.color_pool, b->stream>>>(c, pool);
 }
 
-
+void launch_colored_unpack(shp_batch* b) {
+  k_colour_unpack<<<b->plan.g_unpack, 256, 0, b->stream>>>(b->ctx);
+}

@@ -16,10 +16,33 @@
 #include "../../include/shapes_gpu.h"
 #include "shp_mlp.h"
 
+// LDS of one workgroup on gfx950 (160 KiB per CU); dynamic allocations
+// stay 512 B below it for the kernels' own static LDS
+#define LDS_BYTES_PER_CU (160 * 1024)
+#define LDS_MAX_BYTES (LDS_BYTES_PER_CU - 512)
+// coloured solver: colour offsets per world; ordered solver: level bins
+#define COLOR_CAP 256
+#define LEVEL_CAP 4095
+// the coloured solver keeps a world's FP32 velocities (12 B per body) in
+// LDS up to this many bodies; beyond, in global memory
+#define VEL_LDS_MAX_BODIES ((LDS_MAX_BYTES - (2 * (COLOR_CAP + 3) + 2) * 4) / 12)
+// solver records: the ordered solver's compact FP64 record (doubles per
+// contact); the coloured solver's FP32 fields of both contact slots, their
+// FP64 lagrangians and the per-manifold fields (floats for CC contact
+// slots and PC manifolds)
+#define FLOW_RECORD_DOUBLES 21
+// an LDS array's footprint: the compiler places each on a 16-byte boundary
+constexpr size_t lds16(size_t bytes) { return (bytes + 15) & ~(size_t)15; }
+#define COLOR_PACK_FLOATS(CC, PC) (16L * (CC) + 8L * (PC))
+
 // pair_ab words (coloured solver): world-local body ids a | b << 16, NO_BODY
 // for a static body, NO_PAIR for a pair without contacts
 #define NO_BODY 0xFFFFu
 #define NO_PAIR 0xFFFFFFFFu
+
+// phases of a step on the batch's stream (the colouring, on the second
+// stream, is phase SHP_MAIN_PHASES of the timing ABI)
+#define SHP_MAIN_PHASES (SHP_NUM_PHASES - 1)
 
 enum { CNT_FLAGS = 0, CNT_WARM = 1, CNT_LEARNED = 2, CNT_LEVELS = 3, CNT_MISS = 4 };
 // grid broadphase: partners per body kept by the count pass for the emit
@@ -89,9 +112,9 @@ struct Ctx {
   uint64_t* cache_key;
   double *cache_ln, *cache_lt;
   int32_t* cache_woff;  // W + 1
-  // coloured schedule: per-pair colour, pairs sorted by colour in world
-  int32_t* sched_level;   // contact_cap
-  int32_t* sched_order;   // contact_cap
+  // coloured schedule: per pair its colour and its manifold slot
+  int32_t* sched_level;   // pair_cap
+  int32_t* sched_order;   // pair_cap
   // ordered (dataflow) schedule: every (contact, dynamic side) listed in
   // its body's segment, per contact {rank a, degree a, rank b, degree b},
   // per-body offsets/cursors/counters for worlds too large for LDS
@@ -105,8 +128,8 @@ struct Ctx {
   // contact order for the ordered solver, FP32 records + FP64 lambdas in
   // colour order for the coloured one)
   double* pk;
-  int32_t* pk_g;  // 2 x contact_cap: bodies a, b
-  uint8_t* pk_f;  // flags: 1 a static, 2 b static, 4 warm, 8 pair continues
+  int32_t* pk_g;  // bodies a, b: 2 x contact_cap (ordered), 2 x pair_cap (coloured)
+  uint8_t* pk_f;  // flags per contact (ordered) or manifold (coloured)
   unsigned long long* col_best;  // colouring priorities per body (large worlds)
   uint32_t* col_used;            // colours taken at each body (large worlds)
   uint32_t* pair_ab;             // per pair: world-local bodies for the colouring (k_sat)
@@ -149,14 +172,77 @@ struct Ctx {
   unsigned long long* quality;  // Q_ENTRIES entries, see Q_*
 };
 
+// ---- launch plan ----------------------------------------------------------
+// Every launch of a step takes its grid from the plan, which is computed on
+// the host from the worlds and the configuration before anything is
+// allocated; shp_batch_plan returns it (and its audit) without a GPU.
+
+// An exclusive sum of n counts: one workgroup (k_scan_small<small_v>) when
+// parts == 0, else k_scan_up over `parts` workgroups of SCAN_ITEMS counts,
+// k_scan_mid over the parts, k_scan_down.
+struct ScanPlan {
+  long n = 0;
+  int parts = 0;
+  int small_v = 0;
+};
+
+struct StepPlan {
+  long NB = 0, NV = 0, W = 0, max_nb = 0;
+  long pair_cap = 0, contact_cap = 0, miss_cap = 0, grid_buckets = 0, n_large = 0;
+  int ppb = 8;
+  bool use_grid = false, grid_wide = false;  // grid broadphase; 16 lanes per body
+  int solver = 0, warm = 0;
+  // ordered solver: lanes per world, dynamic LDS, velocities / counters in LDS
+  int solve_threads = 256;
+  size_t solve_lds = 0;
+  bool solve_vlds = false, solve_clds = false;
+  // coloured solver: lanes per world, dynamic LDS of the colouring, LDS pool
+  int color_threads = 256;
+  size_t color_lds = 0, color_pool = 0;
+  ScanPlan scan_buckets, scan_bodies, scan_pairs;
+  long scan_parts_max = 0;  // scratch entries of the multi-workgroup scans
+  // grids (workgroups) of the step's launches
+  long g_reset = 0, g_bodies = 0, g_pairs = 0, g_broadphase = 0, g_sat = 0, g_ranges = 0;
+  long g_contacts = 0, g_infer = 0, g_pack = 0, g_unpack = 0, g_log = 0, g_finish_ni = 0,
+       g_finish_nq = 0, g_cache = 0;
+  // device memory allocated at creation (bytes)
+  long device_bytes = 0;
+};
+
+// One kernel launch of a step, for the audit.
+struct LaunchRec {
+  const char* name;
+  long grid;
+  int block;
+  size_t dyn_lds, static_lds;
+};
+
+// The host side of a batch before upload: validated worlds in SoA with -0
+// normalised, inverse masses, hull CSR, the grid set-up, and the plan.
+struct HostBatch {
+  StepPlan P;
+  std::vector<double> h[10];  // px py th vx vy w iml imr mu e
+  std::vector<uint8_t> is_static, is_large;
+  std::vector<int32_t> body_world, vert_off, body_off, order, tab_off, large_off, large_ids;
+  std::vector<double> lx, ly, cell;
+};
+
+shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_config& cfg,
+                         HostBatch* hb);
+void plan_launches(const StepPlan& P, bool infer, bool logging, std::vector<LaunchRec>* out);
+size_t mlp_infer_static_lds();
+// every launch within the hardware's limits and every 32-bit index in range
+shp_status audit_plan(const StepPlan& P, const std::vector<LaunchRec>& L);
+
 struct shp_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   shp_config cfg{};
   Ctx ctx{};
+  StepPlan plan{};
   std::vector<int32_t> h_body_off, h_vert_off;
-  void* scan_tmp = nullptr;
-  size_t scan_tmp_bytes = 0;
+  int32_t* scan_part = nullptr;  // per-workgroup sums of the multi-workgroup scans
+  long device_bytes = 0;         // allocated so far
   // the coloured solver's colouring runs on side_stream, concurrently with
   // the contact generation on `stream` (fork and join through the events)
   hipStream_t side_stream = nullptr;
@@ -169,16 +255,10 @@ struct shp_batch {
   // per-phase event timing (shp_batch_set_timing)
   bool timing = false;
   bool events_made = false;
-  hipEvent_t events[SHP_NUM_PHASES + 1]{};
+  hipEvent_t events[SHP_MAIN_PHASES + 1]{};  // main-stream phase boundaries
+  hipEvent_t side_events[2]{};                // around the colouring
   MlpParams mlp{};
   bool logging = false;
-  int ppb = 16;             // pair capacity per body
-  bool solve_vlds = false;  // ordered solver keeps velocities in LDS
-  bool solve_clds = false;  // ordered solver keeps body counters in LDS
-  int solve_threads = 256;  // ordered solver workgroup size
-  size_t solve_lds = 0;     // dynamic LDS bytes of the ordered solver
-  size_t color_lds = 0;     // dynamic LDS bytes of the coloured solver
-  int color_threads = 256;    // coloured solver workgroup size
   double phase_ms[SHP_NUM_PHASES]{};
   double solve_us[8]{};  // accumulated solver sub-phase durations (world 0)
   int32_t timed_steps = 0;
@@ -186,16 +266,24 @@ struct shp_batch {
 
 void shp_set_error(const std::string& msg);
 
-shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds, long NB, long NV,
-                       long pair_cap);
+shp_status batch_alloc(shp_batch* b, const HostBatch& hb);
+// device buffers of `bytes` (at least 16) recorded for release at destroy
+shp_status dalloc_bytes(shp_batch* b, void** p, size_t bytes);
 shp_status solver_alloc(shp_batch* b);
 shp_status colored_alloc(shp_batch* b);
+// plan values of the solvers (lanes, LDS) from the largest world
+void plan_ordered_solver(StepPlan* P, int solver_threads);
+void plan_colored_solver(StepPlan* P, int solver_threads);
+// static LDS of the solver kernels at `threads` lanes
+size_t ordered_static_lds(int threads);
+size_t colored_static_lds(bool sweep);
 void launch_ordered_solver(shp_batch* b);
 // coloured solver: colouring (needs pairs and contact counts only), then,
 // after the contacts, manifold packing and the sweeps
 void launch_colouring(shp_batch* b, hipStream_t s);
-void launch_colored_sweeps(shp_batch* b);
+void launch_colored_pack(shp_batch* b);
+void launch_colored_sweep(shp_batch* b);
+void launch_colored_unpack(shp_batch* b);
 hipError_t launch_grid_count(shp_batch* b);
 hipError_t scan_grid_buckets(shp_batch* b);
 hipError_t launch_grid_emit(shp_batch* b);
-size_t grid_scan_bytes(shp_batch* b);

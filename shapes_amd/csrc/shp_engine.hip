@@ -29,7 +29,6 @@
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <stdint.h>
 #include <stdio.h>
@@ -382,17 +381,6 @@ __global__ void k_sat(Ctx c) {
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < tp; p += gridDim.x * blockDim.x) sat_pair(c, p);
 }
 
-// Contact count of pair p for the contact scan: 0 beyond this step's pairs.
-struct PairCount {
-  const int32_t* ccount;
-  const int32_t* total;  // off_rank[NB]
-  int cap;
-  __host__ __device__ int operator()(int p) const {
-    int t = *total;
-    t = t < cap ? t : cap;
-    return p < t ? ccount[p] : 0;
-  }
-};
 
 // Per-world pair and contact ranges after the scans.
 __global__ void k_world_ranges(Ctx c) {
@@ -457,8 +445,8 @@ __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
       }
     }
     for (int q = 0; q < 6; ++q) {
-      c.jn[q * c.contact_cap + dst] = jn[q];
-      c.jt[q * c.contact_cap + dst] = jt[q];
+      c.jn[(size_t)q * c.contact_cap + dst] = jn[q];
+      c.jt[(size_t)q * c.contact_cap + dst] = jt[q];
     }
     // baumgarte (NonPenetration.hs:48-56)
     c.bn[dst] = depth > c.slop ? (c.beta / c.dt) * (c.slop - depth) : 0.0;
@@ -604,16 +592,20 @@ static shp_status fail(shp_status s, const std::string& msg) {
       return fail(SHP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
   } while (0)
 
+shp_status dalloc_bytes(shp_batch* b, void** p, size_t bytes) {
+  void* q = nullptr;
+  if (bytes < 16) bytes = 16;
+  if (hipMalloc(&q, bytes) != hipSuccess)
+    return fail(SHP_E_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  b->allocs.push_back(q);
+  b->device_bytes += (long)bytes;
+  *p = q;
+  return SHP_OK;
+}
+
 template <typename T>
 static shp_status dalloc(shp_batch* b, T** p, size_t n) {
-  void* q = nullptr;
-  size_t bytes = n * sizeof(T);
-  if (bytes == 0) bytes = sizeof(T);
-  hipError_t e = hipMalloc(&q, bytes);
-  if (e != hipSuccess) return fail(SHP_E_OOM, "hipMalloc failed");
-  b->allocs.push_back(q);
-  *p = (T*)q;
-  return SHP_OK;
+  return dalloc_bytes(b, (void**)p, n * sizeof(T));
 }
 
 #define TRY(expr)                  \
@@ -624,9 +616,9 @@ static shp_status dalloc(shp_batch* b, T** p, size_t n) {
 
 static inline double nz(double x) { return x == 0.0 ? 0.0 : x; }
 
-static int blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
+static long blocks(long n, int bs) { return (n + bs - 1) / bs; }
 // grid of a grid-stride kernel over at most n items
-static int gs_blocks(long n) { return std::max(1, std::min(blocks(n, 256), 4096)); }
+static long gs_blocks(long n) { return std::max(1L, std::min(blocks(n, 256), 4096L)); }
 
 // With timing enabled, an event is recorded on the batch's stream before
 // every launch and after the last one, so phase k's duration is
@@ -635,130 +627,95 @@ static inline void mark(shp_batch* b, int k) {
   if (b->timing) (void)hipEventRecord(b->events[k], b->stream);
 }
 
-static hipError_t exclusive_sum(shp_batch* b, const int* in, int* out, long n, hipStream_t s);
 
-static shp_status enqueue_step(shp_batch* b) {
-  Ctx& c = b->ctx;
-  hipStream_t s = b->stream;
-  mark(b, 0);
-  k_reset<<<blocks(std::max<long>(64, c.use_grid ? (long)c.grid_buckets + 1 : 0), 256), 256, 0,
-            s>>>(c);
-  mark(b, 1);
-  if (c.use_grid)
-    HIPCHK(launch_grid_count(b));
-  else
-    k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 0);
-  mark(b, 2);
-  HIPCHK(exclusive_sum(b, c.cnt_rank, c.off_rank, (long)c.NB + 1, s));
-  mark(b, 3);
-  if (c.use_grid)
-    HIPCHK(launch_grid_emit(b));
-  else
-    k_broadphase<<<blocks((long)c.NB * 64, 256), 256, 0, s>>>(c, 1);
-  mark(b, 4);
-  k_sat<<<gs_blocks(c.pair_cap), 256, 0, s>>>(c);
-  // the colouring needs only the pairs and which of them have contacts: it
-  // runs beside the contact scan, contact generation and learned warm start
-  const bool fork = c.solver == SHP_SOLVER_COLORED_F32;
-  if (fork) {
-    HIPCHK(hipEventRecord(b->fork_event, s));
-    HIPCHK(hipStreamWaitEvent(b->side_stream, b->fork_event, 0));
-    launch_colouring(b, b->side_stream);
-    HIPCHK(hipEventRecord(b->join_event, b->side_stream));
-  }
-  mark(b, 5);
-  size_t tb = b->scan_tmp_bytes;
-  {
-    hipcub::CountingInputIterator<int> idx(0);
-    hipcub::TransformInputIterator<int, PairCount, hipcub::CountingInputIterator<int>> cnt(
-        idx, PairCount{c.ccount, c.off_rank + c.NB, c.pair_cap});
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, cnt, c.coff, c.pair_cap + 1, s));
-  }
-  mark(b, 6);
-  k_world_ranges<<<blocks(c.W + 1, 256), 256, 0, s>>>(c);
-  mark(b, 7);
-  k_contacts<<<gs_blocks(c.pair_cap), 256, 0, s>>>(c);
-  if (c.warm_mode == SHP_WARM_LEARNED && c.mlp_ready) mlp_launch_infer(b);
-  mark(b, 8);
-  if (fork) {
-    HIPCHK(hipStreamWaitEvent(s, b->join_event, 0));
-    launch_colored_sweeps(b);
-  } else {
-    launch_ordered_solver(b);
-  }
-  mark(b, 9);
-  if (b->logging) mlp_launch_log(b);
-  {  // advance + moveShapes beside the solve quality, one launch
-    const int ni = blocks(c.NB, INTEGRATE_THREADS);
-    const int nq = (int)std::max<long>(1, std::min<long>((c.contact_cap + 255) / 256, 2048));
-    k_finish<<<ni + nq, INTEGRATE_THREADS, 0, s>>>(c, ni);
-  }
-  mark(b, 10);
-  // the run sums read the quality maxima
-  k_cache_store<<<gs_blocks(std::max(c.contact_cap, c.W + 1)), 256, 0, s>>>(c);
-  mark(b, 11);
-  mark(b, 12);
-  HIPCHK(hipGetLastError());
-  return SHP_OK;
+// ---- exclusive sums ---------------------------------------------------------
+// Counts in[0, n): item i is in[i] for i < valid and 0 beyond, where valid =
+// min(*limit, cap) when a limit is given (the contact counts of this step's
+// pairs) and n otherwise.  out[i] = the sum of the items before i, written
+// for i <= valid (i < n): what the consumers read.  Sums are formed in 64
+// bits and stored saturated at SCAN_SAT (above every capacity, so a
+// saturated offset is an overflow the consumers skip, never a wrapped
+// negative index).  Up to 20480 counts take one workgroup (k_scan_small),
+// more take three launches: k_scan_up sums each workgroup's SCAN_ITEMS
+// counts, k_scan_mid scans those sums in one workgroup, k_scan_down scans
+// each workgroup's counts from its offset.  Workgroups whose counts all lie
+// beyond `valid` skip their work, so a scan sized for the capacity costs
+// what the step's pairs need.
+constexpr int SCAN_T = 256;                       // k_scan_up / k_scan_down lanes
+constexpr int SCAN_V = 8;                         // int4 loads per lane
+constexpr int SCAN_ITEMS = SCAN_T * 4 * SCAN_V;  // counts per workgroup
+constexpr long long SCAN_SAT = 1LL << 30;
+
+__device__ __forceinline__ int sat_int(long long x) { return (int)(x < SCAN_SAT ? x : SCAN_SAT); }
+
+__device__ __forceinline__ long scan_valid(long n, const int* limit, long cap) {
+  if (!limit) return n;
+  const long t = *limit;
+  return t < cap ? t : cap;
 }
 
-// ---- exclusive sums of up to 20480 counts in one workgroup ---------------
-// Chunk u = entries [4096u, 4096u + 4096): thread t loads entries 4t..4t+3
-// of every chunk as int4 (coalesced, all loads in flight at once), the
-// wavefronts scan their thread sums, wavefront 0 scans the (chunk,
-// wavefront) totals in chunk-major order, and every thread writes its
-// entries.  One launch against the library scan's two (state set-up and
-// scan), for the per-body pair counts and grid bucket counts of batches up
-// to 20479 bodies / buckets.
-template <int V>
-__global__ __launch_bounds__(1024) void k_scan_small(const int* __restrict__ in, int* out, int n) {
-  constexpr int P = V * 16, E = (P + 63) / 64;
-  __shared__ int s_part[P];  // (chunk, wavefront) totals, then their exclusive sums
+// four counts at i (16-byte aligned), zero at and beyond `valid`
+__device__ __forceinline__ int4 load_counts(const int* in, long i, long valid) {
+  if (i + 3 < valid) return *(const int4*)(in + i);
+  int4 x;
+  x.x = i < valid ? in[i] : 0;
+  x.y = i + 1 < valid ? in[i + 1] : 0;
+  x.z = i + 2 < valid ? in[i + 2] : 0;
+  x.w = i + 3 < valid ? in[i + 3] : 0;
+  return x;
+}
+
+__device__ __forceinline__ long long sum4(int4 x) {
+  return ((long long)x.x + x.y) + ((long long)x.z + x.w);
+}
+
+// Workgroup of T lanes scans the V * 4 * T counts from `base` (lane t:
+// counts base + u*4T + 4t .. +3 of chunk u): wavefront scans of the lane
+// sums, wavefront 0 scans the (chunk, wavefront) totals in order, and each
+// lane writes its counts' exclusive sums plus `offset`, for indices below
+// `wend`.
+template <int T, int V>
+__device__ __forceinline__ void scan_block(const int* __restrict__ in, int* out, long base,
+                                           long valid, long wend, long long offset) {
+  constexpr int NW = T / 64, P = V * NW, E = (P + 63) / 64;
+  __shared__ long long s_part[P];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   int4 x[V];
-  int sum[V], incl[V];
+  long long sum[V], incl[V];
 #pragma unroll
   for (int u = 0; u < V; ++u) {
-    const int i = u * 4096 + 4 * tid;
-    if (i + 3 < n) {
-      x[u] = *(const int4*)(in + i);
-    } else {
-      x[u].x = i < n ? in[i] : 0;
-      x[u].y = i + 1 < n ? in[i + 1] : 0;
-      x[u].z = i + 2 < n ? in[i + 2] : 0;
-      x[u].w = 0;
-    }
-    sum[u] = (x[u].x + x[u].y) + (x[u].z + x[u].w);
+    x[u] = load_counts(in, base + (long)u * 4 * T + 4 * tid, valid);
+    sum[u] = sum4(x[u]);
     incl[u] = sum[u];
   }
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
 #pragma unroll
     for (int u = 0; u < V; ++u) {
-      const int y = __shfl_up(incl[u], o);
+      const long long y = __shfl_up(incl[u], o);
       if (lane >= o) incl[u] += y;
     }
   }
   if (lane == 63) {
 #pragma unroll
-    for (int u = 0; u < V; ++u) s_part[u * 16 + wv] = incl[u];
+    for (int u = 0; u < V; ++u) s_part[u * NW + wv] = incl[u];
   }
   __syncthreads();
   if (wv == 0) {
-    int v[E], t = 0;
+    long long v[E], t = 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = lane * E + e;
       v[e] = k < P ? s_part[k] : 0;
       t += v[e];
     }
-    int sc = t;
+    long long sc = t;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(sc, o);
+      const long long y = __shfl_up(sc, o);
       if (lane >= o) sc += y;
     }
-    int ex = sc - t;
+    long long ex = sc - t;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = lane * E + e;
@@ -769,47 +726,203 @@ __global__ __launch_bounds__(1024) void k_scan_small(const int* __restrict__ in,
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < V; ++u) {
-    const int i = u * 4096 + 4 * tid;
+    const long i = base + (long)u * 4 * T + 4 * tid;
+    const long long e0 = offset + s_part[u * NW + wv] + incl[u] - sum[u];
+    const long long e1 = e0 + x[u].x, e2 = e1 + x[u].y, e3 = e2 + x[u].z;
     int4 e;
-    e.x = s_part[u * 16 + wv] + incl[u] - sum[u];
-    e.y = e.x + x[u].x;
-    e.z = e.y + x[u].y;
-    e.w = e.z + x[u].z;
-    if (i + 3 < n) {
+    e.x = sat_int(e0);
+    e.y = sat_int(e1);
+    e.z = sat_int(e2);
+    e.w = sat_int(e3);
+    if (i + 3 < wend) {
       *(int4*)(out + i) = e;
     } else {
-      if (i < n) out[i] = e.x;
-      if (i + 1 < n) out[i + 1] = e.y;
-      if (i + 2 < n) out[i + 2] = e.z;
+      if (i < wend) out[i] = e.x;
+      if (i + 1 < wend) out[i + 1] = e.y;
+      if (i + 2 < wend) out[i + 2] = e.z;
     }
   }
 }
 
-// out[i] = in[0] + ... + in[i-1] for i < n, on stream s: one workgroup for
-// n <= 20480 (int4-aligned arrays), the library scan beyond.
-static hipError_t exclusive_sum(shp_batch* b, const int* in, int* out, long n, hipStream_t s) {
-  if (n <= 4096) {
-    k_scan_small<1><<<1, 1024, 0, s>>>(in, out, (int)n);
-  } else if (n <= 8192) {
-    k_scan_small<2><<<1, 1024, 0, s>>>(in, out, (int)n);
-  } else if (n <= 16384) {
-    k_scan_small<4><<<1, 1024, 0, s>>>(in, out, (int)n);
-  } else if (n <= 20480) {  // e.g. a power-of-two bucket table + 1
-    k_scan_small<5><<<1, 1024, 0, s>>>(in, out, (int)n);
-  } else {  // (measured: from 8 chunks on, the wavefront scans cost more than the library's launch)
-    size_t tb = b->scan_tmp_bytes;
-    return hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, tb, in, out, (int)n, s);
+// up to V * 4096 counts in one workgroup of 1024 lanes
+template <int V>
+__global__ __launch_bounds__(1024) void k_scan_small(const int* __restrict__ in, int* out, long n,
+                                                     const int* limit, long cap) {
+  const long valid = scan_valid(n, limit, cap);
+  scan_block<1024, V>(in, out, 0, valid, valid + 1 < n ? valid + 1 : n, 0);
+}
+
+// per-workgroup sums, saturated (a saturated sum saturates every later offset)
+__global__ __launch_bounds__(SCAN_T) void k_scan_up(const int* __restrict__ in, long n,
+                                                    const int* limit, long cap, int* part) {
+  __shared__ long long s_w[SCAN_T / 64];
+  const long valid = scan_valid(n, limit, cap);
+  const long base = (long)blockIdx.x * SCAN_ITEMS;
+  long long s = 0;
+  if (base < valid) {
+#pragma unroll
+    for (int u = 0; u < SCAN_V; ++u)
+      s += sum4(load_counts(in, base + (long)u * 4 * SCAN_T + 4 * threadIdx.x, valid));
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int k = 0; k < SCAN_T / 64; ++k) t += s_w[k];
+    part[blockIdx.x] = sat_int(t);
+  }
+}
+
+// exclusive sums of part[0, parts) in place (saturated), 1024 at a time
+// with a carry
+__global__ __launch_bounds__(1024) void k_scan_mid(int* part, int parts) {
+  __shared__ long long s_w[16];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  long long carry = 0;
+  for (int c0 = 0; c0 < parts; c0 += 1024) {
+    const int i = c0 + tid;
+    const long long x = i < parts ? part[i] : 0;
+    long long incl = x;
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_w[wv] = incl;
+    __syncthreads();
+    long long before = 0, total = 0;
+    for (int k = 0; k < 16; ++k) {
+      const long long v = s_w[k];
+      before += k < wv ? v : 0;
+      total += v;
+    }
+    if (i < parts) part[i] = sat_int(carry + before + incl - x);
+    carry += total;
+    __syncthreads();  // s_w is rewritten by the next chunk
+  }
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_down(const int* __restrict__ in, int* out, long n,
+                                                      const int* limit, long cap,
+                                                      const int* part) {
+  const long valid = scan_valid(n, limit, cap);
+  const long base = (long)blockIdx.x * SCAN_ITEMS;
+  if (base > valid) return;  // no output of this workgroup is read
+  scan_block<SCAN_T, SCAN_V>(in, out, base, valid, valid + 1 < n ? valid + 1 : n,
+                             part[blockIdx.x]);
+}
+
+// The plan of an exclusive sum of n counts.
+static ScanPlan plan_scan(long n) {
+  ScanPlan s;
+  s.n = n;
+  if (n <= 20480) {
+    s.small_v = n <= 4096 ? 1 : n <= 8192 ? 2 : n <= 16384 ? 4 : 5;
+  } else {
+    s.parts = (int)((n + SCAN_ITEMS - 1) / SCAN_ITEMS);
+  }
+  return s;
+}
+
+static void scan_launches(const ScanPlan& s, const char* name, std::vector<LaunchRec>* out) {
+  if (s.parts == 0) {
+    out->push_back({name, 1, 1024, 0, lds16((size_t)s.small_v * 16 * 8)});
+  } else {
+    out->push_back({name, s.parts, SCAN_T, 0, lds16((SCAN_T / 64) * 8)});
+    out->push_back({name, 1, 1024, 0, lds16(16 * 8)});
+    out->push_back({name, s.parts, SCAN_T, 0, lds16((size_t)SCAN_V * (SCAN_T / 64) * 8)});
+  }
+}
+
+// out[i] = in[0] + ... + in[i-1] on stream st, as planned
+static hipError_t exclusive_sum(shp_batch* b, const ScanPlan& sp, const int* in, int* out,
+                                const int* limit, long cap, hipStream_t st) {
+  const long n = sp.n;
+  switch (sp.small_v) {
+    case 1: k_scan_small<1><<<1, 1024, 0, st>>>(in, out, n, limit, cap); break;
+    case 2: k_scan_small<2><<<1, 1024, 0, st>>>(in, out, n, limit, cap); break;
+    case 4: k_scan_small<4><<<1, 1024, 0, st>>>(in, out, n, limit, cap); break;
+    case 5: k_scan_small<5><<<1, 1024, 0, st>>>(in, out, n, limit, cap); break;
+    default:
+      k_scan_up<<<sp.parts, SCAN_T, 0, st>>>(in, n, limit, cap, b->scan_part);
+      k_scan_mid<<<1, 1024, 0, st>>>(b->scan_part, sp.parts);
+      k_scan_down<<<sp.parts, SCAN_T, 0, st>>>(in, out, n, limit, cap, b->scan_part);
   }
   return hipGetLastError();
 }
 
 hipError_t scan_grid_buckets(shp_batch* b) {
-  return exclusive_sum(b, b->ctx.bucket_cnt, b->ctx.bucket_start, (long)b->ctx.grid_buckets + 1,
-                       b->stream);
+  return exclusive_sum(b, b->plan.scan_buckets, b->ctx.bucket_cnt, b->ctx.bucket_start, nullptr,
+                       0, b->stream);
+}
+
+static shp_status enqueue_step(shp_batch* b) {
+  Ctx& c = b->ctx;
+  const StepPlan& P = b->plan;
+  hipStream_t s = b->stream;
+  mark(b, 0);
+  k_reset<<<P.g_reset, 256, 0, s>>>(c);
+  mark(b, 1);
+  if (c.use_grid)
+    HIPCHK(launch_grid_count(b));
+  else
+    k_broadphase<<<P.g_broadphase, 256, 0, s>>>(c, 0);
+  mark(b, 2);
+  HIPCHK(exclusive_sum(b, P.scan_bodies, c.cnt_rank, c.off_rank, nullptr, 0, s));
+  mark(b, 3);
+  if (c.use_grid)
+    HIPCHK(launch_grid_emit(b));
+  else
+    k_broadphase<<<P.g_broadphase, 256, 0, s>>>(c, 1);
+  mark(b, 4);
+  k_sat<<<P.g_sat, 256, 0, s>>>(c);
+  // the colouring needs only the pairs and which of them have contacts: it
+  // runs beside the contact scan, contact generation and learned warm start
+  const bool fork = c.solver == SHP_SOLVER_COLORED_F32;
+  if (fork) {
+    HIPCHK(hipEventRecord(b->fork_event, s));
+    HIPCHK(hipStreamWaitEvent(b->side_stream, b->fork_event, 0));
+    if (b->timing) HIPCHK(hipEventRecord(b->side_events[0], b->side_stream));
+    launch_colouring(b, b->side_stream);
+    if (b->timing) HIPCHK(hipEventRecord(b->side_events[1], b->side_stream));
+    HIPCHK(hipEventRecord(b->join_event, b->side_stream));
+  }
+  mark(b, 5);
+  // contacts per pair: this step's pairs only (the pair count is on the device)
+  HIPCHK(exclusive_sum(b, P.scan_pairs, c.ccount, c.coff, c.off_rank + c.NB, c.pair_cap, s));
+  mark(b, 6);
+  k_world_ranges<<<P.g_ranges, 256, 0, s>>>(c);
+  mark(b, 7);
+  k_contacts<<<P.g_contacts, 256, 0, s>>>(c);
+  mark(b, 8);
+  if (c.warm_mode == SHP_WARM_LEARNED && c.mlp_ready) mlp_launch_infer(b);
+  mark(b, 9);
+  if (fork) {
+    HIPCHK(hipStreamWaitEvent(s, b->join_event, 0));
+    launch_colored_pack(b);
+    mark(b, 10);
+    launch_colored_sweep(b);
+    mark(b, 11);
+    launch_colored_unpack(b);
+  } else {
+    mark(b, 10);
+    launch_ordered_solver(b);
+    mark(b, 11);
+  }
+  if (b->logging) mlp_launch_log(b);
+  mark(b, 12);
+  // advance + moveShapes beside the solve quality, one launch
+  k_finish<<<P.g_finish_ni + P.g_finish_nq, INTEGRATE_THREADS, 0, s>>>(c, (int)P.g_finish_ni);
+  mark(b, 13);
+  // the run sums read the quality maxima
+  k_cache_store<<<P.g_cache, 256, 0, s>>>(c);
+  mark(b, SHP_MAIN_PHASES);
+  HIPCHK(hipGetLastError());
+  return SHP_OK;
 }
 
 static shp_status accumulate_timing(shp_batch* b) {
-  HIPCHK(hipEventSynchronize(b->events[SHP_NUM_PHASES]));
+  HIPCHK(hipEventSynchronize(b->events[SHP_MAIN_PHASES]));
   unsigned long long ts[8];
   HIPCHK(hipMemcpy(ts, b->ctx.tstamp, sizeof(ts), hipMemcpyDeviceToHost));
   // wall_clock64 runs at 100 MHz on CDNA3/4
@@ -817,40 +930,159 @@ static shp_status accumulate_timing(shp_batch* b) {
     if (ts[k + 1] >= ts[k] && ts[k] != 0) b->solve_us[k] += (double)(ts[k + 1] - ts[k]) * 0.01;
   unsigned long long zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   HIPCHK(hipMemcpy(b->ctx.tstamp, zero, sizeof(zero), hipMemcpyHostToDevice));
-  for (int k = 0; k < SHP_NUM_PHASES; ++k) {
+  for (int k = 0; k < SHP_MAIN_PHASES; ++k) {
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, b->events[k], b->events[k + 1]));
     b->phase_ms[k] += ms;
+  }
+  if (b->ctx.solver == SHP_SOLVER_COLORED_F32) {  // the colouring, on the second stream
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, b->side_events[0], b->side_events[1]));
+    b->phase_ms[SHP_MAIN_PHASES] += ms;
   }
   b->timed_steps += 1;
   return SHP_OK;
 }
 
 
-extern "C" {
+// ---- batch creation: validation, host staging, plan, audit, upload --------
 
-const char* shp_last_error(void) { return g_err.c_str(); }
-const char* shp_version(void) { return "shapes-mi355x 0.1 (gfx950)"; }
+// static LDS of the engine's kernels (the __shared__ arrays they declare)
+constexpr size_t FINISH_STATIC_LDS =
+    lds16(6 * INTEGRATE_THREADS * 8) + 2 * lds16(INTEGRATE_VERTS * 8) + lds16(INTEGRATE_VERTS) +
+    lds16((INTEGRATE_THREADS + 1) * 4) + lds16(3 * (QUALITY_THREADS / 64) * 8);
+// 16 lanes per body in the grid's pair passes up to this many bodies
+#define GRID_WIDE_MAX_BODIES 131072
 
-shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, const shp_config* cfg,
-                            shp_batch** out) {
-  if (!worlds || n_worlds <= 0 || !cfg || !out) return fail(SHP_E_INVAL, "null argument");
-  if (!(cfg->dt > 0.0)) return fail(SHP_E_INVAL, "dt must be positive");
-  if (cfg->iterations < 0) return fail(SHP_E_INVAL, "iterations must be >= 0");
-  if (cfg->solver != SHP_SOLVER_ORDERED_F64 && cfg->solver != SHP_SOLVER_COLORED_F32)
+// Every device buffer of a batch and its size in bytes, in allocation
+// order; both the footprint of the plan and the allocation walk this list.
+struct Buf {
+  void** p;
+  size_t bytes;
+};
+
+static void batch_buffers(const StepPlan& P, Ctx& c, int32_t** scan_part, std::vector<Buf>* out) {
+  const size_t NB = P.NB, NV = P.NV, W = P.W, PC = P.pair_cap, CC = P.contact_cap;
+  auto add = [&](auto** ptr, size_t bytes) { out->push_back({(void**)ptr, bytes}); };
+  double** body[10] = {&c.px, &c.py, &c.th, &c.vx, &c.vy, &c.w, &c.iml, &c.imr, &c.mu, &c.e};
+  for (auto* q : body) add(q, NB * 8);
+  add(&c.is_static, NB);
+  add(&c.body_world, NB * 4);
+  add(&c.vert_off, (NB + 1) * 4);
+  add(&c.body_off, (W + 1) * 4);
+  add(&c.world_order, W * 4);
+  add(&c.lx, NV * 8);
+  add(&c.ly, NV * 8);
+  add(&c.ext_min, NV);
+  add(&c.ext_max, NV);
+  for (double** q : {&c.wx, &c.wy, &c.wnx, &c.wny}) add(q, NV * 8);
+  for (double** q : {&c.bminx, &c.bmaxx, &c.bminy, &c.bmaxy}) add(q, NB * 8);
+  add(&c.cnt_rank, (NB + 1) * 4);
+  add(&c.off_rank, (NB + 1) * 4);
+  add(&c.pa, PC * 4);
+  add(&c.pb, PC * 4);
+  add(&c.ccount, (PC + 1) * 4);
+  add(&c.coff, (PC + 1) * 4);
+  add(&c.s_feat, 2 * PC * 2);
+  add(&c.s_flip, 2 * PC);
+  for (double** q : {&c.s_nx, &c.s_ny, &c.s_px, &c.s_py, &c.s_depth}) add(q, 2 * PC * 8);
+  add(&c.wpair_off, (W + 1) * 4);
+  add(&c.wcont_off, (W + 1) * 4);
+  add(&c.key, CC * 8);
+  add(&c.ga, CC * 4);
+  add(&c.gb, CC * 4);
+  add(&c.flip, CC);
+  for (double** q : {&c.cnx, &c.cny, &c.cpx, &c.cpy, &c.depth}) add(q, CC * 8);
+  add(&c.jn, 6 * CC * 8);
+  add(&c.jt, 6 * CC * 8);
+  for (double** q : {&c.bn, &c.rax, &c.ray, &c.rbx, &c.rby, &c.mcn, &c.mct, &c.ln, &c.lt})
+    add(q, CC * 8);
+  add(&c.warm, CC);
+  add(&c.cache_key, CC * 8);
+  add(&c.cache_ln, CC * 8);
+  add(&c.cache_lt, CC * 8);
+  add(&c.cache_woff, (W + 1) * 4);
+  if (P.miss_cap > 0) {
+    add(&c.miss_idx, (size_t)P.miss_cap * 4);
+    add(&c.miss_feat, (size_t)P.miss_cap * MLP_IN * 4);
+  }
+  add(&c.log_count, 8);
+  add(&c.counters, 8 * 4);
+  add(&c.quality, Q_ENTRIES * 8);
+  if (P.scan_parts_max > 0) add(scan_part, (size_t)P.scan_parts_max * 4);
+  if (P.use_grid) {
+    const size_t TB = P.grid_buckets;
+    add(&c.cell_h, W * 8);
+    add(&c.is_large, NB);
+    add(&c.body_cell, 2 * NB * 4);
+    add(&c.tab_off, (W + 1) * 4);
+    add(&c.bucket_cnt, (TB + 1) * 4);
+    add(&c.bucket_start, (TB + 1) * 4);
+    add(&c.body_slot, NB * 4);
+    add(&c.grid_rec, 6 * NB * 8);
+    add(&c.grid_slots, GRID_SLOTS * NB * 4);
+    add(&c.large_off, (W + 1) * 4);
+    add(&c.large_ids, std::max<size_t>(P.n_large, 1) * 4);
+  }
+  if (P.solver == SHP_SOLVER_ORDERED_F64) {
+    add(&c.pk, FLOW_RECORD_DOUBLES * CC * 8);
+    add(&c.pk_g, 2 * CC * 4);
+    add(&c.pk_f, CC);
+    add(&c.flow_list, 2 * CC * 4);
+    add(&c.flow_rk, CC * 16);
+    add(&c.flow_off, NB * 4);
+    add(&c.flow_cur, NB * 4);
+    add(&c.flow_lvl, CC * 4);
+    add(&c.flow_order, CC * 4);
+  } else {
+    add(&c.pk, COLOR_PACK_FLOATS(CC, PC) * 4);
+    add(&c.pk_g, 2 * PC * 4);
+    add(&c.pk_f, PC);
+    add(&c.sched_level, PC * 4);
+    add(&c.sched_order, PC * 4);
+    add(&c.fvx, NB * 4);
+    add(&c.fvy, NB * 4);
+    add(&c.fw, NB * 4);
+    add(&c.col_best, NB * 8);
+    add(&c.col_used, NB * 4);
+    add(&c.col_off, W * (COLOR_CAP + 3) * 4);
+    add(&c.col_info, W * 4);
+    add(&c.pair_ab, PC * 4);
+  }
+}
+
+static long footprint(const StepPlan& P) {
+  Ctx c{};
+  int32_t* part = nullptr;
+  std::vector<Buf> bufs;
+  batch_buffers(P, c, &part, &bufs);
+  long t = 0;
+  for (const Buf& x : bufs) t += (long)std::max<size_t>(x.bytes, 16);
+  return t;
+}
+
+shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_config& cfg,
+                         HostBatch* hb) {
+  if (!worlds || n_worlds <= 0) return fail(SHP_E_INVAL, "null argument");
+  if (!(cfg.dt > 0.0)) return fail(SHP_E_INVAL, "dt must be positive");
+  if (cfg.iterations < 0) return fail(SHP_E_INVAL, "iterations must be >= 0");
+  if (cfg.solver != SHP_SOLVER_ORDERED_F64 && cfg.solver != SHP_SOLVER_COLORED_F32)
     return fail(SHP_E_INVAL, "unknown solver mode");
-  if (cfg->warm_start != SHP_WARM_ZERO && cfg->warm_start != SHP_WARM_LEARNED)
+  if (cfg.warm_start != SHP_WARM_ZERO && cfg.warm_start != SHP_WARM_LEARNED)
     return fail(SHP_E_INVAL, "unknown warm-start mode");
-  if (cfg->broadphase < SHP_BROADPHASE_AUTO || cfg->broadphase > SHP_BROADPHASE_GRID)
+  if (cfg.broadphase < SHP_BROADPHASE_AUTO || cfg.broadphase > SHP_BROADPHASE_GRID)
     return fail(SHP_E_INVAL, "unknown broadphase mode");
-  if (cfg->solver_threads != 0 && cfg->solver_threads != 256 && cfg->solver_threads != 512 &&
-      !(cfg->solver_threads == 1024 && cfg->solver == SHP_SOLVER_COLORED_F32))
+  if (cfg.solver_threads != 0 && cfg.solver_threads != 256 && cfg.solver_threads != 512 &&
+      !(cfg.solver_threads == 1024 && cfg.solver == SHP_SOLVER_COLORED_F32))
     return fail(SHP_E_INVAL, "solver_threads must be 0, 256 or 512 (or 1024, coloured solver)");
-  long NB = 0, NV = 0;
+  if (cfg.max_pairs_per_body < 0) return fail(SHP_E_INVAL, "max_pairs_per_body must be >= 0");
+  long NB = 0, NV = 0, max_nb = 0;
   for (int w = 0; w < n_worlds; ++w) {
     const shp_world_desc& d = worlds[w];
     if (d.n_bodies <= 0) return fail(SHP_E_INVAL, "world with no bodies");
     if (d.n_bodies >= (1 << 24)) return fail(SHP_E_UNSUPPORTED, "world exceeds 2^24 bodies");
+    if (cfg.solver == SHP_SOLVER_COLORED_F32 && d.n_bodies >= (int)NO_BODY)
+      return fail(SHP_E_UNSUPPORTED, "world exceeds 65534 bodies (coloured solver)");
     if (!d.px || !d.py || !d.theta || !d.vx || !d.vy || !d.omega || !d.mass || !d.inertia ||
         !d.mu || !d.bounce || !d.hull_off || !d.hull_x || !d.hull_y)
       return fail(SHP_E_INVAL, "null array in world description");
@@ -862,18 +1094,275 @@ shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, cons
     }
     NB += d.n_bodies;
     NV += d.hull_off[d.n_bodies];
+    max_nb = std::max<long>(max_nb, d.n_bodies);
   }
-  if (NB >= (1L << 30)) return fail(SHP_E_UNSUPPORTED, "too many bodies");
-  int ppb = cfg->max_pairs_per_body > 0 ? cfg->max_pairs_per_body : 8;
-  long pair_cap = NB * (long)ppb + 1024;
-  if (pair_cap >= (1L << 30)) return fail(SHP_E_UNSUPPORTED, "pair capacity too large");
+  if (NB >= (1L << 30) || NV >= (1L << 30)) return fail(SHP_E_UNSUPPORTED, "too many bodies");
+  StepPlan& P = hb->P;
+  P = StepPlan{};
+  P.NB = NB;
+  P.NV = NV;
+  P.W = n_worlds;
+  P.max_nb = max_nb;
+  P.solver = cfg.solver;
+  P.warm = cfg.warm_start;
+  P.ppb = cfg.max_pairs_per_body > 0 ? cfg.max_pairs_per_body : 8;
+  P.pair_cap = NB * (long)P.ppb + 1024;
+  // pair and contact offsets stay below the scans' saturation value (2^30)
+  if (P.pair_cap >= (1L << 29)) return fail(SHP_E_UNSUPPORTED, "pair capacity too large (2^29)");
   // the coloured sweeps address a world's records with 32-bit byte offsets
-  if (cfg->solver == SHP_SOLVER_COLORED_F32 && pair_cap >= (1L << 28))
+  if (cfg.solver == SHP_SOLVER_COLORED_F32 && P.pair_cap >= (1L << 28))
     return fail(SHP_E_UNSUPPORTED, "pair capacity too large for the coloured solver (2^28)");
+  P.contact_cap = 2 * P.pair_cap;
+  P.miss_cap = cfg.warm_start == SHP_WARM_LEARNED ? std::min(2 * P.pair_cap, 4 * NB + 4096) : 0;
 
+  // host staging in SoA, -0 normalised to +0 (static velocities stay as loaded)
+  for (auto& v : hb->h) v.assign(NB, 0.0);
+  hb->body_world.assign(NB, 0);
+  hb->vert_off.assign(NB + 1, 0);
+  hb->body_off.assign(n_worlds + 1, 0);
+  hb->is_static.assign(NB, 0);
+  hb->lx.assign(NV, 0.0);
+  hb->ly.assign(NV, 0.0);
+  long g = 0, vo = 0;
+  for (int w = 0; w < n_worlds; ++w) {
+    const shp_world_desc& d = worlds[w];
+    hb->body_off[w] = (int32_t)g;
+    for (int i = 0; i < d.n_bodies; ++i, ++g) {
+      hb->h[0][g] = nz(d.px[i]);
+      hb->h[1][g] = nz(d.py[i]);
+      hb->h[2][g] = nz(d.theta[i]);
+      hb->h[3][g] = nz(d.vx[i]);
+      hb->h[4][g] = nz(d.vy[i]);
+      hb->h[5][g] = nz(d.omega[i]);
+      // toInvMass2 (Constraint.hs:79-83)
+      hb->h[6][g] = d.mass[i] == 0.0 ? 0.0 : 1.0 / d.mass[i];
+      hb->h[7][g] = d.inertia[i] == 0.0 ? 0.0 : 1.0 / d.inertia[i];
+      hb->h[8][g] = d.mu[i];
+      hb->h[9][g] = d.bounce[i];
+      hb->is_static[g] = (hb->h[6][g] == 0.0 && hb->h[7][g] == 0.0) ? 1 : 0;
+      hb->body_world[g] = w;
+      hb->vert_off[g] = (int32_t)vo;
+      for (int k = d.hull_off[i]; k < d.hull_off[i + 1]; ++k, ++vo) {
+        hb->lx[vo] = d.hull_x[k];
+        hb->ly[vo] = d.hull_y[k];
+      }
+    }
+  }
+  hb->body_off[n_worlds] = (int32_t)g;
+  hb->vert_off[NB] = (int32_t)vo;
+  const std::vector<int32_t>& body_off = hb->body_off;
+  // per-world solver workgroups take the largest worlds first, so the
+  // longest solves start at once instead of trailing the batch
+  hb->order.resize(n_worlds);
+  for (int w = 0; w < n_worlds; ++w) hb->order[w] = w;
+  std::stable_sort(hb->order.begin(), hb->order.end(), [&](int32_t x, int32_t y) {
+    return body_off[x + 1] - body_off[x] > body_off[y + 1] - body_off[y];
+  });
+
+  // grid broadphase set-up: cell size per world from the largest vertex
+  // radius of its dynamic bodies; static bodies larger than a cell are
+  // "large" and tested explicitly
+  const int mode = cfg.broadphase;
+  P.use_grid = mode == SHP_BROADPHASE_GRID || (mode == SHP_BROADPHASE_AUTO && max_nb > 2048);
+  if (P.use_grid) {
+    std::vector<double> rmax(NB, 0.0);
+    for (long gg = 0; gg < NB; ++gg)
+      for (int k = hb->vert_off[gg]; k < hb->vert_off[gg + 1]; ++k)
+        rmax[gg] = std::max(rmax[gg], std::sqrt(hb->lx[k] * hb->lx[k] + hb->ly[k] * hb->ly[k]));
+    hb->cell.assign(n_worlds, 1.0);
+    hb->tab_off.assign(n_worlds + 1, 0);
+    hb->large_off.assign(n_worlds + 1, 0);
+    hb->large_ids.clear();
+    hb->is_large.assign(NB, 0);
+    long tb_total = 0;
+    for (int w = 0; w < n_worlds; ++w) {
+      double hmax = 0.0;
+      for (int gg = body_off[w]; gg < body_off[w + 1]; ++gg)
+        if (!hb->is_static[gg]) hmax = std::max(hmax, 2.0 * rmax[gg]);
+      hb->cell[w] = hmax > 0.0 ? hmax * (1.0 + 1e-6) + 1e-9 : 1.0;
+      hb->large_off[w] = (int32_t)hb->large_ids.size();
+      for (int gg = body_off[w]; gg < body_off[w + 1]; ++gg)
+        if (hb->is_static[gg] && 2.0 * rmax[gg] >= hb->cell[w]) {
+          hb->is_large[gg] = 1;
+          hb->large_ids.push_back(gg);
+        }
+      const int nw = body_off[w + 1] - body_off[w];
+      long t = 64;
+      while (t < nw) t <<= 1;
+      hb->tab_off[w] = (int32_t)tb_total;
+      tb_total += t;
+    }
+    if (tb_total >= (1L << 30)) return fail(SHP_E_UNSUPPORTED, "grid table too large");
+    hb->tab_off[n_worlds] = (int32_t)tb_total;
+    hb->large_off[n_worlds] = (int32_t)hb->large_ids.size();
+    P.grid_buckets = tb_total;
+    P.n_large = (long)hb->large_ids.size();
+    P.grid_wide = NB <= GRID_WIDE_MAX_BODIES;
+    P.scan_buckets = plan_scan(tb_total + 1);
+  }
+  P.scan_bodies = plan_scan(NB + 1);
+  P.scan_pairs = plan_scan(P.pair_cap + 1);
+  P.scan_parts_max =
+      std::max({(long)P.scan_buckets.parts, (long)P.scan_bodies.parts, (long)P.scan_pairs.parts});
+  if (cfg.solver == SHP_SOLVER_ORDERED_F64)
+    plan_ordered_solver(&P, cfg.solver_threads);
+  else
+    plan_colored_solver(&P, cfg.solver_threads);
+
+  const long PC = P.pair_cap, CC = P.contact_cap;
+  P.g_reset = blocks(std::max<long>(64, P.use_grid ? P.grid_buckets + 1 : 0), 256);
+  P.g_bodies = blocks(NB, 256);
+  P.g_pairs = P.grid_wide ? blocks(NB * 16, 256) : blocks(NB, 256);
+  P.g_broadphase = blocks(NB * 64, 256);
+  P.g_sat = gs_blocks(PC);
+  P.g_ranges = blocks(n_worlds + 1, 256);
+  P.g_contacts = gs_blocks(PC);
+  P.g_infer = std::min<long>(std::max<long>(1, blocks(P.miss_cap, 64)), 512);
+  P.g_pack = std::max(1L, std::min(blocks(PC, 256), 4096L));
+  P.g_unpack = std::max(1L, std::min(blocks(std::max(PC, NB), 256), 4096L));
+  P.g_log = std::max<long>(1, blocks(P.miss_cap, 256));
+  P.g_finish_ni = blocks(NB, INTEGRATE_THREADS);
+  P.g_finish_nq = std::max(1L, std::min(blocks(CC, 256), 2048L));
+  P.g_cache = gs_blocks(std::max<long>(CC, n_worlds + 1));
+  P.device_bytes = footprint(P);
+
+  std::vector<LaunchRec> L;
+  plan_launches(P, cfg.warm_start == SHP_WARM_LEARNED, false, &L);
+  TRY(audit_plan(P, L));
+  // The learned warm start has not run with the coloured solver's
+  // global-memory paths (worlds whose FP32 velocities do not fit in LDS):
+  // refused until it is tested there.
+  if (cfg.warm_start == SHP_WARM_LEARNED && cfg.solver == SHP_SOLVER_COLORED_F32 &&
+      max_nb > VEL_LDS_MAX_BODIES)
+    return fail(SHP_E_UNSUPPORTED,
+                "learned warm start with the coloured solver is limited to worlds of at most " +
+                    std::to_string(VEL_LDS_MAX_BODIES) + " bodies (largest world: " +
+                    std::to_string(max_nb) + ")");
+  return SHP_OK;
+}
+
+void plan_launches(const StepPlan& P, bool infer, bool logging, std::vector<LaunchRec>* out) {
+  out->clear();
+  auto add = [&](const char* n, long g, int b, size_t dyn, size_t st) {
+    out->push_back({n, g, b, dyn, st});
+  };
+  add("k_reset", P.g_reset, 256, 0, 0);
+  if (P.use_grid) {
+    add("k_grid_assign", P.g_bodies, 256, 0, 0);
+    scan_launches(P.scan_buckets, "scan_buckets", out);
+    add("k_grid_fill", P.g_bodies, 256, 0, 0);
+    add("k_grid_pairs", P.g_pairs, 256, 0, 0);
+  } else {
+    add("k_broadphase", P.g_broadphase, 256, 0, 0);
+  }
+  scan_launches(P.scan_bodies, "scan_bodies", out);
+  if (P.use_grid)
+    add("k_grid_pairs", P.g_pairs, 256, 0, 0);
+  else
+    add("k_broadphase", P.g_broadphase, 256, 0, 0);
+  add("k_sat", P.g_sat, 256, 0, 0);
+  const bool col = P.solver == SHP_SOLVER_COLORED_F32;
+  if (col) add("k_colour", P.W, P.color_threads >= 512 ? 1024 : 256, P.color_lds,
+               colored_static_lds(false));
+  scan_launches(P.scan_pairs, "scan_pairs", out);
+  add("k_world_ranges", P.g_ranges, 256, 0, 0);
+  add("k_contacts", P.g_contacts, 256, 0, lds16(4));
+  if (infer) add("k_mlp_infer", P.g_infer, 256, 0, mlp_infer_static_lds());
+  if (col) {
+    add("k_pack_manifolds", P.g_pack, 256, 0, 0);
+    add("k_sweep_colours", P.W, P.color_threads, P.color_pool, colored_static_lds(true));
+    add("k_colour_unpack", P.g_unpack, 256, 0, 0);
+  } else {
+    add("k_solve_flow", P.W, P.solve_threads, std::max<size_t>(P.solve_lds, 16),
+        ordered_static_lds(P.solve_threads));
+  }
+  if (logging) add("k_mlp_log", P.g_log, 256, 0, 0);
+  add("k_finish", P.g_finish_ni + P.g_finish_nq, INTEGRATE_THREADS, 0, FINISH_STATIC_LDS);
+  add("k_cache_store", P.g_cache, 256, 0, 0);
+}
+
+shp_status audit_plan(const StepPlan& P, const std::vector<LaunchRec>& L) {
+  for (const LaunchRec& r : L) {
+    const std::string n = r.name;
+    if (r.grid < 1 || r.grid > 0x7FFFFFFFL)
+      return fail(SHP_E_UNSUPPORTED, n + ": " + std::to_string(r.grid) + " workgroups");
+    if (r.block < 64 || r.block > 1024)
+      return fail(SHP_E_UNSUPPORTED, n + ": workgroup of " + std::to_string(r.block));
+    if (r.grid * (long)r.block > 0xFFFFFFFFL)
+      return fail(SHP_E_UNSUPPORTED, n + ": more than 2^32 work-items");
+    if (r.dyn_lds + r.static_lds > (size_t)LDS_BYTES_PER_CU)
+      return fail(SHP_E_UNSUPPORTED, n + ": " + std::to_string(r.dyn_lds + r.static_lds) +
+                                         " bytes of LDS");
+  }
+  // 32-bit index arithmetic: contact slots 2p + k, contact offsets, the
+  // grid table, vertex offsets, the coloured solver's 16-byte record offsets
+  const long I32 = 0x7FFFFFFFL;
+  if (2 * P.pair_cap + 1 > I32 || P.contact_cap + 1 > I32 || P.grid_buckets + 1 > I32 ||
+      P.NV + 1 > I32 || P.NB * 16 > 0xFFFFFFFFL)
+    return fail(SHP_E_UNSUPPORTED, "an index exceeds 32 bits");
+  if (P.solver == SHP_SOLVER_COLORED_F32 && 16 * P.pair_cap > 0xFFFFFFFFL)
+    return fail(SHP_E_UNSUPPORTED, "coloured record offsets exceed 32 bits");
+  if (P.miss_cap > 0 && P.miss_cap > P.contact_cap)
+    return fail(SHP_E_UNSUPPORTED, "miss capacity above the contact capacity");
+  return SHP_OK;
+}
+
+static void fill_plan_out(const StepPlan& P, const std::vector<LaunchRec>& L, shp_plan* o,
+                          shp_launch* launches, int32_t cap) {
+  memset(o, 0, sizeof(*o));
+  o->bodies = P.NB;
+  o->vertices = P.NV;
+  o->worlds = P.W;
+  o->max_world_bodies = P.max_nb;
+  o->pair_capacity = P.pair_cap;
+  o->contact_capacity = P.contact_cap;
+  o->miss_capacity = P.miss_cap;
+  o->grid_buckets = P.grid_buckets;
+  o->device_bytes = P.device_bytes;
+  o->launches = (int32_t)L.size();
+  o->broadphase = P.use_grid ? SHP_BROADPHASE_GRID : SHP_BROADPHASE_SCAN;
+  o->solver_threads = P.solver == SHP_SOLVER_COLORED_F32 ? P.color_threads : P.solve_threads;
+  for (size_t k = 0; k < L.size(); ++k) {
+    o->max_workitems = std::max<int64_t>(o->max_workitems, L[k].grid * (int64_t)L[k].block);
+    o->max_lds_bytes = std::max<int64_t>(o->max_lds_bytes, (int64_t)(L[k].dyn_lds + L[k].static_lds));
+    if (launches && (int32_t)k < cap) {
+      shp_launch& x = launches[k];
+      memset(&x, 0, sizeof(x));
+      strncpy(x.name, L[k].name, sizeof(x.name) - 1);
+      x.workgroups = L[k].grid;
+      x.workgroup_size = L[k].block;
+      x.lds_bytes = (int32_t)(L[k].dyn_lds + L[k].static_lds);
+    }
+  }
+}
+
+extern "C" {
+
+const char* shp_last_error(void) { return g_err.c_str(); }
+const char* shp_version(void) { return "shapes-mi355x 0.2 (gfx950)"; }
+
+shp_status shp_batch_plan(const shp_world_desc* worlds, int32_t n_worlds, const shp_config* cfg,
+                          shp_plan* out, shp_launch* launches, int32_t launch_cap) {
+  if (!worlds || n_worlds <= 0 || !cfg || !out || launch_cap < 0)
+    return fail(SHP_E_INVAL, "bad argument");
+  HostBatch hb;
+  shp_status st = prepare_batch(worlds, n_worlds, *cfg, &hb);
+  if (hb.P.NB > 0) {  // validated: the plan is filled in even when refused
+    std::vector<LaunchRec> L;
+    plan_launches(hb.P, cfg->warm_start == SHP_WARM_LEARNED, false, &L);
+    fill_plan_out(hb.P, L, out, launches, launch_cap);
+  }
+  return st;
+}
+
+shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, const shp_config* cfg,
+                            shp_batch** out) {
+  if (!worlds || n_worlds <= 0 || !cfg || !out) return fail(SHP_E_INVAL, "null argument");
+  HostBatch hb;
+  TRY(prepare_batch(worlds, n_worlds, *cfg, &hb));
   shp_batch* b = new shp_batch();
   b->cfg = *cfg;
-  b->ppb = ppb;
+  b->plan = hb.P;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) {
     delete b;
@@ -890,7 +1379,7 @@ shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, cons
     shp_batch_destroy(b);
     return fail(SHP_E_HIP, "stream or event creation failed");
   }
-  shp_status st = batch_alloc(b, worlds, n_worlds, NB, NV, pair_cap);
+  shp_status st = batch_alloc(b, hb);
   if (st != SHP_OK) {
     std::string keep = g_err;
     shp_batch_destroy(b);
@@ -903,15 +1392,28 @@ shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, cons
 
 }  // extern "C"
 
-shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds, long NB, long NV,
-                       long pair_cap) {
+// The plan's static LDS of a kernel must cover what the compiler gave it.
+static shp_status check_static_lds(const void* fn, size_t planned, const char* name) {
+  hipFuncAttributes a{};
+  HIPCHK(hipFuncGetAttributes(&a, fn));
+  if (a.sharedSizeBytes > planned)
+    return fail(SHP_E_HIP, std::string(name) + " uses " + std::to_string(a.sharedSizeBytes) +
+                               " bytes of static LDS, the plan " + std::to_string(planned));
+  return SHP_OK;
+}
+
+shp_status batch_alloc(shp_batch* b, const HostBatch& hb) {
+  const StepPlan& P = b->plan;
   Ctx& c = b->ctx;
   memset(&c, 0, sizeof(c));
+  const long NB = P.NB, NV = P.NV;
+  const int n_worlds = (int)P.W;
   c.W = n_worlds;
   c.NB = (int)NB;
   c.NV = (int)NV;
-  c.pair_cap = (int)pair_cap;
-  c.contact_cap = (int)(2 * pair_cap);
+  c.pair_cap = (int)P.pair_cap;
+  c.contact_cap = (int)P.contact_cap;
+  c.miss_cap = (int)P.miss_cap;
   c.dt = b->cfg.dt;
   c.beta = b->cfg.baumgarte;
   c.slop = b->cfg.slop;
@@ -920,193 +1422,39 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
   c.iterations = b->cfg.iterations;
   c.solver = b->cfg.solver;
   c.warm_mode = b->cfg.warm_start;
+  c.use_grid = P.use_grid ? 1 : 0;
+  c.grid_buckets = (int)P.grid_buckets;
+  b->h_body_off = hb.body_off;
+  b->h_vert_off = hb.vert_off;
 
-  // host staging in SoA, -0 normalised to +0 (static velocities stay as loaded)
-  std::vector<double> h[10];
-  for (auto& v : h) v.resize(NB);
-  std::vector<int32_t> body_world(NB), vert_off(NB + 1), body_off(n_worlds + 1);
-  std::vector<uint8_t> is_static(NB);
-  std::vector<double> lx(NV), ly(NV);
-  long g = 0, vo = 0;
-  for (int w = 0; w < n_worlds; ++w) {
-    const shp_world_desc& d = worlds[w];
-    body_off[w] = (int32_t)g;
-    for (int i = 0; i < d.n_bodies; ++i, ++g) {
-      h[0][g] = nz(d.px[i]);
-      h[1][g] = nz(d.py[i]);
-      h[2][g] = nz(d.theta[i]);
-      h[3][g] = nz(d.vx[i]);
-      h[4][g] = nz(d.vy[i]);
-      h[5][g] = nz(d.omega[i]);
-      // toInvMass2 (Constraint.hs:79-83)
-      h[6][g] = d.mass[i] == 0.0 ? 0.0 : 1.0 / d.mass[i];
-      h[7][g] = d.inertia[i] == 0.0 ? 0.0 : 1.0 / d.inertia[i];
-      h[8][g] = d.mu[i];
-      h[9][g] = d.bounce[i];
-      is_static[g] = (h[6][g] == 0.0 && h[7][g] == 0.0) ? 1 : 0;
-      body_world[g] = w;
-      vert_off[g] = (int32_t)vo;
-      for (int k = d.hull_off[i]; k < d.hull_off[i + 1]; ++k, ++vo) {
-        lx[vo] = d.hull_x[k];
-        ly[vo] = d.hull_y[k];
-      }
-    }
-  }
-  body_off[n_worlds] = (int32_t)g;
-  vert_off[NB] = (int32_t)vo;
-  b->h_body_off = body_off;
-  b->h_vert_off = vert_off;
+  std::vector<Buf> bufs;
+  batch_buffers(P, c, &b->scan_part, &bufs);
+  for (const Buf& x : bufs) TRY(dalloc_bytes(b, x.p, x.bytes));
+  if (b->device_bytes != P.device_bytes)
+    return fail(SHP_E_HIP, "allocated " + std::to_string(b->device_bytes) + " bytes, planned " +
+                               std::to_string(P.device_bytes));
 
   double** dptr[10] = {&c.px, &c.py, &c.th, &c.vx, &c.vy, &c.w, &c.iml, &c.imr, &c.mu, &c.e};
-  for (int k = 0; k < 10; ++k) {
-    TRY(dalloc(b, dptr[k], NB));
-    HIPCHK(hipMemcpy(*dptr[k], h[k].data(), NB * sizeof(double), hipMemcpyHostToDevice));
+  for (int k = 0; k < 10; ++k)
+    HIPCHK(hipMemcpy(*dptr[k], hb.h[k].data(), NB * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.is_static, hb.is_static.data(), NB, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.body_world, hb.body_world.data(), NB * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.vert_off, hb.vert_off.data(), (NB + 1) * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.body_off, hb.body_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.world_order, hb.order.data(), n_worlds * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.lx, hb.lx.data(), NV * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.ly, hb.ly.data(), NV * 8, hipMemcpyHostToDevice));
+  if (P.use_grid) {
+    HIPCHK(hipMemcpy(c.cell_h, hb.cell.data(), n_worlds * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.is_large, hb.is_large.data(), NB, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.tab_off, hb.tab_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.large_off, hb.large_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
+    if (!hb.large_ids.empty())
+      HIPCHK(hipMemcpy(c.large_ids, hb.large_ids.data(), hb.large_ids.size() * 4,
+                       hipMemcpyHostToDevice));
   }
-  TRY(dalloc(b, &c.is_static, NB));
-  HIPCHK(hipMemcpy(c.is_static, is_static.data(), NB, hipMemcpyHostToDevice));
-  TRY(dalloc(b, &c.body_world, NB));
-  HIPCHK(hipMemcpy(c.body_world, body_world.data(), NB * 4, hipMemcpyHostToDevice));
-  TRY(dalloc(b, &c.vert_off, NB + 1));
-  HIPCHK(hipMemcpy(c.vert_off, vert_off.data(), (NB + 1) * 4, hipMemcpyHostToDevice));
-  TRY(dalloc(b, &c.body_off, n_worlds + 1));
-  HIPCHK(hipMemcpy(c.body_off, body_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
-  // per-world solver workgroups take the largest worlds first, so the
-  // longest solves start at once instead of trailing the batch
-  std::vector<int32_t> order(n_worlds);
-  for (int w = 0; w < n_worlds; ++w) order[w] = w;
-  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-    return body_off[x + 1] - body_off[x] > body_off[y + 1] - body_off[y];
-  });
-  TRY(dalloc(b, &c.world_order, n_worlds));
-  HIPCHK(hipMemcpy(c.world_order, order.data(), n_worlds * 4, hipMemcpyHostToDevice));
-  TRY(dalloc(b, &c.lx, NV));
-  TRY(dalloc(b, &c.ly, NV));
-  HIPCHK(hipMemcpy(c.lx, lx.data(), NV * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c.ly, ly.data(), NV * 8, hipMemcpyHostToDevice));
-  TRY(dalloc(b, &c.ext_min, NV));
-  TRY(dalloc(b, &c.ext_max, NV));
-  TRY(dalloc(b, &c.wx, NV));
-  TRY(dalloc(b, &c.wy, NV));
-  TRY(dalloc(b, &c.wnx, NV));
-  TRY(dalloc(b, &c.wny, NV));
-  TRY(dalloc(b, &c.bminx, NB));
-  TRY(dalloc(b, &c.bmaxx, NB));
-  TRY(dalloc(b, &c.bminy, NB));
-  TRY(dalloc(b, &c.bmaxy, NB));
-
-  long PC = pair_cap, CC = 2 * pair_cap;
-  TRY(dalloc(b, &c.cnt_rank, NB + 1));
-  TRY(dalloc(b, &c.off_rank, NB + 1));
-  TRY(dalloc(b, &c.pa, PC));
-  TRY(dalloc(b, &c.pb, PC));
-  TRY(dalloc(b, &c.ccount, PC + 1));
-  TRY(dalloc(b, &c.coff, PC + 1));
-  TRY(dalloc(b, &c.s_feat, 2 * PC));
-  TRY(dalloc(b, &c.s_flip, 2 * PC));
-  TRY(dalloc(b, &c.s_nx, 2 * PC));
-  TRY(dalloc(b, &c.s_ny, 2 * PC));
-  TRY(dalloc(b, &c.s_px, 2 * PC));
-  TRY(dalloc(b, &c.s_py, 2 * PC));
-  TRY(dalloc(b, &c.s_depth, 2 * PC));
-  TRY(dalloc(b, &c.wpair_off, n_worlds + 1));
-  TRY(dalloc(b, &c.wcont_off, n_worlds + 1));
-  TRY(dalloc(b, &c.key, CC));
-  TRY(dalloc(b, &c.ga, CC));
-  TRY(dalloc(b, &c.gb, CC));
-  TRY(dalloc(b, &c.flip, CC));
-  TRY(dalloc(b, &c.cnx, CC));
-  TRY(dalloc(b, &c.cny, CC));
-  TRY(dalloc(b, &c.cpx, CC));
-  TRY(dalloc(b, &c.cpy, CC));
-  TRY(dalloc(b, &c.depth, CC));
-  TRY(dalloc(b, &c.jn, 6 * CC));
-  TRY(dalloc(b, &c.jt, 6 * CC));
-  TRY(dalloc(b, &c.bn, CC));
-  TRY(dalloc(b, &c.rax, CC));
-  TRY(dalloc(b, &c.ray, CC));
-  TRY(dalloc(b, &c.rbx, CC));
-  TRY(dalloc(b, &c.rby, CC));
-  TRY(dalloc(b, &c.mcn, CC));
-  TRY(dalloc(b, &c.mct, CC));
-  TRY(dalloc(b, &c.ln, CC));
-  TRY(dalloc(b, &c.lt, CC));
-  TRY(dalloc(b, &c.warm, CC));
-  TRY(dalloc(b, &c.cache_key, CC));
-  TRY(dalloc(b, &c.cache_ln, CC));
-  TRY(dalloc(b, &c.cache_lt, CC));
-  TRY(dalloc(b, &c.cache_woff, n_worlds + 1));
-  c.miss_cap = (int)std::min<long>(2 * pair_cap, 4 * NB + 4096);
-  TRY(dalloc(b, &c.miss_idx, c.miss_cap));
-  TRY(dalloc(b, &c.miss_feat, (long)c.miss_cap * MLP_IN));
-  TRY(dalloc(b, &c.log_count, 1));
-  TRY(dalloc(b, &c.counters, 8));
-  TRY(dalloc(b, &c.quality, Q_ENTRIES));
   TRY(solver_alloc(b));
-
-  // grid broadphase set-up: cell size per world from the largest vertex
-  // radius of its dynamic bodies; static bodies larger than a cell are
-  // "large" and tested explicitly
-  {
-    int max_n = 0;
-    for (int w = 0; w < n_worlds; ++w) max_n = std::max(max_n, body_off[w + 1] - body_off[w]);
-    int mode = b->cfg.broadphase;
-    c.use_grid = mode == SHP_BROADPHASE_GRID || (mode == SHP_BROADPHASE_AUTO && max_n > 2048);
-    std::vector<double> rmax(NB, 0.0), cell(n_worlds, 1.0);
-    for (long gg = 0; gg < NB; ++gg)
-      for (int k = vert_off[gg]; k < vert_off[gg + 1]; ++k)
-        rmax[gg] = std::max(rmax[gg], std::sqrt(lx[k] * lx[k] + ly[k] * ly[k]));
-    std::vector<int32_t> tab_off(n_worlds + 1), large_off(n_worlds + 1), large_ids;
-    std::vector<uint8_t> large(NB, 0);
-    long tb_total = 0;
-    for (int w = 0; w < n_worlds; ++w) {
-      double hmax = 0.0;
-      for (int gg = body_off[w]; gg < body_off[w + 1]; ++gg)
-        if (!is_static[gg]) hmax = std::max(hmax, 2.0 * rmax[gg]);
-      cell[w] = hmax > 0.0 ? hmax * (1.0 + 1e-6) + 1e-9 : 1.0;
-      large_off[w] = (int32_t)large_ids.size();
-      for (int gg = body_off[w]; gg < body_off[w + 1]; ++gg)
-        if (is_static[gg] && 2.0 * rmax[gg] >= cell[w]) {
-          large[gg] = 1;
-          large_ids.push_back(gg);
-        }
-      int nw = body_off[w + 1] - body_off[w];
-      long t = 64;
-      while (t < nw) t <<= 1;
-      tab_off[w] = (int32_t)tb_total;
-      tb_total += t;
-    }
-    tab_off[n_worlds] = (int32_t)tb_total;
-    large_off[n_worlds] = (int32_t)large_ids.size();
-    c.grid_buckets = (int)tb_total;
-    TRY(dalloc(b, &c.cell_h, n_worlds));
-    HIPCHK(hipMemcpy(c.cell_h, cell.data(), n_worlds * 8, hipMemcpyHostToDevice));
-    TRY(dalloc(b, &c.is_large, NB));
-    HIPCHK(hipMemcpy(c.is_large, large.data(), NB, hipMemcpyHostToDevice));
-    TRY(dalloc(b, &c.body_cell, 2 * NB));
-    TRY(dalloc(b, &c.tab_off, n_worlds + 1));
-    HIPCHK(hipMemcpy(c.tab_off, tab_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
-    TRY(dalloc(b, &c.bucket_cnt, tb_total + 1));
-    TRY(dalloc(b, &c.bucket_start, tb_total + 1));
-    TRY(dalloc(b, &c.body_slot, NB));
-    TRY(dalloc(b, &c.grid_rec, 6L * NB));
-    TRY(dalloc(b, &c.grid_slots, (size_t)GRID_SLOTS * NB));
-    TRY(dalloc(b, &c.large_off, n_worlds + 1));
-    HIPCHK(hipMemcpy(c.large_off, large_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
-    TRY(dalloc(b, &c.large_ids, std::max<size_t>(large_ids.size(), 1)));
-    if (!large_ids.empty())
-      HIPCHK(hipMemcpy(c.large_ids, large_ids.data(), large_ids.size() * 4, hipMemcpyHostToDevice));
-  }
-
-  size_t t1 = 0, t2 = 0;
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, c.cnt_rank, c.off_rank, c.NB + 1, b->stream));
-  {
-    hipcub::CountingInputIterator<int> idx(0);
-    hipcub::TransformInputIterator<int, PairCount, hipcub::CountingInputIterator<int>> cnt(
-        idx, PairCount{c.ccount, c.off_rank + c.NB, c.pair_cap});
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, cnt, c.coff, c.pair_cap + 1, b->stream));
-  }
-  b->scan_tmp_bytes = std::max(std::max(t1, t2), grid_scan_bytes(b));
-  TRY(dalloc(b, (uint8_t**)&b->scan_tmp, b->scan_tmp_bytes));
+  TRY(check_static_lds((const void*)k_finish, FINISH_STATIC_LDS, "k_finish"));
 
   hipStream_t s = b->stream;
   k_fill_i32<<<1, 1, 0, s>>>(c.cnt_rank + NB, 0, 1);
@@ -1116,7 +1464,7 @@ shp_status batch_alloc(shp_batch* b, const shp_world_desc* worlds, int n_worlds,
   k_fill_i32<<<1, 64, 0, s>>>(c.counters, 0, 8);
   k_fill_i32<<<1, 64, 0, s>>>((int32_t*)c.quality, 0, 2 * Q_ENTRIES);
   k_fill_i32<<<1, 64, 0, s>>>((int32_t*)c.log_count, 0, 2);
-  // makeWorldObj places each shape with moveShape (World.hs:169-176)
+  // makeWorldObj places each shape with moveShape (World.hs:142-149)
   k_init_hulls<<<blocks(NB, 256), 256, 0, s>>>(c);
   k_integrate<<<blocks(NB, INTEGRATE_THREADS), INTEGRATE_THREADS, 0, s>>>(c, 0);
   HIPCHK(hipGetLastError());
@@ -1166,7 +1514,9 @@ shp_status shp_batch_destroy(shp_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->graph_made) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->events_made)
-    for (int k = 0; k <= SHP_NUM_PHASES; ++k) (void)hipEventDestroy(b->events[k]);
+    for (int k = 0; k <= SHP_MAIN_PHASES; ++k) (void)hipEventDestroy(b->events[k]);
+  if (b->events_made)
+    for (int k = 0; k < 2; ++k) (void)hipEventDestroy(b->side_events[k]);
   mlp_params_free(&b->mlp);
   for (void* p : b->allocs) (void)hipFree(p);
   if (b->side_stream) {
@@ -1198,7 +1548,8 @@ shp_status shp_batch_set_timing(shp_batch* b, int32_t enable) {
   if (!b) return fail(SHP_E_INVAL, "null batch");
   HIPCHK(hipStreamSynchronize(b->stream));
   if (enable && !b->events_made) {
-    for (int k = 0; k <= SHP_NUM_PHASES; ++k) HIPCHK(hipEventCreate(&b->events[k]));
+    for (int k = 0; k <= SHP_MAIN_PHASES; ++k) HIPCHK(hipEventCreate(&b->events[k]));
+    for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreate(&b->side_events[k]));
     b->events_made = true;
     TRY(dalloc(b, &b->ctx.tstamp_store, 8));
     unsigned long long zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1384,6 +1735,14 @@ shp_status shp_batch_set_logging(shp_batch* b, int64_t capacity) {
   if (!b || capacity < 0) return fail(SHP_E_INVAL, "bad argument");
   HIPCHK(hipStreamSynchronize(b->stream));
   Ctx& c = b->ctx;
+  if (capacity > 0 && c.miss_cap == 0) {  // a zero-start batch: the miss buffers now
+    StepPlan& P = b->plan;
+    P.miss_cap = std::min(2 * P.pair_cap, 4 * P.NB + 4096);
+    P.g_log = std::max<long>(1, blocks(P.miss_cap, 256));
+    TRY(dalloc(b, &c.miss_idx, P.miss_cap));
+    TRY(dalloc(b, &c.miss_feat, (size_t)P.miss_cap * MLP_IN));
+    c.miss_cap = (int)P.miss_cap;
+  }
   if (capacity > c.log_cap) {
     TRY(dalloc(b, &c.log_feat, capacity * MLP_IN));
     TRY(dalloc(b, &c.log_tgt, capacity * 2));
@@ -1435,13 +1794,18 @@ shp_status shp_batch_set_cache(shp_batch* b, int32_t world, int32_t count, const
   std::vector<uint64_t> nk;
   std::vector<double> nln, nlt;
   std::vector<int32_t> noff(W + 1);
+  const int nbw = b->h_body_off[world + 1] - b->h_body_off[world];
   for (int w = 0; w < W; ++w) {
     noff[w] = (int32_t)nk.size();
     if (w == world) {
       for (int q = 0; q < count; ++q) {
         const int32_t* kk = keys + 4 * q;
-        if (kk[0] < 0 || kk[1] < 0 || kk[2] < 0 || kk[3] < 0 || kk[2] > 255 || kk[3] > 255)
-          return fail(SHP_E_INVAL, "bad cache key");
+        // bodies of this world (below 2^24, the key's body fields), feature
+        // indices below 256
+        if (kk[0] < 0 || kk[1] < 0 || kk[0] >= nbw || kk[1] >= nbw || kk[2] < 0 || kk[3] < 0 ||
+            kk[2] > 255 || kk[3] > 255)
+          return fail(SHP_E_INVAL, "bad cache key (bodies must be below the world's body count, "
+                                   "features below 256)");
         uint64_t pk = ((uint64_t)kk[0] << 40) | ((uint64_t)kk[1] << 16) | ((uint64_t)kk[2] << 8) |
                       (uint64_t)kk[3];
         if (q > 0 && !(pk < nk.back())) return fail(SHP_E_INVAL, "cache keys must be strictly descending");

@@ -20,7 +20,6 @@
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include "shp_engine.h"
 
@@ -102,8 +101,6 @@ __global__ void k_grid_fill(Ctx c) {
   *(int4*)(p + 2) = make_int4(cx, cy, g, c.is_static[g] != 0);
 }
 
-static int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
-
 // LANES lanes per body i (1 or 16): count (fill = 0) or emit (fill = 1)
 // its pairs.  With one lane, the count pass also keeps the first
 // GRID_SLOTS partners, and the emit pass writes those without a second
@@ -123,7 +120,16 @@ __device__ __forceinline__ void grid_visit(const Ctx& c, int g, int sl, F&& hit)
   const int b0 = c.body_off[w];
   const bool si = c.is_static[g] != 0;
   const double aminx = c.bminx[g], amaxx = c.bmaxx[g], aminy = c.bminy[g], amaxy = c.bmaxy[g];
-  if (c.is_large[g]) {
+  const double h = c.cell_h[w];
+  const int cx0 = cell_of(aminx, h) - 1, cx1 = cell_of(amaxx, h);
+  const int cy0 = cell_of(aminy, h) - 1, cy1 = cell_of(amaxy, h);
+  const long ncx = (long)cx1 - cx0 + 1, ncy = (long)cy1 - cy0 + 1;
+  // A large body scans every j < i of its world.  So does a body whose AABB
+  // is not finite (a small body's finite AABB spans at most 3 x 3 candidate
+  // cells; a non-finite one would span up to 2^62 of them): the same
+  // bounded work as a large body, by all lanes.
+  const double span = (amaxx - aminx) + (amaxy - aminy);
+  if (c.is_large[g] || !(span - span == 0.0) || ncx > 3 || ncy > 3) {
     for (int gj = g - 1 - sl; gj >= b0; gj -= LANES)
       if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) hit(gj);
     return;
@@ -137,12 +143,6 @@ __device__ __forceinline__ void grid_visit(const Ctx& c, int g, int sl, F&& hit)
     }
     if (LANES != 1) return;
   }
-  const double h = c.cell_h[w];
-  const int cx0 = cell_of(aminx, h) - 1, cx1 = cell_of(amaxx, h);
-  const int cy0 = cell_of(aminy, h) - 1, cy1 = cell_of(amaxy, h);
-  // a small body's AABB spans at most 2 cells per axis, so there are <= 9
-  // cells; the stride keeps any other case correct
-  const long ncx = (long)cx1 - cx0 + 1, ncy = (long)cy1 - cy0 + 1;
   const int t0 = c.tab_off[w];
   const uint32_t mask = (uint32_t)(c.tab_off[w + 1] - t0 - 1);
   for (long ci = sl; ci < ncx * ncy; ci += CELL_LANES) {
@@ -178,15 +178,17 @@ __global__ void k_grid_pairs(Ctx c, int fill) {
     c.cnt_rank[rank] = count;
     return;
   }
+  // Past the capacity the step reports an overflow; the slots below it are
+  // still all written (a segment that straddles the capacity keeps its
+  // first pairs), so no later kernel of the step reads an unwritten slot.
   if (LANES == 1) {
     // one lane: the count pass sized the segment
     base = c.off_rank[rank];
     total = c.cnt_rank[rank];
-    if (base + total > c.pair_cap) {
-      atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_PAIR_OVERFLOW);
-      return;
-    }
-    if (total <= GRID_SLOTS) {
+    if ((long)base + total > c.pair_cap) atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_PAIR_OVERFLOW);
+    if (base >= c.pair_cap) return;
+    total = min(total, c.pair_cap - base);
+    if (total <= GRID_SLOTS && c.cnt_rank[rank] <= GRID_SLOTS) {
       // the kept partners, in descending order
       int v[GRID_SLOTS];
 #pragma unroll
@@ -209,9 +211,12 @@ __global__ void k_grid_pairs(Ctx c, int fill) {
       return;
     }
     pos = base;
+    const int end = base + total;
     grid_visit<LANES>(c, g, sl, [&](int gj) {
-      c.pa[pos] = g;
-      c.pb[pos] = gj;
+      if (pos < end) {
+        c.pa[pos] = g;
+        c.pb[pos] = gj;
+      }
       ++pos;
     });
   } else {
@@ -228,14 +233,17 @@ __global__ void k_grid_pairs(Ctx c, int fill) {
       return;
     }
     base = c.off_rank[rank];
-    if (base + total > c.pair_cap) {
-      if (sl == 0) atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_PAIR_OVERFLOW);
-      return;
-    }
+    if ((long)base + total > c.pair_cap && sl == 0)
+      atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_PAIR_OVERFLOW);
+    if (base >= c.pair_cap) return;  // the same for the group's lanes
+    total = min(total, c.pair_cap - base);
+    const int end = base + total;
     pos = base + incl - count;
     grid_visit<LANES>(c, g, sl, [&](int gj) {
-      c.pa[pos] = g;
-      c.pb[pos] = gj;
+      if (pos < end) {
+        c.pa[pos] = g;
+        c.pb[pos] = gj;
+      }
       ++pos;
     });
     // the group's stores precede lane 0's sort in this wavefront's program
@@ -256,15 +264,15 @@ __global__ void k_grid_pairs(Ctx c, int fill) {
 }
 
 // 16 lanes per body while that still fits about one wave of work per SIMD
-// slot of the GPU several times over; beyond, one lane per body.
-#define GRID_WIDE_MAX_BODIES 131072
-
+// slot of the GPU several times over (batches up to 131072 bodies); beyond,
+// one lane per body (the plan's grid_wide).
 static void launch_grid_pairs(shp_batch* b, int fill) {
   const Ctx& c = b->ctx;
-  if (c.NB <= GRID_WIDE_MAX_BODIES)
-    k_grid_pairs<16><<<nblk((long)c.NB * 16, 256), 256, 0, b->stream>>>(c, fill);
+  const StepPlan& P = b->plan;
+  if (P.grid_wide)
+    k_grid_pairs<16><<<P.g_pairs, 256, 0, b->stream>>>(c, fill);
   else
-    k_grid_pairs<1><<<nblk(c.NB, 256), 256, 0, b->stream>>>(c, fill);
+    k_grid_pairs<1><<<P.g_pairs, 256, 0, b->stream>>>(c, fill);
 }
 
 // Grid broadphase, first half: binning and per-body pair counts into
@@ -272,10 +280,10 @@ static void launch_grid_pairs(shp_batch* b, int fill) {
 hipError_t launch_grid_count(shp_batch* b) {
   Ctx& c = b->ctx;
   hipStream_t s = b->stream;
-  k_grid_assign<<<nblk(c.NB, 256), 256, 0, s>>>(c);
+  k_grid_assign<<<b->plan.g_bodies, 256, 0, s>>>(c);
   hipError_t e = scan_grid_buckets(b);
   if (e != hipSuccess) return e;
-  k_grid_fill<<<nblk(c.NB, 256), 256, 0, s>>>(c);
+  k_grid_fill<<<b->plan.g_bodies, 256, 0, s>>>(c);
   launch_grid_pairs(b, 0);
   return hipGetLastError();
 }
@@ -284,11 +292,4 @@ hipError_t launch_grid_count(shp_batch* b) {
 hipError_t launch_grid_emit(shp_batch* b) {
   launch_grid_pairs(b, 1);
   return hipGetLastError();
-}
-
-size_t grid_scan_bytes(shp_batch* b) {
-  size_t t = 0;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, t, b->ctx.bucket_cnt, b->ctx.bucket_start,
-                                         b->ctx.grid_buckets + 1, b->stream);
-  return t;
 }

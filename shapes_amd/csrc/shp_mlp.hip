@@ -490,13 +490,16 @@ void mlp_launch_infer(shp_batch* b) {
   m.b2 = b->ctx.mlp_w + MLP_OFF_B2;
   m.b3 = b->ctx.mlp_w + MLP_OFF_B3;
   m.b4 = b->ctx.mlp_w + MLP_OFF_B4;
-  int tiles = (b->ctx.miss_cap + TILE - 1) / TILE;
-  int grid = tiles < 512 ? tiles : 512;
-  k_mlp_infer<<<grid, 256, 0, b->stream>>>(b->ctx, m);
+  k_mlp_infer<<<b->plan.g_infer, 256, 0, b->stream>>>(b->ctx, m);
+}
+
+// sx, s1, s2 (BF16) and sy (FP32) of k_mlp_infer
+size_t mlp_infer_static_lds() {
+  return lds16(TILE * 16 * 2) + 2 * lds16(TILE * 128 * 2) + lds16(TILE * 16 * 4);
 }
 
 void mlp_launch_log(shp_batch* b) {
-  k_mlp_log<<<(b->ctx.miss_cap + 255) / 256, 256, 0, b->stream>>>(b->ctx);
+  k_mlp_log<<<b->plan.g_log, 256, 0, b->stream>>>(b->ctx);
 }
 
 // ---- training handle -------------------------------------------------------

@@ -39,8 +39,8 @@ __device__ __forceinline__ void apply_lag(double l, const double* j, const doubl
 __device__ __forceinline__ void load_rows(const Ctx& c, int k, double* jn, double* jt) {
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
-    jn[q] = c.jn[q * c.contact_cap + k];
-    jt[q] = c.jt[q * c.contact_cap + k];
+    jn[q] = c.jn[(size_t)q * c.contact_cap + k];
+    jt[q] = c.jt[(size_t)q * c.contact_cap + k];
   }
 }
 

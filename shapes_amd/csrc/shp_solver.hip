@@ -64,8 +64,6 @@
 using namespace shp;
 
 #define WAIT_LIMIT (1u << 20)
-#define LEVEL_CAP 4095
-#define LDS_MAX_BYTES (160 * 1024 - 512)
 
 // compact record fields (doubles), stride contact_cap
 enum {
@@ -659,72 +657,69 @@ static const void* flow_kernel(int threads) {
                         : (const void*)k_solve_flow<CLDS, VLDS, 256>;
 }
 
-shp_status solver_alloc(shp_batch* b) {
-  Ctx& c = b->ctx;
-  const long CC = c.contact_cap;
-  void** bufs[] = {(void**)&c.sched_level, (void**)&c.sched_order, (void**)&c.pk,
-                   (void**)&c.pk_g,        (void**)&c.pk_f,        (void**)&c.fvx,
-                   (void**)&c.fvy,         (void**)&c.fw,          (void**)&c.flow_list,
-                   (void**)&c.flow_rk,     (void**)&c.flow_off,    (void**)&c.flow_cur,
-                   (void**)&c.flow_lvl,    (void**)&c.flow_order};
-  // pk also holds the coloured solver's records (28 FP32 words + 2 FP64
-  // lambdas per slot, less than 29 doubles)
-  // levels and the level order of the contacts (ordered solver only)
-  const long items = c.solver == SHP_SOLVER_ORDERED_F64 ? CC : 0;
-  size_t sizes[] = {CC * 4UL,        CC * 4UL,        29UL * CC * 8,   2UL * CC * 4,
-                    (size_t)CC,      (size_t)c.NB * 4, (size_t)c.NB * 4, (size_t)c.NB * 4,
-                    2UL * CC * 4,    CC * 16UL,       (size_t)c.NB * 4, (size_t)c.NB * 4,
-                    4UL * items,     4UL * items};
-  static_assert(FL_FIELDS <= 29, "compact record exceeds the packed buffer");
-  for (int k = 0; k < 14; ++k) {
-    if (hipMalloc(bufs[k], sizes[k] ? sizes[k] : 16) != hipSuccess) return SHP_E_OOM;
-    b->allocs.push_back(*bufs[k]);
-  }
-  int max_nb = 0;
-  for (int w = 0; w < c.W; ++w) max_nb = std::max(max_nb, b->h_body_off[w + 1] - b->h_body_off[w]);
-  // LDS per world: level bins, then velocities (24 B/body) and offsets +
-  // cursors (8 B/body) when they fit; offsets + cursors alone; else both in
-  // global memory
+static_assert(FL_FIELDS <= FLOW_RECORD_DOUBLES, "compact record exceeds the packed buffer");
+
+// LDS per world: level bins, then velocities (24 B/body) and offsets +
+// cursors (8 B/body) when they fit; offsets + cursors alone; else both in
+// global memory.  4 waves per world; a large world gets 8 (a
+// level-synchronous pass keeps two records per lane, so 8 waves leave it
+// 256 registers per lane).
+void plan_ordered_solver(StepPlan* P, int solver_threads) {
   const size_t bins = 4UL * (LEVEL_CAP + 1);
-  size_t both = bins + 32UL * max_nb, cnt_only = bins + 8UL * max_nb;
+  const size_t both = bins + 32UL * P->max_nb, cnt_only = bins + 8UL * P->max_nb;
   if (both <= LDS_MAX_BYTES) {
-    b->solve_clds = true;
-    b->solve_vlds = true;
-    b->solve_lds = both;
+    P->solve_clds = true;
+    P->solve_vlds = true;
+    P->solve_lds = both;
   } else if (cnt_only <= LDS_MAX_BYTES) {
-    b->solve_clds = true;
-    b->solve_vlds = false;
-    b->solve_lds = cnt_only;
+    P->solve_clds = true;
+    P->solve_vlds = false;
+    P->solve_lds = cnt_only;
   } else {
-    b->solve_clds = false;
-    b->solve_vlds = false;
-    b->solve_lds = bins;
+    P->solve_clds = false;
+    P->solve_vlds = false;
+    P->solve_lds = bins;
   }
-  // 4 waves per world; a large world gets 8 (a level-synchronous pass
-  // keeps two records per lane, so 8 waves leave it 256 registers per lane)
-  b->solve_threads = b->cfg.solver_threads ? b->cfg.solver_threads : (max_nb > 2048 ? 512 : 256);
-  const void* fn = b->solve_clds ? (b->solve_vlds ? flow_kernel<true, true>(b->solve_threads)
-                                                  : flow_kernel<true, false>(b->solve_threads))
-                                 : flow_kernel<false, false>(b->solve_threads);
+  P->solve_threads = solver_threads ? solver_threads : (P->max_nb > 2048 ? 512 : 256);
+}
+
+// wsum[T / 64], progress, s_maxl
+size_t ordered_static_lds(int threads) { return lds16((size_t)(threads / 64) * 4) + 2 * lds16(4); }
+
+shp_status solver_alloc(shp_batch* b) {
+  const StepPlan& P = b->plan;
+  if (P.solver != SHP_SOLVER_ORDERED_F64) return colored_alloc(b);
+  const void* fn = P.solve_clds ? (P.solve_vlds ? flow_kernel<true, true>(P.solve_threads)
+                                                : flow_kernel<true, false>(P.solve_threads))
+                                : flow_kernel<false, false>(P.solve_threads);
   if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)std::max<size_t>(b->solve_lds, 16)) != hipSuccess)
+                          (int)std::max<size_t>(P.solve_lds, 16)) != hipSuccess) {
+    shp_set_error("hipFuncSetAttribute (ordered solver LDS) failed");
     return SHP_E_HIP;
-  return colored_alloc(b);
+  }
+  hipFuncAttributes a{};
+  if (hipFuncGetAttributes(&a, fn) != hipSuccess ||
+      a.sharedSizeBytes > ordered_static_lds(P.solve_threads)) {
+    shp_set_error("ordered solver: static LDS above the plan's");
+    return SHP_E_HIP;
+  }
+  return SHP_OK;
 }
 
 template <int T>
 static void launch_flow(shp_batch* b) {
-  const size_t lds = std::max<size_t>(b->solve_lds, 16);
-  if (b->solve_clds && b->solve_vlds)
+  const StepPlan& P = b->plan;
+  const size_t lds = std::max<size_t>(P.solve_lds, 16);
+  if (P.solve_clds && P.solve_vlds)
     k_solve_flow<true, true, T><<<b->ctx.W, T, lds, b->stream>>>(b->ctx);
-  else if (b->solve_clds)
+  else if (P.solve_clds)
     k_solve_flow<true, false, T><<<b->ctx.W, T, lds, b->stream>>>(b->ctx);
   else
     k_solve_flow<false, false, T><<<b->ctx.W, T, lds, b->stream>>>(b->ctx);
 }
 
 void launch_ordered_solver(shp_batch* b) {
-  if (b->solve_threads == 512)
+  if (b->plan.solve_threads == 512)
     launch_flow<512>(b);
   else
     launch_flow<256>(b);

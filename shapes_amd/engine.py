@@ -72,6 +72,19 @@ class StepStatsC(C.Structure):
                 ("mean_normal_residual", C.c_double), ("mean_friction_residual", C.c_double)]
 
 
+class PlanC(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "bodies", "vertices", "worlds", "max_world_bodies", "pair_capacity",
+        "contact_capacity", "miss_capacity", "grid_buckets", "device_bytes",
+        "max_workitems", "max_lds_bytes")] + [
+        (n, C.c_int32) for n in ("launches", "broadphase", "solver_threads", "reserved")]
+
+
+class LaunchC(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("workgroups", C.c_int64),
+                ("workgroup_size", C.c_int32), ("lds_bytes", C.c_int32)]
+
+
 EXPORTS = [
     "shp_batch_create", "shp_batch_destroy", "shp_batch_step",
     "shp_batch_step_async", "shp_batch_sync", "shp_batch_num_bodies",
@@ -83,11 +96,23 @@ EXPORTS = [
     "shp_batch_set_logging", "shp_batch_read_log", "shp_mlp_create",
     "shp_mlp_destroy", "shp_mlp_get_params", "shp_mlp_forward",
     "shp_mlp_gradient", "shp_mlp_set_data", "shp_mlp_train", "shp_mlp_set_loss",
+    "shp_batch_plan",
 ]
 
 PHASES = ["reset", "broadphase_count", "scan_pairs", "broadphase_fill", "sat",
-          "scan_contacts", "world_ranges", "contacts", "solve", "quality",
-          "cache_store", "integrate"]
+          "scan_contacts", "world_ranges", "contacts", "infer", "solve_setup", "sweeps",
+          "solve_writeback", "finish", "cache_store", "colouring"]
+# the kernel(s) of each phase (colouring: on the second stream)
+PHASE_KERNELS = {
+    "broadphase_count": ["k_grid_assign", "scan", "k_grid_fill", "k_grid_pairs", "k_broadphase"],
+    "scan_pairs": ["k_scan_small", "k_scan_up", "k_scan_mid", "k_scan_down"],
+    "broadphase_fill": ["k_grid_pairs", "k_broadphase"],
+    "sat": ["k_sat"], "scan_contacts": ["k_scan_small", "k_scan_up", "k_scan_mid", "k_scan_down"],
+    "world_ranges": ["k_world_ranges"], "contacts": ["k_contacts"], "infer": ["k_mlp_infer"],
+    "solve_setup": ["k_pack_manifolds"], "sweeps": ["k_sweep_colours", "k_solve_flow"],
+    "solve_writeback": ["k_colour_unpack"], "finish": ["k_finish"],
+    "cache_store": ["k_cache_store"], "colouring": ["k_colour"],
+}
 
 _lib = None
 
@@ -131,6 +156,8 @@ def load_library(path: str = LIB_PATH):
     L.shp_batch_set_timing.argtypes = [C.c_void_p, C.c_int32]
     L.shp_batch_read_timing.argtypes = [C.c_void_p, _dp, _ip]
     L.shp_batch_read_solve_profile.argtypes = [C.c_void_p, _dp]
+    L.shp_batch_plan.argtypes = [C.POINTER(WorldDescC), C.c_int32, C.POINTER(ConfigC),
+                                 C.POINTER(PlanC), C.POINTER(LaunchC), C.c_int32]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L
@@ -147,6 +174,56 @@ def _dptr(a):
     return a.ctypes.data_as(_dp)
 
 
+def _world_descs(worlds: Sequence[WorldDesc]):
+    """C descriptors of the worlds and the arrays they point into."""
+    keep = []
+    descs = (WorldDescC * len(worlds))()
+    for k, w in enumerate(worlds):
+        d = descs[k]
+        d.n_bodies = w.n
+        for name in ("px", "py", "theta", "vx", "vy", "omega", "mass",
+                     "inertia", "mu", "bounce"):
+            a = np.ascontiguousarray(getattr(w, name), dtype=np.float64)
+            keep.append(a)
+            setattr(d, name, _dptr(a))
+        ho = np.ascontiguousarray(w.hull_off, dtype=np.int32)
+        hx = np.ascontiguousarray(w.hull_x, dtype=np.float64)
+        hy = np.ascontiguousarray(w.hull_y, dtype=np.float64)
+        keep += [ho, hx, hy]
+        d.hull_off = ho.ctypes.data_as(_ip)
+        d.hull_x = _dptr(hx)
+        d.hull_y = _dptr(hy)
+    return descs, keep
+
+
+def _config(config: SceneConfig, solver: int, warm_start: int, max_pairs_per_body: int,
+            broadphase: int, solver_threads: int) -> ConfigC:
+    return ConfigC(config.dt, config.baumgarte, config.slop, config.gx,
+                   config.gy, config.iterations, solver, warm_start,
+                   max_pairs_per_body, broadphase, solver_threads)
+
+
+def plan(worlds: Sequence[WorldDesc], config: SceneConfig, solver: int = SOLVER_ORDERED_F64,
+         warm_start: int = WARM_ZERO, max_pairs_per_body: int = 0,
+         broadphase: int = BROADPHASE_AUTO, solver_threads: int = 0):
+    """shp_batch_plan: (status, message, plan dict, launch list) of what
+    make_batch would allocate and launch, computed on the host (no GPU)."""
+    L = load_library()
+    descs, keep = _world_descs(worlds)
+    cfg = _config(config, solver, warm_start, max_pairs_per_body, broadphase, solver_threads)
+    out = PlanC()
+    cap = 64
+    launches = (LaunchC * cap)()
+    st = L.shp_batch_plan(descs, len(worlds), C.byref(cfg), C.byref(out), launches, cap)
+    del keep
+    msg = L.shp_last_error().decode() if st else ""
+    p = {f: getattr(out, f) for f, _ in PlanC._fields_}
+    rows = [{"name": launches[k].name.decode(), "workgroups": launches[k].workgroups,
+             "workgroup_size": launches[k].workgroup_size, "lds_bytes": launches[k].lds_bytes}
+            for k in range(min(out.launches, cap))]
+    return st, msg, p, rows
+
+
 class Batch:
     """Independent worlds stepped together on one GPU (n = 1 is the
     reference engine).  Each world keeps its own warm-start cache."""
@@ -157,30 +234,13 @@ class Batch:
                  solver_threads: int = 0):
         L = load_library()
         self._L = L
-        self._keep = []
-        descs = (WorldDescC * len(worlds))()
-        for k, w in enumerate(worlds):
-            d = descs[k]
-            d.n_bodies = w.n
-            for name in ("px", "py", "theta", "vx", "vy", "omega", "mass",
-                         "inertia", "mu", "bounce"):
-                a = np.ascontiguousarray(getattr(w, name), dtype=np.float64)
-                self._keep.append(a)
-                setattr(d, name, _dptr(a))
-            ho = np.ascontiguousarray(w.hull_off, dtype=np.int32)
-            hx = np.ascontiguousarray(w.hull_x, dtype=np.float64)
-            hy = np.ascontiguousarray(w.hull_y, dtype=np.float64)
-            self._keep += [ho, hx, hy]
-            d.hull_off = ho.ctypes.data_as(_ip)
-            d.hull_x = _dptr(hx)
-            d.hull_y = _dptr(hy)
-        cfg = ConfigC(config.dt, config.baumgarte, config.slop, config.gx,
-                      config.gy, config.iterations, solver, warm_start,
-                      max_pairs_per_body, broadphase, solver_threads)
+        descs, keep = _world_descs(worlds)
+        cfg = _config(config, solver, warm_start, max_pairs_per_body, broadphase,
+                      solver_threads)
         h = C.c_void_p()
         _check(L.shp_batch_create(descs, len(worlds), C.byref(cfg), C.byref(h)))
+        del keep
         self._h = h
-        self._keep = []
         self.n_worlds = len(worlds)
         self.sizes = [w.n for w in worlds]
         self._vert_counts = [int(w.hull_off[-1]) for w in worlds]

@@ -77,6 +77,98 @@ def test_create_rejects_bad_arguments():
     # the coloured solver addresses a world's records with 32-bit offsets:
     # pair capacities from 2^28 are refused before any device work
     ppb = (1 << 28) // w.n + 1
+    assert ppb * w.n + 1024 < (1 << 29)
     big = engine.ConfigC(1 / 60, 0.01, 0.02, 0, -9.8, 4, engine.SOLVER_COLORED_F32, 0, ppb, 0)
     assert L.shp_batch_create(C.byref(d), 1, C.byref(big), C.byref(h)) == 4
     assert b"coloured" in L.shp_last_error()
+
+
+# ---- the launch plan and its audit (host only) ------------------------------
+
+LDS_PER_CU = 160 * 1024
+
+
+def _check_launches(rows):
+    for r in rows:
+        assert 1 <= r["workgroups"] <= 2**31 - 1, r
+        assert r["workgroups"] * r["workgroup_size"] <= 2**32 - 1, r
+        assert 64 <= r["workgroup_size"] <= 1024, r
+        assert r["lds_bytes"] <= LDS_PER_CU, r
+
+
+@pytest.fixture(scope="module")
+def pile_batch():
+    # 1024 piles of 2k-32k random polygons, 11.3 M bodies (about 20 s to build)
+    return scenes.pile_batch()
+
+
+@pytest.mark.parametrize("ppb", [4, 8, 16])
+def test_plan_combined_at_scale(pile_batch, ppb):
+    """Every launch of a step of the 1024-pile batch, coloured solver, stays
+    within the hardware's limits, every 32-bit index in range, and the
+    footprint fits one MI355X's 288 GB."""
+    sc = pile_batch
+    st, msg, p, rows = engine.plan(sc.worlds, sc.config, solver=engine.SOLVER_COLORED_F32,
+                                   max_pairs_per_body=ppb)
+    assert st == 0, msg
+    assert p["bodies"] == sc.n_bodies == 11344603
+    assert p["pair_capacity"] == ppb * p["bodies"] + 1024
+    assert p["contact_capacity"] == 2 * p["pair_capacity"]
+    assert p["broadphase"] == engine.BROADPHASE_GRID
+    assert p["launches"] == len(rows) <= 64
+    _check_launches(rows)
+    names = [r["name"] for r in rows]
+    assert names.count("k_colour") == 1 and names.count("k_sweep_colours") == 1
+    assert "k_mlp_infer" not in names
+    print(f"ppb {ppb}: {p['device_bytes'] / 1e9:.1f} GB, {len(rows)} launches, largest "
+          f"{p['max_workitems']} work-items, {p['max_lds_bytes']} B LDS")
+    assert p["device_bytes"] < 288e9
+    # the 16-byte record offsets of the coloured sweeps fit 32 bits
+    assert 16 * p["pair_capacity"] < 2**32
+
+
+def test_plan_learned_refused_beyond_lds_worlds(pile_batch):
+    """The learned warm start with the coloured solver is refused for worlds
+    whose FP32 velocities live in global memory; its launches are audited
+    all the same, and the batch's smaller worlds alone are accepted."""
+    sc = pile_batch
+    st, msg, p, rows = engine.plan(sc.worlds, sc.config, solver=engine.SOLVER_COLORED_F32,
+                                   warm_start=engine.WARM_LEARNED, max_pairs_per_body=4)
+    assert st == 4 and "learned warm start" in msg
+    _check_launches(rows)
+    assert "k_mlp_infer" in [r["name"] for r in rows]
+    assert p["miss_capacity"] == min(2 * p["pair_capacity"], 4 * p["bodies"] + 4096)
+    small = [w for w in sc.worlds if w.n <= 13437][:64]
+    st, msg, p, rows = engine.plan(small, sc.config, solver=engine.SOLVER_COLORED_F32,
+                                   warm_start=engine.WARM_LEARNED)
+    assert st == 0, msg
+
+
+def test_plan_refuses_capacities_beyond_32_bit_offsets():
+    w = scenes.pyramid(rows=2).worlds[0]
+    cfg = scenes.SceneConfig()
+    ppb = (1 << 29) // w.n + 1
+    st, msg, _, _ = engine.plan([w], cfg, max_pairs_per_body=ppb)
+    assert st == 4 and "2^29" in msg
+    ppb = (1 << 28) // w.n + 1
+    st, msg, _, _ = engine.plan([w], cfg, solver=engine.SOLVER_COLORED_F32, max_pairs_per_body=ppb)
+    assert st == 4 and "coloured" in msg
+    st, msg, p, rows = engine.plan([w], cfg, max_pairs_per_body=ppb)
+    assert st == 0, msg
+    _check_launches(rows)
+
+
+def test_plan_scans_cover_their_counts():
+    """One workgroup scans up to 20480 counts; beyond, three launches whose
+    first and last cover every count in 8192-count workgroups."""
+    for n_worlds, expect_parts in ((4, 0), (64, None)):
+        sc = scenes.world_batch_fast(n_worlds=n_worlds)
+        st, msg, p, rows = engine.plan(sc.worlds, sc.config)
+        assert st == 0, msg
+        scans = [r for r in rows if r["name"] == "scan_pairs"]
+        n = p["pair_capacity"] + 1
+        if n <= 20480:
+            assert len(scans) == 1 and scans[0]["workgroups"] == 1
+        else:
+            assert len(scans) == 3
+            assert scans[0]["workgroups"] == scans[2]["workgroups"] == -(-n // 8192)

@@ -1,0 +1,92 @@
+"""Drop-in (ordered FP64) mode on the GPU: the hand-derived known answers of
+the contact update, restitution, the longer pile run, overflow handling and
+cache validation, all through the C ABI."""
+import numpy as np
+import pytest
+
+import kat_scenes as K
+from shapes_amd import engine, scenes
+from test_gpu_parity import assert_bitwise, compare_state, run_parity
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", sorted(K.CASES))
+def test_contact_update_known_answers_gpu(orc, name):
+    """The GPU's drop-in step gives the hand-derived answers (within 1e-14)
+    and the oracle's bits."""
+    case = K.CASES[name]
+    w = K.world(**case["world"])
+    cfg = K.config(case["iterations"])
+    gpu = engine.make_batch([w], cfg)
+    ref = orc.OracleEngine(w, cfg)
+    if case["cache"] is not None:
+        gpu.set_cache(0, np.array(K.KEYS, dtype=np.int32), np.array(case["cache"]))
+        ref.set_cache(np.array(K.KEYS, dtype=np.int32), np.array(case["cache"]))
+    gpu.step(1)
+    ref.step(1)
+    c = gpu.contacts()
+    assert c["keys"].tolist() == K.KEYS
+    want_lam = np.array([[float(x) for x in r] for r in case["lam"]])
+    np.testing.assert_allclose(c["lam"], want_lam, rtol=1e-14, atol=1e-15)
+    want_v = np.array([float(x) for x in case["v_b"]])
+    np.testing.assert_allclose(gpu.bodies()[0, 3:6], want_v, rtol=1e-14, atol=1e-15)
+    compare_state(gpu, ref, 1)
+    gpu.close()
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_restitution_bitwise(orc, seed):
+    """Bodies with bounce in (0, 1] (bounceB with e > 0 on every contact):
+    pairs, keys, lagrangians and state bit for bit over 120 steps."""
+    w = scenes.random_world(seed, count=60, spacing=1.25)
+    rng = np.random.default_rng(seed)
+    w.bounce = rng.uniform(0.2, 1.0, size=w.n)
+    w.vy = w.vy - 3.0 * (w.mass != 0)  # falling onto the floor and each other
+    cfg = scenes.SceneConfig(iterations=4)
+    ref = run_parity(orc, w, cfg, steps=120, every=4)
+    assert ref.stats()["contacts"] > 40
+
+
+def test_polygon_pile_bitwise_full_run(orc):
+    """The 8192-polygon world over all 600 steps of its configuration,
+    compared every 50 steps (pairs, keys, geometry, lagrangians, state)."""
+    sc = scenes.polygon_pile()
+    run_parity(orc, sc.worlds[0], sc.config, steps=600, every=50)
+
+
+def test_pair_overflow_reported_and_state_kept():
+    """With too little pair capacity (one pair per body) the step reports
+    SHP_E_OVERFLOW; graph-replayed steps after the overflow stay within the
+    capacity (every pair slot below it holds a pair of the step) and the
+    batch can still be read back."""
+    rng = np.random.Generator(np.random.Philox(5))
+    w = scenes.grid_world(rng, 3000, cols=60, spacing=0.55, y0=1.0, square_frac=0.5, vel=0.0)
+    cfg = scenes.SceneConfig(iterations=2)
+    for bp in (engine.BROADPHASE_SCAN, engine.BROADPHASE_GRID):
+        b = engine.make_batch([w], cfg, max_pairs_per_body=1, broadphase=bp)
+        b.step_async(5, use_graph=True)
+        with pytest.raises(engine.ShapesError) as ei:
+            b.sync()
+        assert ei.value.status == 5
+        s = b.stats()
+        assert s["flags"] & 1
+        assert s["pairs"] == w.n * 1 + 1024
+        p = b.pairs(0)
+        assert len(p) == w.n + 1024
+        assert (p[:, 0] > p[:, 1]).all() and (p[:, 1] >= 0).all() and (p[:, 0] < w.n).all()
+        b.bodies(0)
+        b.close()
+
+
+def test_set_cache_rejects_out_of_range_keys():
+    w = scenes.pyramid(rows=3).worlds[0]
+    b = engine.make_batch([w], scenes.SceneConfig())
+    lam = np.zeros((1, 2))
+    for bad in ([[w.n, 0, 0, 0]], [[1, w.n, 0, 0]], [[1, 0, 256, 0]], [[1, -1, 0, 0]]):
+        with pytest.raises(engine.ShapesError) as ei:
+            b.set_cache(0, np.array(bad, dtype=np.int32), lam)
+        assert ei.value.status == 1
+    b.set_cache(0, np.array([[w.n - 1, 0, 0, 0]], dtype=np.int32), lam)
+    b.step(1)
+    b.close()

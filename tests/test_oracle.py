@@ -338,3 +338,65 @@ def test_coloured_restatement_tracks_fp64_pyramid(orc):
     assert np.array_equal(b.bodies().view(np.uint64), c.bodies().view(np.uint64))
     lam = b.contacts()["lam"]
     assert (lam[:, 0] >= 0).all()
+
+
+# ------------------------------------------------- contact update known answers
+
+@pytest.mark.parametrize("name", ["warm", "bounce", "bounce_clamped", "bounce_two_sweeps"])
+def test_contact_update_known_answers(orc, name):
+    """Hand-derived answers of one step of the contact-benchmark boxes
+    (tests/kat_scenes.py): cached lagrangians applied before the sweeps
+    (useCache, Solvers/Contact.hs:100-114); both rows computed from the same
+    pre-update velocities and applied normal first (improveContactSln,
+    :124-143, applySln :55-65); bounceB with e > 0 (Restitution.hs:34-47);
+    the friction bound from the accumulated normal impulse
+    (Constraints/Contact.hs:101-110).  Friction from the post-normal
+    velocity, a bound from the cached or the new normal impulse, an
+    unapplied cache, or a bounce term with swapped radii each change these
+    numbers."""
+    import kat_scenes as K
+
+    case = K.CASES[name]
+    w = K.world(**case["world"])
+    e = orc.OracleEngine(w, K.config(case["iterations"]))
+    if case["cache"] is not None:
+        e.set_cache(np.array(K.KEYS, dtype=np.int32), np.array(case["cache"]))
+    e.step(1)
+    c = e.contacts()
+    assert c["keys"].tolist() == K.KEYS
+    assert c["flip"].tolist() == [1, 1]
+    assert c["warm"].tolist() == ([1, 1] if case["cache"] is not None else [0, 0])
+    want_lam = np.array([[float(x) for x in r] for r in case["lam"]])
+    np.testing.assert_allclose(c["lam"], want_lam, rtol=1e-14, atol=1e-15)
+    body = e.bodies()
+    want_v = np.array([float(x) for x in case["v_b"]])
+    np.testing.assert_allclose(body[0, 3:6], want_v, rtol=1e-14, atol=1e-15)
+    # the static box keeps its (zero) velocity; B advanced by dt * v
+    assert body[1, 3:6].tolist() == [0.0, 0.0, 0.0]
+    np.testing.assert_allclose(body[0, :2], [1.0 + want_v[0] / 64.0, 3.0 + want_v[1] / 64.0],
+                               rtol=1e-15)
+
+
+@pytest.mark.parametrize("scene", ["random", "pile", "stacks"])
+def test_reference_grid_broadphase_equals_brute_force(orc, scene):
+    """The oracle's restatement of the reference's grid (Grid.hs:67-141,
+    axes GridAxis 20 1 (-10), whose flattened cells alias across rows)
+    gives exactly the brute-force culledKeys list (Aabb.hs:168-183), step
+    after step, on scattered, piled and stacked worlds."""
+    if scene == "random":
+        w, cfg, steps = scenes.random_world(9, count=80, spacing=1.2), scenes.SceneConfig(), 40
+    elif scene == "pile":
+        sc = scenes.polygon_pile(count=1500, cols=40)
+        w, cfg, steps = sc.worlds[0], sc.config, 30
+    else:
+        sc = scenes.reference_stacks()
+        w, cfg, steps = sc.worlds[0], sc.config, 10
+    a = orc.OracleEngine(w, cfg, broadphase=orc.BROADPHASE_GRID)
+    b = orc.OracleEngine(w, cfg, broadphase=orc.BROADPHASE_BRUTE)
+    for _ in range(steps):
+        a.step(1)
+        b.step(1)
+        pa, pb = a.pairs(), b.pairs()
+        assert pa.shape == pb.shape and (pa == pb).all()
+    assert len(pa) > 0
+    assert (a.bodies().view(np.uint64) == b.bodies().view(np.uint64)).all()
