@@ -96,6 +96,9 @@ def matches(q, ref, tol=QUALITY_TOL):
             q["mean_step_normal_residual"] <= ref["mean_step_normal_residual"] * (1 + tol))
 
 
+GRAPH = True  # steps replayed from a captured HIP graph (--no-graph: eager launches)
+
+
 def run(scene, solver, warm, iterations, preroll, steps, sched, ppb, model=None,
         positions=False, instrument=0):
     """One batch from the initial state: `preroll` untimed graph-replayed
@@ -111,10 +114,10 @@ def run(scene, solver, warm, iterations, preroll, steps, sched, ppb, model=None,
         if warm == engine.WARM_LEARNED:
             b.set_mlp(model)
         if preroll:
-            b.step_async(preroll, use_graph=True)
+            b.step_async(preroll, use_graph=GRAPH)
         b.sync()
         t0 = time.perf_counter()
-        b.step_async(steps, use_graph=True)
+        b.step_async(steps, use_graph=GRAPH)
         b.sync()
         elapsed = time.perf_counter() - t0
         st = b.stats()
@@ -336,6 +339,7 @@ def fp64_reference(scene, args, sched, ppb, preroll, runs):
     base = scene.config.iterations
     total = preroll + args.steps
     ordered = engine.SOLVER_ORDERED_F64
+    sched = (sched[0], 0)  # the ordered solver's lanes by world size
     ref = run(scene, ordered, engine.WARM_ZERO, base, 0, total, sched, ppb, positions=True)
     moved = []
     for w in scene.worlds:
@@ -410,6 +414,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="eager launches instead of graph replays (the same kernels; for "
+                         "profilers)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -432,6 +439,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    global GRAPH
+    GRAPH = not args.no_graph
     t_start = time.perf_counter()
     make, settle, ppb_default, candidates, desc = WORKLOADS[args.workload]
     scene = make(args.worlds)
@@ -452,7 +461,7 @@ def main():
 
     tuning = None
     if args.broadphase == "tune":
-        sched, tuning = autotune(scene, ppb, candidates, min(preroll, 200))
+        sched, tuning = autotune(scene, ppb, candidates, preroll)
     else:
         sched = (args.broadphase, args.solver_threads)
     note(f"schedule: {sched[0]}, {sched[1]} lanes")
@@ -537,7 +546,7 @@ def main():
             "solver": solver_name,
             "warm_start": warm_name,
             "mode": args.mode,
-            "graph": True,
+            "graph": GRAPH,
             "timed_window": [preroll, preroll + args.steps],
             "broadphase": sched[0],
             "solver_threads": sched[1],

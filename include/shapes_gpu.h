@@ -16,7 +16,7 @@
  *     distinct threads.  Each handle owns one HIP stream on the current
  *     device at creation time.
  *   - Body indices are world slot indices, i.e. the reference's fromList
- *     order (shapes/src/Physics/World.hs:144-151).
+ *     order (shapes/src/Physics/World.hs:118-124).
  */
 #ifndef SHAPES_GPU_H
 #define SHAPES_GPU_H
@@ -52,9 +52,9 @@ typedef int32_t shp_status;
 #define SHP_WARM_LEARNED 1 /* MLP-predicted initial impulses */
 
 /* One world, structure of arrays (n bodies).  Mirrors makePhysicalObj vel
- * rotvel pos rotpos (mass, inertia) (shapes/src/Physics/Engine.hs:125-137),
- * makeWorldObj phys mu bounce shape (World.hs:169-176) and makeHull
- * (Engine.hs:145-146).  mass/inertia 0 = infinite (toInvMass2,
+ * rotvel pos rotpos (mass, inertia) (shapes/src/Physics/Engine.hs:27-39),
+ * makeWorldObj phys mu bounce shape (World.hs:142-149) and makeHull
+ * (Engine.hs:47-48).  mass/inertia 0 = infinite (toInvMass2,
  * Constraint.hs:79-83).  Hull vertices are local-space, counter-clockwise,
  * 3..8 per body, CSR: body b owns hull_x/y[hull_off[b] .. hull_off[b+1]). */
 typedef struct shp_world_desc {
@@ -75,7 +75,7 @@ typedef struct shp_world_desc {
 } shp_world_desc;
 
 /* EngineConfig + ContactBehavior (Engine/Main.hs:34-37, Contact/Types.hs:
- * 65-70), the constant-acceleration External (World/External.hs:249-254)
+ * 20-25), the constant-acceleration External (World/External.hs:23-28)
  * and the sweep count, which the reference fixes at 2 (Main.hs:81-82). */
 typedef struct shp_config {
   double dt;
@@ -145,7 +145,7 @@ shp_status shp_batch_step_async(shp_batch* b, int32_t n_steps,
                                 int32_t use_graph);
 shp_status shp_batch_sync(shp_batch* b);
 
-/* ---- readback (readPhysObj, World.hs:120-121; debug as IOWorld.hs:74-81) */
+/* ---- readback (readPhysObj, World.hs:93-94; debug as IOWorld.hs:74-81) */
 
 shp_status shp_batch_num_bodies(const shp_batch* b, int32_t world,
                                 int32_t* n);
@@ -167,7 +167,7 @@ shp_status shp_batch_read_contacts(shp_batch* b, int32_t world,
                                    int32_t* flip, double* geom, double* lam,
                                    int32_t* warm);
 /* World-space vertices and unit edge normals (4 doubles per vertex, CSR
- * order) as left by moveShapes (World.hs:163-167). */
+ * order) as left by moveShapes (World.hs:136-140). */
 shp_status shp_batch_read_hulls(shp_batch* b, int32_t world, double* out);
 shp_status shp_batch_last_stats(shp_batch* b, shp_step_stats* out);
 

@@ -74,7 +74,7 @@ inline double invert_mass(double m) { return m == 0.0 ? 0.0 : 1.0 / m; }
 inline bool is_static(const Phys& p) { return p.iml == 0.0 && p.imr == 0.0; }
 inline bool is_static_lin(const Phys& p) { return 0.0 == p.iml; }
 
-struct Material {  // World.hs:62-65
+struct Material {  // World.hs:35-38
   double mu, bounce;
 };
 
@@ -161,7 +161,7 @@ inline V2 afmul(const Affine& t, V2 v) {
             (t.m10 * v.x + t.m11 * v.y) + t.m12 * 1.0);
 }
 
-// setHullTransform (ConvexHull.hs:184-195) via moveShape (World.hs:159-161)
+// setHullTransform (ConvexHull.hs:184-195) via moveShape (World.hs:132-134)
 void set_hull_transform(Hull& h, const Phys& p, int trig) {
   Affine t = to_transform(p.pos, p.th, trig);
   for (int i = 0; i < h.n; ++i) h.verts[i] = afmul(t, h.local[i]);
@@ -337,7 +337,7 @@ int clip_contacts(const Hull& se, const Hull& sp, const Overlap& ov,
   return np;
 }
 
-// HH.generateContacts (HullVsHull.hs:170-174) via contactDebug
+// HH.generateContacts (HullVsHull.hs:85-90) via contactDebug
 // (SAT.hs:238-249) and eitherBranchBoth (Utils.hs:230-235).
 int hull_contacts(const Hull& A, const Hull& B, ContactGeom* out, bool* sep) {
   Overlap oab, oba;
@@ -353,7 +353,7 @@ int hull_contacts(const Hull& A, const Hull& B, ContactGeom* out, bool* sep) {
 
 // ---- constraints -------------------------------------------------------------
 
-struct Key {  // ObjectFeatureKey (Constraints/Contact.hs:194-197)
+struct Key {  // ObjectFeatureKey (Constraints/Contact.hs:36-39)
   int i, j, fi, fj;
 };
 inline bool key_less(const Key& a, const Key& b) {
@@ -366,7 +366,7 @@ inline bool key_eq(const Key& a, const Key& b) {
   return a.i == b.i && a.j == b.j && a.fi == b.fi && a.fj == b.fj;
 }
 
-struct Constraint {  // ContactConstraint (Constraints/Types.hs:494-498)
+struct Constraint {  // ContactConstraint (Constraints/Types.hs:47-51)
   double jn[6], bn;  // non-penetration row and Baumgarte bias
   double jt[6];      // friction row (bias 0)
   V2 ra, rb, nr;     // RestitutionConstraint
@@ -400,7 +400,7 @@ inline void flip3v3(double* j) {  // Linear.hs:149-151
   for (int k = 0; k < 3; ++k) std::swap(j[k], j[k + 3]);
 }
 
-// constraintGen (Constraints/Contact.hs:218-229): NP and F are generated
+// constraintGen (Constraints/Contact.hs:60-72): NP and F are generated
 // for the swapped pair and their halves swapped back when the contact is
 // Flip (flipMap + flipExtract, Utils.hs:168-209); the restitution radii
 // always use (a,b) = (obj_i, obj_j) (Restitution.hs:21-31).
@@ -573,7 +573,7 @@ void culled_keys(orc_engine* e, std::vector<std::pair<int, int>>& out) {
 }
 
 // prepareFrame (Solvers/Contact.hs:40-52) + keyedContacts
-// (Constraints/Contact.hs:207-215)
+// (Constraints/Contact.hs:49-57)
 void prepare_frame(orc_engine* e, std::vector<Contact>& out) {
   out.clear();
   e->pair_first.clear();
@@ -627,8 +627,8 @@ void apply_cached_slns(orc_engine* e, std::vector<Contact>& cs,
 }
 
 // improveContactSln (Solvers/Contact.hs:124-143) with contactLagrangian
-// (Constraints/Contact.hs:244-253) and solutionProcessor (:259-268):
-// positive (SolutionProcessors.hs:427-429) then clampAbs (:436-447).
+// (Constraints/Contact.hs:86-95) and solutionProcessor (:101-110):
+// positive (SolutionProcessors.hs:33-35) then clampAbs (:42-53).
 void improve_contact(orc_engine* e, Contact& c) {
   Phys& a = e->objs[c.key.i];
   Phys& b = e->objs[c.key.j];
@@ -643,7 +643,7 @@ void improve_contact(orc_engine* e, Contact& c) {
   // positive: apply = max new (-cached)
   double app_n = hs_max(new_n, -c.ln);
   double acc_n = c.ln + app_n;
-  // clampAbs with threshold acc_n * pairMu (Friction.hs:46-48,341-347)
+  // clampAbs with threshold acc_n * pairMu (Friction.hs:46-48,50-55)
   double mu = (ma.mu + mb.mu) / 2.0;
   double thr = acc_n * mu;
   double acc_t = c.lt + new_t;
@@ -918,7 +918,7 @@ static void step_once(orc_engine* e) {
   int n = (int)e->objs.size();
   // 1. broadphase on the shapes placed by the previous moveShapes
   culled_keys(e, e->pairs);
-  // 2. applyExternal constantAccel (World.hs:183-185, External.hs:249-254)
+  // 2. applyExternal constantAccel (World.hs:156-158, External.hs:23-28)
   for (int i = 0; i < n; ++i) {
     Phys& o = e->objs[i];
     if (!is_static_lin(o)) o.vel = add(o.vel, v2(p.gx * p.dt, p.gy * p.dt));
@@ -940,13 +940,13 @@ static void step_once(orc_engine* e) {
   }
   compute_stats(e);
   residuals(e);
-  // 6. advance (World.hs:194-196, Constraint.hs:225-229)
+  // 6. advance (World.hs:167-169, Constraint.hs:225-229)
   for (int i = 0; i < n; ++i) {
     Phys& o = e->objs[i];
     o.pos = add(v2(p.dt * o.vel.x, p.dt * o.vel.y), o.pos);
     o.th = (p.dt * o.w) + o.th;
   }
-  // 7. moveShapes (World.hs:163-167)
+  // 7. moveShapes (World.hs:136-140)
   for (int i = 0; i < n; ++i) set_hull_transform(e->hulls[i], e->objs[i], p.trig);
 }
 
@@ -967,7 +967,7 @@ orc_engine* orc_engine_create(int32_t n, const double* px, const double* py,
   memset(&e->stats, 0, sizeof(e->stats));
   for (int i = 0; i < n; ++i) {
     Phys& o = e->objs[i];
-    // makePhysicalObj (Engine.hs:125-137); -0 normalised to +0 at load
+    // makePhysicalObj (Engine.hs:27-39); -0 normalised to +0 at load
     o.vel = v2(nz(vx[i]), nz(vy[i]));
     o.w = nz(omega[i]);
     o.pos = v2(nz(px[i]), nz(py[i]));
@@ -979,7 +979,7 @@ orc_engine* orc_engine_create(int32_t n, const double* px, const double* py,
     for (int k = hull_off[i]; k < hull_off[i + 1]; ++k)
       vs.push_back(v2(hull_x[k], hull_y[k]));
     e->hulls[i] = list_to_hull(vs);
-    // makeWorldObj places the shape with moveShape (World.hs:169-176)
+    // makeWorldObj places the shape with moveShape (World.hs:142-149)
     set_hull_transform(e->hulls[i], o, prm->trig);
   }
   return e;

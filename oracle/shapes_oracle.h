@@ -9,7 +9,7 @@
  * library never links or calls it.
  *
  * All indices are world slot indices (the reference's fromList order,
- * shapes/src/Physics/World.hs:144-151).
+ * shapes/src/Physics/World.hs:118-124).
  */
 #ifndef SHAPES_ORACLE_H
 #define SHAPES_ORACLE_H

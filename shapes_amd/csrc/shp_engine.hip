@@ -50,7 +50,7 @@ using namespace shp;
 // ---------------------------------------------------------------- helpers
 
 
-// Key = ObjectFeatureKey ((i, j), (fi, fj)) (Constraints/Contact.hs:194-197)
+// Key = ObjectFeatureKey ((i, j), (fi, fj)) (Constraints/Contact.hs:36-39)
 // packed so that unsigned order equals the derived lexicographic order.
 __device__ __forceinline__ uint64_t pack_key(int i, int j, int fi, int fj) {
   return ((uint64_t)(uint32_t)i << 40) | ((uint64_t)(uint32_t)j << 16) |
@@ -107,8 +107,8 @@ __global__ void k_init_hulls(Ctx c) {
   }
 }
 
-// advance (World.hs:194-196, Constraint.hs:225-229) when `advance`, then
-// moveShapes (World.hs:163-167): T(pos) R(theta) applied to the local
+// advance (World.hs:167-169, Constraint.hs:225-229) when `advance`, then
+// moveShapes (World.hs:136-140): T(pos) R(theta) applied to the local
 // vertices (Transform.hs:34-38, Linear.hs:217-220), unit edge normals
 // (ConvexHull.hs:184-195) and the AABB fold (Broadphase/Aabb.hs:81-110).
 // A workgroup takes 256 consecutive bodies: one lane per body builds the
@@ -290,8 +290,8 @@ __global__ __launch_bounds__(INTEGRATE_THREADS) void k_finish(Ctx c, int ni) {
 // Broadphase: one wavefront per body i scans j = i-1 .. 0 of its world in
 // chunks of 64 lanes (culledKeys, Broadphase/Aabb.hs:168-183, which is the
 // pair set the grid of Grid.hs:67-86 returns).  Pass 0 counts and applies
-// the constant acceleration (applyExternal, World.hs:183-185 with
-// constantAccel, World/External.hs:249-254); pass 1 writes the pairs, in
+// the constant acceleration (applyExternal, World.hs:156-158 with
+// constantAccel, World/External.hs:23-28); pass 1 writes the pairs, in
 // descending j within descending i, using ballot + prefix popcount.
 __global__ void k_broadphase(Ctx c, int fill) {
   int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -395,7 +395,7 @@ __global__ void k_world_ranges(Ctx c) {
   if (w == c.W && c.coff[ps] > c.contact_cap) atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_CONTACT_OVERFLOW);
 }
 
-// Compaction + constraintGen (Constraints/Contact.hs:218-229) + the warm
+// Compaction + constraintGen (Constraints/Contact.hs:60-72) + the warm
 // start join of applyCachedSlns (Solvers/Contact.hs:74-121): a binary search
 // of the key in last frame's cache of the same world, which is strictly
 // descending, finds exactly the element descZipVector (Utils/Descending.hs:
@@ -1229,15 +1229,6 @@ shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_c
   std::vector<LaunchRec> L;
   plan_launches(P, cfg.warm_start == SHP_WARM_LEARNED, false, &L);
   TRY(audit_plan(P, L));
-  // The learned warm start has not run with the coloured solver's
-  // global-memory paths (worlds whose FP32 velocities do not fit in LDS):
-  // refused until it is tested there.
-  if (cfg.warm_start == SHP_WARM_LEARNED && cfg.solver == SHP_SOLVER_COLORED_F32 &&
-      max_nb > VEL_LDS_MAX_BODIES)
-    return fail(SHP_E_UNSUPPORTED,
-                "learned warm start with the coloured solver is limited to worlds of at most " +
-                    std::to_string(VEL_LDS_MAX_BODIES) + " bodies (largest world: " +
-                    std::to_string(max_nb) + ")");
   return SHP_OK;
 }
 

@@ -235,9 +235,9 @@ __device__ __forceinline__ void warm_frec(const FRec& r, const Vel<VLDS>& V) {
 }
 
 // improveContactSln (Solvers/Contact.hs:124-143): contactLagrangian with
-// bounceB (Constraints/Contact.hs:232-253, Restitution.hs:34-47), then
-// solutionProcessor = positive / clampAbs (Constraints/Contact.hs:259-268,
-// SolutionProcessors.hs:427-447), then applySln, on a compact record.  The
+// bounceB (Constraints/Contact.hs:74-95, Restitution.hs:34-47), then
+// solutionProcessor = positive / clampAbs (Constraints/Contact.hs:101-110,
+// SolutionProcessors.hs:33-53), then applySln, on a compact record.  The
 // new lagrangians go to the record and to the packed array.
 template <bool VLDS>
 __device__ __forceinline__ void improve_frec(const Ctx& c, int k, FRec& r, const Vel<VLDS>& V) {

@@ -443,7 +443,7 @@ def make_batch(worlds, config, **kw) -> Batch:
 
 @dataclass
 class ContactBehavior:
-    """ContactBehavior (shapes/src/Physics/Contact/Types.hs:65-70)."""
+    """ContactBehavior (shapes/src/Physics/Contact/Types.hs:20-25)."""
     contact_baumgarte: float = 0.0
     contact_penetration_slop: float = 0.0
 
@@ -482,7 +482,7 @@ def run_world(dt: float, scene: Scene, steps: int) -> Batch:
 
 
 def read_phys_obj(state: Batch, i: int, world: int = 0) -> dict:
-    """readPhysObj (World.hs:120-121)."""
+    """readPhysObj (World.hs:93-94)."""
     b = state.bodies(world)[i]
     return {"pos": (b[0], b[1]), "rot_pos": b[2], "vel": (b[3], b[4]),
             "rot_vel": b[5]}

@@ -3,7 +3,7 @@
 A scene is a list of worlds; a world is a `WorldDesc` of structure-of-arrays
 body state plus CSR hull vertices in local space, counter-clockwise and
 centred on the centroid.  Body order is the world's slot order (the
-reference's `fromList` order, shapes/src/Physics/World.hs:144-151): the
+reference's `fromList` order, shapes/src/Physics/World.hs:118-124): the
 static floor is body 0.
 
 Random numbers come from numpy's Philox counter-based generator, so a seed
@@ -54,7 +54,7 @@ class WorldDesc:
 @dataclass
 class SceneConfig:
     """Engine parameters: the reference's EngineConfig + ContactBehavior
-    (shapes/src/Physics/Engine/Main.hs:34-37, Contact/Types.hs:65-70), the
+    (shapes/src/Physics/Engine/Main.hs:34-37, Contact/Types.hs:20-25), the
     constant acceleration External, and the sweep count the reference
     hard-codes to 2 (Engine/Main.hs:81-82)."""
 

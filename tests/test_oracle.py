@@ -223,7 +223,7 @@ def test_one_contact_closed_form(orc):
 
 def test_solver_invariants(orc):
     """After every step: lambda_n >= 0 and |lambda_t| <= mu_pair * lambda_n
-    (positive / clampAbs, SolutionProcessors.hs:427-447); the static floor
+    (positive / clampAbs, SolutionProcessors.hs:33-53); the static floor
     never moves."""
     w = scenes.random_world(11, count=50, spacing=1.3)
     cfg = scenes.SceneConfig(iterations=4)
@@ -299,7 +299,7 @@ def test_warm_start_join_matches_lookup(orc):
 
 def test_free_flight_closed_form(orc):
     """No contacts: v_n = v0 + n*(g*dt) and pos += dt*v (applyExternal then
-    advance, World.hs:183-196), with the same float operations."""
+    advance, World.hs:156-169), with the same float operations."""
     b = scenes._Builder()
     b.add_dynamic(scenes.rectangle_vertices(1, 1), (3.0, 50.0), vel=(0.5, 2.0),
                   omega=0.3)
