@@ -134,6 +134,7 @@ def run(scene, solver, warm, iterations, preroll, steps, sched, ppb, model=None,
                 tot["contacts"] += s["contacts"]
             t = b.timing()
             out["phases_ms"] = {p: t[p] for p in engine.PHASES}
+            out["solve_profile_us"] = b.solve_profile()
             out["instrumented"] = {k: v / instrument for k, v in tot.items()}
             out["instrumented"]["steps"] = instrument
     finally:
@@ -560,6 +561,7 @@ def main():
                           "achieved": round(step_gbs, 3), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 6)},
         "phase_us": {p: round(v * 1e3, 2) for p, v in instr["phases_ms"].items()},
+        "largest_world_solve_us": instr["solve_profile_us"],
         "avg_pairs": round(instr["instrumented"]["pairs"], 1),
         "avg_contacts": round(instr["instrumented"]["contacts"], 1),
     }

@@ -32,7 +32,7 @@ typedef int32_t shp_status;
 #define SHP_E_INVAL 1       /* bad argument */
 #define SHP_E_OOM 2         /* device allocation failed */
 #define SHP_E_HIP 3         /* HIP runtime error */
-#define SHP_E_UNSUPPORTED 4 /* circles, hulls with > 8 vertices, ... */
+#define SHP_E_UNSUPPORTED 4 /* hulls with > 8 vertices, capacities, ... */
 #define SHP_E_OVERFLOW 5    /* pair/contact capacity exceeded */
 #define SHP_E_NONFINITE 6   /* a body reached a non-finite state */
 
@@ -72,6 +72,10 @@ typedef struct shp_world_desc {
   const int32_t* hull_off; /* n_bodies + 1 entries */
   const double* hull_x;
   const double* hull_y;
+  /* NULL: every body is a hull.  Otherwise radius[i] > 0 makes body i a
+   * circle of that radius (makeCircle, Engine.hs:53-54; Circle.hs:19-25)
+   * whose hull range must be empty; radius[i] == 0 keeps it a hull. */
+  const double* radius;
 } shp_world_desc;
 
 /* EngineConfig + ContactBehavior (Engine/Main.hs:34-37, Contact/Types.hs:

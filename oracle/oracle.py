@@ -60,7 +60,7 @@ def lib():
             build()
         L = C.CDLL(LIB_PATH)
         L.orc_engine_create.restype = C.c_void_p
-        L.orc_engine_create.argtypes = [C.c_int32] + [_dp] * 10 + [_ip, _dp, _dp,
+        L.orc_engine_create.argtypes = [C.c_int32] + [_dp] * 10 + [_ip, _dp, _dp, _dp,
                                                                   C.POINTER(Params)]
         L.orc_engine_destroy.argtypes = [C.c_void_p]
         L.orc_engine_step.argtypes = [C.c_void_p, C.c_int32]
@@ -78,6 +78,11 @@ def lib():
         L.orc_hull_contacts.argtypes = [C.c_int32, _dp, _dp, C.c_int32, _dp, _dp,
                                         _ip, _ip, _dp, _ip]
         L.orc_hull_contacts.restype = C.c_int32
+        L.orc_shape_contacts.argtypes = [C.c_int32, _dp, _dp, _dp, C.c_int32, _dp, _dp, _dp,
+                                         _ip, _ip, _dp]
+        L.orc_shape_contacts.restype = C.c_int32
+        L.orc_closest_simplex.argtypes = [C.c_int32, _dp, _dp, C.c_double, C.c_double, _ip]
+        L.orc_closest_simplex.restype = C.c_int32
         L.orc_solve_constraint.argtypes = [_dp, C.c_double, _dp, _dp]
         L.orc_unordered_pairs.argtypes = [C.c_int32, _ip]
         L.orc_unordered_pairs.restype = C.c_int32
@@ -119,7 +124,13 @@ class OracleEngine:
         prm = Params(config.dt, config.baumgarte, config.slop, config.gx,
                      config.gy, config.iterations, trig, solver, world_index, broadphase)
         self.n = world.n
-        self._h = L.orc_engine_create(world.n, *args, hop, hxp, hyp, C.byref(prm))
+        rad = getattr(world, "radius", None)
+        if rad is not None:
+            rad, radp = _d(rad)
+            keep.append(rad)
+        else:
+            radp = None
+        self._h = L.orc_engine_create(world.n, *args, hop, hxp, hyp, radp, C.byref(prm))
         self._L = L
 
     def close(self):
@@ -205,6 +216,43 @@ def hull_contacts(a_verts, b_verts):
         out.append((int(keys[2 * k]), int(keys[2 * k + 1]), int(flip[k]),
                     *[float(g) for g in geom[5 * k:5 * k + 5]]))
     return bool(sep[0]), out
+
+
+def _shape_args(shape):
+    """(n, xs, ys, circle) of a shape: a list of world vertices, or a
+    ("circle", radius, (cx, cy)) tuple."""
+    if isinstance(shape, tuple) and shape and shape[0] == "circle":
+        return 0, [0.0], [0.0], [float(shape[1]), float(shape[2][0]), float(shape[2][1])]
+    return len(shape), [v[0] for v in shape], [v[1] for v in shape], [0.0, 0.0, 0.0]
+
+
+def shape_contacts(a, b):
+    """Contact.generateContacts for two shapes (hull vertex lists or
+    ("circle", r, (x, y))): list of (fa, fb, flip, nx, ny, px, py, depth)."""
+    L = lib()
+    na, ax, ay, ac = _shape_args(a)
+    nb, bx, by, bc = _shape_args(b)
+    arrs = [_d(v) for v in (ax, ay, ac, bx, by, bc)]
+    keys = np.zeros(4, np.int32)
+    flip = np.zeros(2, np.int32)
+    geom = np.zeros(10, np.float64)
+    n = L.orc_shape_contacts(na, arrs[0][1], arrs[1][1], arrs[2][1], nb, arrs[3][1],
+                             arrs[4][1], arrs[5][1], keys.ctypes.data_as(_ip),
+                             flip.ctypes.data_as(_ip), geom.ctypes.data_as(_dp))
+    return [(int(keys[2 * k]), int(keys[2 * k + 1]), int(flip[k]),
+             *[float(g) for g in geom[5 * k:5 * k + 5]]) for k in range(n)]
+
+
+def closest_simplex(verts, origin):
+    """GJK.closestSimplex of a world-space hull to a point: vertex indices,
+    most recently added first."""
+    L = lib()
+    x, xp = _d([v[0] for v in verts])
+    y, yp = _d([v[1] for v in verts])
+    idx = np.zeros(3, np.int32)
+    n = L.orc_closest_simplex(len(verts), xp, yp, float(origin[0]), float(origin[1]),
+                              idx.ctypes.data_as(_ip))
+    return idx[:n].tolist()
 
 
 def solve_constraint(j, b, vel6, im4):

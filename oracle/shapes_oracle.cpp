@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <vector>
 
@@ -80,8 +81,12 @@ struct Material {  // World.hs:35-38
 
 // ConvexHull (ConvexHull.hs:61-68): vertex count, world vertices, world unit
 // edge normals, per-edge cached extent indices, local vertices.
+// A circle (Contact/Circle.hs:19-22) is a Hull with radius > 0, no
+// vertices and its world centre.
 struct Hull {
   int n;
+  double radius = 0.0;
+  V2 center{0.0, 0.0};
   std::vector<V2> verts;
   std::vector<V2> normals;
   std::vector<int> ext_min, ext_max;
@@ -161,9 +166,15 @@ inline V2 afmul(const Affine& t, V2 v) {
             (t.m10 * v.x + t.m11 * v.y) + t.m12 * 1.0);
 }
 
-// setHullTransform (ConvexHull.hs:184-195) via moveShape (World.hs:132-134)
+// setHullTransform (ConvexHull.hs:184-195) via moveShape (World.hs:132-134);
+// for a circle setCircleTransform (Circle.hs:60-64): the image of the local
+// origin.
 void set_hull_transform(Hull& h, const Phys& p, int trig) {
   Affine t = to_transform(p.pos, p.th, trig);
+  if (h.radius > 0.0) {
+    h.center = afmul(t, v2(0.0, 0.0));
+    return;
+  }
   for (int i = 0; i < h.n; ++i) h.verts[i] = afmul(t, h.local[i]);
   for (int i = 0; i < h.n; ++i) h.normals[i] = unit_edge_normal(h.verts, i);
 }
@@ -172,8 +183,13 @@ struct Aabb {
   double minx, maxx, miny, maxy;
 };
 
-// hullToAabb = foldl1 mergeAabb (Aabb.hs:81-110)
+// toAabb (Aabb.hs:81-93): hullToAabb = foldl1 mergeAabb over the
+// vertices, or circleToAabb = (x - r, x + r) x (y - r, y + r).
 Aabb hull_aabb(const Hull& h) {
+  if (h.radius > 0.0) {
+    const double x = h.center.x, y = h.center.y, r = h.radius;
+    return Aabb{x - r, x + r, y - r, y + r};
+  }
   Aabb a{h.verts[0].x, h.verts[0].x, h.verts[0].y, h.verts[0].y};
   for (int i = 1; i < h.n; ++i) {
     double x = h.verts[i].x, y = h.verts[i].y;
@@ -349,6 +365,169 @@ int hull_contacts(const Hull& A, const Hull& B, ContactGeom* out, bool* sep) {
   *sep = false;
   if (oab.depth < oba.depth) return clip_contacts(A, B, oab, false, out);
   return clip_contacts(B, A, oba, true, out);
+}
+
+// ---- circles and GJK --------------------------------------------------------
+
+// Circle.contact (Circle.hs:30-57): A penetrated, B penetrating; the normal
+// points out of A.  Contact when (ra + rb)^2 >= |b - a|^2.
+bool circle_contact(V2 a, double ra, V2 b, double rb, ContactGeom* c) {
+  const V2 ab = sub(b, a);                          // diffP2 b a
+  const double rab = ra + rb;
+  const double ab_sq = ab.x * ab.x + ab.y * ab.y;   // sqLengthV2
+  if (!(rab * rab >= ab_sq)) return false;
+  const double len = std::sqrt(ab_sq);
+  const V2 abn = v2(ab.x / len, ab.y / len);        // sdivV2 abLength ab
+  const V2 ea = v2(abn.x * ra + a.x, abn.y * ra + a.y);        // vplusP2 (smulV2 ra abN) a
+  const V2 eb = v2(abn.x * (-rb) + b.x, abn.y * (-rb) + b.y);  // vplusP2 (smulV2 (-rb) abN) b
+  c->p = v2((ea.x + eb.x) / 2.0, (ea.y + eb.y) / 2.0);         // midpointP2
+  c->depth = (ra + rb) - len;
+  c->n = abn;
+  return true;
+}
+
+// GJK.closestSimplex (GJK.hs:52-146) of a hull to a point: vertex indices of
+// the simplex, most recently added first (a 1-, 2- or 3-simplex).
+struct Simplex {
+  int n;
+  int v[3];
+};
+
+// support (ConvexHull.hs:124-128): the first vertex of maximal dot(v, dir)
+inline int support(const Hull& h, V2 dir) {
+  int best = 0;
+  double bd = dot(h.verts[0], dir);
+  for (int k = 1; k < h.n; ++k) {
+    const double d = dot(h.verts[k], dir);
+    if (d > bd) {
+      bd = d;
+      best = k;
+    }
+  }
+  return best;
+}
+
+inline bool same_direction(V2 a, V2 b) { return dot(a, b) > 0.0; }  // GJK.hs:139-140
+// crossV2V2 (Linear.hs:135-139): (a x b) x c
+inline V2 cross_v2v2(V2 a, V2 b, V2 c) {
+  const double abz = a.x * b.y - a.y * b.x;
+  return v2(-(abz * c.y), abz * c.x);
+}
+// crosszV2 (Linear.hs:122-125)
+inline V2 crossz(V2 a, double z) { return v2(a.y * z, -(a.x * z)); }
+
+// The loop stops when the support point repeats (a found simplex) or the
+// 2-simplex encloses the point.  The reference has no iteration bound; the
+// loop here stops after GJK_MAX_ITERATIONS, keeping the current simplex
+// (no hull of <= 8 vertices takes more than a few).
+#define GJK_MAX_ITERATIONS 32
+Simplex closest_simplex(const Hull& h, V2 o) {
+  Simplex s{1, {0, 0, 0}};
+  V2 d = sub(o, h.verts[0]);
+  for (int it = 0; it < GJK_MAX_ITERATIONS; ++it) {
+    const int ai = support(h, d);
+    // extendSimplex (GJK.hs:70-86): a repeat ends the search
+    for (int k = 0; k < s.n; ++k)
+      if (s.v[k] == ai) return s;
+    if (s.n == 1) {
+      // shiftSimplex2 (GJK.hs:95-109) on (a, b)
+      const int bi = s.v[0];
+      const V2 a = h.verts[ai], b = h.verts[bi];
+      const V2 ab = sub(b, a), ao = sub(o, a);
+      if (same_direction(ab, ao)) {
+        s = Simplex{2, {ai, bi, 0}};
+        d = cross_v2v2(ab, ao, ab);
+      } else {
+        s = Simplex{1, {ai, 0, 0}};
+        d = ao;
+      }
+    } else {
+      // shiftSimplex3 (GJK.hs:111-137) on (a, b, c)
+      const int bi = s.v[0], ci = s.v[1];
+      const V2 a = h.verts[ai], b = h.verts[bi], c = h.verts[ci];
+      const V2 ab = sub(b, a), ac = sub(c, a), ao = sub(o, a);
+      const double abc = cross(ab, ac);
+      const V2 abcac = zcross(abc, ac);
+      const V2 ababc = crossz(ab, abc);
+      bool star;
+      if (same_direction(abcac, ao)) {
+        if (same_direction(ac, ao)) {
+          s = Simplex{2, {ai, ci, 0}};
+          d = cross_v2v2(ac, ao, ac);
+          continue;
+        }
+        star = true;
+      } else {
+        star = same_direction(ababc, ao);
+      }
+      if (!star) return Simplex{3, {ai, bi, ci}};  // encloses the point
+      if (same_direction(ab, ao)) {
+        s = Simplex{2, {ai, bi, 0}};
+        d = cross_v2v2(ab, ao, ab);
+      } else {
+        s = Simplex{1, {ai, 0, 0}};
+        d = ao;
+      }
+    }
+  }
+  return s;
+}
+
+// CircleVsHull.generateContacts (CircleVsHull.hs:18-69): the hull feature
+// (the simplex's most recent vertex) and the contact, the circle always the
+// penetrator; none when the centre is inside the hull (3-simplex) or
+// farther than the radius from the closest point.
+bool circle_hull_contact(V2 center, double r, const Hull& h, int* feature, ContactGeom* c) {
+  const Simplex s = closest_simplex(h, center);
+  if (s.n == 3) return false;
+  V2 a = h.verts[s.v[0]];
+  if (s.n == 2) {  // closestAlong (:135-144)
+    const V2 p = h.verts[s.v[0]], q = h.verts[s.v[1]];
+    const V2 ao = sub(center, p);
+    const V2 abn = normalize(sub(q, p));
+    const double t = dot(ao, abn);
+    a = v2(abn.x * t + p.x, abn.y * t + p.y);
+  }
+  // processSimplex_ (:117-133)
+  const V2 ab = sub(center, a);
+  const double ab_sq = ab.x * ab.x + ab.y * ab.y;
+  if (r * r < ab_sq) return false;
+  const double len = std::sqrt(ab_sq);
+  const V2 nrm = v2(ab.x / len, ab.y / len);
+  c->n = neg(nrm);
+  c->p = a;
+  c->depth = r - len;
+  *feature = s.v[0];
+  return true;
+}
+
+// Contact.generateContacts (Contact.hs:22-40) for a pair (A, B) = (shape i,
+// shape j): circle-circle (0, 0) Same; circle-hull (0, f) Same; hull-circle
+// (f, 0) Flip (the contact computed circle against hull); hull-hull as SAT.
+int shape_contacts(const Hull& A, const Hull& B, ContactGeom* out, bool* sep) {
+  const bool ca = A.radius > 0.0, cb = B.radius > 0.0;
+  if (!ca && !cb) return hull_contacts(A, B, out, sep);
+  *sep = false;
+  ContactGeom g;
+  int f = 0;
+  if (ca && cb) {
+    if (!circle_contact(A.center, A.radius, B.center, B.radius, &g)) return 0;
+    g.fa = 0;
+    g.fb = 0;
+    g.flip = false;
+  } else if (ca) {
+    if (!circle_hull_contact(A.center, A.radius, B, &f, &g)) return 0;
+    g.fa = 0;
+    g.fb = f;
+    g.flip = false;
+  } else {
+    if (!circle_hull_contact(B.center, B.radius, A, &f, &g)) return 0;
+    g.fa = f;
+    g.fb = 0;
+    g.flip = true;
+  }
+  out[0] = g;
+  return 1;
 }
 
 // ---- constraints -------------------------------------------------------------
@@ -581,7 +760,7 @@ void prepare_frame(orc_engine* e, std::vector<Contact>& out) {
     e->pair_first.push_back((int)out.size());
     ContactGeom g[2];
     bool sep;
-    int k = hull_contacts(e->hulls[pr.first], e->hulls[pr.second], g, &sep);
+    int k = shape_contacts(e->hulls[pr.first], e->hulls[pr.second], g, &sep);
     for (int q = 0; q < k; ++q) {
       Contact c;
       c.key = Key{pr.first, pr.second, g[q].fa, g[q].fb};
@@ -958,7 +1137,8 @@ orc_engine* orc_engine_create(int32_t n, const double* px, const double* py,
                               const double* mass, const double* inertia,
                               const double* mu, const double* bounce,
                               const int32_t* hull_off, const double* hull_x,
-                              const double* hull_y, const orc_params* prm) {
+                              const double* hull_y, const double* radius,
+                              const orc_params* prm) {
   orc_engine* e = new orc_engine();
   e->prm = *prm;
   e->objs.resize(n);
@@ -978,7 +1158,13 @@ orc_engine* orc_engine_create(int32_t n, const double* px, const double* py,
     std::vector<V2> vs;
     for (int k = hull_off[i]; k < hull_off[i + 1]; ++k)
       vs.push_back(v2(hull_x[k], hull_y[k]));
-    e->hulls[i] = list_to_hull(vs);
+    if (radius && radius[i] > 0.0) {  // makeCircle (Engine.hs:53-54)
+      e->hulls[i] = Hull();
+      e->hulls[i].n = 0;
+      e->hulls[i].radius = radius[i];
+    } else {
+      e->hulls[i] = list_to_hull(vs);
+    }
     // makeWorldObj places the shape with moveShape (World.hs:142-149)
     set_hull_transform(e->hulls[i], o, prm->trig);
   }
@@ -1110,6 +1296,50 @@ int32_t orc_hull_contacts(int32_t na, const double* ax, const double* ay,
     out_geom[5 * k + 4] = g[k].depth;
   }
   return n;
+}
+
+// A shape for the test entry points: n > 0 world-space hull vertices, or a
+// circle (n == 0) of radius r at (cx, cy).
+static Hull test_shape(int32_t n, const double* x, const double* y, double r, double cx, double cy) {
+  if (n == 0) {
+    Hull h;
+    h.n = 0;
+    h.radius = r;
+    h.center = v2(cx, cy);
+    return h;
+  }
+  std::vector<V2> v;
+  for (int k = 0; k < n; ++k) v.push_back(v2(x[k], y[k]));
+  return list_to_hull(v);
+}
+
+int32_t orc_shape_contacts(int32_t na, const double* ax, const double* ay, const double* ac,
+                           int32_t nb, const double* bx, const double* by, const double* bc,
+                           int32_t* out_keys, int32_t* out_flip, double* out_geom) {
+  const Hull A = test_shape(na, ax, ay, ac[0], ac[1], ac[2]);
+  const Hull B = test_shape(nb, bx, by, bc[0], bc[1], bc[2]);
+  ContactGeom g[2];
+  bool sep;
+  const int n = shape_contacts(A, B, g, &sep);
+  for (int k = 0; k < n; ++k) {
+    out_keys[2 * k] = g[k].fa;
+    out_keys[2 * k + 1] = g[k].fb;
+    out_flip[k] = g[k].flip ? 1 : 0;
+    out_geom[5 * k + 0] = g[k].n.x;
+    out_geom[5 * k + 1] = g[k].n.y;
+    out_geom[5 * k + 2] = g[k].p.x;
+    out_geom[5 * k + 3] = g[k].p.y;
+    out_geom[5 * k + 4] = g[k].depth;
+  }
+  return n;
+}
+
+int32_t orc_closest_simplex(int32_t n, const double* x, const double* y, double ox, double oy,
+                            int32_t* idx) {
+  const Hull h = test_shape(n, x, y, 0.0, 0.0, 0.0);
+  const Simplex s = closest_simplex(h, v2(ox, oy));
+  for (int k = 0; k < s.n; ++k) idx[k] = s.v[k];
+  return s.n;
 }
 
 void orc_solve_constraint(const double* j, double b, double* vel,

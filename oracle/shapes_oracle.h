@@ -56,7 +56,8 @@ orc_engine* orc_engine_create(int32_t n, const double* px, const double* py,
                               const double* mass, const double* inertia,
                               const double* mu, const double* bounce,
                               const int32_t* hull_off, const double* hull_x,
-                              const double* hull_y, const orc_params* prm);
+                              const double* hull_y, const double* radius,
+                              const orc_params* prm);
 void orc_engine_destroy(orc_engine* e);
 void orc_engine_step(orc_engine* e, int32_t steps);
 
@@ -108,6 +109,15 @@ int32_t orc_hull_contacts(int32_t na, const double* ax, const double* ay,
 /* solveConstraint (shapes/src/Physics/Constraint.hs:207-212).
  * j[6], b; obj[6*2] = vx,vy,w per body then ...; im[4] = (ml,mr) a, b.
  * Writes updated velocities into obj. */
+/* Contacts between two shapes (Contact.generateContacts, Contact.hs:22-40):
+ * a hull of na > 0 world-space vertices, or a circle (na == 0) with
+ * ac = {radius, centre x, centre y}; the same for b. */
+int32_t orc_shape_contacts(int32_t na, const double* ax, const double* ay, const double* ac,
+                           int32_t nb, const double* bx, const double* by, const double* bc,
+                           int32_t* out_keys, int32_t* out_flip, double* out_geom);
+/* GJK.closestSimplex (GJK.hs:52-146): vertex indices, most recent first. */
+int32_t orc_closest_simplex(int32_t n, const double* x, const double* y, double ox, double oy,
+                            int32_t* idx);
 void orc_solve_constraint(const double* j, double b, double* vel6,
                           const double* im4);
 

@@ -830,39 +830,61 @@ __device__ void colour_world_lds(const Ctx& c, int w, int p0, int np, int b0, in
 
 // The same colouring for worlds whose colouring does not fit in LDS:
 // 64-bit priorities and colour masks per body (col_best, col_used) and
-// colours per pair (sched_level) in global memory (L2-resident).
+// colours per pair (sched_level) in global memory (L2-resident).  The
+// uncoloured manifolds of a round are kept as a list (col_list, two
+// buffers of the world's pair range): a round visits only them, clears
+// only their bodies' priorities, and passes the manifolds it does not
+// colour on to the next round's list.  The selection of a round depends
+// only on which manifolds are uncoloured and on their priorities, so the
+// colours are those of a round over every pair.
 template <int T>
 __device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int b0, int nb,
                                     int* s_off, int* s_cur, int* sm) {
-  int &s_left = sm[0], &s_taken = sm[1], &s_top = sm[2];
+  int &s_left = sm[0], &s_taken = sm[1], &s_top = sm[2], &s_fill = sm[3];
   const int tid = threadIdx.x;
   unsigned long long* best = c.col_best + b0;
   uint32_t* used_at = c.col_used + b0;
   int32_t* colour = c.sched_level + p0;  // per pair: colour, -1 uncoloured, -2 none
+  int32_t* list[2] = {c.col_list + p0, c.col_list + c.pair_cap + p0};
+  const unsigned long long below = (1ull << (tid & 63)) - 1ull;
   if (tid == 0) {
-    s_left = 0;
     s_top = 0;
+    s_fill = 0;
   }
   for (int i = tid; i < nb; i += T) used_at[i] = 0u;
   __syncthreads();
-  int mine = 0;
-  for (int q = tid; q < np; q += T) {
-    const int p = p0 + q;
-    const bool has = c.ccount[p] > 0;
-    colour[q] = has ? -1 : -2;
-    mine += has;
+  // the first list: every pair with contacts
+  for (int q0 = 0; q0 < np; q0 += T) {
+    const int q = q0 + tid;
+    const bool has = q < np && c.ccount[p0 + q] > 0;
+    if (q < np) colour[q] = has ? -1 : -2;
+    const unsigned long long bal = __ballot(has);
+    int base = 0;
+    if ((tid & 63) == 0 && bal) base = atomicAdd(&s_fill, (int)__popcll(bal));
+    base = __shfl(base, 0);
+    if (has) list[0][base + (int)__popcll(bal & below)] = q;
   }
-  mine = wave_sum(mine);
-  if ((tid & 63) == 0) atomicAdd(&s_left, mine);
   __syncthreads();
-  int rounds = 0;
+  if (tid == 0) s_left = s_fill;
+  __syncthreads();
+  int rounds = 0, cur = 0;
   while (s_left > 0 && rounds < JP_ROUNDS) {
-    for (int i = tid; i < nb; i += T) best[i] = 0ull;
-    if (tid == 0) s_taken = 0;
+    const int n = s_left;
+    const int32_t* in = list[cur];
+    int32_t* out = list[cur ^ 1];
+    for (int k = tid; k < n; k += T) {
+      const int p = p0 + in[k];
+      const int ga = c.pa[p], gb = c.pb[p];
+      if (!c.is_static[ga]) best[ga - b0] = 0ull;
+      if (!c.is_static[gb]) best[gb - b0] = 0ull;
+    }
+    if (tid == 0) {
+      s_taken = 0;
+      s_fill = 0;
+    }
     __syncthreads();
-    for (int q = tid; q < np; q += T) {
-      if (colour[q] != -1) continue;
-      const int p = p0 + q;
+    for (int k = tid; k < n; k += T) {
+      const int q = in[k], p = p0 + q;
       const unsigned long long prio = prio64(q, w);
       const int ga = c.pa[p], gb = c.pb[p];
       if (!c.is_static[ga]) atomicMax(&best[ga - b0], prio);
@@ -870,23 +892,35 @@ __device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int b0,
     }
     __syncthreads();
     int taken = 0, top = 0;
-    for (int q = tid; q < np; q += T) {
-      if (colour[q] != -1) continue;
-      const int p = p0 + q;
-      const unsigned long long prio = prio64(q, w);
-      const int ga = c.pa[p], gb = c.pb[p];
-      const bool sa = c.is_static[ga] != 0, sb = c.is_static[gb] != 0;
-      if ((sa || best[ga - b0] == prio) && (sb || best[gb - b0] == prio)) {
-        const uint32_t used = (sa ? 0u : used_at[ga - b0]) | (sb ? 0u : used_at[gb - b0]);
-        const int l = greedy_colour(used, rounds);
-        colour[q] = l;
-        if (l < GREEDY_COLOURS) {
-          if (!sa) used_at[ga - b0] = used_at[ga - b0] | (1u << l);
-          if (!sb) used_at[gb - b0] = used_at[gb - b0] | (1u << l);
+    for (int k0 = 0; k0 < n; k0 += T) {
+      const int k = k0 + tid;
+      bool keep = false;
+      int q = 0;
+      if (k < n) {
+        q = in[k];
+        const int p = p0 + q;
+        const unsigned long long prio = prio64(q, w);
+        const int ga = c.pa[p], gb = c.pb[p];
+        const bool sa = c.is_static[ga] != 0, sb = c.is_static[gb] != 0;
+        if ((sa || best[ga - b0] == prio) && (sb || best[gb - b0] == prio)) {
+          const uint32_t used = (sa ? 0u : used_at[ga - b0]) | (sb ? 0u : used_at[gb - b0]);
+          const int l = greedy_colour(used, rounds);
+          colour[q] = l;
+          if (l < GREEDY_COLOURS) {
+            if (!sa) used_at[ga - b0] = used_at[ga - b0] | (1u << l);
+            if (!sb) used_at[gb - b0] = used_at[gb - b0] | (1u << l);
+          }
+          top = max(top, l + 1);
+          ++taken;
+        } else {
+          keep = true;
         }
-        top = max(top, l + 1);
-        ++taken;
       }
+      const unsigned long long bal = __ballot(keep);
+      int base = 0;
+      if ((tid & 63) == 0 && bal) base = atomicAdd(&s_fill, (int)__popcll(bal));
+      base = __shfl(base, 0);
+      if (keep) out[base + (int)__popcll(bal & below)] = q;
     }
     taken = wave_sum(taken);
     top = wave_max(top);
@@ -895,14 +929,15 @@ __device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int b0,
       atomicMax(&s_top, top);
     }
     __syncthreads();
-    if (tid == 0) s_left -= s_taken;
+    if (tid == 0) s_left = s_fill;
+    cur ^= 1;
     ++rounds;
     __syncthreads();
   }
   const bool serial_tail = s_left > 0;
   const int top = s_top;
-  for (int q = tid; q < np; q += T)
-    if (colour[q] == -1) colour[q] = top;
+  // the manifolds the rounds did not reach form the serial tail colour
+  for (int k = tid; k < s_left; k += T) colour[list[cur][k]] = top;
   __syncthreads();
   colour_finish<T>(c, w, p0, np, colour, top, serial_tail, s_off, s_cur);
 }

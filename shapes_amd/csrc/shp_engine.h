@@ -80,6 +80,7 @@ struct Ctx {
 
   // bodies (NB)
   double *px, *py, *th, *vx, *vy, *w, *iml, *imr, *mu, *e;
+  double* rad;  // circle radius, 0 for a hull (circles have no vertices)
   uint8_t* is_static;
   int32_t* body_world;
   int32_t* body_off;  // W + 1
@@ -132,6 +133,7 @@ struct Ctx {
   uint8_t* pk_f;  // flags per contact (ordered) or manifold (coloured)
   unsigned long long* col_best;  // colouring priorities per body (large worlds)
   uint32_t* col_used;            // colours taken at each body (large worlds)
+  int32_t* col_list;             // 2 x pair_cap: uncoloured manifolds per round (large worlds)
   uint32_t* pair_ab;             // per pair: world-local bodies for the colouring (k_sat)
   int32_t* col_off;   // per world: COLOR_CAP + 3 colour offsets (manifolds)
   int32_t* col_info;  // per world: colours | serial tail << 16 (0: nothing to solve)
@@ -222,6 +224,7 @@ struct LaunchRec {
 struct HostBatch {
   StepPlan P;
   std::vector<double> h[10];  // px py th vx vy w iml imr mu e
+  std::vector<double> rad;    // circle radius, 0 for a hull
   std::vector<uint8_t> is_static, is_large;
   std::vector<int32_t> body_world, vert_off, body_off, order, tab_off, large_off, large_ids;
   std::vector<double> lx, ly, cell;

@@ -180,6 +180,18 @@ __device__ __forceinline__ void integrate_tile(const Ctx& c, int advance, int vb
   }
   __syncthreads();
   if (t < nbk) {
+    const double r = c.rad[g];
+    if (r > 0.0) {
+      // circleToAabb (Aabb.hs:86-88) around the centre, which is the
+      // body's position (setCircleTransform maps the local origin there:
+      // (m00*0 + m01*0) + m02*1 = px, positions never being -0)
+      const double x = c.px[g], y = c.py[g];
+      c.bminx[g] = x - r;
+      c.bmaxx[g] = x + r;
+      c.bminy[g] = y - r;
+      c.bmaxy[g] = y + r;
+      return;
+    }
     const int o = s_voff[t] - v0, e = s_voff[t + 1] - v0;
     double bminx = s_wx[o], bmaxx = s_wx[o], bminy = s_wy[o], bmaxy = s_wy[o];
     for (int i = o + 1; i < e; ++i) {
@@ -345,12 +357,49 @@ __device__ __forceinline__ int total_pairs(const Ctx& c) {
 
 // SAT narrowphase, one lane per pair (generateContacts, Contact.hs:22-40,
 // for two hulls).  Writes up to two contacts into the pair's slots.
+// Contact.generateContacts (Contact.hs:22-40) for shapes (i, j) = (pa, pb)
+// when one is a circle: circle-circle keyed (0, 0) Same; circle-hull
+// (0, f) Same; hull-circle (f, 0) Flip, computed circle against hull.
+__device__ __forceinline__ int circle_pair(const Ctx& c, int ga, int gb, double ra, double rb,
+                                           Cgeom* gm, int* flip) {
+  const d2 ca = mk(c.px[ga], c.py[ga]), cb = mk(c.px[gb], c.py[gb]);
+  *flip = 0;
+  if (ra > 0.0 && rb > 0.0) {
+    if (!circle_contact(ca, ra, cb, rb, gm)) return 0;
+    gm->fa = 0;
+    gm->fb = 0;
+    return 1;
+  }
+  HullV hv;
+  int f = 0;
+  if (ra > 0.0) {
+    load_hull_v(hull_of(c, gb), hv);
+    if (!circle_hull_contact(ca, ra, hv, &f, gm)) return 0;
+    gm->fa = 0;
+    gm->fb = f;
+    return 1;
+  }
+  load_hull_v(hull_of(c, ga), hv);
+  if (!circle_hull_contact(cb, rb, hv, &f, gm)) return 0;
+  gm->fa = f;
+  gm->fb = 0;
+  *flip = 1;
+  return 1;
+}
+
 __device__ __forceinline__ void sat_pair(const Ctx& c, int p) {
-  HullRef A = hull_of(c, c.pa[p]);
-  HullRef B = hull_of(c, c.pb[p]);
+  const int pga = c.pa[p], pgb = c.pb[p];
+  const double ra = c.rad[pga], rb = c.rad[pgb];
   Cgeom gm[2];
   int flip = 0;
-  int n = hull_contacts(A, B, gm, &flip);
+  int n;
+  if (ra > 0.0 || rb > 0.0) {
+    n = circle_pair(c, pga, pgb, ra, rb, gm, &flip);
+  } else {
+    HullRef A = hull_of(c, pga);
+    HullRef B = hull_of(c, pgb);
+    n = hull_contacts(A, B, gm, &flip);
+  }
   for (int k = 0; k < n; ++k) {
     int s = 2 * p + k;
     c.s_feat[s] = (uint16_t)((gm[k].fa << 8) | gm[k].fb);
@@ -966,6 +1015,7 @@ static void batch_buffers(const StepPlan& P, Ctx& c, int32_t** scan_part, std::v
   auto add = [&](auto** ptr, size_t bytes) { out->push_back({(void**)ptr, bytes}); };
   double** body[10] = {&c.px, &c.py, &c.th, &c.vx, &c.vy, &c.w, &c.iml, &c.imr, &c.mu, &c.e};
   for (auto* q : body) add(q, NB * 8);
+  add(&c.rad, NB * 8);
   add(&c.is_static, NB);
   add(&c.body_world, NB * 4);
   add(&c.vert_off, (NB + 1) * 4);
@@ -1045,6 +1095,7 @@ static void batch_buffers(const StepPlan& P, Ctx& c, int32_t** scan_part, std::v
     add(&c.fw, NB * 4);
     add(&c.col_best, NB * 8);
     add(&c.col_used, NB * 4);
+    add(&c.col_list, 2 * PC * 4);
     add(&c.col_off, W * (COLOR_CAP + 3) * 4);
     add(&c.col_info, W * 4);
     add(&c.pair_ab, PC * 4);
@@ -1089,6 +1140,12 @@ shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_c
     if (d.hull_off[0] != 0) return fail(SHP_E_INVAL, "hull_off[0] must be 0");
     for (int i = 0; i < d.n_bodies; ++i) {
       int nv = d.hull_off[i + 1] - d.hull_off[i];
+      const double r = d.radius ? d.radius[i] : 0.0;
+      if (!(r >= 0.0) || r - r != 0.0) return fail(SHP_E_INVAL, "radius must be finite and >= 0");
+      if (r > 0.0) {  // a circle
+        if (nv != 0) return fail(SHP_E_INVAL, "a circle's hull range must be empty");
+        continue;
+      }
       if (nv < 3) return fail(SHP_E_INVAL, "hull with fewer than 3 vertices");
       if (nv > SHP_MAXV) return fail(SHP_E_UNSUPPORTED, "hull with more than 8 vertices");
     }
@@ -1117,6 +1174,7 @@ shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_c
 
   // host staging in SoA, -0 normalised to +0 (static velocities stay as loaded)
   for (auto& v : hb->h) v.assign(NB, 0.0);
+  hb->rad.assign(NB, 0.0);
   hb->body_world.assign(NB, 0);
   hb->vert_off.assign(NB + 1, 0);
   hb->body_off.assign(n_worlds + 1, 0);
@@ -1139,6 +1197,7 @@ shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_c
       hb->h[7][g] = d.inertia[i] == 0.0 ? 0.0 : 1.0 / d.inertia[i];
       hb->h[8][g] = d.mu[i];
       hb->h[9][g] = d.bounce[i];
+      hb->rad[g] = d.radius ? d.radius[i] : 0.0;
       hb->is_static[g] = (hb->h[6][g] == 0.0 && hb->h[7][g] == 0.0) ? 1 : 0;
       hb->body_world[g] = w;
       hb->vert_off[g] = (int32_t)vo;
@@ -1165,7 +1224,7 @@ shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_c
   const int mode = cfg.broadphase;
   P.use_grid = mode == SHP_BROADPHASE_GRID || (mode == SHP_BROADPHASE_AUTO && max_nb > 2048);
   if (P.use_grid) {
-    std::vector<double> rmax(NB, 0.0);
+    std::vector<double> rmax(hb->rad);  // a circle's extent is its radius
     for (long gg = 0; gg < NB; ++gg)
       for (int k = hb->vert_off[gg]; k < hb->vert_off[gg + 1]; ++k)
         rmax[gg] = std::max(rmax[gg], std::sqrt(hb->lx[k] * hb->lx[k] + hb->ly[k] * hb->ly[k]));
@@ -1428,6 +1487,7 @@ shp_status batch_alloc(shp_batch* b, const HostBatch& hb) {
   double** dptr[10] = {&c.px, &c.py, &c.th, &c.vx, &c.vy, &c.w, &c.iml, &c.imr, &c.mu, &c.e};
   for (int k = 0; k < 10; ++k)
     HIPCHK(hipMemcpy(*dptr[k], hb.h[k].data(), NB * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.rad, hb.rad.data(), NB * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.is_static, hb.is_static.data(), NB, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.body_world, hb.body_world.data(), NB * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.vert_off, hb.vert_off.data(), (NB + 1) * 4, hipMemcpyHostToDevice));

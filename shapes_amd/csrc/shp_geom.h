@@ -227,6 +227,150 @@ __device__ __forceinline__ int hull_contacts(const HullRef& A, const HullRef& B,
   return clip_contacts(B, A, oba, true, out);
 }
 
+// ---- circles and GJK ---------------------------------------------------------
+
+// Circle.contact (Contact/Circle.hs:30-57): A penetrated, B penetrating, the
+// normal out of A; a contact when (ra + rb)^2 >= |b - a|^2.
+__device__ __forceinline__ bool circle_contact(d2 a, double ra, d2 b, double rb, Cgeom* g) {
+  const d2 ab = sub(b, a);
+  const double rab = ra + rb;
+  const double ab_sq = ab.x * ab.x + ab.y * ab.y;  // sqLengthV2
+  if (!(rab * rab >= ab_sq)) return false;
+  const double len = __builtin_sqrt(ab_sq);
+  const d2 abn = mk(ab.x / len, ab.y / len);                      // sdivV2
+  const d2 ea = mk(abn.x * ra + a.x, abn.y * ra + a.y);           // edge of A
+  const d2 eb = mk(abn.x * (-rb) + b.x, abn.y * (-rb) + b.y);     // edge of B
+  g->px = (ea.x + eb.x) / 2.0;                                    // midpointP2
+  g->py = (ea.y + eb.y) / 2.0;
+  g->depth = (ra + rb) - len;
+  g->nx = abn.x;
+  g->ny = abn.y;
+  return true;
+}
+
+// support (Contact/ConvexHull.hs:124-128): the first vertex of maximal
+// dot(v, dir)
+__device__ __forceinline__ int gjk_support(const HullV& h, d2 dir) {
+  int best = 0;
+  double bd = dot(mk(h.x[0], h.y[0]), dir);
+#pragma unroll
+  for (int k = 1; k < SHP_MAXV; ++k) {
+    if (k >= h.n) break;
+    const double d = dot(mk(h.x[k], h.y[k]), dir);
+    if (d > bd) {
+      bd = d;
+      best = k;
+    }
+  }
+  return best;
+}
+
+__device__ __forceinline__ bool same_direction(d2 a, d2 b) { return dot(a, b) > 0.0; }
+// crossV2V2 (Linear.hs:135-139) and crosszV2 (:122-125)
+__device__ __forceinline__ d2 cross_v2v2(d2 a, d2 b, d2 c) {
+  const double abz = a.x * b.y - a.y * b.x;
+  return mk(-(abz * c.y), abz * c.x);
+}
+__device__ __forceinline__ d2 crossz(d2 a, double z) { return mk(a.y * z, -(a.x * z)); }
+
+#define GJK_MAX_ITERATIONS 32
+
+// GJK.closestSimplex (Contact/GJK.hs:52-146) of a hull to point o: the
+// simplex's vertex indices, most recently added first; 3 = encloses o.  The
+// reference's loop has no bound; this one keeps the current simplex after
+// GJK_MAX_ITERATIONS (a hull of <= 8 vertices needs a few).
+__device__ __forceinline__ int closest_simplex(const HullV& h, d2 o, int* sv) {
+  int n = 1;
+  sv[0] = 0;
+  d2 d = sub(o, mk(h.x[0], h.y[0]));
+  for (int it = 0; it < GJK_MAX_ITERATIONS; ++it) {
+    const int ai = gjk_support(h, d);
+    // extendSimplex (GJK.hs:70-86): a repeat ends the search
+    if (sv[0] == ai || (n == 2 && sv[1] == ai)) return n;
+    const d2 a = mk(h.x[ai], h.y[ai]);
+    const d2 ao = sub(o, a);
+    if (n == 1) {  // shiftSimplex2 (GJK.hs:95-109)
+      const int bi = sv[0];
+      const d2 ab = sub(mk(h.x[bi], h.y[bi]), a);
+      if (same_direction(ab, ao)) {
+        n = 2;
+        sv[0] = ai;
+        sv[1] = bi;
+        d = cross_v2v2(ab, ao, ab);
+      } else {
+        sv[0] = ai;
+        d = ao;
+      }
+      continue;
+    }
+    // shiftSimplex3 (GJK.hs:111-137) on (a, b, c)
+    const int bi = sv[0], ci = sv[1];
+    const d2 ab = sub(mk(h.x[bi], h.y[bi]), a), ac = sub(mk(h.x[ci], h.y[ci]), a);
+    const double abc = cross(ab, ac);
+    const d2 abcac = zcross(abc, ac);
+    const d2 ababc = crossz(ab, abc);
+    bool star;
+    if (same_direction(abcac, ao)) {
+      if (same_direction(ac, ao)) {
+        sv[0] = ai;
+        sv[1] = ci;
+        d = cross_v2v2(ac, ao, ac);
+        continue;
+      }
+      star = true;
+    } else {
+      star = same_direction(ababc, ao);
+    }
+    if (!star) {  // the triangle encloses o
+      sv[0] = ai;
+      sv[1] = bi;
+      sv[2] = ci;
+      return 3;
+    }
+    if (same_direction(ab, ao)) {
+      sv[0] = ai;
+      sv[1] = bi;
+      d = cross_v2v2(ab, ao, ab);
+    } else {
+      n = 1;
+      sv[0] = ai;
+      d = ao;
+    }
+  }
+  return n;
+}
+
+// CircleVsHull.generateContacts (Contact/CircleVsHull.hs:18-69): the hull
+// feature (the simplex's most recent vertex) and one contact, the circle
+// the penetrator; none for a centre inside the hull or farther than the
+// radius.
+__device__ __forceinline__ bool circle_hull_contact(d2 center, double r, const HullV& h,
+                                                    int* feature, Cgeom* g) {
+  int sv[3];
+  const int n = closest_simplex(h, center, sv);
+  if (n == 3) return false;
+  d2 a = mk(h.x[sv[0]], h.y[sv[0]]);
+  if (n == 2) {  // closestAlong (:135-144)
+    const d2 p = a, q = mk(h.x[sv[1]], h.y[sv[1]]);
+    const d2 ao = sub(center, p);
+    const d2 abn = normalize(sub(q, p));
+    const double t = dot(ao, abn);
+    a = mk(abn.x * t + p.x, abn.y * t + p.y);
+  }
+  // processSimplex_ (:117-133)
+  const d2 ab = sub(center, a);
+  const double ab_sq = ab.x * ab.x + ab.y * ab.y;
+  if (r * r < ab_sq) return false;
+  const double len = __builtin_sqrt(ab_sq);
+  g->nx = -(ab.x / len);
+  g->ny = -(ab.y / len);
+  g->px = a.x;
+  g->py = a.y;
+  g->depth = r - len;
+  *feature = sv[0];
+  return true;
+}
+
 // NonPenetration.jacobian (Constraints/Contact/NonPenetration.hs:34-44)
 __device__ __forceinline__ void np_jacobian(d2 n, d2 p, d2 xa, d2 xb, double* j) {
   d2 na = neg(n);

@@ -48,7 +48,7 @@ class WorldDescC(C.Structure):
     _fields_ = [("n_bodies", C.c_int32)] + [
         (n, _dp) for n in ("px", "py", "theta", "vx", "vy", "omega", "mass",
                            "inertia", "mu", "bounce")
-    ] + [("hull_off", _ip), ("hull_x", _dp), ("hull_y", _dp)]
+    ] + [("hull_off", _ip), ("hull_x", _dp), ("hull_y", _dp), ("radius", _dp)]
 
 
 class ConfigC(C.Structure):
@@ -193,6 +193,10 @@ def _world_descs(worlds: Sequence[WorldDesc]):
         d.hull_off = ho.ctypes.data_as(_ip)
         d.hull_x = _dptr(hx)
         d.hull_y = _dptr(hy)
+        if getattr(w, "radius", None) is not None:
+            r = np.ascontiguousarray(w.radius, dtype=np.float64)
+            keep.append(r)
+            d.radius = _dptr(r)
     return descs, keep
 
 

@@ -14,6 +14,8 @@ from __future__ import annotations
 import math
 from dataclasses import dataclass, field
 
+from typing import Optional
+
 import numpy as np
 
 
@@ -22,7 +24,9 @@ class WorldDesc:
     """Structure-of-arrays description of one world.
 
     mass/inertia of 0 mean infinite (the reference's `toInvMass2`,
-    shapes/src/Physics/Constraint.hs:79-83).
+    shapes/src/Physics/Constraint.hs:79-83).  `radius` (None: every body is
+    a hull) makes body i a circle of radius[i] > 0 (makeCircle,
+    Engine.hs:53-54), whose hull range is then empty.
     """
 
     px: np.ndarray
@@ -38,6 +42,7 @@ class WorldDesc:
     hull_off: np.ndarray  # int32, n + 1
     hull_x: np.ndarray
     hull_y: np.ndarray
+    radius: Optional[np.ndarray] = None
 
     @property
     def n(self) -> int:
@@ -48,7 +53,7 @@ class WorldDesc:
         return int(np.count_nonzero((self.mass != 0) | (self.inertia != 0)))
 
     def copy(self) -> "WorldDesc":
-        return WorldDesc(**{k: v.copy() for k, v in self.__dict__.items()})
+        return WorldDesc(**{k: (None if v is None else v.copy()) for k, v in self.__dict__.items()})
 
 
 @dataclass
@@ -130,6 +135,7 @@ class _Builder:
         self.off = [0]
         self.hx = []
         self.hy = []
+        self.radius = []
 
     def add(self, verts, pos, vel=(0.0, 0.0), omega=0.0, theta=0.0,
             mass=0.0, inertia=0.0, mu=0.5, bounce=0.0):
@@ -142,6 +148,12 @@ class _Builder:
             self.hx.append(x)
             self.hy.append(y)
         self.off.append(len(self.hx))
+        self.radius.append(0.0)
+
+    def add_circle(self, radius, pos, **kw):
+        """A circle of `radius` (makeCircle, Engine.hs:53-54)."""
+        self.add([], pos, **kw)
+        self.radius[-1] = float(radius)
 
     def add_dynamic(self, verts, pos, density=1.0, **kw):
         m, _, inertia = polygon_mass_properties([v[0] for v in verts],
@@ -150,9 +162,11 @@ class _Builder:
 
     def build(self) -> WorldDesc:
         arr = {k: np.asarray(v, dtype=np.float64) for k, v in self.cols.items()}
+        rad = np.asarray(self.radius, dtype=np.float64)
         return WorldDesc(hull_off=np.asarray(self.off, dtype=np.int32),
                          hull_x=np.asarray(self.hx, dtype=np.float64),
-                         hull_y=np.asarray(self.hy, dtype=np.float64), **arr)
+                         hull_y=np.asarray(self.hy, dtype=np.float64),
+                         radius=rad if (rad > 0).any() else None, **arr)
 
 
 # ------------------------------------------------------------------ scenes
@@ -201,6 +215,61 @@ def reference_stacks(dims=(30, 30), spacing=0.0, steps: int = 10) -> Scene:
     cfg = SceneConfig(dt=0.01, gy=-2.0, iterations=2, steps=steps)
     return Scene([b.build()], cfg, name="reference_stacks_30x30",
                  meta={"dims": dims, "spacing": spacing})
+
+
+def balls(dims=(10, 10), diameter: float = 0.5, spacing: float = 0.0, steps: int = 300,
+          iterations: int = 2) -> Scene:
+    """The reference's circle scene Balls.makeScene dims diameter spacing
+    (shapes/src/Physics/Scenes/Balls.hs:16-76): on the static 18x1 floor at
+    (0, -6) (boxFloor', Stacks.hs:19-32), `dims` = (columns, rows) stacks
+    side by side from x = -(diameter * (columns - 1) / 2), alternating a
+    column of circles of diameter `diameter` (mass (2, 1), mu 0.2, bounce
+    0.5: circle', Balls.hs:16-22) and a column of diameter x diameter boxes
+    (mass (2, 1), mu 0.2, bounce 0: box', Stacks.hs:22-28), bottoms at
+    y = -4.5 stepping up by diameter + spacing; constant acceleration
+    (0, -2), ContactBehavior 0.01 0.02 (Stacks.hs:84-88).  Positions
+    accumulate with the same additions as the Haskell `iterate`."""
+    n_w, n_h = dims
+    b = _Builder()
+    b.add(rectangle_vertices(18.0, 1.0), (0.0, -6.0), mu=0.2)
+    left = 0.0 - (diameter * float(n_w - 1) / 2.0)
+    circle = True
+    for _ in range(n_w):
+        y = -4.5
+        for _ in range(n_h):
+            if circle:
+                b.add_circle(diameter / 2.0, (left, y), mass=2.0, inertia=1.0, mu=0.2, bounce=0.5)
+            else:
+                b.add(rectangle_vertices(diameter, diameter), (left, y), mass=2.0, inertia=1.0,
+                      mu=0.2)
+            y = y + (diameter + spacing)
+        left = left + diameter
+        circle = not circle
+    cfg = SceneConfig(dt=0.01, gy=-2.0, iterations=iterations, steps=steps)
+    return Scene([b.build()], cfg, name=f"balls_{n_w}x{n_h}",
+                 meta={"dims": dims, "diameter": diameter, "spacing": spacing})
+
+
+def circle_world(seed: int, count: int = 60, cols: int = 10, spacing: float = 1.2,
+                 circle_frac: float = 0.5) -> WorldDesc:
+    """Circles (radius U[0.2, 0.6], density 1) mixed with random 3-8-gons on
+    a grid above a static floor, with seeded velocities."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    b = _Builder()
+    b.add(rectangle_vertices(cols * spacing + 10.0, 1.0), (0.0, -0.5), mu=0.5)
+    x0 = -(cols - 1) * spacing / 2.0
+    for k in range(count):
+        r, c = divmod(k, cols)
+        pos = (x0 + c * spacing + rng.uniform(-0.1, 0.1), 1.0 + r * spacing)
+        vel = (rng.uniform(-1.0, 1.0), rng.uniform(-1.0, 1.0))
+        if rng.uniform() < circle_frac:
+            rad = rng.uniform(0.2, 0.6)
+            m = np.pi * rad * rad
+            b.add_circle(rad, pos, vel=vel, mass=m, inertia=0.5 * m * rad * rad,
+                         mu=rng.uniform(0.2, 0.8), bounce=rng.uniform(0.0, 0.5))
+        else:
+            b.add_dynamic(random_polygon(rng), pos, vel=vel, mu=rng.uniform(0.2, 0.8))
+    return b.build()
 
 
 def _polygon_field(rng, b: _Builder, count: int, cols: int, spacing: float,

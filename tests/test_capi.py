@@ -169,3 +169,24 @@ def test_plan_scans_cover_their_counts():
         else:
             assert len(scans) == 3
             assert scans[0]["workgroups"] == scans[2]["workgroups"] == -(-n // 8192)
+
+
+def test_circle_descriptions_validated_on_the_host():
+    """A circle must have an empty hull range and a finite radius >= 0
+    (validated before any device work)."""
+    sc = scenes.balls(dims=(2, 2))
+    w = sc.worlds[0]
+    st, msg, p, rows = engine.plan([w], sc.config)
+    assert st == 0, msg
+    assert p["vertices"] == 4 * 3  # the floor and two boxes; circles have none
+    bad = w.copy()
+    bad.radius[0] = 0.3  # the floor keeps its four vertices
+    st, msg, _, _ = engine.plan([bad], sc.config)
+    assert st == 1 and "empty" in msg
+    bad = w.copy()
+    bad.radius[1] = -1.0
+    st, msg, _, _ = engine.plan([bad], sc.config)
+    assert st == 1 and "radius" in msg
+    bad.radius[1] = float("nan")
+    st, msg, _, _ = engine.plan([bad], sc.config)
+    assert st == 1

@@ -400,3 +400,115 @@ def test_reference_grid_broadphase_equals_brute_force(orc, scene):
         assert pa.shape == pb.shape and (pa == pb).all()
     assert len(pa) > 0
     assert (a.bodies().view(np.uint64) == b.bodies().view(np.uint64)).all()
+
+
+# ------------------------------------------------------------ circles and GJK
+
+def test_circle_benchmark_contact(orc):
+    """contact (Circle (-1,0) 1.1) (Circle (1,0) 1.1)
+    (shapes/bench/Physics/Contact/Circle/Benchmark.hs:8-11), derived from
+    Circle.hs:30-57: |ab| = 2, normal (1, 0) out of the first circle, the
+    edges at x = 0.1 and -0.1 give the midpoint (0, 0), depth 2.2 - 2."""
+    cs = orc.shape_contacts(("circle", 1.1, (-1.0, 0.0)), ("circle", 1.1, (1.0, 0.0)))
+    assert cs == [(0, 0, 0, 1.0, 0.0, 0.0, 0.0, (1.1 + 1.1) - 2.0)]
+    # apart by more than the radii: none; touching: depth 0
+    assert orc.shape_contacts(("circle", 1.0, (-1.0, 0.0)), ("circle", 0.9, (1.0, 0.0))) == []
+    t = orc.shape_contacts(("circle", 1.0, (-1.0, 0.0)), ("circle", 1.0, (1.0, 0.0)))
+    assert t == [(0, 0, 0, 1.0, 0.0, 0.0, 0.0, 0.0)]
+
+
+def test_gjk_benchmark_simplices(orc):
+    """closestSimplex (makeOptBox 0 0 1.9 1.9) at (1,1), (0,1), (0,0.9)
+    (shapes/bench/Physics/Contact/GJK/Benchmark.hs:21-28), derived by hand
+    from GJK.hs:52-146: past the corner vertex 0 the search repeats vertex 0
+    (a 1-simplex); above the top edge it extends to vertex 1 and stops on
+    the edge (1, 0); inside, the 2-simplex (2, 0) extends to vertex 1 and
+    the triangle (1, 2, 0) encloses the point."""
+    b = box(0.0, 0.0, 1.9, 1.9)
+    assert orc.closest_simplex(b, (1.0, 1.0)) == [0]
+    assert orc.closest_simplex(b, (0.0, 1.0)) == [1, 0]
+    assert orc.closest_simplex(b, (0.0, 0.9)) == [1, 2, 0]
+
+
+def test_circle_hull_contacts_known_answers(orc):
+    """CircleVsHull (CircleVsHull.hs:18-69) on the same box: a circle of
+    radius 0.5 at (1,1) touches vertex 0 (normal from the centre towards the
+    hull, depth 0.5 - sqrt(0.005)); at (0,1) the closest point of edge
+    (1, 0) is (0, 0.95), depth 0.45, feature 1; a centre inside the hull
+    gives no contact (the reference's 3-simplex case).  Circle as the second
+    shape gives the same contact keyed (feature, 0) and Flip
+    (Contact.hs:35-39)."""
+    b = box(0.0, 0.0, 1.9, 1.9)
+    (c,) = orc.shape_contacts(("circle", 0.5, (1.0, 1.0)), b)
+    dx = 1.0 - 0.95  # the centre minus vertex 0, per axis
+    d = math.sqrt(dx * dx + dx * dx)
+    assert c[:3] == (0, 0, 0)
+    assert c[3:7] == (-(dx / d), -(dx / d), 0.95, 0.95)
+    assert c[7] == 0.5 - d
+    (c,) = orc.shape_contacts(("circle", 0.5, (0.0, 1.0)), b)
+    assert c[:3] == (0, 1, 0) and c[4:7] == (-1.0, 0.0, 0.95)
+    assert abs(c[7] - 0.45) < 1e-15
+    assert orc.shape_contacts(("circle", 0.5, (0.0, 0.9)), b) == []
+    (f,) = orc.shape_contacts(b, ("circle", 0.5, (0.0, 1.0)))
+    assert f[:3] == (1, 0, 1) and f[3:] == c[3:]
+
+
+def _closest_on_hull(verts, o):
+    """Closest point of a convex polygon's boundary to an outside point, by
+    brute force over its edges."""
+    best, bp = None, None
+    n = len(verts)
+    for k in range(n):
+        a, b = np.array(verts[k]), np.array(verts[(k + 1) % n])
+        t = np.clip(np.dot(np.array(o) - a, b - a) / np.dot(b - a, b - a), 0.0, 1.0)
+        p = a + t * (b - a)
+        dd = np.hypot(*(np.array(o) - p))
+        if best is None or dd < best:
+            best, bp = dd, p
+    return best, bp
+
+
+def test_circle_hull_closest_point_bruteforce(orc):
+    """For points outside random convex polygons, GJK's contact point is the
+    polygon's closest boundary point and the depth is r - distance, against
+    a brute-force search over the edges (a wrong simplex shift or
+    projection moves the point)."""
+    rng = np.random.default_rng(3)
+    checked = 0
+    for _ in range(300):
+        verts = [(x + 0.3, y - 0.2) for (x, y) in scenes.random_polygon(rng)]
+        ang = rng.uniform(0, 2 * np.pi)
+        o = (1.6 * np.cos(ang) + rng.uniform(-0.2, 0.2), 1.6 * np.sin(ang) + rng.uniform(-0.2, 0.2))
+        dist, p = _closest_on_hull(verts, o)
+        cs = orc.shape_contacts(("circle", 2.0, o), verts)
+        assert len(cs) == 1
+        (fa, fb, flip, nx, ny, px, py, depth) = cs[0]
+        assert fa == 0 and flip == 0 and 0 <= fb < len(verts)
+        assert np.hypot(px - p[0], py - p[1]) < 1e-9, (verts, o)
+        assert abs(depth - (2.0 - dist)) < 1e-9
+        assert abs(np.hypot(nx, ny) - 1.0) < 1e-12
+        checked += 1
+    assert checked == 300
+
+
+def test_balls_scene_invariants(orc):
+    """The reference's circle scene (Balls.makeScene, Balls.hs:68-71) on the
+    oracle: every contact involving a circle carries a circle feature key 0,
+    and the lagrangians stay in the friction cone with lambda_n >= 0."""
+    sc = scenes.balls(dims=(6, 6))
+    w = sc.worlds[0]
+    e = orc.OracleEngine(w, sc.config)
+    for _ in range(80):
+        e.step(1)
+        c = e.contacts()
+        circ = w.radius > 0
+        for (i, j, fi, fj), fl in zip(c["keys"].tolist(), c["flip"].tolist()):
+            if circ[i]:
+                assert fi == 0
+            if circ[j]:
+                assert fj == 0
+            if circ[i] != circ[j]:
+                assert fl == (0 if circ[i] else 1)
+        lam = c["lam"]
+        assert (lam[:, 0] >= 0).all()
+    assert len(c["keys"]) > 30
