@@ -213,6 +213,17 @@ shp_status shp_mlp_get_params(shp_mlp* m, float* params);
 /* out[2*i + k] = output k of row i (n x 14 features) */
 shp_status shp_mlp_forward(shp_mlp* m, const float* feats, int32_t n,
                            float* out);
+/* No counterpart in the reference.  shp_mlp_forward for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc): d_feats (n x 14 FP32) and d_out (n x 2 FP32) are
+ * device pointers on the handle's GPU, n <= 65536.  The staging lives in
+ * the library's static device scratch (6 MiB per GPU, kept until the process
+ * ends) instead of buffers the handle allocates; the call allocates nothing.
+ * Enqueued on `stream` (a hipStream_t; NULL: the handle's stream) and
+ * returns without waiting.  Every call on a GPU shares the scratch: calls
+ * that overlap on one GPU race. */
+shp_status shp_mlp_forward_noalloc(shp_mlp* m, const float* d_feats,
+                                   int32_t n, float* d_out, void* stream);
 /* Gradient of the mean squared error over n rows w.r.t. every parameter
  * (blob layout) and the loss itself (with a Huber threshold set, the
  * gradient is that of twice the mean Huber loss; the loss reported stays

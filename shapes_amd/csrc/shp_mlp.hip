@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -505,6 +506,7 @@ void mlp_launch_log(shp_batch* b) {
 // ---- training handle -------------------------------------------------------
 
 struct shp_mlp {
+  int device = 0;  // the GPU the handle's weights live on
   hipStream_t stream = nullptr;
   MlpParams p{};
   float *mom = nullptr, *vel = nullptr, *grad = nullptr, *partial = nullptr;
@@ -526,6 +528,36 @@ struct shp_mlp {
 static shp_status mlp_fail(shp_status s, const char* m) {
   shp_set_error(m);
   return s;
+}
+
+// ---- forward pass that allocates nothing (shp_mlp_forward_noalloc) --------
+// Its staging - the padded BF16 input rows and the FP32 output rows of up
+// to MLP_NOALLOC_MAX_ROWS rows (96 B a row, 6 MiB) - lives in this
+// file-scope device array instead of the handle's buffers.  Each GPU holds
+// its own copy until the process ends; every call on a GPU shares it.
+#define MLP_NOALLOC_MAX_ROWS 65536
+static __device__ __align__(256) unsigned char scratch[(size_t)MLP_NOALLOC_MAX_ROWS * 16 * (2 + 4)];
+#define MLP_MAX_DEVICES 64
+static void* g_scratch_ptr[MLP_MAX_DEVICES];  // device address of `scratch` per GPU
+static std::mutex g_scratch_mu;
+
+// The device address of `scratch` on the current GPU, taken once per GPU
+// (its host-side name is not a device address).
+static shp_status scratch_setup(int dev) {
+  if (dev < 0 || dev >= MLP_MAX_DEVICES) return mlp_fail(SHP_E_UNSUPPORTED, "device index");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return SHP_OK;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+    return mlp_fail(SHP_E_HIP, "hipGetSymbolAddress(scratch) failed");
+  g_scratch_ptr[dev] = ptr;
+  return SHP_OK;
+}
+
+// out[2 r + k] = y[16 r + k], k < 2, r < n
+__global__ void k_take_outputs(const float* y, int n, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2 * n) out[i] = y[(size_t)(i >> 1) * 16 + (i & 1)];
 }
 
 #define MCHK(x) \
@@ -588,6 +620,14 @@ int32_t shp_mlp_param_count(void) { return MLP_NPARAM; }
 shp_status shp_mlp_create(const float* params, shp_mlp** out) {
   if (!params || !out) return mlp_fail(SHP_E_INVAL, "null argument");
   shp_mlp* h = new shp_mlp();
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return mlp_fail(SHP_E_HIP, "no HIP device");
+  }
+  if (scratch_setup(h->device) != SHP_OK) {
+    delete h;
+    return SHP_E_HIP;
+  }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return mlp_fail(SHP_E_HIP, "stream");
@@ -656,6 +696,29 @@ shp_status shp_mlp_forward(shp_mlp* h, const float* feats, int32_t n, float* out
     out[2 * r] = y[(size_t)r * 16];
     out[2 * r + 1] = y[(size_t)r * 16 + 1];
   }
+  return SHP_OK;
+}
+
+shp_status shp_mlp_forward_noalloc(shp_mlp* h, const float* d_feats, int32_t n, float* d_out,
+                                   void* stream) {
+  if (!h || n < 0 || (n > 0 && (!d_feats || !d_out))) return mlp_fail(SHP_E_INVAL, "bad argument");
+  if (n > MLP_NOALLOC_MAX_ROWS)
+    return mlp_fail(SHP_E_INVAL, "n exceeds the 65536 rows of the static scratch");
+  if (n == 0) return SHP_OK;
+  int dev = -1;
+  MCHK(hipGetDevice(&dev));
+  if (dev != h->device) return mlp_fail(SHP_E_INVAL, "the current device is not the handle's");
+  void* base = g_scratch_ptr[dev];
+  if (!base) return mlp_fail(SHP_E_HIP, "scratch not set up on this device");
+  uint16_t* X = (uint16_t*)base;                                                    // rows x 16 BF16
+  float* Y = (float*)((unsigned char*)base + (size_t)MLP_NOALLOC_MAX_ROWS * 16 * 2);  // rows x 16
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  const int rows = ((n + TILE - 1) / TILE) * TILE;
+  // the rows n..rows-1 are packed as zeros, so nothing needs clearing
+  k_pack_features<<<(rows * 16 + 255) / 256, 256, 0, s>>>(d_feats, n, rows, X);
+  mlp_launch_forward(h->p, X, rows, Y, s);
+  k_take_outputs<<<(2 * n + 255) / 256, 256, 0, s>>>(Y, n, d_out);
+  MCHK(hipGetLastError());
   return SHP_OK;
 }
 

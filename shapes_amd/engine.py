@@ -96,7 +96,7 @@ EXPORTS = [
     "shp_batch_set_logging", "shp_batch_read_log", "shp_mlp_create",
     "shp_mlp_destroy", "shp_mlp_get_params", "shp_mlp_forward",
     "shp_mlp_gradient", "shp_mlp_set_data", "shp_mlp_train", "shp_mlp_set_loss",
-    "shp_batch_plan",
+    "shp_batch_plan", "shp_mlp_forward_noalloc",
 ]
 
 PHASES = ["reset", "broadphase_count", "scan_pairs", "broadphase_fill", "sat",
@@ -149,6 +149,8 @@ def load_library(path: str = LIB_PATH):
     L.shp_mlp_destroy.argtypes = [C.c_void_p]
     L.shp_mlp_get_params.argtypes = [C.c_void_p, _fp]
     L.shp_mlp_forward.argtypes = [C.c_void_p, _fp, C.c_int32, _fp]
+    L.shp_mlp_forward_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                          C.c_void_p]
     L.shp_mlp_gradient.argtypes = [C.c_void_p, _fp, _fp, C.c_int32, _fp, _fp]
     L.shp_mlp_set_data.argtypes = [C.c_void_p, _fp, _fp, C.c_int64]
     L.shp_mlp_train.argtypes = [C.c_void_p, _ip, C.c_int32, C.c_int32, C.c_float, _fp]
@@ -405,6 +407,12 @@ class Mlp:
         _check(self._L.shp_mlp_forward(self._h, f.ctypes.data_as(_fp), len(f),
                                        out.ctypes.data_as(_fp)))
         return out
+
+    def forward_noalloc(self, d_feats: int, n: int, d_out: int, stream: int = 0):
+        """shp_mlp_forward_noalloc on device addresses (n x 14 features in,
+        n x 2 outputs out), enqueued on `stream` (0: the handle's)."""
+        _check(self._L.shp_mlp_forward_noalloc(self._h, C.c_void_p(d_feats), int(n),
+                                               C.c_void_p(d_out), C.c_void_p(stream or None)))
 
     def gradient(self, feats, targets):
         f = np.ascontiguousarray(feats, dtype=np.float32)
