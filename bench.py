@@ -188,7 +188,11 @@ def kernel_bytes(phase, nb, nv, st, word):
         # radii, normal, bias, lagrangians, masses, materials (~273 B)
         return nb * (13 * word + 4) + nv * 6 * word + C * 273
     if phase == "sat":
-        return P * (8 + 2 * (nv / max(nb, 1)) * (4 * 8 + 2) + 8) + C * 43
+        # per pair its two body indices read, its contact count and (coloured)
+        # its colouring word written; every hull's world vertices, normals
+        # and extent indices read once (FP64 in both modes); per contact
+        # slot its normal, point, depth, feature and flip written
+        return P * (8 + 4 + (4 if word == 4 else 0)) + nv * (4 * 8 + 2) + C * 43
     if phase == "contacts":
         return C * (43 + 2 * 40 + 24) + C * 226
     if phase == "solve_setup":  # manifold packing: contact rows in, records out
