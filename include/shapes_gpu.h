@@ -76,7 +76,23 @@ typedef struct shp_world_desc {
    * circle of that radius (makeCircle, Engine.hs:53-54; Circle.hs:19-25)
    * whose hull range must be empty; radius[i] == 0 keeps it a hull. */
   const double* radius;
+  /* Slots of the world (World.new capacity, World.hs:61-75): 0 means
+   * n_bodies (fromList, :118-124); otherwise >= n_bodies, and slots
+   * [n_bodies, capacity) start empty, available to shp_batch_append. */
+  int32_t capacity;
 } shp_world_desc;
+
+/* One body for shp_batch_append: makePhysicalObj + makeWorldObj as in
+ * shp_world_desc, one body; radius > 0 with n_vertices 0 for a circle,
+ * else radius 0 and 3..8 local-space counter-clockwise vertices. */
+typedef struct shp_body_desc {
+  double px, py, theta, vx, vy, omega;
+  double mass, inertia, mu, bounce;
+  double radius;
+  int32_t n_vertices;
+  const double* hull_x;
+  const double* hull_y;
+} shp_body_desc;
 
 /* EngineConfig + ContactBehavior (Engine/Main.hs:34-37, Contact/Types.hs:
  * 20-25), the constant-acceleration External (World/External.hs:23-28)
@@ -171,14 +187,49 @@ shp_status shp_batch_read_contacts(shp_batch* b, int32_t world,
                                    int32_t* flip, double* geom, double* lam,
                                    int32_t* warm);
 /* World-space vertices and unit edge normals (4 doubles per vertex, CSR
- * order) as left by moveShapes (World.hs:136-140). */
+ * order) as left by moveShapes (World.hs:136-140); shp_batch_num_vertices
+ * gives the world's vertex count (the hulls' sizes, slot by slot). */
 shp_status shp_batch_read_hulls(shp_batch* b, int32_t world, double* out);
+shp_status shp_batch_num_vertices(const shp_batch* b, int32_t world,
+                                  int32_t* n);
 shp_status shp_batch_last_stats(shp_batch* b, shp_step_stats* out);
 
 /* Replace `world`'s warm-start cache (keys descending, 4 ints each;
  * lam: 2 doubles each). */
 shp_status shp_batch_set_cache(shp_batch* b, int32_t world, int32_t count,
                                const int32_t* keys, const double* lam);
+
+/* ---- world mutation between steps (World.hs:77-91 over
+ * Utils/EmptiesVector.hs:45-85) ------------------------------------------
+ * Each world's slots keep the reference's free list: delete pushes the slot
+ * onto the list of empty slots, append takes the most recently emptied
+ * slot (or the next never-filled one).  Empty slots are skipped by every
+ * pass of the step (no acceleration, pairs, advance or placement), keep
+ * their last state (read back by shp_batch_read_bodies), and the warm-start
+ * cache is kept as it is, so a body appended into a slot emptied since the
+ * last step can reuse that slot's cached lagrangians, as in the reference. */
+
+/* World.append (World.hs:77-84, EmptiesVector.append :70-85): the body into
+ * the head empty slot, placed by moveShape (World.hs:142-149); *out_slot is
+ * its slot.  SHP_E_INVAL when the world has no empty slot (the reference's
+ * out-of-bounds read).  A body with as many vertices as the slot's previous
+ * occupant (and, with the grid broadphase, no wider than the world's cells)
+ * is written in place; otherwise the batch's arrays are laid out again
+ * (the cost of a batch creation; state, warm-start caches, run statistics,
+ * model and training log carried over; the last step's pairs and contacts
+ * read back empty until the next step). */
+shp_status shp_batch_append(shp_batch* b, int32_t world,
+                            const shp_body_desc* body, int32_t* out_slot);
+/* World.delete = EmptiesVector.safeDelete (World.hs:86-87,
+ * EmptiesVector.hs:57-68): SHP_E_INVAL when the slot is already empty. */
+shp_status shp_batch_delete(shp_batch* b, int32_t world, int32_t slot);
+/* World.filled (World.hs:90-91): the number of filled slots. */
+shp_status shp_batch_filled(const shp_batch* b, int32_t world,
+                            int32_t* out_filled);
+/* EmptiesVector.read (EmptiesVector.hs:87-92) of every slot: 1 filled,
+ * 0 empty (shp_batch_num_bodies entries). */
+shp_status shp_batch_read_slots(const shp_batch* b, int32_t world,
+                                uint8_t* out_filled);
 
 /* ---- learned warm start (no reference counterpart: the reference always
  * zero-starts new contacts, Solvers/Contact.hs:95) ------------------------ */

@@ -60,8 +60,15 @@ def lib():
             build()
         L = C.CDLL(LIB_PATH)
         L.orc_engine_create.restype = C.c_void_p
+        L.orc_engine_append.restype = C.c_int32
+        L.orc_engine_append.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int32, _dp, _dp]
+        L.orc_engine_delete.restype = C.c_int32
+        L.orc_engine_delete.argtypes = [C.c_void_p, C.c_int32]
+        L.orc_engine_filled.restype = C.c_int32
+        L.orc_engine_filled.argtypes = [C.c_void_p]
+        L.orc_engine_read_slots.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
         L.orc_engine_create.argtypes = [C.c_int32] + [_dp] * 10 + [_ip, _dp, _dp, _dp,
-                                                                  C.POINTER(Params)]
+                                                                  C.POINTER(Params), C.c_int32]
         L.orc_engine_destroy.argtypes = [C.c_void_p]
         L.orc_engine_step.argtypes = [C.c_void_p, C.c_int32]
         for f in ("orc_engine_read_bodies", "orc_engine_read_hulls",
@@ -123,15 +130,38 @@ class OracleEngine:
         keep += [ho, hx, hy]
         prm = Params(config.dt, config.baumgarte, config.slop, config.gx,
                      config.gy, config.iterations, trig, solver, world_index, broadphase)
-        self.n = world.n
+        capacity = int(getattr(world, "capacity", 0) or 0)
+        self.n = max(world.n, capacity)  # slots
         rad = getattr(world, "radius", None)
         if rad is not None:
             rad, radp = _d(rad)
             keep.append(rad)
         else:
             radp = None
-        self._h = L.orc_engine_create(world.n, *args, hop, hxp, hyp, radp, C.byref(prm))
+        self._h = L.orc_engine_create(world.n, *args, hop, hxp, hyp, radp, C.byref(prm),
+                                      capacity)
         self._L = L
+
+    def append(self, px, py, theta=0.0, vx=0.0, vy=0.0, omega=0.0, mass=1.0, inertia=1.0,
+               mu=0.5, bounce=0.0, hull=None, radius=0.0) -> int:
+        """World.append; returns the slot, or -1 when the world is full."""
+        b10, b10p = _d([px, py, theta, vx, vy, omega, mass, inertia, mu, bounce])
+        hx, hxp = _d([v[0] for v in hull] if hull is not None else [0.0])
+        hy, hyp = _d([v[1] for v in hull] if hull is not None else [0.0])
+        return self._L.orc_engine_append(self._h, b10p, radius,
+                                         len(hull) if hull is not None else 0, hxp, hyp)
+
+    def delete(self, slot: int) -> int:
+        """World.delete (safeDelete); -1 when the slot is already empty."""
+        return self._L.orc_engine_delete(self._h, slot)
+
+    def filled(self) -> int:
+        return self._L.orc_engine_filled(self._h)
+
+    def slots(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.uint8)
+        self._L.orc_engine_read_slots(self._h, out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return out.astype(bool)
 
     def close(self):
         if self._h:

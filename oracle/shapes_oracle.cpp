@@ -663,6 +663,10 @@ struct orc_engine {
   std::vector<Phys> objs;
   std::vector<Material> mats;
   std::vector<Hull> hulls;
+  // EmptiesVector (Utils/EmptiesVector.hs:17-22): ev[s] == -1 for a filled
+  // slot, else the next empty slot (the capacity ends the list)
+  std::vector<int> ev;
+  int ev_head = 0, ev_filled = 0;
   std::vector<std::pair<int, int>> pairs;  // last step
   std::vector<int> pair_first;             // pairs + 1: first contact of each pair
   std::vector<Contact> contacts;           // last step == the cache
@@ -737,18 +741,27 @@ void culled_keys_grid(const std::vector<Aabb>& boxes, const std::vector<char>& s
   out.erase(std::unique(out.begin(), out.end()), out.end());
 }
 
+inline bool filled(const orc_engine* e, int i) { return e->ev[i] == -1; }
+
+// toTaggedAabbs (Aabb.hs:140-148) lists the filled slots only (E.mapM),
+// keyed by slot: the pairs are found among them and named by their slots.
 void culled_keys(orc_engine* e, std::vector<std::pair<int, int>>& out) {
   int n = (int)e->objs.size();
-  std::vector<Aabb> boxes(n);
-  std::vector<char> st(n);
-  for (int i = 0; i < n; ++i) {
-    boxes[i] = hull_aabb(e->hulls[i]);
-    st[i] = is_static(e->objs[i]);
+  std::vector<int> slot;
+  for (int i = 0; i < n; ++i)
+    if (filled(e, i)) slot.push_back(i);
+  const int m = (int)slot.size();
+  std::vector<Aabb> boxes(m);
+  std::vector<char> st(m);
+  for (int k = 0; k < m; ++k) {
+    boxes[k] = hull_aabb(e->hulls[slot[k]]);
+    st[k] = is_static(e->objs[slot[k]]);
   }
   if (e->prm.broadphase == ORC_BROADPHASE_BRUTE)
     culled_keys_brute(boxes, st, out);
   else
     culled_keys_grid(boxes, st, out);
+  for (auto& p : out) p = std::make_pair(slot[p.first], slot[p.second]);
 }
 
 // prepareFrame (Solvers/Contact.hs:40-52) + keyedContacts
@@ -1032,7 +1045,7 @@ void solve_coloured(orc_engine* e, std::vector<Contact>& cs) {
         solve_manifold_f32(e, cs, e->pair_first[q], e->pair_first[q + 1], pass, vf);
   for (int i = 0; i < n; ++i) {
     Phys& o = e->objs[i];
-    if (is_static(o)) continue;
+    if (is_static(o) || !filled(e, i)) continue;
     o.vel = v2(o.vel.x + (double)(vf[3 * i] - (float)o.vel.x),
                o.vel.y + (double)(vf[3 * i + 1] - (float)o.vel.y));
     o.w = o.w + (double)(vf[3 * i + 2] - (float)o.w);
@@ -1100,7 +1113,7 @@ static void step_once(orc_engine* e) {
   // 2. applyExternal constantAccel (World.hs:156-158, External.hs:23-28)
   for (int i = 0; i < n; ++i) {
     Phys& o = e->objs[i];
-    if (!is_static_lin(o)) o.vel = add(o.vel, v2(p.gx * p.dt, p.gy * p.dt));
+    if (filled(e, i) && !is_static_lin(o)) o.vel = add(o.vel, v2(p.gx * p.dt, p.gy * p.dt));
   }
   // 3. prepareFrame
   std::vector<Contact> cs;
@@ -1119,14 +1132,16 @@ static void step_once(orc_engine* e) {
   }
   compute_stats(e);
   residuals(e);
-  // 6. advance (World.hs:167-169, Constraint.hs:225-229)
+  // 6. advance (World.hs:167-169, Constraint.hs:225-229), filled slots
   for (int i = 0; i < n; ++i) {
+    if (!filled(e, i)) continue;
     Phys& o = e->objs[i];
     o.pos = add(v2(p.dt * o.vel.x, p.dt * o.vel.y), o.pos);
     o.th = (p.dt * o.w) + o.th;
   }
-  // 7. moveShapes (World.hs:136-140)
-  for (int i = 0; i < n; ++i) set_hull_transform(e->hulls[i], e->objs[i], p.trig);
+  // 7. moveShapes (World.hs:136-140), filled slots
+  for (int i = 0; i < n; ++i)
+    if (filled(e, i)) set_hull_transform(e->hulls[i], e->objs[i], p.trig);
 }
 
 extern "C" {
@@ -1138,7 +1153,7 @@ orc_engine* orc_engine_create(int32_t n, const double* px, const double* py,
                               const double* mu, const double* bounce,
                               const int32_t* hull_off, const double* hull_x,
                               const double* hull_y, const double* radius,
-                              const orc_params* prm) {
+                              const orc_params* prm, int32_t capacity) {
   orc_engine* e = new orc_engine();
   e->prm = *prm;
   e->objs.resize(n);
@@ -1168,7 +1183,73 @@ orc_engine* orc_engine_create(int32_t n, const double* px, const double* py,
     // makeWorldObj places the shape with moveShape (World.hs:142-149)
     set_hull_transform(e->hulls[i], o, prm->trig);
   }
+  // World.new capacity + fromList's appends (World.hs:61-75,118-124,
+  // EmptiesVector.hs:27-42,70-85): slots [0, n) filled, [n, capacity) empty
+  const int cap = capacity > n ? capacity : n;
+  e->objs.resize(cap, Phys{});
+  e->mats.resize(cap, Material{0.0, 0.0});
+  e->hulls.resize(cap);
+  e->ev.assign(cap, -1);
+  for (int s = n; s < cap; ++s) {
+    e->ev[s] = s + 1;
+    e->hulls[s].n = 0;
+    e->hulls[s].radius = 0.0;
+  }
+  e->ev_head = n;
+  e->ev_filled = n;
   return e;
+}
+
+// World.append (World.hs:77-84) over EmptiesVector.append
+// (EmptiesVector.hs:70-85): the head empty slot takes the object, placed by
+// moveShape (makeWorldObj, World.hs:142-149); its successor becomes the
+// head.  Returns the slot, or -1 when the world is full (the reference's
+// out-of-bounds read of the vector).
+int32_t orc_engine_append(orc_engine* e, const double* b10, double radius, int32_t nv,
+                          const double* hx, const double* hy) {
+  const int cap = (int)e->ev.size();
+  const int slot = e->ev_head;
+  if (slot >= cap || e->ev[slot] == -1) return -1;
+  Phys o;
+  o.vel = v2(nz(b10[3]), nz(b10[4]));
+  o.w = nz(b10[5]);
+  o.pos = v2(nz(b10[0]), nz(b10[1]));
+  o.th = nz(b10[2]);
+  o.iml = invert_mass(b10[6]);
+  o.imr = invert_mass(b10[7]);
+  e->objs[slot] = o;
+  e->mats[slot] = Material{b10[8], b10[9]};
+  if (radius > 0.0) {
+    e->hulls[slot] = Hull();
+    e->hulls[slot].n = 0;
+    e->hulls[slot].radius = radius;
+  } else {
+    std::vector<V2> vs;
+    for (int k = 0; k < nv; ++k) vs.push_back(v2(hx[k], hy[k]));
+    e->hulls[slot] = list_to_hull(vs);
+  }
+  set_hull_transform(e->hulls[slot], o, e->prm.trig);
+  e->ev_head = e->ev[slot];
+  e->ev[slot] = -1;
+  e->ev_filled += 1;
+  return slot;
+}
+
+// World.delete = EmptiesVector.safeDelete (World.hs:86-87,
+// EmptiesVector.hs:57-68): 0, or -1 when the slot is already empty.
+int32_t orc_engine_delete(orc_engine* e, int32_t slot) {
+  if (slot < 0 || slot >= (int)e->ev.size() || e->ev[slot] != -1) return -1;
+  e->ev[slot] = e->ev_head;
+  e->ev_head = slot;
+  e->ev_filled -= 1;
+  return 0;
+}
+
+int32_t orc_engine_filled(const orc_engine* e) { return e->ev_filled; }
+
+// EmptiesVector.read (EmptiesVector.hs:87-92) of every slot
+void orc_engine_read_slots(const orc_engine* e, uint8_t* out) {
+  for (size_t s = 0; s < e->ev.size(); ++s) out[s] = e->ev[s] == -1 ? 1 : 0;
 }
 
 void orc_engine_destroy(orc_engine* e) { delete e; }
@@ -1208,6 +1289,10 @@ void orc_engine_read_hulls(const orc_engine* e, double* out) {
 
 void orc_engine_read_aabbs(const orc_engine* e, double* out) {
   for (size_t i = 0; i < e->hulls.size(); ++i) {
+    if (e->hulls[i].n == 0 && !(e->hulls[i].radius > 0.0)) {  // never filled
+      for (int k = 0; k < 4; ++k) out[4 * i + k] = NAN;
+      continue;
+    }
     Aabb a = hull_aabb(e->hulls[i]);
     out[4 * i + 0] = a.minx;
     out[4 * i + 1] = a.maxx;

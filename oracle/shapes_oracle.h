@@ -49,7 +49,8 @@ typedef struct orc_params {
 } orc_params;
 
 /* One world.  hull_off has n+1 entries (CSR into hull_x/hull_y); vertices
- * are local-space, counter-clockwise.  mass/inertia 0 means infinite. */
+ * are local-space, counter-clockwise.  mass/inertia 0 means infinite.
+ * capacity > n adds empty slots [n, capacity) (World.new capacity). */
 orc_engine* orc_engine_create(int32_t n, const double* px, const double* py,
                               const double* theta, const double* vx,
                               const double* vy, const double* omega,
@@ -57,7 +58,16 @@ orc_engine* orc_engine_create(int32_t n, const double* px, const double* py,
                               const double* mu, const double* bounce,
                               const int32_t* hull_off, const double* hull_x,
                               const double* hull_y, const double* radius,
-                              const orc_params* prm);
+                              const orc_params* prm, int32_t capacity);
+/* World.append: b10 = px, py, theta, vx, vy, omega, mass, inertia, mu,
+ * bounce; a circle when radius > 0 (nv 0).  Returns the slot or -1 (full). */
+int32_t orc_engine_append(orc_engine* e, const double* b10, double radius, int32_t nv,
+                          const double* hx, const double* hy);
+/* World.delete (safeDelete): 0, or -1 when the slot is already empty. */
+int32_t orc_engine_delete(orc_engine* e, int32_t slot);
+int32_t orc_engine_filled(const orc_engine* e);
+/* 1 per filled slot, 0 per empty one (capacity entries) */
+void orc_engine_read_slots(const orc_engine* e, uint8_t* out);
 void orc_engine_destroy(orc_engine* e);
 void orc_engine_step(orc_engine* e, int32_t steps);
 

@@ -82,6 +82,7 @@ struct Ctx {
   double *px, *py, *th, *vx, *vy, *w, *iml, *imr, *mu, *e;
   double* rad;  // circle radius, 0 for a hull (circles have no vertices)
   uint8_t* is_static;
+  uint8_t* dead;  // 1: an empty slot (World.delete), skipped by every body pass
   int32_t* body_world;
   int32_t* body_off;  // W + 1
   int32_t* world_order;  // W: worlds by descending body count (per-world workgroups)
@@ -225,13 +226,14 @@ struct HostBatch {
   StepPlan P;
   std::vector<double> h[10];  // px py th vx vy w iml imr mu e
   std::vector<double> rad;    // circle radius, 0 for a hull
-  std::vector<uint8_t> is_static, is_large;
+  std::vector<uint8_t> is_static, is_large, dead;
   std::vector<int32_t> body_world, vert_off, body_off, order, tab_off, large_off, large_ids;
   std::vector<double> lx, ly, cell;
 };
 
 shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_config& cfg,
                          HostBatch* hb);
+shp_status plan_staged(const shp_config& cfg, HostBatch* hb);
 void plan_launches(const StepPlan& P, bool infer, bool logging, std::vector<LaunchRec>* out);
 size_t mlp_infer_static_lds();
 // every launch within the hardware's limits and every 32-bit index in range
@@ -244,8 +246,17 @@ struct shp_batch {
   Ctx ctx{};
   StepPlan plan{};
   std::vector<int32_t> h_body_off, h_vert_off;
+  std::vector<double> h_cell;       // grid cell size per world (grid broadphase)
+  std::vector<uint8_t> h_is_large;  // bodies tested explicitly by the grid
+  // per world, the slots' free list (Utils/EmptiesVector.hs:17-22): ev_next[s]
+  // is -1 for a filled slot, else the empty slot after s (capacity ends the
+  // list); ev_head is the first empty slot, ev_filled the filled count
+  std::vector<std::vector<int32_t>> ev_next;
+  std::vector<int32_t> ev_head, ev_filled;
   int32_t* scan_part = nullptr;  // per-workgroup sums of the multi-workgroup scans
   long device_bytes = 0;         // allocated so far
+  size_t table_allocs = 0;       // allocs[0, table_allocs) are the plan's buffers
+  long table_bytes = 0;          // ... of this many bytes
   // the coloured solver's colouring runs on side_stream, concurrently with
   // the contact generation on `stream` (fork and join through the events)
   hipStream_t side_stream = nullptr;

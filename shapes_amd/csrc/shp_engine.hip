@@ -111,6 +111,11 @@ __global__ void k_init_hulls(Ctx c) {
 // moveShapes (World.hs:136-140): T(pos) R(theta) applied to the local
 // vertices (Transform.hs:34-38, Linear.hs:217-220), unit edge normals
 // (ConvexHull.hs:184-195) and the AABB fold (Broadphase/Aabb.hs:81-110).
+// With `advance`, empty slots are skipped, as E.mapM_ skips them
+// (World.hs:137,168): their state and shape stay as they were when the slot
+// was emptied.  Without (placement at creation or after a mutation), every
+// slot is placed from its state, which for an empty slot reproduces the
+// shape it was left with.
 // A workgroup takes 256 consecutive bodies: one lane per body builds the
 // transform, one lane per vertex transforms it and its successor and writes
 // the vertex and edge normal (coalesced), and one lane per body folds its
@@ -124,14 +129,17 @@ __device__ __forceinline__ void integrate_tile(const Ctx& c, int advance, int vb
   __shared__ double s_wx[INTEGRATE_VERTS], s_wy[INTEGRATE_VERTS];
   __shared__ uint8_t s_body[INTEGRATE_VERTS];
   __shared__ int s_voff[INTEGRATE_THREADS + 1];
+  __shared__ uint8_t s_dead[INTEGRATE_THREADS];
   const int t = threadIdx.x;
   const int g0 = vb * INTEGRATE_THREADS;
   const int nbk = min(INTEGRATE_THREADS, c.NB - g0);
   const int g = g0 + t;
   const double one = 1.0, zer = 0.0;
   if (t < nbk) {
+    const bool dead = c.dead[g] != 0;
+    s_dead[t] = dead && advance;
     double px = c.px[g], py = c.py[g], th = c.th[g];
-    if (advance) {
+    if (advance && !dead) {
       double dt = c.dt;
       px = (dt * c.vx[g]) + px;
       py = (dt * c.vy[g]) + py;
@@ -151,7 +159,7 @@ __device__ __forceinline__ void integrate_tile(const Ctx& c, int advance, int vb
     s_voff[t] = c.vert_off[g];
     if (t == nbk - 1) s_voff[nbk] = c.vert_off[g + 1];
     double chk = px + py + th + c.vx[g] + c.vy[g] + c.w[g];
-    if (!(chk - chk == 0.0)) atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_NONFINITE);
+    if (!dead && !(chk - chk == 0.0)) atomicOr(&c.counters[CNT_FLAGS], SHP_FLAG_NONFINITE);
   }
   __syncthreads();
   const int v0 = s_voff[0], nv = s_voff[nbk] - v0;
@@ -160,6 +168,7 @@ __device__ __forceinline__ void integrate_tile(const Ctx& c, int advance, int vb
   __syncthreads();
   for (int i = t; i < nv; i += INTEGRATE_THREADS) {
     const int b = s_body[i];
+    if (s_dead[b]) continue;
     const int last = s_voff[b + 1] - v0 - 1;
     const int i1 = i < last ? i + 1 : s_voff[b] - v0;
     const double m00 = s_m[0][b], m01 = s_m[1][b], m02 = s_m[2][b];
@@ -180,6 +189,7 @@ __device__ __forceinline__ void integrate_tile(const Ctx& c, int advance, int vb
   }
   __syncthreads();
   if (t < nbk) {
+    if (s_dead[t]) return;
     const double r = c.rad[g];
     if (r > 0.0) {
       // circleToAabb (Aabb.hs:86-88) around the centre, which is the
@@ -193,6 +203,7 @@ __device__ __forceinline__ void integrate_tile(const Ctx& c, int advance, int vb
       return;
     }
     const int o = s_voff[t] - v0, e = s_voff[t + 1] - v0;
+    if (e == o) return;  // no shape: a slot never filled since creation
     double bminx = s_wx[o], bmaxx = s_wx[o], bminy = s_wy[o], bmaxy = s_wy[o];
     for (int i = o + 1; i < e; ++i) {
       const double x = s_wx[i], y = s_wy[i];
@@ -315,7 +326,9 @@ __global__ void k_broadphase(Ctx c, int fill) {
   int li = g - b0;
   int nw = c.body_off[w + 1] - b0;
   int rank = b0 + (nw - 1 - li);
-  if (!fill && lane == 0 && c.iml[g] != 0.0) {
+  // an empty slot (World.delete) gets no acceleration and no pairs
+  const bool dead = c.dead[g] != 0;
+  if (!fill && lane == 0 && c.iml[g] != 0.0 && !dead) {
     c.vx[g] = c.vx[g] + c.gx * c.dt;
     c.vy[g] = c.vy[g] + c.gy * c.dt;
   }
@@ -323,12 +336,12 @@ __global__ void k_broadphase(Ctx c, int fill) {
   double aminx = c.bminx[g], amaxx = c.bmaxx[g], aminy = c.bminy[g], amaxy = c.bmaxy[g];
   int count = 0;
   int base = fill ? c.off_rank[rank] : 0;
-  for (int j0 = li - 1; j0 >= 0; j0 -= 64) {
+  for (int j0 = dead ? -1 : li - 1; j0 >= 0; j0 -= 64) {
     int lj = j0 - lane;
     bool hit = false;
     if (lj >= 0) {
       int gj = b0 + lj;
-      if (!(si && body_static(c, gj))) {
+      if (!(si && body_static(c, gj)) && !c.dead[gj]) {
         // aabbCheck (Aabb.hs:69-78), inclusive
         double bminx = c.bminx[gj], bmaxx = c.bmaxx[gj];
         double bminy = c.bminy[gj], bmaxy = c.bmaxy[gj];
@@ -999,7 +1012,8 @@ static shp_status accumulate_timing(shp_batch* b) {
 // static LDS of the engine's kernels (the __shared__ arrays they declare)
 constexpr size_t FINISH_STATIC_LDS =
     lds16(6 * INTEGRATE_THREADS * 8) + 2 * lds16(INTEGRATE_VERTS * 8) + lds16(INTEGRATE_VERTS) +
-    lds16((INTEGRATE_THREADS + 1) * 4) + lds16(3 * (QUALITY_THREADS / 64) * 8);
+    lds16((INTEGRATE_THREADS + 1) * 4) + lds16(INTEGRATE_THREADS) +
+    lds16(3 * (QUALITY_THREADS / 64) * 8);
 // 16 lanes per body in the grid's pair passes up to this many bodies
 #define GRID_WIDE_MAX_BODIES 131072
 
@@ -1017,6 +1031,7 @@ static void batch_buffers(const StepPlan& P, Ctx& c, int32_t** scan_part, std::v
   for (auto* q : body) add(q, NB * 8);
   add(&c.rad, NB * 8);
   add(&c.is_static, NB);
+  add(&c.dead, NB);
   add(&c.body_world, NB * 4);
   add(&c.vert_off, (NB + 1) * 4);
   add(&c.body_off, (W + 1) * 4);
@@ -1127,13 +1142,16 @@ shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_c
       !(cfg.solver_threads == 1024 && cfg.solver == SHP_SOLVER_COLORED_F32))
     return fail(SHP_E_INVAL, "solver_threads must be 0, 256 or 512 (or 1024, coloured solver)");
   if (cfg.max_pairs_per_body < 0) return fail(SHP_E_INVAL, "max_pairs_per_body must be >= 0");
-  long NB = 0, NV = 0, max_nb = 0;
+  long NB = 0, NV = 0;
   for (int w = 0; w < n_worlds; ++w) {
     const shp_world_desc& d = worlds[w];
     if (d.n_bodies <= 0) return fail(SHP_E_INVAL, "world with no bodies");
-    if (d.n_bodies >= (1 << 24)) return fail(SHP_E_UNSUPPORTED, "world exceeds 2^24 bodies");
-    if (cfg.solver == SHP_SOLVER_COLORED_F32 && d.n_bodies >= (int)NO_BODY)
-      return fail(SHP_E_UNSUPPORTED, "world exceeds 65534 bodies (coloured solver)");
+    if (d.capacity != 0 && d.capacity < d.n_bodies)
+      return fail(SHP_E_INVAL, "capacity below the number of bodies");
+    const long slots = std::max(d.capacity, d.n_bodies);
+    if (slots >= (1 << 24)) return fail(SHP_E_UNSUPPORTED, "world exceeds 2^24 slots");
+    if (cfg.solver == SHP_SOLVER_COLORED_F32 && slots >= (long)NO_BODY)
+      return fail(SHP_E_UNSUPPORTED, "world exceeds 65534 slots (coloured solver)");
     if (!d.px || !d.py || !d.theta || !d.vx || !d.vy || !d.omega || !d.mass || !d.inertia ||
         !d.mu || !d.bounce || !d.hull_off || !d.hull_x || !d.hull_y)
       return fail(SHP_E_INVAL, "null array in world description");
@@ -1149,11 +1167,67 @@ shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_c
       if (nv < 3) return fail(SHP_E_INVAL, "hull with fewer than 3 vertices");
       if (nv > SHP_MAXV) return fail(SHP_E_UNSUPPORTED, "hull with more than 8 vertices");
     }
-    NB += d.n_bodies;
+    NB += slots;
     NV += d.hull_off[d.n_bodies];
-    max_nb = std::max<long>(max_nb, d.n_bodies);
+    if (NB >= (1L << 30) || NV >= (1L << 30)) return fail(SHP_E_UNSUPPORTED, "too many bodies");
   }
-  if (NB >= (1L << 30) || NV >= (1L << 30)) return fail(SHP_E_UNSUPPORTED, "too many bodies");
+
+  // host staging in SoA, -0 normalised to +0 (static velocities stay as
+  // loaded); slots [n_bodies, capacity) start empty, with no vertices
+  for (auto& v : hb->h) v.assign(NB, 0.0);
+  hb->rad.assign(NB, 0.0);
+  hb->body_world.assign(NB, 0);
+  hb->vert_off.assign(NB + 1, 0);
+  hb->body_off.assign(n_worlds + 1, 0);
+  hb->is_static.assign(NB, 0);
+  hb->dead.assign(NB, 0);
+  hb->lx.assign(NV, 0.0);
+  hb->ly.assign(NV, 0.0);
+  long g = 0, vo = 0;
+  for (int w = 0; w < n_worlds; ++w) {
+    const shp_world_desc& d = worlds[w];
+    hb->body_off[w] = (int32_t)g;
+    const int slots = std::max(d.capacity, d.n_bodies);
+    for (int i = 0; i < slots; ++i, ++g) {
+      hb->body_world[g] = w;
+      hb->vert_off[g] = (int32_t)vo;
+      if (i >= d.n_bodies) {
+        hb->dead[g] = 1;
+        hb->is_static[g] = 1;
+        continue;
+      }
+      hb->h[0][g] = nz(d.px[i]);
+      hb->h[1][g] = nz(d.py[i]);
+      hb->h[2][g] = nz(d.theta[i]);
+      hb->h[3][g] = nz(d.vx[i]);
+      hb->h[4][g] = nz(d.vy[i]);
+      hb->h[5][g] = nz(d.omega[i]);
+      // toInvMass2 (Constraint.hs:79-83)
+      hb->h[6][g] = d.mass[i] == 0.0 ? 0.0 : 1.0 / d.mass[i];
+      hb->h[7][g] = d.inertia[i] == 0.0 ? 0.0 : 1.0 / d.inertia[i];
+      hb->h[8][g] = d.mu[i];
+      hb->h[9][g] = d.bounce[i];
+      hb->rad[g] = d.radius ? d.radius[i] : 0.0;
+      hb->is_static[g] = (hb->h[6][g] == 0.0 && hb->h[7][g] == 0.0) ? 1 : 0;
+      for (int k = d.hull_off[i]; k < d.hull_off[i + 1]; ++k, ++vo) {
+        hb->lx[vo] = d.hull_x[k];
+        hb->ly[vo] = d.hull_y[k];
+      }
+    }
+  }
+  hb->body_off[n_worlds] = (int32_t)g;
+  hb->vert_off[NB] = (int32_t)vo;
+  return plan_staged(cfg, hb);
+}
+
+// The plan of a staged batch: capacities, world order, grid set-up, scans,
+// launch sizes and their audit.
+shp_status plan_staged(const shp_config& cfg, HostBatch* hb) {
+  const int n_worlds = (int)hb->body_off.size() - 1;
+  const long NB = hb->body_off[n_worlds], NV = hb->vert_off[NB];
+  long max_nb = 0;
+  for (int w = 0; w < n_worlds; ++w)
+    max_nb = std::max<long>(max_nb, hb->body_off[w + 1] - hb->body_off[w]);
   StepPlan& P = hb->P;
   P = StepPlan{};
   P.NB = NB;
@@ -1171,44 +1245,6 @@ shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_c
     return fail(SHP_E_UNSUPPORTED, "pair capacity too large for the coloured solver (2^28)");
   P.contact_cap = 2 * P.pair_cap;
   P.miss_cap = cfg.warm_start == SHP_WARM_LEARNED ? std::min(2 * P.pair_cap, 4 * NB + 4096) : 0;
-
-  // host staging in SoA, -0 normalised to +0 (static velocities stay as loaded)
-  for (auto& v : hb->h) v.assign(NB, 0.0);
-  hb->rad.assign(NB, 0.0);
-  hb->body_world.assign(NB, 0);
-  hb->vert_off.assign(NB + 1, 0);
-  hb->body_off.assign(n_worlds + 1, 0);
-  hb->is_static.assign(NB, 0);
-  hb->lx.assign(NV, 0.0);
-  hb->ly.assign(NV, 0.0);
-  long g = 0, vo = 0;
-  for (int w = 0; w < n_worlds; ++w) {
-    const shp_world_desc& d = worlds[w];
-    hb->body_off[w] = (int32_t)g;
-    for (int i = 0; i < d.n_bodies; ++i, ++g) {
-      hb->h[0][g] = nz(d.px[i]);
-      hb->h[1][g] = nz(d.py[i]);
-      hb->h[2][g] = nz(d.theta[i]);
-      hb->h[3][g] = nz(d.vx[i]);
-      hb->h[4][g] = nz(d.vy[i]);
-      hb->h[5][g] = nz(d.omega[i]);
-      // toInvMass2 (Constraint.hs:79-83)
-      hb->h[6][g] = d.mass[i] == 0.0 ? 0.0 : 1.0 / d.mass[i];
-      hb->h[7][g] = d.inertia[i] == 0.0 ? 0.0 : 1.0 / d.inertia[i];
-      hb->h[8][g] = d.mu[i];
-      hb->h[9][g] = d.bounce[i];
-      hb->rad[g] = d.radius ? d.radius[i] : 0.0;
-      hb->is_static[g] = (hb->h[6][g] == 0.0 && hb->h[7][g] == 0.0) ? 1 : 0;
-      hb->body_world[g] = w;
-      hb->vert_off[g] = (int32_t)vo;
-      for (int k = d.hull_off[i]; k < d.hull_off[i + 1]; ++k, ++vo) {
-        hb->lx[vo] = d.hull_x[k];
-        hb->ly[vo] = d.hull_y[k];
-      }
-    }
-  }
-  hb->body_off[n_worlds] = (int32_t)g;
-  hb->vert_off[NB] = (int32_t)vo;
   const std::vector<int32_t>& body_off = hb->body_off;
   // per-world solver workgroups take the largest worlds first, so the
   // longest solves start at once instead of trailing the batch
@@ -1237,11 +1273,11 @@ shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_c
     for (int w = 0; w < n_worlds; ++w) {
       double hmax = 0.0;
       for (int gg = body_off[w]; gg < body_off[w + 1]; ++gg)
-        if (!hb->is_static[gg]) hmax = std::max(hmax, 2.0 * rmax[gg]);
+        if (!hb->is_static[gg] && !hb->dead[gg]) hmax = std::max(hmax, 2.0 * rmax[gg]);
       hb->cell[w] = hmax > 0.0 ? hmax * (1.0 + 1e-6) + 1e-9 : 1.0;
       hb->large_off[w] = (int32_t)hb->large_ids.size();
       for (int gg = body_off[w]; gg < body_off[w + 1]; ++gg)
-        if (hb->is_static[gg] && 2.0 * rmax[gg] >= hb->cell[w]) {
+        if (hb->is_static[gg] && !hb->dead[gg] && 2.0 * rmax[gg] >= hb->cell[w]) {
           hb->is_large[gg] = 1;
           hb->large_ids.push_back(gg);
         }
@@ -1436,6 +1472,21 @@ shp_status shp_batch_create(const shp_world_desc* worlds, int32_t n_worlds, cons
     g_err = keep;
     return st;
   }
+  b->table_allocs = b->allocs.size();
+  b->table_bytes = b->device_bytes;
+  // EmptiesVector.new capacity, then fromList's appends fill [0, n)
+  // (EmptiesVector.hs:27-42, World.hs:118-124)
+  b->ev_next.resize(n_worlds);
+  b->ev_head.resize(n_worlds);
+  b->ev_filled.resize(n_worlds);
+  for (int w = 0; w < n_worlds; ++w) {
+    const int n = worlds[w].n_bodies, cap = b->h_body_off[w + 1] - b->h_body_off[w];
+    std::vector<int32_t>& nx = b->ev_next[w];
+    nx.assign(cap, -1);
+    for (int s = n; s < cap; ++s) nx[s] = s + 1;
+    b->ev_head[w] = n;
+    b->ev_filled[w] = n;
+  }
   *out = b;
   return SHP_OK;
 }
@@ -1476,6 +1527,8 @@ shp_status batch_alloc(shp_batch* b, const HostBatch& hb) {
   c.grid_buckets = (int)P.grid_buckets;
   b->h_body_off = hb.body_off;
   b->h_vert_off = hb.vert_off;
+  b->h_cell = hb.cell;
+  b->h_is_large = hb.is_large;
 
   std::vector<Buf> bufs;
   batch_buffers(P, c, &b->scan_part, &bufs);
@@ -1489,6 +1542,7 @@ shp_status batch_alloc(shp_batch* b, const HostBatch& hb) {
     HIPCHK(hipMemcpy(*dptr[k], hb.h[k].data(), NB * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.rad, hb.rad.data(), NB * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.is_static, hb.is_static.data(), NB, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.dead, hb.dead.data(), NB, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.body_world, hb.body_world.data(), NB * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.vert_off, hb.vert_off.data(), (NB + 1) * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.body_off, hb.body_off.data(), (n_worlds + 1) * 4, hipMemcpyHostToDevice));
@@ -1880,6 +1934,259 @@ shp_status shp_batch_set_cache(shp_batch* b, int32_t world, int32_t count, const
     HIPCHK(hipMemcpy(c.cache_lt, nlt.data(), nlt.size() * 8, hipMemcpyHostToDevice));
   }
   HIPCHK(hipMemcpy(c.cache_woff, noff.data(), (W + 1) * 4, hipMemcpyHostToDevice));
+  return SHP_OK;
+}
+
+// ---- world mutation (World.append / World.delete, World.hs:77-87) --------
+
+// A body description staged like prepare_batch stages one; validated as
+// shp_batch_create validates a body.
+struct StagedBody {
+  double h[10];
+  double rad;
+  uint8_t is_static;
+  std::vector<double> x, y;
+};
+
+static shp_status stage_body(const shp_body_desc* d, StagedBody* s) {
+  const double r = d->radius;
+  if (!(r >= 0.0) || r - r != 0.0) return fail(SHP_E_INVAL, "radius must be finite and >= 0");
+  if (r > 0.0 && d->n_vertices != 0) return fail(SHP_E_INVAL, "a circle has no vertices");
+  if (r == 0.0 && d->n_vertices < 3) return fail(SHP_E_INVAL, "hull with fewer than 3 vertices");
+  if (d->n_vertices > SHP_MAXV) return fail(SHP_E_UNSUPPORTED, "hull with more than 8 vertices");
+  if (d->n_vertices > 0 && (!d->hull_x || !d->hull_y)) return fail(SHP_E_INVAL, "null hull");
+  const double in[10] = {d->px, d->py, d->theta, d->vx, d->vy, d->omega,
+                         d->mass, d->inertia, d->mu, d->bounce};
+  for (int k = 0; k < 6; ++k) s->h[k] = nz(in[k]);
+  s->h[6] = d->mass == 0.0 ? 0.0 : 1.0 / d->mass;  // toInvMass2 (Constraint.hs:79-83)
+  s->h[7] = d->inertia == 0.0 ? 0.0 : 1.0 / d->inertia;
+  s->h[8] = d->mu;
+  s->h[9] = d->bounce;
+  s->rad = r;
+  s->is_static = (s->h[6] == 0.0 && s->h[7] == 0.0) ? 1 : 0;
+  s->x.assign(d->hull_x, d->hull_x + d->n_vertices);
+  s->y.assign(d->hull_y, d->hull_y + d->n_vertices);
+  return SHP_OK;
+}
+
+// The body's state, materials and shape into slot g, which keeps its
+// vertex range (same vertex count); then its hull's normals and extents
+// and every body's placement (moveShape, World.hs:142-149: the other
+// bodies' placements are recomputed from unchanged state, bit for bit).
+static shp_status write_slot_in_place(shp_batch* b, long g, const StagedBody& s) {
+  Ctx& c = b->ctx;
+  double* dst[10] = {c.px, c.py, c.th, c.vx, c.vy, c.w, c.iml, c.imr, c.mu, c.e};
+  for (int k = 0; k < 10; ++k)
+    HIPCHK(hipMemcpy(dst[k] + g, &s.h[k], 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.rad + g, &s.rad, 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.is_static + g, &s.is_static, 1, hipMemcpyHostToDevice));
+  const long v0 = b->h_vert_off[g];
+  if (!s.x.empty()) {
+    HIPCHK(hipMemcpy(c.lx + v0, s.x.data(), s.x.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.ly + v0, s.y.data(), s.y.size() * 8, hipMemcpyHostToDevice));
+  }
+  const uint8_t zero = 0;
+  HIPCHK(hipMemcpy(c.dead + g, &zero, 1, hipMemcpyHostToDevice));
+  k_init_hulls<<<blocks(c.NB, 256), 256, 0, b->stream>>>(c);
+  k_integrate<<<blocks(c.NB, INTEGRATE_THREADS), INTEGRATE_THREADS, 0, b->stream>>>(c, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return SHP_OK;
+}
+
+// The batch re-laid with slot g holding the staged body: the current state
+// is read back, staged again with the new vertex ranges and grid set-up,
+// and uploaded into new buffers; the warm-start caches, the run's quality
+// sums, the training log and the learned model carry over.
+static shp_status rebuild_with(shp_batch* b, long g, const StagedBody& s) {
+  Ctx& c = b->ctx;
+  const StepPlan& P = b->plan;
+  const long NB = P.NB, NV = P.NV;
+  const int W = (int)P.W;
+  HostBatch hb;
+  double* src[10] = {c.px, c.py, c.th, c.vx, c.vy, c.w, c.iml, c.imr, c.mu, c.e};
+  for (int k = 0; k < 10; ++k) {
+    hb.h[k].resize(NB);
+    HIPCHK(hipMemcpy(hb.h[k].data(), src[k], NB * 8, hipMemcpyDeviceToHost));
+  }
+  hb.rad.resize(NB);
+  hb.is_static.resize(NB);
+  hb.dead.resize(NB);
+  HIPCHK(hipMemcpy(hb.rad.data(), c.rad, NB * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(hb.is_static.data(), c.is_static, NB, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(hb.dead.data(), c.dead, NB, hipMemcpyDeviceToHost));
+  std::vector<double> lx(NV), ly(NV);
+  if (NV) {
+    HIPCHK(hipMemcpy(lx.data(), c.lx, NV * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ly.data(), c.ly, NV * 8, hipMemcpyDeviceToHost));
+  }
+  for (int k = 0; k < 10; ++k) hb.h[k][g] = s.h[k];
+  hb.rad[g] = s.rad;
+  hb.is_static[g] = s.is_static;
+  hb.dead[g] = 0;
+  hb.body_off = b->h_body_off;
+  hb.body_world.resize(NB);
+  for (int w = 0; w < W; ++w)
+    for (int q = hb.body_off[w]; q < hb.body_off[w + 1]; ++q) hb.body_world[q] = w;
+  hb.vert_off.resize(NB + 1);
+  long vo = 0;
+  for (long q = 0; q < NB; ++q) {
+    hb.vert_off[q] = (int32_t)vo;
+    if (q == g) {
+      for (size_t k = 0; k < s.x.size(); ++k, ++vo) {
+        hb.lx.push_back(s.x[k]);
+        hb.ly.push_back(s.y[k]);
+      }
+    } else {
+      for (int k = b->h_vert_off[q]; k < b->h_vert_off[q + 1]; ++k, ++vo) {
+        hb.lx.push_back(lx[k]);
+        hb.ly.push_back(ly[k]);
+      }
+    }
+  }
+  hb.vert_off[NB] = (int32_t)vo;
+  if (vo >= (1L << 30)) return fail(SHP_E_UNSUPPORTED, "too many vertices");
+  TRY(plan_staged(b->cfg, &hb));
+
+  // what one frame hands to the next
+  std::vector<int32_t> woff(W + 1);
+  HIPCHK(hipMemcpy(woff.data(), c.cache_woff, (W + 1) * 4, hipMemcpyDeviceToHost));
+  const long total = woff[W];
+  std::vector<uint64_t> ck(total);
+  std::vector<double> cln(total), clt(total);
+  if (total) {
+    HIPCHK(hipMemcpy(ck.data(), c.cache_key, total * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cln.data(), c.cache_ln, total * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(clt.data(), c.cache_lt, total * 8, hipMemcpyDeviceToHost));
+  }
+  unsigned long long q[Q_ENTRIES], logc = 0;
+  HIPCHK(hipMemcpy(q, c.quality, sizeof(q), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&logc, c.log_count, 8, hipMemcpyDeviceToHost));
+  const Ctx keep = c;
+  const StepPlan old_plan = P;
+
+  // the plan's buffers are replaced; buffers added later (timing stamps,
+  // logging) are kept
+  if (b->graph_made) {
+    HIPCHK(hipGraphExecDestroy(b->graph_exec));
+    b->graph_made = false;
+  }
+  b->graph_ready = false;
+  for (size_t k = 0; k < b->table_allocs; ++k) (void)hipFree(b->allocs[k]);
+  b->allocs.erase(b->allocs.begin(), b->allocs.begin() + b->table_allocs);
+  const long extra = b->device_bytes - b->table_bytes;
+  b->device_bytes = 0;
+  b->plan = hb.P;
+  const size_t a0 = b->allocs.size();
+  const shp_status st = batch_alloc(b, hb);
+  std::rotate(b->allocs.begin(), b->allocs.begin() + a0, b->allocs.end());
+  b->table_allocs = b->allocs.size() - a0;
+  b->table_bytes = b->device_bytes;
+  b->device_bytes += extra;
+  if (st != SHP_OK) return st;
+
+  c.tstamp_store = keep.tstamp_store;
+  c.tstamp = keep.tstamp;
+  c.mlp_w = keep.mlp_w;
+  c.mlp_wbf = keep.mlp_wbf;
+  c.mlp_ready = keep.mlp_ready;
+  c.collect_misses = keep.collect_misses;
+  c.log_feat = keep.log_feat;
+  c.log_tgt = keep.log_tgt;
+  c.log_cap = keep.log_cap;
+  if (b->plan.miss_cap == 0 && keep.miss_cap > 0) {  // miss buffers added for logging
+    c.miss_idx = keep.miss_idx;
+    c.miss_feat = keep.miss_feat;
+    c.miss_cap = keep.miss_cap;
+    b->plan.miss_cap = old_plan.miss_cap;
+    b->plan.g_log = old_plan.g_log;
+  }
+  if (total) {
+    HIPCHK(hipMemcpy(c.cache_key, ck.data(), total * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.cache_ln, cln.data(), total * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.cache_lt, clt.data(), total * 8, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(c.cache_woff, woff.data(), (W + 1) * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.quality, q, sizeof(q), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.log_count, &logc, 8, hipMemcpyHostToDevice));
+  return SHP_OK;
+}
+
+static double body_extent(const StagedBody& s) {
+  double r = s.rad;
+  for (size_t k = 0; k < s.x.size(); ++k) r = std::max(r, std::sqrt(s.x[k] * s.x[k] + s.y[k] * s.y[k]));
+  return 2.0 * r;
+}
+
+shp_status shp_batch_append(shp_batch* b, int32_t world, const shp_body_desc* body,
+                            int32_t* out_slot) {
+  if (!b || !body || !out_slot || world < 0 || world >= b->ctx.W)
+    return fail(SHP_E_INVAL, "bad argument");
+  StagedBody s;
+  TRY(stage_body(body, &s));
+  std::vector<int32_t>& nx = b->ev_next[world];
+  const int cap = (int)nx.size();
+  const int slot = b->ev_head[world];
+  // EmptiesVector.append (EmptiesVector.hs:70-85): the head slot is taken
+  // and its successor becomes the head; a full world has no head slot
+  if (slot >= cap) return fail(SHP_E_INVAL, "world is full: no empty slot to append into");
+  const int next = nx[slot];
+  if (next == -1) return fail(SHP_E_INVAL, "the empty-slot list points at a filled slot");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const long g = (long)b->h_body_off[world] + slot;
+  bool in_place = (long)s.x.size() == (long)(b->h_vert_off[g + 1] - b->h_vert_off[g]);
+  if (in_place && b->plan.use_grid) {
+    // the grid's cells must stay at least as wide as every small body, and
+    // the large static bodies are listed at set-up
+    const double ext = body_extent(s), cell = b->h_cell[world];
+    if (b->h_is_large[g]) in_place = false;
+    else if (s.is_static) in_place = ext < cell;
+    else in_place = ext * (1.0 + 1e-6) + 1e-9 <= cell;
+  }
+  if (in_place)
+    TRY(write_slot_in_place(b, g, s));
+  else
+    TRY(rebuild_with(b, g, s));
+  nx[slot] = -1;
+  b->ev_head[world] = next;
+  b->ev_filled[world] += 1;
+  *out_slot = slot;
+  return SHP_OK;
+}
+
+shp_status shp_batch_delete(shp_batch* b, int32_t world, int32_t slot) {
+  if (!b || world < 0 || world >= b->ctx.W) return fail(SHP_E_INVAL, "bad argument");
+  std::vector<int32_t>& nx = b->ev_next[world];
+  if (slot < 0 || slot >= (int)nx.size()) return fail(SHP_E_INVAL, "slot out of range");
+  // safeDelete (EmptiesVector.hs:57-68): the slot must be filled; it becomes
+  // the head of the empty-slot list
+  if (nx[slot] != -1) return fail(SHP_E_INVAL, "slot " + std::to_string(slot) + " is empty");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const long g = (long)b->h_body_off[world] + slot;
+  const uint8_t one = 1;
+  HIPCHK(hipMemcpy(b->ctx.dead + g, &one, 1, hipMemcpyHostToDevice));
+  nx[slot] = b->ev_head[world];
+  b->ev_head[world] = slot;
+  b->ev_filled[world] -= 1;
+  return SHP_OK;
+}
+
+shp_status shp_batch_num_vertices(const shp_batch* b, int32_t world, int32_t* n) {
+  if (!b || !n || world < 0 || world >= b->ctx.W) return fail(SHP_E_INVAL, "bad argument");
+  *n = b->h_vert_off[b->h_body_off[world + 1]] - b->h_vert_off[b->h_body_off[world]];
+  return SHP_OK;
+}
+
+shp_status shp_batch_filled(const shp_batch* b, int32_t world, int32_t* out_filled) {
+  if (!b || !out_filled || world < 0 || world >= b->ctx.W) return fail(SHP_E_INVAL, "bad argument");
+  *out_filled = b->ev_filled[world];
+  return SHP_OK;
+}
+
+shp_status shp_batch_read_slots(const shp_batch* b, int32_t world, uint8_t* out_filled) {
+  if (!b || !out_filled || world < 0 || world >= b->ctx.W) return fail(SHP_E_INVAL, "bad argument");
+  const std::vector<int32_t>& nx = b->ev_next[world];
+  for (size_t s = 0; s < nx.size(); ++s) out_filled[s] = nx[s] == -1 ? 1 : 0;
   return SHP_OK;
 }
 

@@ -55,10 +55,11 @@ __device__ __forceinline__ bool aabb_hit(const Ctx& c, double aminx, double amax
   return !(bminx > amaxx || bmaxx < aminx) && !(bminy > amaxy || bmaxy < aminy);
 }
 
-// gravity (applyExternal) + binning of small bodies
+// gravity (applyExternal) + binning of small bodies; empty slots
+// (World.delete) get neither
 __global__ void k_grid_assign(Ctx c) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= c.NB) return;
+  if (g >= c.NB || c.dead[g]) return;
   if (c.iml[g] != 0.0) {
     c.vx[g] = c.vx[g] + c.gx * c.dt;
     c.vy[g] = c.vy[g] + c.gy * c.dt;
@@ -88,7 +89,7 @@ __device__ __forceinline__ GridRec load_rec(const Ctx& c, int q) {
 
 __global__ void k_grid_fill(Ctx c) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= c.NB || c.is_large[g]) return;
+  if (g >= c.NB || c.is_large[g] || c.dead[g]) return;
   int w = c.body_world[g];
   int t0 = c.tab_off[w];
   uint32_t mask = (uint32_t)(c.tab_off[w + 1] - t0 - 1);
@@ -116,6 +117,7 @@ __global__ void k_grid_fill(Ctx c) {
 template <int LANES, class F>
 __device__ __forceinline__ void grid_visit(const Ctx& c, int g, int sl, F&& hit) {
   constexpr int CELL_LANES = LANES < 9 ? LANES : 9;
+  if (c.dead[g]) return;  // an empty slot has no pairs; empty slots are not binned
   const int w = c.body_world[g];
   const int b0 = c.body_off[w];
   const bool si = c.is_static[g] != 0;
@@ -131,7 +133,8 @@ __device__ __forceinline__ void grid_visit(const Ctx& c, int g, int sl, F&& hit)
   const double span = (amaxx - aminx) + (amaxy - aminy);
   if (c.is_large[g] || !(span - span == 0.0) || ncx > 3 || ncy > 3) {
     for (int gj = g - 1 - sl; gj >= b0; gj -= LANES)
-      if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) hit(gj);
+      if (!(si && c.is_static[gj]) && !c.dead[gj] && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj))
+        hit(gj);
     return;
   }
   if (LANES == 1 || sl >= CELL_LANES) {
@@ -139,7 +142,8 @@ __device__ __forceinline__ void grid_visit(const Ctx& c, int g, int sl, F&& hit)
     for (int q = c.large_off[w] + l0; q < c.large_off[w + 1]; q += ls) {
       const int gj = c.large_ids[q];
       if (gj >= g) break;  // ascending
-      if (!(si && c.is_static[gj]) && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj)) hit(gj);
+      if (!(si && c.is_static[gj]) && !c.dead[gj] && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj))
+        hit(gj);
     }
     if (LANES != 1) return;
   }

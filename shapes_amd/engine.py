@@ -48,7 +48,14 @@ class WorldDescC(C.Structure):
     _fields_ = [("n_bodies", C.c_int32)] + [
         (n, _dp) for n in ("px", "py", "theta", "vx", "vy", "omega", "mass",
                            "inertia", "mu", "bounce")
-    ] + [("hull_off", _ip), ("hull_x", _dp), ("hull_y", _dp), ("radius", _dp)]
+    ] + [("hull_off", _ip), ("hull_x", _dp), ("hull_y", _dp), ("radius", _dp),
+         ("capacity", C.c_int32)]
+
+
+class BodyDescC(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("px", "py", "theta", "vx", "vy", "omega", "mass",
+                                          "inertia", "mu", "bounce", "radius")] + [
+        ("n_vertices", C.c_int32), ("hull_x", _dp), ("hull_y", _dp)]
 
 
 class ConfigC(C.Structure):
@@ -97,6 +104,8 @@ EXPORTS = [
     "shp_mlp_destroy", "shp_mlp_get_params", "shp_mlp_forward",
     "shp_mlp_gradient", "shp_mlp_set_data", "shp_mlp_train", "shp_mlp_set_loss",
     "shp_batch_plan", "shp_mlp_forward_noalloc",
+    "shp_batch_append", "shp_batch_delete", "shp_batch_filled", "shp_batch_read_slots",
+    "shp_batch_num_vertices",
 ]
 
 PHASES = ["reset", "broadphase_count", "scan_pairs", "broadphase_fill", "sat",
@@ -199,6 +208,7 @@ def _world_descs(worlds: Sequence[WorldDesc]):
             r = np.ascontiguousarray(w.radius, dtype=np.float64)
             keep.append(r)
             d.radius = _dptr(r)
+        d.capacity = int(getattr(w, "capacity", 0) or 0)
     return descs, keep
 
 
@@ -248,8 +258,8 @@ class Batch:
         del keep
         self._h = h
         self.n_worlds = len(worlds)
-        self.sizes = [w.n for w in worlds]
-        self._vert_counts = [int(w.hull_off[-1]) for w in worlds]
+        # slots per world (World.new capacity; n when no capacity is given)
+        self.sizes = [max(w.n, int(getattr(w, "capacity", 0) or 0)) for w in worlds]
         self.config = config
 
     def close(self):
@@ -316,9 +326,44 @@ class Batch:
                 "warm": warm}
 
     def hulls(self, world: int = 0) -> np.ndarray:
-        out = np.empty((self._vert_counts[world], 4), dtype=np.float64)
+        nv = C.c_int32()
+        _check(self._L.shp_batch_num_vertices(self._h, world, C.byref(nv)))
+        out = np.empty((nv.value, 4), dtype=np.float64)
         _check(self._L.shp_batch_read_hulls(self._h, world, _dptr(out)))
         return out
+
+    # ---- world mutation (World.append / World.delete, World.hs:77-87)
+    def append(self, world: int, px: float, py: float, theta: float = 0.0, vx: float = 0.0,
+               vy: float = 0.0, omega: float = 0.0, mass: float = 1.0, inertia: float = 1.0,
+               mu: float = 0.5, bounce: float = 0.0, hull=None, radius: float = 0.0) -> int:
+        """World.append: the body into the world's head empty slot; returns
+        the slot.  `hull` is a sequence of local-space (x, y) vertices, or
+        None for a circle of `radius`."""
+        hx = np.ascontiguousarray([v[0] for v in hull] if hull is not None else [0.0],
+                                  dtype=np.float64)
+        hy = np.ascontiguousarray([v[1] for v in hull] if hull is not None else [0.0],
+                                  dtype=np.float64)
+        d = BodyDescC(px, py, theta, vx, vy, omega, mass, inertia, mu, bounce, radius,
+                      len(hull) if hull is not None else 0, _dptr(hx), _dptr(hy))
+        slot = C.c_int32()
+        _check(self._L.shp_batch_append(self._h, world, C.byref(d), C.byref(slot)))
+        return slot.value
+
+    def delete(self, world: int, slot: int):
+        """World.delete (EmptiesVector.safeDelete): the slot becomes empty."""
+        _check(self._L.shp_batch_delete(self._h, world, slot))
+
+    def filled(self, world: int = 0) -> int:
+        n = C.c_int32()
+        _check(self._L.shp_batch_filled(self._h, world, C.byref(n)))
+        return n.value
+
+    def slots(self, world: int = 0) -> np.ndarray:
+        """Per slot: True when filled (EmptiesVector.read)."""
+        out = np.empty(self.sizes[world], dtype=np.uint8)
+        _check(self._L.shp_batch_read_slots(self._h, world,
+                                            out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out.astype(bool)
 
     def stats(self) -> dict:
         s = StepStatsC()

@@ -43,6 +43,7 @@ class WorldDesc:
     hull_x: np.ndarray
     hull_y: np.ndarray
     radius: Optional[np.ndarray] = None
+    capacity: int = 0  # slots (World.new capacity); 0 = n
 
     @property
     def n(self) -> int:
@@ -53,7 +54,8 @@ class WorldDesc:
         return int(np.count_nonzero((self.mass != 0) | (self.inertia != 0)))
 
     def copy(self) -> "WorldDesc":
-        return WorldDesc(**{k: (None if v is None else v.copy()) for k, v in self.__dict__.items()})
+        return WorldDesc(**{k: (v.copy() if isinstance(v, np.ndarray) else v)
+                            for k, v in self.__dict__.items()})
 
 
 @dataclass

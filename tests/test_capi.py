@@ -190,3 +190,26 @@ def test_circle_descriptions_validated_on_the_host():
     bad.radius[1] = float("nan")
     st, msg, _, _ = engine.plan([bad], sc.config)
     assert st == 1
+
+
+def test_capacity_adds_empty_slots_to_the_plan():
+    """World.new capacity: slots beyond the bodies count in the plan (no
+    vertices); a capacity below the body count is refused."""
+    w = scenes.random_world(3, count=10)
+    cfg = scenes.SceneConfig()
+    st, msg, p0, _ = engine.plan([w], cfg)
+    assert st == 0, msg
+    w2 = w.copy()
+    w2.capacity = w.n + 7
+    st, msg, p, rows = engine.plan([w2], cfg)
+    assert st == 0, msg
+    assert p["bodies"] == w.n + 7 and p["vertices"] == p0["vertices"]
+    assert p["max_world_bodies"] == w.n + 7
+    _check_launches(rows)
+    w2.capacity = w.n - 1
+    st, msg, _, _ = engine.plan([w2], cfg)
+    assert st == 1 and "capacity" in msg
+    L = engine.load_library()
+    for name in ("shp_batch_append", "shp_batch_delete", "shp_batch_filled",
+                 "shp_batch_read_slots", "shp_batch_num_vertices"):
+        assert hasattr(L, name)
