@@ -223,6 +223,51 @@ shp_status shp_batch_append(shp_batch* b, int32_t world,
 /* World.delete = EmptiesVector.safeDelete (World.hs:86-87,
  * EmptiesVector.hs:57-68): SHP_E_INVAL when the slot is already empty. */
 shp_status shp_batch_delete(shp_batch* b, int32_t world, int32_t slot);
+/* ---- debug overlays and dumps (the demo's worldContacts / worldAabbs,
+ * shapes-demo/src/Physics/Demo/IOWorld.hs:74-90) -------------------------- */
+
+/* worldContacts (IOWorld.hs:74-81): G.culledKeys and prepareFrame of the
+ * worlds as they stand, without stepping (no acceleration, no solve, no
+ * advance; the warm-start caches untouched).  shp_batch_read_pairs and
+ * shp_batch_read_contacts then return these pairs and contacts (lam: the
+ * cached lagrangians joined by applyCachedSlns, 0 for new contacts) until
+ * the next step. */
+shp_status shp_batch_probe_contacts(shp_batch* b);
+/* worldAabbs = B.toAabbs (IOWorld.hs:86-90, Broadphase/Aabb.hs:119-125):
+ * the filled slots in slot order and their AABBs (minx, maxx, miny, maxy).
+ * Call with slots = boxes = NULL for the count. */
+shp_status shp_batch_read_aabbs(shp_batch* b, int32_t world, int32_t* count,
+                                int32_t* slots, double* boxes);
+/* The EngineCache (Engine/Main.hs:32) the next step joins against: keys
+ * descending (4 ints) and the accumulated (normal, friction) lagrangians. */
+shp_status shp_batch_read_cache(shp_batch* b, int32_t world, int32_t* count,
+                                int32_t* keys, double* lam);
+
+/* One world's dump, a self-describing little-endian byte stream:
+ *   header  "SHPD", u16 version (1), u16 sections, i32 world, i32 0,
+ *           i64 steps since creation                           (24 bytes)
+ *   section char tag[4], u32 record bytes, u64 records, then the records:
+ *   "BODY"  filled slots: i32 slot, i32 0, f64 px py theta vx vy omega (56)
+ *   "AABB"  as shp_batch_read_aabbs: i32 slot, i32 0, f64 minx maxx miny
+ *           maxy                                                      (40)
+ *   "PAIR"  as shp_batch_read_pairs: i32 i, i32 j                      (8)
+ *   "CONT"  as shp_batch_read_contacts: i32 i, j; u8 fi, fj, flip, warm;
+ *           u32 0; f64 nx ny px py depth lambda_n lambda_t             (72)
+ *   "CACH"  as shp_batch_read_cache: i32 i, j, fi, fj; f64 lambda_n,
+ *           lambda_t                                                   (32)
+ * in that order, those selected by `sections` (SHP_DUMP_* bits).  With
+ * buf = NULL only *size is set; otherwise capacity must hold *size bytes.
+ * Dumps of several worlds or steps concatenate. */
+#define SHP_DUMP_VERSION 1
+#define SHP_DUMP_BODIES 1u
+#define SHP_DUMP_AABBS 2u
+#define SHP_DUMP_PAIRS 4u
+#define SHP_DUMP_CONTACTS 8u
+#define SHP_DUMP_CACHE 16u
+#define SHP_DUMP_ALL 31u
+shp_status shp_batch_dump(shp_batch* b, int32_t world, uint32_t sections,
+                          void* buf, int64_t capacity, int64_t* size);
+
 /* World.filled (World.hs:90-91): the number of filled slots. */
 shp_status shp_batch_filled(const shp_batch* b, int32_t world,
                             int32_t* out_filled);

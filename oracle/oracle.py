@@ -67,6 +67,10 @@ def lib():
         L.orc_engine_filled.restype = C.c_int32
         L.orc_engine_filled.argtypes = [C.c_void_p]
         L.orc_engine_read_slots.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
+        L.orc_engine_probe.argtypes = [C.c_void_p]
+        L.orc_engine_num_cache.argtypes = [C.c_void_p]
+        L.orc_engine_num_cache.restype = C.c_int32
+        L.orc_engine_read_cache.argtypes = [C.c_void_p, _ip, _dp]
         L.orc_engine_create.argtypes = [C.c_int32] + [_dp] * 10 + [_ip, _dp, _dp, _dp,
                                                                   C.POINTER(Params), C.c_int32]
         L.orc_engine_destroy.argtypes = [C.c_void_p]
@@ -214,6 +218,20 @@ class OracleEngine:
                 warm.ctypes.data_as(_ip))
         return {"keys": keys, "flip": flip, "geom": geom, "lam": lam,
                 "warm": warm}
+
+    def probe(self):
+        """worldContacts: pairs and contacts of the world as it stands."""
+        self._L.orc_engine_probe(self._h)
+
+    def cache(self):
+        """(keys n x 4, lam n x 2) of the warm-start cache."""
+        n = self._L.orc_engine_num_cache(self._h)
+        keys = np.empty((n, 4), dtype=np.int32)
+        lam = np.empty((n, 2), dtype=np.float64)
+        if n:
+            self._L.orc_engine_read_cache(self._h, keys.ctypes.data_as(_ip),
+                                          lam.ctypes.data_as(_dp))
+        return keys, lam
 
     def stats(self) -> dict:
         s = Stats()

@@ -670,6 +670,8 @@ struct orc_engine {
   std::vector<std::pair<int, int>> pairs;  // last step
   std::vector<int> pair_first;             // pairs + 1: first contact of each pair
   std::vector<Contact> contacts;           // last step == the cache
+  std::vector<Contact> probe;              // contacts of the last probe
+  bool probed = false;                     // readbacks show the probe
   orc_stats stats;
 };
 
@@ -1108,6 +1110,7 @@ static void residuals(orc_engine* e) {
 static void step_once(orc_engine* e) {
   const orc_params& p = e->prm;
   int n = (int)e->objs.size();
+  e->probed = false;
   // 1. broadphase on the shapes placed by the previous moveShapes
   culled_keys(e, e->pairs);
   // 2. applyExternal constantAccel (World.hs:156-158, External.hs:23-28)
@@ -1312,15 +1315,40 @@ void orc_engine_read_pairs(const orc_engine* e, int32_t* ij) {
   }
 }
 
+// worldContacts (shapes-demo IOWorld.hs:74-81): culledKeys and prepareFrame
+// of the world as it stands, joined against the cache without applying it
+// (the lagrangians a step would start from); the cache is left as it is.
+void orc_engine_probe(orc_engine* e) {
+  culled_keys(e, e->pairs);
+  prepare_frame(e, e->probe);
+  apply_cached_slns(e, e->probe, e->contacts, false);
+  e->probed = true;
+}
+
+int32_t orc_engine_num_cache(const orc_engine* e) { return (int32_t)e->contacts.size(); }
+
+void orc_engine_read_cache(const orc_engine* e, int32_t* keys, double* lam) {
+  for (size_t k = 0; k < e->contacts.size(); ++k) {
+    const Contact& c = e->contacts[k];
+    keys[4 * k + 0] = c.key.i;
+    keys[4 * k + 1] = c.key.j;
+    keys[4 * k + 2] = c.key.fi;
+    keys[4 * k + 3] = c.key.fj;
+    lam[2 * k] = c.ln;
+    lam[2 * k + 1] = c.lt;
+  }
+}
+
 int32_t orc_engine_num_contacts(const orc_engine* e) {
-  return (int32_t)e->contacts.size();
+  return (int32_t)(e->probed ? e->probe : e->contacts).size();
 }
 
 void orc_engine_read_contacts(const orc_engine* e, int32_t* keys,
                               int32_t* flip, double* geom, double* lam,
                               int32_t* warm) {
-  for (size_t k = 0; k < e->contacts.size(); ++k) {
-    const Contact& c = e->contacts[k];
+  const std::vector<Contact>& shown = e->probed ? e->probe : e->contacts;
+  for (size_t k = 0; k < shown.size(); ++k) {
+    const Contact& c = shown[k];
     if (keys) {
       keys[4 * k + 0] = c.key.i;
       keys[4 * k + 1] = c.key.j;

@@ -90,6 +90,13 @@ void orc_engine_read_contacts(const orc_engine* e, int32_t* keys,
                               int32_t* flip, double* geom, double* lam,
                               int32_t* warm);
 
+/* worldContacts: pairs and contacts of the world as it stands (no step);
+ * read_pairs / read_contacts show them until the next step. */
+void orc_engine_probe(orc_engine* e);
+/* the warm-start cache (keys descending, 4 ints; lam 2 doubles each) */
+int32_t orc_engine_num_cache(const orc_engine* e);
+void orc_engine_read_cache(const orc_engine* e, int32_t* keys, double* lam);
+
 /* per-step statistics of the last step */
 typedef struct orc_stats {
   int32_t pairs, contacts, warm_hits;

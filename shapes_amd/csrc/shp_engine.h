@@ -159,6 +159,7 @@ struct Ctx {
   // offsets, per-body cell, large static bodies listed per world
   int use_grid;
   int grid_buckets;
+  int probe;  // 1: contact generation of the current world only (no acceleration)
   double* cell_h;
   uint8_t* is_large;
   int32_t* body_cell;

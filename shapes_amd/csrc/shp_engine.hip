@@ -328,7 +328,7 @@ __global__ void k_broadphase(Ctx c, int fill) {
   int rank = b0 + (nw - 1 - li);
   // an empty slot (World.delete) gets no acceleration and no pairs
   const bool dead = c.dead[g] != 0;
-  if (!fill && lane == 0 && c.iml[g] != 0.0 && !dead) {
+  if (!fill && lane == 0 && c.iml[g] != 0.0 && !dead && !c.probe) {
     c.vx[g] = c.vx[g] + c.gx * c.dt;
     c.vy[g] = c.vy[g] + c.gy * c.dt;
   }
@@ -918,7 +918,9 @@ hipError_t scan_grid_buckets(shp_batch* b) {
                        0, b->stream);
 }
 
-static shp_status enqueue_step(shp_batch* b) {
+// The step's front: broadphase, SAT, contact generation and the warm-start
+// join; with `fork` the colouring starts on the second stream after k_sat.
+static shp_status enqueue_front(shp_batch* b, bool fork) {
   Ctx& c = b->ctx;
   const StepPlan& P = b->plan;
   hipStream_t s = b->stream;
@@ -940,7 +942,6 @@ static shp_status enqueue_step(shp_batch* b) {
   k_sat<<<P.g_sat, 256, 0, s>>>(c);
   // the colouring needs only the pairs and which of them have contacts: it
   // runs beside the contact scan, contact generation and learned warm start
-  const bool fork = c.solver == SHP_SOLVER_COLORED_F32;
   if (fork) {
     HIPCHK(hipEventRecord(b->fork_event, s));
     HIPCHK(hipStreamWaitEvent(b->side_stream, b->fork_event, 0));
@@ -957,6 +958,16 @@ static shp_status enqueue_step(shp_batch* b) {
   mark(b, 7);
   k_contacts<<<P.g_contacts, 256, 0, s>>>(c);
   mark(b, 8);
+  HIPCHK(hipGetLastError());
+  return SHP_OK;
+}
+
+static shp_status enqueue_step(shp_batch* b) {
+  Ctx& c = b->ctx;
+  const StepPlan& P = b->plan;
+  hipStream_t s = b->stream;
+  const bool fork = c.solver == SHP_SOLVER_COLORED_F32;
+  TRY(enqueue_front(b, fork));
   if (c.warm_mode == SHP_WARM_LEARNED && c.mlp_ready) mlp_launch_infer(b);
   mark(b, 9);
   if (fork) {
@@ -2187,6 +2198,195 @@ shp_status shp_batch_read_slots(const shp_batch* b, int32_t world, uint8_t* out_
   if (!b || !out_filled || world < 0 || world >= b->ctx.W) return fail(SHP_E_INVAL, "bad argument");
   const std::vector<int32_t>& nx = b->ev_next[world];
   for (size_t s = 0; s < nx.size(); ++s) out_filled[s] = nx[s] == -1 ? 1 : 0;
+  return SHP_OK;
+}
+
+// ---- debug overlays and the dump format (shapes-demo IOWorld.hs:74-90) ---
+
+shp_status shp_batch_probe_contacts(shp_batch* b) {
+  if (!b) return fail(SHP_E_INVAL, "null batch");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const bool timing = b->timing;
+  b->timing = false;
+  b->ctx.probe = 1;
+  const shp_status st = enqueue_front(b, false);
+  b->ctx.probe = 0;
+  b->timing = timing;
+  TRY(st);
+  return shp_batch_sync(b);
+}
+
+shp_status shp_batch_read_aabbs(shp_batch* b, int32_t world, int32_t* count, int32_t* slots,
+                                double* boxes) {
+  if (!b || !count || world < 0 || world >= b->ctx.W) return fail(SHP_E_INVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const std::vector<int32_t>& nx = b->ev_next[world];
+  const int o = b->h_body_off[world], n = (int)nx.size();
+  int m = 0;
+  for (int s = 0; s < n; ++s) m += nx[s] == -1;
+  *count = m;
+  if (!slots && !boxes) return SHP_OK;
+  std::vector<double> t[4];
+  double* src[4] = {b->ctx.bminx, b->ctx.bmaxx, b->ctx.bminy, b->ctx.bmaxy};
+  for (int k = 0; k < 4; ++k) {
+    t[k].resize(n);
+    if (boxes && n) HIPCHK(hipMemcpy(t[k].data(), src[k] + o, n * 8, hipMemcpyDeviceToHost));
+  }
+  int q = 0;
+  for (int s = 0; s < n; ++s) {
+    if (nx[s] != -1) continue;
+    if (slots) slots[q] = s;
+    if (boxes)
+      for (int k = 0; k < 4; ++k) boxes[4 * q + k] = t[k][s];
+    ++q;
+  }
+  return SHP_OK;
+}
+
+shp_status shp_batch_read_cache(shp_batch* b, int32_t world, int32_t* count, int32_t* keys,
+                                double* lam) {
+  if (!b || !count || world < 0 || world >= b->ctx.W) return fail(SHP_E_INVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const Ctx& c = b->ctx;
+  int32_t r[2];
+  HIPCHK(hipMemcpy(r, c.cache_woff + world, 8, hipMemcpyDeviceToHost));
+  const int n = r[1] - r[0];
+  *count = n;
+  if (n == 0) return SHP_OK;
+  if (keys) {
+    std::vector<uint64_t> k(n);
+    HIPCHK(hipMemcpy(k.data(), c.cache_key + r[0], n * 8, hipMemcpyDeviceToHost));
+    for (int q = 0; q < n; ++q) {
+      keys[4 * q + 0] = (int32_t)(k[q] >> 40);
+      keys[4 * q + 1] = (int32_t)((k[q] >> 16) & 0xFFFFFF);
+      keys[4 * q + 2] = (int32_t)((k[q] >> 8) & 0xFF);
+      keys[4 * q + 3] = (int32_t)(k[q] & 0xFF);
+    }
+  }
+  if (lam) {
+    std::vector<double> ln(n), lt(n);
+    HIPCHK(hipMemcpy(ln.data(), c.cache_ln + r[0], n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lt.data(), c.cache_lt + r[0], n * 8, hipMemcpyDeviceToHost));
+    for (int q = 0; q < n; ++q) {
+      lam[2 * q] = ln[q];
+      lam[2 * q + 1] = lt[q];
+    }
+  }
+  return SHP_OK;
+}
+
+}  // extern "C"
+
+// The dump's byte stream: little-endian, every record a multiple of 8 bytes.
+struct DumpWriter {
+  std::vector<uint8_t> out;
+  template <class T>
+  void put(T v) {
+    const uint8_t* p = (const uint8_t*)&v;
+    out.insert(out.end(), p, p + sizeof(T));
+  }
+  void tag(const char* t, uint32_t record_bytes, uint64_t count) {
+    out.insert(out.end(), t, t + 4);
+    put<uint32_t>(record_bytes);
+    put<uint64_t>(count);
+  }
+};
+
+extern "C" {
+
+shp_status shp_batch_dump(shp_batch* b, int32_t world, uint32_t sections, void* buf,
+                          int64_t capacity, int64_t* size) {
+  if (!b || !size || world < 0 || world >= b->ctx.W || capacity < 0 ||
+      (sections & ~(uint32_t)SHP_DUMP_ALL) != 0)
+    return fail(SHP_E_INVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  unsigned long long q[Q_ENTRIES];
+  HIPCHK(hipMemcpy(q, b->ctx.quality, sizeof(q), hipMemcpyDeviceToHost));
+  DumpWriter w;
+  int n_sec = 0;
+  for (int k = 0; k < 5; ++k) n_sec += (sections >> k) & 1;
+  w.out.insert(w.out.end(), {'S', 'H', 'P', 'D'});
+  w.put<uint16_t>(SHP_DUMP_VERSION);
+  w.put<uint16_t>((uint16_t)n_sec);
+  w.put<int32_t>(world);
+  w.put<int32_t>(0);
+  w.put<int64_t>((int64_t)q[Q_STEPS]);
+  const std::vector<int32_t>& nx = b->ev_next[world];
+  const int n = (int)nx.size();
+  if (sections & SHP_DUMP_BODIES) {
+    std::vector<double> v[6];
+    for (auto& x : v) x.resize(n);
+    TRY(shp_batch_read_bodies(b, world, v[0].data(), v[1].data(), v[2].data(), v[3].data(),
+                              v[4].data(), v[5].data()));
+    int m = 0;
+    for (int s = 0; s < n; ++s) m += nx[s] == -1;
+    w.tag("BODY", 56, (uint64_t)m);
+    for (int s = 0; s < n; ++s) {
+      if (nx[s] != -1) continue;
+      w.put<int32_t>(s);
+      w.put<int32_t>(0);
+      for (int k = 0; k < 6; ++k) w.put<double>(v[k][s]);
+    }
+  }
+  if (sections & SHP_DUMP_AABBS) {
+    int32_t m = 0;
+    TRY(shp_batch_read_aabbs(b, world, &m, nullptr, nullptr));
+    std::vector<int32_t> sl(m);
+    std::vector<double> bx(4 * (size_t)m);
+    TRY(shp_batch_read_aabbs(b, world, &m, sl.data(), bx.data()));
+    w.tag("AABB", 40, (uint64_t)m);
+    for (int k = 0; k < m; ++k) {
+      w.put<int32_t>(sl[k]);
+      w.put<int32_t>(0);
+      for (int e = 0; e < 4; ++e) w.put<double>(bx[4 * k + e]);
+    }
+  }
+  if (sections & SHP_DUMP_PAIRS) {
+    int32_t m = 0;
+    TRY(shp_batch_read_pairs(b, world, &m, nullptr));
+    std::vector<int32_t> ij(2 * (size_t)m + 2);
+    TRY(shp_batch_read_pairs(b, world, &m, ij.data()));
+    w.tag("PAIR", 8, (uint64_t)m);
+    for (int k = 0; k < 2 * m; ++k) w.put<int32_t>(ij[k]);
+  }
+  if (sections & SHP_DUMP_CONTACTS) {
+    int32_t m = 0;
+    TRY(shp_batch_read_contacts(b, world, &m, nullptr, nullptr, nullptr, nullptr, nullptr));
+    std::vector<int32_t> keys(4 * (size_t)m + 4), flip(m + 1), warm(m + 1);
+    std::vector<double> geom(5 * (size_t)m + 5), lam(2 * (size_t)m + 2);
+    TRY(shp_batch_read_contacts(b, world, &m, keys.data(), flip.data(), geom.data(), lam.data(),
+                                warm.data()));
+    w.tag("CONT", 72, (uint64_t)m);
+    for (int k = 0; k < m; ++k) {
+      w.put<int32_t>(keys[4 * k]);
+      w.put<int32_t>(keys[4 * k + 1]);
+      w.put<uint8_t>((uint8_t)keys[4 * k + 2]);
+      w.put<uint8_t>((uint8_t)keys[4 * k + 3]);
+      w.put<uint8_t>((uint8_t)flip[k]);
+      w.put<uint8_t>((uint8_t)warm[k]);
+      w.put<uint32_t>(0);
+      for (int e = 0; e < 5; ++e) w.put<double>(geom[5 * k + e]);
+      w.put<double>(lam[2 * k]);
+      w.put<double>(lam[2 * k + 1]);
+    }
+  }
+  if (sections & SHP_DUMP_CACHE) {
+    int32_t m = 0;
+    TRY(shp_batch_read_cache(b, world, &m, nullptr, nullptr));
+    std::vector<int32_t> keys(4 * (size_t)m + 4);
+    std::vector<double> lam(2 * (size_t)m + 2);
+    TRY(shp_batch_read_cache(b, world, &m, keys.data(), lam.data()));
+    w.tag("CACH", 32, (uint64_t)m);
+    for (int k = 0; k < m; ++k) {
+      for (int e = 0; e < 4; ++e) w.put<int32_t>(keys[4 * k + e]);
+      w.put<double>(lam[2 * k]);
+      w.put<double>(lam[2 * k + 1]);
+    }
+  }
+  *size = (int64_t)w.out.size();
+  if (!buf) return SHP_OK;
+  if (capacity < *size) return fail(SHP_E_INVAL, "dump buffer too small");
+  memcpy(buf, w.out.data(), w.out.size());
   return SHP_OK;
 }
 

@@ -60,7 +60,7 @@ __device__ __forceinline__ bool aabb_hit(const Ctx& c, double aminx, double amax
 __global__ void k_grid_assign(Ctx c) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= c.NB || c.dead[g]) return;
-  if (c.iml[g] != 0.0) {
+  if (c.iml[g] != 0.0 && !c.probe) {
     c.vx[g] = c.vx[g] + c.gx * c.dt;
     c.vy[g] = c.vy[g] + c.gy * c.dt;
   }

@@ -105,7 +105,8 @@ EXPORTS = [
     "shp_mlp_gradient", "shp_mlp_set_data", "shp_mlp_train", "shp_mlp_set_loss",
     "shp_batch_plan", "shp_mlp_forward_noalloc",
     "shp_batch_append", "shp_batch_delete", "shp_batch_filled", "shp_batch_read_slots",
-    "shp_batch_num_vertices",
+    "shp_batch_num_vertices", "shp_batch_probe_contacts", "shp_batch_read_aabbs",
+    "shp_batch_read_cache", "shp_batch_dump",
 ]
 
 PHASES = ["reset", "broadphase_count", "scan_pairs", "broadphase_fill", "sat",
@@ -169,6 +170,16 @@ def load_library(path: str = LIB_PATH):
     L.shp_batch_read_solve_profile.argtypes = [C.c_void_p, _dp]
     L.shp_batch_plan.argtypes = [C.POINTER(WorldDescC), C.c_int32, C.POINTER(ConfigC),
                                  C.POINTER(PlanC), C.POINTER(LaunchC), C.c_int32]
+    L.shp_batch_append.argtypes = [C.c_void_p, C.c_int32, C.POINTER(BodyDescC), _ip]
+    L.shp_batch_delete.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    L.shp_batch_filled.argtypes = [C.c_void_p, C.c_int32, _ip]
+    L.shp_batch_read_slots.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_uint8)]
+    L.shp_batch_num_vertices.argtypes = [C.c_void_p, C.c_int32, _ip]
+    L.shp_batch_probe_contacts.argtypes = [C.c_void_p]
+    L.shp_batch_read_aabbs.argtypes = [C.c_void_p, C.c_int32, _ip, _ip, _dp]
+    L.shp_batch_read_cache.argtypes = [C.c_void_p, C.c_int32, _ip, _ip, _dp]
+    L.shp_batch_dump.argtypes = [C.c_void_p, C.c_int32, C.c_uint32, C.c_void_p, C.c_int64,
+                                 C.POINTER(C.c_int64)]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L
@@ -331,6 +342,43 @@ class Batch:
         out = np.empty((nv.value, 4), dtype=np.float64)
         _check(self._L.shp_batch_read_hulls(self._h, world, _dptr(out)))
         return out
+
+    # ---- debug overlays and dumps (shapes-demo IOWorld.hs:74-90)
+    def probe_contacts(self):
+        """worldContacts: pairs and contacts of the worlds as they stand (no
+        step); pairs() / contacts() return them until the next step."""
+        _check(self._L.shp_batch_probe_contacts(self._h))
+
+    def aabbs(self, world: int = 0):
+        """worldAabbs: (slots, n x 4 minx/maxx/miny/maxy) of the filled slots."""
+        n = C.c_int32()
+        _check(self._L.shp_batch_read_aabbs(self._h, world, C.byref(n), None, None))
+        slots = np.empty(n.value, dtype=np.int32)
+        boxes = np.empty((n.value, 4), dtype=np.float64)
+        if n.value:
+            _check(self._L.shp_batch_read_aabbs(self._h, world, C.byref(n),
+                                                slots.ctypes.data_as(_ip), _dptr(boxes)))
+        return slots, boxes
+
+    def cache(self, world: int = 0):
+        """The warm-start cache: (keys n x 4, lam n x 2), keys descending."""
+        n = C.c_int32()
+        _check(self._L.shp_batch_read_cache(self._h, world, C.byref(n), None, None))
+        keys = np.empty((n.value, 4), dtype=np.int32)
+        lam = np.empty((n.value, 2), dtype=np.float64)
+        if n.value:
+            _check(self._L.shp_batch_read_cache(self._h, world, C.byref(n),
+                                                keys.ctypes.data_as(_ip), _dptr(lam)))
+        return keys, lam
+
+    def dump(self, world: int = 0, sections: int = 31) -> bytes:
+        """shp_batch_dump: the world's dump bytes (format: shapes_amd.wire)."""
+        size = C.c_int64()
+        _check(self._L.shp_batch_dump(self._h, world, sections, None, 0, C.byref(size)))
+        buf = C.create_string_buffer(max(size.value, 1))
+        _check(self._L.shp_batch_dump(self._h, world, sections, buf, size.value,
+                                      C.byref(size)))
+        return buf.raw[:size.value]
 
     # ---- world mutation (World.append / World.delete, World.hs:77-87)
     def append(self, world: int, px: float, py: float, theta: float = 0.0, vx: float = 0.0,
