@@ -56,16 +56,17 @@ QUALITY_TOL = 0.02
 SOLVERS = {"ordered": engine.SOLVER_ORDERED_F64, "colored": engine.SOLVER_COLORED_F32}
 DTYPES = {"ordered": "f64", "colored": "f32"}
 BROADPHASES = {"auto": engine.BROADPHASE_AUTO, "scan": engine.BROADPHASE_SCAN,
-               "grid": engine.BROADPHASE_GRID}
+               "grid": engine.BROADPHASE_GRID, "grid1": engine.BROADPHASE_GRID_1,
+               "grid16": engine.BROADPHASE_GRID_16}
 
 # workload -> (scene factory, settling pre-roll, pair capacity per body,
 # autotune candidates, description)
 WORKLOADS = {
     "pile_batch": (lambda w: scenes.pile_batch(n_worlds=w or 1024), 500, 4,
-                   [("grid", 512), ("grid", 1024)],
+                   [(b, t) for b in ("grid1", "grid16") for t in (512, 1024)],
                    "worlds of 2k-32k random 3-8-gons stacked in piles up to 60 layers"),
     "world_batch": (lambda w: scenes.world_batch_fast(n_worlds=w or 4096), 150, 4,
-                    [(b, t) for b in ("grid", "scan") for t in (256, 512, 1024)],
+                    [(b, t) for b in ("grid1", "grid16", "scan") for t in (256, 512, 1024)],
                     "worlds of 8-1024 bodies (squares and 3-8-gons), each on its own floor"),
     "polygon_pile": (lambda w: scenes.polygon_pile(), 300, 0,
                      [(b, t) for b in ("grid", "scan") for t in (256, 512, 1024)],
@@ -142,12 +143,15 @@ def run(scene, solver, warm, iterations, preroll, steps, sched, ppb, model=None,
     return out
 
 
-def autotune(scene, ppb, candidates, preroll, steps=30):
+def autotune(scene, ppb, candidates, preroll, steps=30, solver=engine.SOLVER_COLORED_F32):
     """Pick broadphase x solver lanes per world by timing `steps` graph
-    replays of the plain coloured solver after `preroll` untimed steps."""
+    replays of the plain solver after `preroll` untimed steps (the ordered
+    solver sizes its lanes by world size: only the broadphase is tuned)."""
+    if solver == engine.SOLVER_ORDERED_F64:
+        candidates = sorted({(b, 0) for b, _ in candidates})
     times = {}
     for sched in candidates:
-        r = run(scene, engine.SOLVER_COLORED_F32, engine.WARM_ZERO, scene.config.iterations,
+        r = run(scene, solver, engine.WARM_ZERO, scene.config.iterations,
                 preroll, steps, sched, ppb)
         times[sched] = r["seconds"] / steps * 1e3
         note(f"autotune {sched[0]}/{sched[1]}: {times[sched]:.3f} ms/step")
@@ -466,7 +470,7 @@ def main():
 
     tuning = None
     if args.broadphase == "tune":
-        sched, tuning = autotune(scene, ppb, candidates, preroll)
+        sched, tuning = autotune(scene, ppb, candidates, preroll, solver=SOLVERS[solver_name])
     else:
         sched = (args.broadphase, args.solver_threads)
     note(f"schedule: {sched[0]}, {sched[1]} lanes")

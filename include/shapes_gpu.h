@@ -44,7 +44,12 @@ typedef int32_t shp_status;
 /* Broadphase algorithms (same pair set and order) */
 #define SHP_BROADPHASE_AUTO 0 /* grid when a world has more than 2048 bodies */
 #define SHP_BROADPHASE_SCAN 1 /* one wavefront per body scans j < i */
-#define SHP_BROADPHASE_GRID 2 /* uniform grid + explicit large static bodies */
+#define SHP_BROADPHASE_GRID 2 /* uniform grid + explicit large static bodies;
+                                 16 lanes per body up to 131072 bodies, one
+                                 lane per body beyond */
+#define SHP_BROADPHASE_GRID_1 3  /* the grid, one lane per body */
+#define SHP_BROADPHASE_GRID_16 4 /* the grid, 16 lanes per body (one per
+                                    candidate cell) */
 
 /* Warm-start modes for contacts absent from last frame's cache */
 #define SHP_WARM_ZERO 0    /* the reference: start at 0

@@ -1147,7 +1147,7 @@ shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_c
     return fail(SHP_E_INVAL, "unknown solver mode");
   if (cfg.warm_start != SHP_WARM_ZERO && cfg.warm_start != SHP_WARM_LEARNED)
     return fail(SHP_E_INVAL, "unknown warm-start mode");
-  if (cfg.broadphase < SHP_BROADPHASE_AUTO || cfg.broadphase > SHP_BROADPHASE_GRID)
+  if (cfg.broadphase < SHP_BROADPHASE_AUTO || cfg.broadphase > SHP_BROADPHASE_GRID_16)
     return fail(SHP_E_INVAL, "unknown broadphase mode");
   if (cfg.solver_threads != 0 && cfg.solver_threads != 256 && cfg.solver_threads != 512 &&
       !(cfg.solver_threads == 1024 && cfg.solver == SHP_SOLVER_COLORED_F32))
@@ -1269,7 +1269,7 @@ shp_status plan_staged(const shp_config& cfg, HostBatch* hb) {
   // radius of its dynamic bodies; static bodies larger than a cell are
   // "large" and tested explicitly
   const int mode = cfg.broadphase;
-  P.use_grid = mode == SHP_BROADPHASE_GRID || (mode == SHP_BROADPHASE_AUTO && max_nb > 2048);
+  P.use_grid = mode >= SHP_BROADPHASE_GRID || (mode == SHP_BROADPHASE_AUTO && max_nb > 2048);
   if (P.use_grid) {
     std::vector<double> rmax(hb->rad);  // a circle's extent is its radius
     for (long gg = 0; gg < NB; ++gg)
@@ -1303,7 +1303,8 @@ shp_status plan_staged(const shp_config& cfg, HostBatch* hb) {
     hb->large_off[n_worlds] = (int32_t)hb->large_ids.size();
     P.grid_buckets = tb_total;
     P.n_large = (long)hb->large_ids.size();
-    P.grid_wide = NB <= GRID_WIDE_MAX_BODIES;
+    P.grid_wide = mode == SHP_BROADPHASE_GRID_16 ||
+                  (mode != SHP_BROADPHASE_GRID_1 && NB <= GRID_WIDE_MAX_BODIES);
     P.scan_buckets = plan_scan(tb_total + 1);
   }
   P.scan_bodies = plan_scan(NB + 1);

@@ -29,6 +29,8 @@ WARM_LEARNED = 1
 BROADPHASE_AUTO = 0
 BROADPHASE_SCAN = 1
 BROADPHASE_GRID = 2
+BROADPHASE_GRID_1 = 3   # the grid, one lane per body
+BROADPHASE_GRID_16 = 4  # the grid, 16 lanes per body
 
 STATUS = {0: "OK", 1: "E_INVAL", 2: "E_OOM", 3: "E_HIP", 4: "E_UNSUPPORTED",
           5: "E_OVERFLOW", 6: "E_NONFINITE"}
