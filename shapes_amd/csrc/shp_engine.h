@@ -89,8 +89,8 @@ struct Ctx {
   // hulls (NV; vert_off NB + 1)
   int32_t* vert_off;
   double *lx, *ly;
-  uint8_t *ext_min, *ext_max;
-  double *wx, *wy, *wnx, *wny;
+  uint16_t* ext;  // per edge: extent vertex indices, min | max << 8
+  double4* wv;    // per vertex: world x, y and the unit normal of its edge
   double *bminx, *bmaxx, *bminy, *bmaxy;
   // broadphase
   int32_t *cnt_rank, *off_rank;  // NB + 1, indexed by descending-body rank

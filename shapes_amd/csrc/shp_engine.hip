@@ -60,12 +60,8 @@ __device__ __forceinline__ uint64_t pack_key(int i, int j, int fi, int fj) {
 __device__ __forceinline__ HullRef hull_of(const Ctx& c, int g) {
   int o = c.vert_off[g];
   HullRef h;
-  h.wx = c.wx + o;
-  h.wy = c.wy + o;
-  h.nx = c.wnx + o;
-  h.ny = c.wny + o;
-  h.emin = c.ext_min + o;
-  h.emax = c.ext_max + o;
+  h.vn = c.wv + o;
+  h.ext = c.ext + o;
   h.n = c.vert_off[g + 1] - o;
   return h;
 }
@@ -102,8 +98,7 @@ __global__ void k_init_hulls(Ctx c) {
         imax = k;
       }
     }
-    c.ext_min[o + e] = (uint8_t)imin;
-    c.ext_max[o + e] = (uint8_t)imax;
+    c.ext[o + e] = (uint16_t)(imin | (imax << 8));
   }
 }
 
@@ -179,13 +174,11 @@ __device__ __forceinline__ void integrate_tile(const Ctx& c, int advance, int vb
     const double y = (m10 * lx + m11 * ly) + m12 * one;
     const double x1 = (m00 * lx1 + m01 * ly1) + m02 * one;
     const double y1 = (m10 * lx1 + m11 * ly1) + m12 * one;
-    c.wx[v0 + i] = x;
-    c.wy[v0 + i] = y;
     s_wx[i] = x;
     s_wy[i] = y;
     d2 nn = normalize(clockwise(sub(mk(x1, y1), mk(x, y))));
-    c.wnx[v0 + i] = nn.x;
-    c.wny[v0 + i] = nn.y;
+    // vertex and edge normal in one 32-byte record (coalesced)
+    c.wv[v0 + i] = make_double4(x, y, nn.x, nn.y);
   }
   __syncthreads();
   if (t < nbk) {
@@ -383,20 +376,16 @@ __device__ __forceinline__ int circle_pair(const Ctx& c, int ga, int gb, double 
     gm->fb = 0;
     return 1;
   }
+  // one of them is a circle: the circle against the other's hull
+  const bool a_circle = ra > 0.0;
+  const HullRef hr = hull_of(c, a_circle ? gb : ga);
   HullV hv;
   int f = 0;
-  if (ra > 0.0) {
-    load_hull_v(hull_of(c, gb), hv);
-    if (!circle_hull_contact(ca, ra, hv, &f, gm)) return 0;
-    gm->fa = 0;
-    gm->fb = f;
-    return 1;
-  }
-  load_hull_v(hull_of(c, ga), hv);
-  if (!circle_hull_contact(cb, rb, hv, &f, gm)) return 0;
-  gm->fa = f;
-  gm->fb = 0;
-  *flip = 1;
+  load_hull_v(hr, hv);
+  if (!circle_hull_contact(a_circle ? ca : cb, a_circle ? ra : rb, hv, hr, &f, gm)) return 0;
+  gm->fa = a_circle ? 0 : f;
+  gm->fb = a_circle ? f : 0;
+  *flip = a_circle ? 0 : 1;
   return 1;
 }
 
@@ -413,7 +402,9 @@ __device__ __forceinline__ void sat_pair(const Ctx& c, int p) {
     HullRef B = hull_of(c, pgb);
     n = hull_contacts(A, B, gm, &flip);
   }
-  for (int k = 0; k < n; ++k) {
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (k >= n) break;
     int s = 2 * p + k;
     c.s_feat[s] = (uint16_t)((gm[k].fa << 8) | gm[k].fb);
     c.s_flip[s] = (uint8_t)flip;
@@ -438,7 +429,7 @@ __device__ __forceinline__ void sat_pair(const Ctx& c, int p) {
 }
 
 // grid-stride over this step's pairs (the launch is sized for the capacity)
-__global__ void k_sat(Ctx c) {
+__global__ __launch_bounds__(256) void k_sat(Ctx c) {
   int tp = total_pairs(c);
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < tp; p += gridDim.x * blockDim.x) sat_pair(c, p);
 }
@@ -1049,9 +1040,8 @@ static void batch_buffers(const StepPlan& P, Ctx& c, int32_t** scan_part, std::v
   add(&c.world_order, W * 4);
   add(&c.lx, NV * 8);
   add(&c.ly, NV * 8);
-  add(&c.ext_min, NV);
-  add(&c.ext_max, NV);
-  for (double** q : {&c.wx, &c.wy, &c.wnx, &c.wny}) add(q, NV * 8);
+  add(&c.ext, NV * 2);
+  add(&c.wv, NV * 32);
   for (double** q : {&c.bminx, &c.bmaxx, &c.bminy, &c.bmaxy}) add(q, NB * 8);
   add(&c.cnt_rank, (NB + 1) * 4);
   add(&c.off_rank, (NB + 1) * 4);
@@ -1729,12 +1719,8 @@ shp_status shp_batch_read_hulls(shp_batch* b, int32_t world, double* out) {
   HIPCHK(hipStreamSynchronize(b->stream));
   int v0 = b->h_vert_off[b->h_body_off[world]], v1 = b->h_vert_off[b->h_body_off[world + 1]];
   int n = v1 - v0;
-  std::vector<double> t(4 * (size_t)n);
-  double* src[4] = {b->ctx.wx, b->ctx.wy, b->ctx.wnx, b->ctx.wny};
-  for (int k = 0; k < 4; ++k)
-    HIPCHK(hipMemcpy(t.data() + (size_t)k * n, src[k] + v0, n * 8, hipMemcpyDeviceToHost));
-  for (int q = 0; q < n; ++q)
-    for (int k = 0; k < 4; ++k) out[4 * q + k] = t[(size_t)k * n + q];
+  // the device keeps (x, y, nx, ny) per vertex, the layout of `out`
+  if (n) HIPCHK(hipMemcpy(out, b->ctx.wv + v0, (size_t)n * 32, hipMemcpyDeviceToHost));
   return SHP_OK;
 }
 

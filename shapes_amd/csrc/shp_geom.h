@@ -71,16 +71,23 @@ __device__ __forceinline__ int clip_segment(d2 bp, d2 bn, d2 ip, d2 inn, d2 a, d
 }
 
 // A hull as the kernels see it: CSR offset, vertex count and device arrays.
+// A hull in world space: per vertex (x, y, the unit normal of the edge to
+// the next vertex) in 32 contiguous bytes, and per edge the extent vertex
+// indices (min | max << 8), so a hull of n vertices spans 32n + 2n bytes.
 struct HullRef {
-  const double* wx;
-  const double* wy;
-  const double* nx;
-  const double* ny;
-  const uint8_t* emin;
-  const uint8_t* emax;
+  const double4* vn;
+  const uint16_t* ext;
   int n;
-  __device__ __forceinline__ d2 v(int k) const { return mk(wx[k], wy[k]); }
-  __device__ __forceinline__ d2 nrm(int k) const { return mk(nx[k], ny[k]); }
+  __device__ __forceinline__ d2 v(int k) const {
+    const double2 p = *(const double2*)(vn + k);
+    return mk(p.x, p.y);
+  }
+  __device__ __forceinline__ d2 nrm(int k) const {
+    const double2 p = *((const double2*)(vn + k) + 1);
+    return mk(p.x, p.y);
+  }
+  __device__ __forceinline__ int emin(int e) const { return ext[e] & 255; }
+  __device__ __forceinline__ int emax(int e) const { return ext[e] >> 8; }
 };
 
 struct Overlap {
@@ -101,20 +108,22 @@ __device__ __forceinline__ void load_hull_v(const HullRef& h, HullV& r) {
 #pragma unroll
   for (int k = 0; k < SHP_MAXV; ++k)
     if (k < h.n) {
-      r.x[k] = h.wx[k];
-      r.y[k] = h.wy[k];
+      const d2 p = h.v(k);
+      r.x[k] = p.x;
+      r.y[k] = p.y;
     }
 }
 
-// overlap + minOverlap fold (Contact/SAT.hs:103-138; extentAlong',
+// overlap + minOverlap fold (Contact/SAT.hs:103-138; extentAlong
 // ConvexHull.hs:81-92): first separating edge wins; otherwise the first
 // strictly smallest depth.  Returns false if separated.
 __device__ __forceinline__ bool min_overlap(const HullRef& se, const HullV& sp, Overlap* out) {
   Overlap best{0, 0, 0.0};
   for (int e = 0; e < se.n; ++e) {
     d2 dir = se.nrm(e);
-    double smin = dot(se.v(se.emin[e]), dir);
-    double smax = dot(se.v(se.emax[e]), dir);
+    const int ex = se.ext[e];
+    double smin = dot(se.v(ex & 255), dir);
+    double smax = dot(se.v(ex >> 8), dir);
     double pmin = dot(mk(sp.x[0], sp.y[0]), dir), pmax = pmin;
     int imin = 0;
 #pragma unroll
@@ -192,7 +201,9 @@ __device__ __forceinline__ int clip_contacts(const HullRef& se, const HullRef& s
     np = 2;
   }
   double da = dot(a, n);
-  for (int k = 0; k < np; ++k) {
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (k >= np) break;
     out[k].depth = da - dot(q[k], n);  // contactDepth (HullVsHull.hs:24-37)
     out[k].nx = n.x;
     out[k].ny = n.y;
@@ -213,12 +224,14 @@ __device__ __forceinline__ int clip_contacts(const HullRef& se, const HullRef& s
 // Utils/Utils.hs:230-235).  *flip_out: 0 Same (A penetrated), 1 Flip.
 __device__ __forceinline__ int hull_contacts(const HullRef& A, const HullRef& B, Cgeom* out,
                                              int* flip_out) {
+  // one hull's vertices in registers at a time: B's against A's edges, then
+  // A's against B's edges (the same reads of the same values)
   Overlap oab, oba;
-  HullV va, vb;
-  load_hull_v(A, va);
-  load_hull_v(B, vb);
-  if (!min_overlap(A, vb, &oab)) return 0;
-  if (!min_overlap(B, va, &oba)) return 0;
+  HullV hv;
+  load_hull_v(B, hv);
+  if (!min_overlap(A, hv, &oab)) return 0;
+  load_hull_v(A, hv);
+  if (!min_overlap(B, hv, &oba)) return 0;
   if (oab.depth < oba.depth) {
     *flip_out = 0;
     return clip_contacts(A, B, oab, false, out);
@@ -277,9 +290,11 @@ __device__ __forceinline__ d2 crossz(d2 a, double z) { return mk(a.y * z, -(a.x 
 
 // GJK.closestSimplex (Contact/GJK.hs:52-146) of a hull to point o: the
 // simplex's vertex indices, most recently added first; 3 = encloses o.  The
+// support search reads the vertices held in registers (h), the simplex's
+// vertices by index come from the hull in memory (hr; the same values).  The
 // reference's loop has no bound; this one keeps the current simplex after
 // GJK_MAX_ITERATIONS (a hull of <= 8 vertices needs a few).
-__device__ __forceinline__ int closest_simplex(const HullV& h, d2 o, int* sv) {
+__device__ __forceinline__ int closest_simplex(const HullV& h, const HullRef& hr, d2 o, int* sv) {
   int n = 1;
   sv[0] = 0;
   d2 d = sub(o, mk(h.x[0], h.y[0]));
@@ -287,11 +302,11 @@ __device__ __forceinline__ int closest_simplex(const HullV& h, d2 o, int* sv) {
     const int ai = gjk_support(h, d);
     // extendSimplex (GJK.hs:70-86): a repeat ends the search
     if (sv[0] == ai || (n == 2 && sv[1] == ai)) return n;
-    const d2 a = mk(h.x[ai], h.y[ai]);
+    const d2 a = hr.v(ai);
     const d2 ao = sub(o, a);
     if (n == 1) {  // shiftSimplex2 (GJK.hs:95-109)
       const int bi = sv[0];
-      const d2 ab = sub(mk(h.x[bi], h.y[bi]), a);
+      const d2 ab = sub(hr.v(bi), a);
       if (same_direction(ab, ao)) {
         n = 2;
         sv[0] = ai;
@@ -305,7 +320,7 @@ __device__ __forceinline__ int closest_simplex(const HullV& h, d2 o, int* sv) {
     }
     // shiftSimplex3 (GJK.hs:111-137) on (a, b, c)
     const int bi = sv[0], ci = sv[1];
-    const d2 ab = sub(mk(h.x[bi], h.y[bi]), a), ac = sub(mk(h.x[ci], h.y[ci]), a);
+    const d2 ab = sub(hr.v(bi), a), ac = sub(hr.v(ci), a);
     const double abc = cross(ab, ac);
     const d2 abcac = zcross(abc, ac);
     const d2 ababc = crossz(ab, abc);
@@ -345,13 +360,13 @@ __device__ __forceinline__ int closest_simplex(const HullV& h, d2 o, int* sv) {
 // the penetrator; none for a centre inside the hull or farther than the
 // radius.
 __device__ __forceinline__ bool circle_hull_contact(d2 center, double r, const HullV& h,
-                                                    int* feature, Cgeom* g) {
+                                                    const HullRef& hr, int* feature, Cgeom* g) {
   int sv[3];
-  const int n = closest_simplex(h, center, sv);
+  const int n = closest_simplex(h, hr, center, sv);
   if (n == 3) return false;
-  d2 a = mk(h.x[sv[0]], h.y[sv[0]]);
+  d2 a = hr.v(sv[0]);
   if (n == 2) {  // closestAlong (:135-144)
-    const d2 p = a, q = mk(h.x[sv[1]], h.y[sv[1]]);
+    const d2 p = a, q = hr.v(sv[1]);
     const d2 ao = sub(center, p);
     const d2 abn = normalize(sub(q, p));
     const double t = dot(ao, abn);
