@@ -63,7 +63,7 @@ BROADPHASES = {"auto": engine.BROADPHASE_AUTO, "scan": engine.BROADPHASE_SCAN,
 # autotune candidates, description)
 WORKLOADS = {
     "pile_batch": (lambda w: scenes.pile_batch(n_worlds=w or 1024), 500, 4,
-                   [(b, t) for b in ("grid1", "grid16") for t in (512, 1024)],
+                   [("grid1", 512), ("grid1", 1024)],
                    "worlds of 2k-32k random 3-8-gons stacked in piles up to 60 layers"),
     "world_batch": (lambda w: scenes.world_batch_fast(n_worlds=w or 4096), 150, 4,
                     [(b, t) for b in ("grid1", "grid16", "scan") for t in (256, 512, 1024)],
