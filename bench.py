@@ -212,7 +212,7 @@ def kernel_bytes(phase, nb, nv, st, word):
     return 0.0
 
 
-def roofline(instr, scene, word):
+def roofline(instr, scene, word, broadphase="grid"):
     """The dominant kernel of the instrumented replay (the phase with the
     largest average HIP-event time among those holding a single kernel):
     algorithmic bytes per launch / its average launch duration."""
@@ -227,7 +227,7 @@ def roofline(instr, scene, word):
     achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {
         "bound": "hbm",
-        "kernel": engine.PHASE_KERNELS[dom][0] if len(engine.PHASE_KERNELS[dom]) == 1 else dom,
+        "kernel": phase_kernel(dom, word, broadphase),
         "phase": dom,
         "achieved": round(achieved, 3),
         "peak": HBM_PEAK_GBS,
@@ -238,6 +238,15 @@ def roofline(instr, scene, word):
         "avg_launch_us": round(ms * 1e3, 2),
         "launches_timed": instr["instrumented"]["steps"],
     }
+
+
+def phase_kernel(phase, word, broadphase):
+    """The kernel a single-kernel phase launches in this configuration."""
+    if phase == "sweeps":
+        return "k_sweep_colours" if word == 4 else "k_solve_flow"
+    if phase == "broadphase_fill":
+        return "k_broadphase" if broadphase == "scan" else "k_grid_pairs"
+    return engine.PHASE_KERNELS[phase][0]
 
 
 def step_bytes(nb, nv, contacts, word):
@@ -528,7 +537,7 @@ def main():
 
     word = 4 if solver_name == "colored" else 8
     nv = sum(int(w.hull_off[-1]) for w in scene.worlds)
-    roof = roofline(instr, scene, word)
+    roof = roofline(instr, scene, word, sched[0])
     sb = step_bytes(scene.n_bodies, nv, instr["instrumented"]["contacts"], word)
     step_gbs = sb / (ms * 1e-3) / 1e9
     out = {
