@@ -66,7 +66,8 @@ WORKLOADS = {
                    [("grid1", 512), ("grid1", 1024)],
                    "worlds of 2k-32k random 3-8-gons stacked in piles up to 60 layers"),
     "world_batch": (lambda w: scenes.world_batch_fast(n_worlds=w or 4096), 150, 4,
-                    [(b, t) for b in ("grid1", "grid16", "scan") for t in (256, 512, 1024)],
+                    [(b, t) for b in ("grid1", "grid16", "scan") for t in (64, 256)] +
+                    [("grid1", 512), ("grid1", 1024)],
                     "worlds of 8-1024 bodies (squares and 3-8-gons), each on its own floor"),
     "polygon_pile": (lambda w: scenes.polygon_pile(), 300, 0,
                      [(b, t) for b in ("grid", "scan") for t in (256, 512, 1024)],
@@ -212,7 +213,7 @@ def kernel_bytes(phase, nb, nv, st, word):
     return 0.0
 
 
-def roofline(instr, scene, word, broadphase="grid"):
+def roofline(instr, scene, word, broadphase="grid", threads=0):
     """The dominant kernel of the instrumented replay (the phase with the
     largest average HIP-event time among those holding a single kernel):
     algorithmic bytes per launch / its average launch duration."""
@@ -227,7 +228,8 @@ def roofline(instr, scene, word, broadphase="grid"):
     achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {
         "bound": "hbm",
-        "kernel": phase_kernel(dom, word, broadphase),
+        "kernel": phase_kernel(dom, word, broadphase, threads,
+                               max(w.n for w in scene.worlds)),
         "phase": dom,
         "achieved": round(achieved, 3),
         "peak": HBM_PEAK_GBS,
@@ -240,10 +242,22 @@ def roofline(instr, scene, word, broadphase="grid"):
     }
 
 
-def phase_kernel(phase, word, broadphase):
-    """The kernel a single-kernel phase launches in this configuration."""
+# worlds of at most this many bodies sweep one wavefront each when the
+# coloured solver runs with 64 lanes
+SWEEP_WAVE_MAX_BODIES = 1024
+
+
+def phase_kernel(phase, word, broadphase, threads=0, largest=0):
+    """The kernel a single-kernel phase launches in this configuration (the
+    coloured sweeps with 64 lanes: k_sweep_waves alone when every world
+    fits a wavefront's slice, else it and k_sweep_colours)."""
     if phase == "sweeps":
-        return "k_sweep_colours" if word == 4 else "k_solve_flow"
+        if word == 8:
+            return "k_solve_flow"
+        if threads != 64:
+            return "k_sweep_colours"
+        return ("k_sweep_waves" if largest <= SWEEP_WAVE_MAX_BODIES
+                else "k_sweep_colours+k_sweep_waves")
     if phase == "broadphase_fill":
         return "k_broadphase" if broadphase == "scan" else "k_grid_pairs"
     return engine.PHASE_KERNELS[phase][0]
@@ -427,7 +441,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="instrumented eager steps after the plain run's window")
     ap.add_argument("--broadphase", default="tune", choices=["tune"] + list(BROADPHASES))
-    ap.add_argument("--solver-threads", type=int, default=0, choices=[0, 256, 512, 1024])
+    ap.add_argument("--solver-threads", type=int, default=0, choices=[0, 64, 256, 512, 1024])
     ap.add_argument("--pairs-per-body", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -537,7 +551,7 @@ def main():
 
     word = 4 if solver_name == "colored" else 8
     nv = sum(int(w.hull_off[-1]) for w in scene.worlds)
-    roof = roofline(instr, scene, word, sched[0])
+    roof = roofline(instr, scene, word, sched[0], sched[1])
     sb = step_bytes(scene.n_bodies, nv, instr["instrumented"]["contacts"], word)
     step_gbs = sb / (ms * 1e-3) / 1e9
     out = {

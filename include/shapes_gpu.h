@@ -117,9 +117,12 @@ typedef struct shp_config {
                                  (SHP_E_UNSUPPORTED otherwise) */
   int32_t broadphase;         /* SHP_BROADPHASE_* */
   int32_t solver_threads;     /* lanes per world in the solve: 0 = by world
-                                 size, or 256 or 512, or 1024 for the coloured
-                                 solver (a schedule choice; the results do
-                                 not depend on it) */
+                                 size, or 256 or 512, or for the coloured
+                                 solver 1024, or 64: one wavefront per world
+                                 (several per workgroup) for worlds of at most
+                                 1024 bodies and 256 lanes for larger ones (a
+                                 schedule choice; the results do not depend
+                                 on it) */
 } shp_config;
 
 typedef struct shp_batch shp_batch;

@@ -942,6 +942,77 @@ __device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int b0,
   colour_finish<T>(c, w, p0, np, colour, top, serial_tail, s_off, s_cur);
 }
 
+// The LDS accesses of a wavefront before this point are done before any of
+// its accesses after it (the wavefront's lanes run together).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Worlds small enough for one wavefront each (world_order[first, W), at
+// most SWEEP_WAVE_MAX_BODIES bodies, FP32 velocities in the wavefront's
+// slice of LDS): the warm pass and the sweeps of sweep_colours, colour by
+// colour, lane t taking manifolds s + t, s + t + 64, ... of a colour, with
+// the wavefront's own ordering of its LDS accesses between colours instead
+// of workgroup barriers; SWEEP_WAVES worlds per workgroup.  The same
+// updates in the same colour order as sweep_world, so the same results.
+__global__ __launch_bounds__(64 * SWEEP_WAVES) void k_sweep_waves(Ctx c, int first_world,
+                                                                  int slice_floats) {
+  extern __shared__ int lds[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k = first_world + (int)blockIdx.x * SWEEP_WAVES + wave;
+  if (k >= c.W) return;  // this wavefront only: no workgroup barrier follows
+  const int w = c.world_order[k];
+  const int info = c.col_info[w];
+  const int ncol = info & 0xFFFF;
+  if (ncol == 0) return;
+  const bool serial_tail = (info >> 16) & 1;
+  const int p0 = c.wpair_off[w];
+  const int b0 = c.body_off[w], nb = c.body_off[w + 1] - b0;
+  const int* off = c.col_off + (long)w * (COLOR_CAP + 3);
+  VelF<true> V;
+  V.lds = (float*)lds + wave * slice_floats;
+  V.c = &c;
+  V.b0 = b0;
+  for (int i = lane; i < nb; i += 64) {
+    const int g = b0 + i;
+    const float v[3] = {(float)c.vx[g], (float)c.vy[g], (float)c.w[g]};
+    V.store(g, v);
+  }
+  wave_sync();
+  const PackF P = packf(c);
+  const RecW R = recw(c, P, p0);
+  const int phases = (c.iterations + 1) * ncol;
+  for (int h = 0; h < phases; ++h) {
+    const int l = h % ncol;
+    const bool warm = h < ncol;
+    const int s = off[l], e = off[l + 1];
+    if (serial_tail && l == ncol - 1) {  // the serial tail: one lane, in order
+      if (lane == 0)
+        for (int i = s; i < e; ++i) {
+          ManF m;
+          load_man(R, i, m);
+          solve_man<true>(R, i, m, warm, V);
+        }
+    } else {
+      for (int i = s + lane; i < e; i += 64) {
+        ManF m;
+        load_man(R, i, m);
+        solve_man<true>(R, i, m, warm, V);
+      }
+    }
+    wave_sync();
+  }
+  for (int i = lane; i < nb; i += 64) {
+    float v[3];
+    V.load(b0 + i, v);
+    c.fvx[b0 + i] = v[0];
+    c.fvy[b0 + i] = v[1];
+    c.fw[b0 + i] = v[2];
+  }
+}
+
 // Colouring, one workgroup per world, the largest worlds first: in LDS when
 // the world's colouring fits (by its pair count, else by its manifold
 // count), else in global memory.
@@ -1125,7 +1196,11 @@ void plan_colored_solver(StepPlan* P, int solver_threads) {
   const size_t fixed = (2 * (COLOR_CAP + 3) + 2) * 4;
   P->color_lds = std::min<size_t>(fixed + need, LDS_MAX_BYTES);
   P->color_pool = P->color_lds - fixed;
-  P->color_threads = solver_threads ? solver_threads : (nb > 2048 ? 512 : 256);
+  // 64: worlds of up to SWEEP_WAVE_MAX_BODIES sweep one wavefront each,
+  // the others (and the colouring) with 256 lanes
+  P->color_threads = solver_threads == 64 ? 256
+                     : solver_threads     ? solver_threads
+                                          : (nb > 2048 ? 512 : 256);
 }
 
 // k_colour: sm[4]; k_sweep_colours: s_off[COLOR_CAP + 3]
@@ -1139,6 +1214,13 @@ shp_status colored_alloc(shp_batch* b) {
                        : t == 512 ? (const void*)k_sweep_colours<512>
                                   : (const void*)k_sweep_colours<256>};
   const size_t ldsz[] = {P.color_lds, P.color_pool};
+  if (P.g_sweep_waves > 0 &&
+      hipFuncSetAttribute((const void*)k_sweep_waves, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)std::max<size_t>((size_t)SWEEP_WAVES * P.sweep_slice * 4, 16)) !=
+          hipSuccess) {
+    shp_set_error("hipFuncSetAttribute (wavefront sweeps LDS) failed");
+    return SHP_E_HIP;
+  }
   for (int k = 0; k < 2; ++k) {
     if (hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsz[k]) !=
         hipSuccess) {
@@ -1173,12 +1255,19 @@ void launch_colored_sweep(shp_batch* b) {
   const Ctx& c = b->ctx;
   const StepPlan& P = b->plan;
   const int pool = (int)P.color_pool;
-  if (P.color_threads == 1024)
-    k_sweep_colours<1024><<<c.W, 1024, P.color_pool, b->stream>>>(c, pool);
-  else if (P.color_threads == 512)
-    k_sweep_colours<512><<<c.W, 512, P.color_pool, b->stream>>>(c, pool);
-  else
-    k_sweep_colours<256><<<c.W, 256, P.color_pool, b->stream>>>(c, pool);
+  const unsigned big = (unsigned)P.sweep_big;
+  if (big > 0) {
+    if (P.color_threads == 1024)
+      k_sweep_colours<1024><<<big, 1024, P.color_pool, b->stream>>>(c, pool);
+    else if (P.color_threads == 512)
+      k_sweep_colours<512><<<big, 512, P.color_pool, b->stream>>>(c, pool);
+    else
+      k_sweep_colours<256><<<big, 256, P.color_pool, b->stream>>>(c, pool);
+  }
+  if (P.g_sweep_waves > 0)
+    k_sweep_waves<<<(unsigned)P.g_sweep_waves, 64 * SWEEP_WAVES,
+                    (size_t)SWEEP_WAVES * P.sweep_slice * 4, b->stream>>>(c, (int)big,
+                                                                         P.sweep_slice);
 }
 
 void launch_colored_unpack(shp_batch* b) {

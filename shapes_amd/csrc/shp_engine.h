@@ -22,6 +22,10 @@
 #define LDS_MAX_BYTES (LDS_BYTES_PER_CU - 512)
 // coloured solver: colour offsets per world; ordered solver: level bins
 #define COLOR_CAP 256
+// one wavefront per world in the coloured sweeps (solver_threads 64) for
+// worlds of up to this many bodies, SWEEP_WAVES wavefronts per workgroup
+#define SWEEP_WAVE_MAX_BODIES 1024
+#define SWEEP_WAVES 4
 #define LEVEL_CAP 4095
 // the coloured solver keeps a world's FP32 velocities (12 B per body) in
 // LDS up to this many bodies; beyond, in global memory
@@ -203,6 +207,11 @@ struct StepPlan {
   // coloured solver: lanes per world, dynamic LDS of the colouring, LDS pool
   int color_threads = 256;
   size_t color_lds = 0, color_pool = 0;
+  // with solver_threads 64: the worlds of at most SWEEP_WAVE_MAX_BODIES
+  // bodies (world_order[sweep_big, W)) sweep one wavefront each, several
+  // per workgroup, velocities in the wavefront's LDS slice (floats)
+  long sweep_big = 0, g_sweep_waves = 0;
+  int sweep_slice = 0;
   ScanPlan scan_buckets, scan_bodies, scan_pairs;
   long scan_parts_max = 0;  // scratch entries of the multi-workgroup scans
   // grids (workgroups) of the step's launches

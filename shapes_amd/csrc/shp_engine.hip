@@ -1140,8 +1140,10 @@ shp_status prepare_batch(const shp_world_desc* worlds, int n_worlds, const shp_c
   if (cfg.broadphase < SHP_BROADPHASE_AUTO || cfg.broadphase > SHP_BROADPHASE_GRID_16)
     return fail(SHP_E_INVAL, "unknown broadphase mode");
   if (cfg.solver_threads != 0 && cfg.solver_threads != 256 && cfg.solver_threads != 512 &&
-      !(cfg.solver_threads == 1024 && cfg.solver == SHP_SOLVER_COLORED_F32))
-    return fail(SHP_E_INVAL, "solver_threads must be 0, 256 or 512 (or 1024, coloured solver)");
+      !((cfg.solver_threads == 1024 || cfg.solver_threads == 64) &&
+        cfg.solver == SHP_SOLVER_COLORED_F32))
+    return fail(SHP_E_INVAL,
+                "solver_threads must be 0, 256 or 512 (or 64 or 1024, coloured solver)");
   if (cfg.max_pairs_per_body < 0) return fail(SHP_E_INVAL, "max_pairs_per_body must be >= 0");
   long NB = 0, NV = 0;
   for (int w = 0; w < n_worlds; ++w) {
@@ -1305,6 +1307,21 @@ shp_status plan_staged(const shp_config& cfg, HostBatch* hb) {
     plan_ordered_solver(&P, cfg.solver_threads);
   else
     plan_colored_solver(&P, cfg.solver_threads);
+  P.sweep_big = n_worlds;
+  if (cfg.solver == SHP_SOLVER_COLORED_F32 && cfg.solver_threads == 64) {
+    // worlds by descending size: the small ones form the tail of the order
+    long big = 0, small_max = 0;
+    for (int k = 0; k < n_worlds; ++k) {
+      const long nw = body_off[hb->order[k] + 1] - body_off[hb->order[k]];
+      if (nw > SWEEP_WAVE_MAX_BODIES)
+        ++big;
+      else
+        small_max = std::max(small_max, nw);
+    }
+    P.sweep_big = big;
+    P.sweep_slice = (int)((3 * small_max + 3) & ~3L);
+    P.g_sweep_waves = (n_worlds - big + SWEEP_WAVES - 1) / SWEEP_WAVES;
+  }
 
   const long PC = P.pair_cap, CC = P.contact_cap;
   P.g_reset = blocks(std::max<long>(64, P.use_grid ? P.grid_buckets + 1 : 0), 256);
@@ -1358,7 +1375,11 @@ void plan_launches(const StepPlan& P, bool infer, bool logging, std::vector<Laun
   if (infer) add("k_mlp_infer", P.g_infer, 256, 0, mlp_infer_static_lds());
   if (col) {
     add("k_pack_manifolds", P.g_pack, 256, 0, 0);
-    add("k_sweep_colours", P.W, P.color_threads, P.color_pool, colored_static_lds(true));
+    if (P.sweep_big > 0)
+      add("k_sweep_colours", P.sweep_big, P.color_threads, P.color_pool, colored_static_lds(true));
+    if (P.g_sweep_waves > 0)
+      add("k_sweep_waves", P.g_sweep_waves, 64 * SWEEP_WAVES,
+          (size_t)SWEEP_WAVES * P.sweep_slice * 4, 0);
     add("k_colour_unpack", P.g_unpack, 256, 0, 0);
   } else {
     add("k_solve_flow", P.W, P.solve_threads, std::max<size_t>(P.solve_lds, 16),

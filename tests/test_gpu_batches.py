@@ -40,14 +40,16 @@ def test_world_batch_full_run_ordered(orc, world_batch):
     gpu.close()
 
 
-def test_world_batch_full_run_colored(orc, world_batch):
-    """The same batch in the coloured FP32 solver (256 lanes per world, the
-    autotuned schedule) against the oracle's restatement of the coloured
-    schedule on 12 sampled worlds, bit for bit at steps 100, 200, 300."""
+@pytest.mark.parametrize("lanes", [256, 64])
+def test_world_batch_full_run_colored(orc, world_batch, lanes):
+    """The same batch in the coloured FP32 solver (256 lanes per world, or
+    one wavefront per world, several per workgroup) against the oracle's
+    restatement of the coloured schedule on 12 sampled worlds, bit for bit
+    at steps 100, 200, 300."""
     sc = world_batch
-    print(f"world batch: {sc.n_bodies} bodies, coloured, 300 steps", flush=True)
+    print(f"world batch: {sc.n_bodies} bodies, coloured, {lanes} lanes, 300 steps", flush=True)
     gpu = engine.make_batch(sc.worlds, sc.config, solver=engine.SOLVER_COLORED_F32,
-                            max_pairs_per_body=4, solver_threads=256)
+                            max_pairs_per_body=4, solver_threads=lanes)
     sample = _sample(sc, 12)
     refs = {k: orc.OracleEngine(sc.worlds[k], sc.config, solver=orc.SOLVER_COLORED, world_index=k)
             for k in sample}

@@ -126,3 +126,32 @@ def test_colored_matches_restatement_global_path(orc):
     rng = np.random.Generator(np.random.Philox(78))
     w = scenes.grid_world(rng, 24000, cols=200, spacing=1.05, y0=1.0, square_frac=0.5, vel=0.3)
     _colored_vs_restatement(orc, [w], scenes.SceneConfig(iterations=3), steps=4, every=2)
+
+
+def test_wavefront_sweeps_mixed_sizes_bitwise(orc):
+    """One wavefront per world for the worlds of at most 1024 bodies and a
+    256-lane workgroup for the larger ones, in one batch: every world equals
+    the oracle's restatement of the coloured schedule bit for bit."""
+    worlds = [scenes.random_world(70 + k, count=n, cols=max(8, int(n ** 0.5)))
+              for k, n in enumerate((1400, 300, 40, 1100, 900, 5))]
+    cfg = scenes.SceneConfig(iterations=6)
+    st, msg, p, rows = engine.plan(worlds, cfg, solver=engine.SOLVER_COLORED_F32,
+                                   solver_threads=64)
+    assert st == 0, msg
+    names = {r["name"]: r for r in rows}
+    assert names["k_sweep_colours"]["workgroups"] == 2  # the two worlds above 1024 bodies
+    assert names["k_sweep_waves"]["workgroups"] == 1    # four small worlds, one workgroup
+    gpu = engine.make_batch(worlds, cfg, solver=engine.SOLVER_COLORED_F32, solver_threads=64)
+    refs = [orc.OracleEngine(w, cfg, solver=orc.SOLVER_COLORED, world_index=k)
+            for k, w in enumerate(worlds)]
+    for step in range(4):
+        gpu.step_async(25, use_graph=True)
+        gpu.sync()
+        assert gpu.stats()["flags"] == 0
+        for k, r in enumerate(refs):
+            r.step(25)
+            assert np.array_equal(gpu.bodies(k).view(np.uint64), r.bodies().view(np.uint64)), \
+                f"world {k} step {25 * (step + 1)}"
+            assert np.array_equal(gpu.contacts(k)["lam"].view(np.uint64),
+                                  r.contacts()["lam"].view(np.uint64))
+    gpu.close()
