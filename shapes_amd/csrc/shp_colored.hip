@@ -19,7 +19,12 @@
 //
 // Records are per manifold: the normal, materials, inverse masses and
 // bodies once, then per contact the two cross terms of each row, bias,
-// lever arms, effective masses and the FP64 lagrangians.  The rows are
+// lever arms, effective masses and the FP64 lagrangians; a second contact
+// has a record of its own (a manifold with one contact reads nothing of
+// it).  Records are interleaved in blocks of 64 manifolds, word by word,
+// so a wavefront reading consecutive manifolds reads each word as one
+// contiguous kilobyte, a lane with 16-byte loads from one address.  The
+// rows are
 // rebuilt from them exactly (jn = (-u, jn2, u, jn5), jt = (-cw(u), jt2,
 // cw(u), jt5) with u the row normal; negation commutes with rounding to
 // FP32).  A lane keeps its slot of a colour in every pass, so the record
@@ -42,32 +47,22 @@ using namespace shp;
 #define GREEDY_COLOURS 16
 #define JP_ROUNDS (COLOR_CAP - GREEDY_COLOURS - 3)
 
-// per-contact FP32 fields (stride contact_cap), then the FP64 lagrangians,
-// then the per-manifold FP32 fields (stride pair_cap); manifold bodies in
-// pk_g (2 x pair_cap), flags in pk_f.
-// Manifold M's contacts sit in slots 2M and 2M + 1, so a field of both
-// loads as one 8-byte pair.
-//   JN2, JN5 / JT2, JT5: the angular entries of the normal / friction rows
-//   ICN, ICT: reciprocal effective masses
-//   WN2, WN5 / WT2, WT5: angular entries times the inverse inertias
-//   AX, AY / BX, BY: the normal times the inverse masses of a / b
-// Products and reciprocals are formed in FP64 and rounded once.
-enum {
-  CF_JN2 = 0,
-  CF_JN5,
-  CF_JT2,
-  CF_JT5,
-  CF_BN,
-  CF_ICN,
-  CF_ICT,
-  CF_WN2,
-  CF_WN5,
-  CF_WT2,
-  CF_WT5,
-  CF_FIELDS,
-  CF_LAMBDA = 12  // float offset of the lambdas (x contact_cap), 16-byte aligned
-};
-enum { MF_UX = 0, MF_UY, MF_MU, MF_E, MF_AX, MF_AY, MF_BX, MF_BY, MF_FIELDS };
+// Manifold M's record: 7 words of 16 bytes, word q at
+// rec[(M / 64) * 7 * 64 + q * 64 + M % 64]
+//   0  ux, uy, mu, e       the row normal (flip applied), materials
+//   1  ax, ay, bx, by      the normal times the inverse masses of a / b
+//   2  ga, gb, flags, 0    bodies (global ids), flags
+//   3  jn2, jn5, jt2, jt5  the first contact: angular entries of the
+//                          normal / friction rows
+//   4  bn, icn, ict, wn2   bias, reciprocal effective masses, angular
+//   5  wn5, wt2, wt5, 0      entries times the inverse inertias
+//   6  ln, lt              its lagrangians (FP64)
+// and the second contact's 4 words (3-6 above) at
+// rec2[(M / 64) * 4 * 64 + q * 64 + M % 64].  Products and reciprocals are
+// formed in FP64 and rounded once.
+enum { REC_WORDS = 7, REC2_WORDS = 4 };
+static_assert(16 * 64 * (REC_WORDS + REC2_WORDS) == 4 * COLOR_PACK_FLOATS(64),
+              "record words differ from the buffer size");
 // manifold flags: 1 a static, 2 b static, 4 first contact warm, 8 second
 // contact present, 16 second contact warm
 enum { MFL_A_STATIC = 1, MFL_B_STATIC = 2, MFL_WARM0 = 4, MFL_TWO = 8, MFL_WARM1 = 16 };
@@ -90,20 +85,24 @@ __device__ __forceinline__ float dot6f(const float* a, const float* b) {
 }
 
 struct PackF {
-  float* f;      // CF_FIELDS x CC, then lambdas, then MF_FIELDS x PC
-  double* lam;   // 2 x CC (normal, friction)
-  float* mf;     // MF_FIELDS x PC
-  long CC, PC;
+  float4* rec;   // REC_WORDS x PC (PC rounded up to blocks)
+  float4* rec2;  // REC2_WORDS x PC
 };
 
 __device__ __forceinline__ PackF packf(const Ctx& c) {
   PackF p;
-  p.CC = c.contact_cap;
-  p.PC = c.pair_cap;
-  p.f = (float*)c.pk;
-  p.lam = (double*)(p.f + (long)CF_LAMBDA * p.CC);
-  p.mf = p.f + (long)(CF_LAMBDA + 4) * p.CC;
+  p.rec = (float4*)c.pk;
+  p.rec2 = p.rec + (long)REC_WORDS * ((c.pair_cap + 63) / 64 * 64);
   return p;
+}
+
+// Word 0 of manifold M's record (words q at + 64 q) and of its second
+// contact's.
+__device__ __forceinline__ float4* rec_of(float4* rec, int M) {
+  return rec + (long)(M >> 6) * (REC_WORDS * 64) + (M & 63);
+}
+__device__ __forceinline__ float4* rec2_of(float4* rec2, int M) {
+  return rec2 + (long)(M >> 6) * (REC2_WORDS * 64) + (M & 63);
 }
 
 struct ConF {
@@ -117,131 +116,102 @@ struct ManF {
   ConF k[2];
 };
 
-// The records of one world: every pointer is offset to the world's first
-// manifold, so a manifold's fields load from a workgroup-uniform base and
-// one 32-bit byte offset per lane (the worlds' manifold counts stay below
-// 2^27, checked at batch creation).
+// The records and the first manifold of one world.
 struct RecW {
-  const float* f;     // field q of slot 2m + j at f[q * CC + 2m + j]
-  double* lam;        // normal lambda of slot 2m + j at lam[2m + j], friction at + CC
-  const float* mf;    // field q of manifold m at mf[q * PC + m]
-  const int* g;       // body a of manifold m at g[m], body b at g[PC + m]
-  const uint8_t* fl;  // flags of manifold m
-  long CC, PC;
+  float4* rec;
+  float4* rec2;
+  int p0;
 };
 
 __device__ __forceinline__ RecW recw(const Ctx& c, const PackF& P, int p0) {
   RecW r;
-  r.f = P.f + 2L * p0;
-  r.lam = P.lam + 2L * p0;
-  r.mf = P.mf + p0;
-  r.g = c.pk_g + p0;
-  r.fl = c.pk_f + p0;
-  r.CC = P.CC;
-  r.PC = P.PC;
+  r.rec = P.rec;
+  r.rec2 = P.rec2;
+  r.p0 = p0;
   return r;
 }
 
-template <class V>
-__device__ __forceinline__ V ldo(const void* base, uint32_t byte_off) {
-  return *(const V*)((const char*)base + byte_off);
+// Words 3-6 of a record: one contact.
+__device__ __forceinline__ void load_contact(const float4* w, ConF& k) {
+  const float4 a = w[0], b = w[64], d = w[128];
+  const double2 l = *(const double2*)(w + 192);
+  k.jn2 = a.x;
+  k.jn5 = a.y;
+  k.jt2 = a.z;
+  k.jt5 = a.w;
+  k.bn = b.x;
+  k.icn = b.y;
+  k.ict = b.z;
+  k.wn2 = b.w;
+  k.wn5 = d.x;
+  k.wt2 = d.y;
+  k.wt5 = d.z;
+  k.ln = l.x;
+  k.lt = l.y;
 }
 
-// Manifold m of the world (local index) and its contact slots 2m, 2m + 1.
-// Both slots load, used or not: slot 2m + 1 of a one-contact manifold
-// holds whatever an earlier step left there and is never solved.
+// Manifold m of the world (local index) and its contacts; the second
+// contact's record loads only when the manifold has one.
 __device__ __forceinline__ void load_man(const RecW& R, int m_loc, ManF& m) {
-  const uint32_t o4 = (uint32_t)m_loc * 4u, o8 = (uint32_t)m_loc * 8u, o16 = (uint32_t)m_loc * 16u;
-  const long PC = R.PC, CC = R.CC;
-  m.ux = ldo<float>(R.mf + MF_UX * PC, o4);
-  m.uy = ldo<float>(R.mf + MF_UY * PC, o4);
-  m.mu = ldo<float>(R.mf + MF_MU * PC, o4);
-  m.e = ldo<float>(R.mf + MF_E * PC, o4);
-  m.ax = ldo<float>(R.mf + MF_AX * PC, o4);
-  m.ay = ldo<float>(R.mf + MF_AY * PC, o4);
-  m.bx = ldo<float>(R.mf + MF_BX * PC, o4);
-  m.by = ldo<float>(R.mf + MF_BY * PC, o4);
-  m.ga = ldo<int>(R.g, o4);
-  m.gb = ldo<int>(R.g + PC, o4);
-  m.flags = ldo<uint8_t>(R.fl, (uint32_t)m_loc);
-  float2 x[CF_FIELDS];
-#pragma unroll
-  for (int q = 0; q < CF_FIELDS; ++q) x[q] = ldo<float2>(R.f + q * CC, o8);
-  const double2 ln = ldo<double2>(R.lam, o16), lt = ldo<double2>(R.lam + CC, o16);
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    ConF& k = m.k[j];
-    k.jn2 = j ? x[CF_JN2].y : x[CF_JN2].x;
-    k.jn5 = j ? x[CF_JN5].y : x[CF_JN5].x;
-    k.jt2 = j ? x[CF_JT2].y : x[CF_JT2].x;
-    k.jt5 = j ? x[CF_JT5].y : x[CF_JT5].x;
-    k.bn = j ? x[CF_BN].y : x[CF_BN].x;
-    k.icn = j ? x[CF_ICN].y : x[CF_ICN].x;
-    k.ict = j ? x[CF_ICT].y : x[CF_ICT].x;
-    k.wn2 = j ? x[CF_WN2].y : x[CF_WN2].x;
-    k.wn5 = j ? x[CF_WN5].y : x[CF_WN5].x;
-    k.wt2 = j ? x[CF_WT2].y : x[CF_WT2].x;
-    k.wt5 = j ? x[CF_WT5].y : x[CF_WT5].x;
-    k.ln = j ? ln.y : ln.x;
-    k.lt = j ? lt.y : lt.x;
-  }
+  const float4* r = rec_of(R.rec, R.p0 + m_loc);
+  const float4 a = r[0], b = r[64];
+  const int4 g = *(const int4*)(r + 128);
+  m.ux = a.x;
+  m.uy = a.y;
+  m.mu = a.z;
+  m.e = a.w;
+  m.ax = b.x;
+  m.ay = b.y;
+  m.bx = b.z;
+  m.by = b.w;
+  m.ga = g.x;
+  m.gb = g.y;
+  m.flags = g.z;
+  load_contact(r + 192, m.k[0]);
+  if (m.flags & MFL_TWO) load_contact(rec2_of(R.rec2, R.p0 + m_loc), m.k[1]);
 }
 
 // Pack the <= 2 contacts [k0, k1) of a pair as manifold M.  Every value is
 // loaded before the first store.
 __device__ __forceinline__ void pack_man(const Ctx& c, const PackF& P, int k0, int k1, int M) {
-  const long PC = P.PC, CC = P.CC;
+  const long CC = c.contact_cap;
   const int ga = c.ga[k0], gb = c.gb[k0];
   const double ux = c.jn[3 * CC + k0], uy = c.jn[4 * CC + k0];  // the row normal (flip applied)
   const double ima = c.iml[ga], ira = c.imr[ga], imb = c.iml[gb], irb = c.imr[gb];
-  float mf[MF_FIELDS];
-  mf[MF_UX] = (float)ux;
-  mf[MF_UY] = (float)uy;
-  mf[MF_MU] = (float)((c.mu[ga] + c.mu[gb]) / 2.0);
-  mf[MF_E] = (float)hs_min(c.e[ga], c.e[gb]);
-  mf[MF_AX] = (float)(ima * ux);
-  mf[MF_AY] = (float)(ima * uy);
-  mf[MF_BX] = (float)(imb * ux);
-  mf[MF_BY] = (float)(imb * uy);
+  const float4 w0 = make_float4((float)ux, (float)uy, (float)((c.mu[ga] + c.mu[gb]) / 2.0),
+                                (float)hs_min(c.e[ga], c.e[gb]));
+  const float4 w1 = make_float4((float)(ima * ux), (float)(ima * uy), (float)(imb * ux),
+                                (float)(imb * uy));
   const bool two = k1 - k0 > 1;
   int flags = (c.is_static[ga] ? MFL_A_STATIC : 0) | (c.is_static[gb] ? MFL_B_STATIC : 0) |
               (c.warm[k0] ? MFL_WARM0 : 0) | (two ? MFL_TWO : 0);
-  float cf[2][CF_FIELDS];
-  double lam[2][2];
+  float4 cw[2][3];
+  double2 lam[2];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     if (j == 1 && !two) break;
     const int k = k0 + j;
     const double jn2 = c.jn[2 * CC + k], jn5 = c.jn[5 * CC + k];
     const double jt2 = c.jt[2 * CC + k], jt5 = c.jt[5 * CC + k];
-    cf[j][CF_JN2] = (float)jn2;
-    cf[j][CF_JN5] = (float)jn5;
-    cf[j][CF_JT2] = (float)jt2;
-    cf[j][CF_JT5] = (float)jt5;
-    cf[j][CF_BN] = (float)c.bn[k];
-    cf[j][CF_ICN] = (float)(1.0 / c.mcn[k]);
-    cf[j][CF_ICT] = (float)(1.0 / c.mct[k]);
-    cf[j][CF_WN2] = (float)(ira * jn2);
-    cf[j][CF_WN5] = (float)(irb * jn5);
-    cf[j][CF_WT2] = (float)(ira * jt2);
-    cf[j][CF_WT5] = (float)(irb * jt5);
-    lam[j][0] = c.ln[k];
-    lam[j][1] = c.lt[k];
+    cw[j][0] = make_float4((float)jn2, (float)jn5, (float)jt2, (float)jt5);
+    cw[j][1] = make_float4((float)c.bn[k], (float)(1.0 / c.mcn[k]), (float)(1.0 / c.mct[k]),
+                           (float)(ira * jn2));
+    cw[j][2] = make_float4((float)(irb * jn5), (float)(ira * jt2), (float)(irb * jt5), 0.0f);
+    lam[j] = make_double2(c.ln[k], c.lt[k]);
     if (j == 1 && c.warm[k]) flags |= MFL_WARM1;
   }
-#pragma unroll
-  for (int q = 0; q < MF_FIELDS; ++q) P.mf[q * PC + M] = mf[q];
-  c.pk_g[M] = ga;
-  c.pk_g[PC + M] = gb;
-  c.pk_f[M] = (uint8_t)flags;
+  float4* r = rec_of(P.rec, M);
+  r[0] = w0;
+  r[64] = w1;
+  *(int4*)(r + 128) = make_int4(ga, gb, flags, 0);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     if (j == 1 && !two) break;
-    const long s = 2L * M + j;
-#pragma unroll
-    for (int q = 0; q < CF_FIELDS; ++q) P.f[q * CC + s] = cf[j][q];
-    P.lam[s] = lam[j][0];
-    P.lam[CC + s] = lam[j][1];
+    float4* w = j ? rec2_of(P.rec2, M) : r + 192;
+    w[0] = cw[j][0];
+    w[64] = cw[j][1];
+    w[128] = cw[j][2];
+    *(double2*)(w + 192) = lam[j];
   }
 }
 
@@ -333,10 +303,10 @@ __device__ __forceinline__ void solve_man(const RecW& R, int m_loc, ManF& m, boo
     changed = true;
   }
   if (!changed) return;
-  if (!warm) {  // both slots' lagrangians, as one 16-byte store each
-    const uint32_t o16 = (uint32_t)m_loc * 16u;
-    *(double2*)((char*)R.lam + o16) = make_double2(m.k[0].ln, m.k[1].ln);
-    *(double2*)((char*)(R.lam + R.CC) + o16) = make_double2(m.k[0].lt, m.k[1].lt);
+  if (!warm) {  // the solved contacts' lagrangians
+    *(double2*)(rec_of(R.rec, R.p0 + m_loc) + 6 * 64) = make_double2(m.k[0].ln, m.k[0].lt);
+    if (nk > 1)
+      *(double2*)(rec2_of(R.rec2, R.p0 + m_loc) + 3 * 64) = make_double2(m.k[1].ln, m.k[1].lt);
   }
   if (!(m.flags & MFL_A_STATIC)) V.store(m.ga, v);
   if (!(m.flags & MFL_B_STATIC)) V.store(m.gb, v + 3);
@@ -1051,16 +1021,15 @@ __global__ __launch_bounds__(T) void k_colour(Ctx c, int pool_bytes) {
 // A manifold whose contacts lie beyond the contact capacity (the step then
 // reports the overflow): a record that touches no velocity and no contact.
 __device__ __forceinline__ void pack_null(const Ctx& c, const PackF& P, int p, int M) {
-  const long PC = P.PC, CC = P.CC, s = 2L * M;
-#pragma unroll
-  for (int q = 0; q < MF_FIELDS; ++q) P.mf[q * PC + M] = 0.0f;
-  c.pk_g[M] = c.pa[p];
-  c.pk_g[PC + M] = c.pb[p];
-  c.pk_f[M] = (uint8_t)(MFL_A_STATIC | MFL_B_STATIC);
-#pragma unroll
-  for (int q = 0; q < CF_FIELDS; ++q) P.f[q * CC + s] = 0.0f;
-  P.lam[s] = 0.0;
-  P.lam[CC + s] = 0.0;
+  float4* r = rec_of(P.rec, M);
+  const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  r[0] = z;
+  r[64] = z;
+  *(int4*)(r + 128) = make_int4(c.pa[p], c.pb[p], MFL_A_STATIC | MFL_B_STATIC, 0);
+  r[192] = z;
+  r[256] = z;
+  r[320] = z;
+  r[384] = z;
 }
 
 // Every coloured pair's <= 2 contacts become its manifold record, over the
@@ -1165,8 +1134,10 @@ __global__ void k_colour_unpack(Ctx c) {
         const int k0 = c.coff[t];
         const int nk = min(c.ccount[t], c.contact_cap - k0);  // contacts that were stored
         for (int j = 0; j < nk; ++j) {
-          c.ln[k0 + j] = P.lam[2L * M + j];
-          c.lt[k0 + j] = P.lam[P.CC + 2L * M + j];
+          const double2 l =
+              *(const double2*)(j ? rec2_of(P.rec2, M) + 3 * 64 : rec_of(P.rec, M) + 6 * 64);
+          c.ln[k0 + j] = l.x;
+          c.lt[k0 + j] = l.y;
         }
       }
     }
@@ -1181,7 +1152,6 @@ __global__ void k_colour_unpack(Ctx c) {
   }
 }
 
-static_assert(CF_LAMBDA + 4 == 16 && MF_FIELDS == 8, "record layout differs from COLOR_PACK_FLOATS");
 
 // LDS of the colouring: colour offsets and cursors, then a pool for the
 // per-body priorities and colour masks and the manifold list (pairs per

@@ -30,14 +30,13 @@
 // the coloured solver keeps a world's FP32 velocities (12 B per body) in
 // LDS up to this many bodies; beyond, in global memory
 #define VEL_LDS_MAX_BODIES ((LDS_MAX_BYTES - (2 * (COLOR_CAP + 3) + 2) * 4) / 12)
-// solver records: the ordered solver's compact FP64 record (doubles per
-// contact); the coloured solver's FP32 fields of both contact slots, their
-// FP64 lagrangians and the per-manifold fields (floats for CC contact
-// slots and PC manifolds)
+// the ordered solver's compact FP64 record (doubles per contact)
 #define FLOW_RECORD_DOUBLES 21
 // an LDS array's footprint: the compiler places each on a 16-byte boundary
 constexpr size_t lds16(size_t bytes) { return (bytes + 15) & ~(size_t)15; }
-#define COLOR_PACK_FLOATS(CC, PC) (16L * (CC) + 8L * (PC))
+// coloured solver records: 112 B per manifold and 64 B for its second
+// contact, in blocks of 64 manifolds (PC manifolds)
+#define COLOR_PACK_FLOATS(PC) (44L * (((PC) + 63) / 64 * 64))
 
 // pair_ab words (coloured solver): world-local body ids a | b << 16, NO_BODY
 // for a static body, NO_PAIR for a pair without contacts
@@ -134,8 +133,8 @@ struct Ctx {
   // contact order for the ordered solver, FP32 records + FP64 lambdas in
   // colour order for the coloured one)
   double* pk;
-  int32_t* pk_g;  // bodies a, b: 2 x contact_cap (ordered), 2 x pair_cap (coloured)
-  uint8_t* pk_f;  // flags per contact (ordered) or manifold (coloured)
+  int32_t* pk_g;  // bodies a, b: 2 x contact_cap (ordered solver)
+  uint8_t* pk_f;  // flags per contact (ordered solver)
   unsigned long long* col_best;  // colouring priorities per body (large worlds)
   uint32_t* col_used;            // colours taken at each body (large worlds)
   int32_t* col_list;             // 2 x pair_cap: uncoloured manifolds per round (large worlds)

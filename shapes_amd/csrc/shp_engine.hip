@@ -1101,9 +1101,7 @@ static void batch_buffers(const StepPlan& P, Ctx& c, int32_t** scan_part, std::v
     add(&c.flow_lvl, CC * 4);
     add(&c.flow_order, CC * 4);
   } else {
-    add(&c.pk, COLOR_PACK_FLOATS(CC, PC) * 4);
-    add(&c.pk_g, 2 * PC * 4);
-    add(&c.pk_f, PC);
+    add(&c.pk, COLOR_PACK_FLOATS(PC) * 4);
     add(&c.sched_level, PC * 4);
     add(&c.sched_order, PC * 4);
     add(&c.fvx, NB * 4);
