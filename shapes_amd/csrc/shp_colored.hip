@@ -406,14 +406,21 @@ __device__ void sweep_colours(const Ctx& c, const RecW& R, int ncol, bool serial
       const int end = ce;
       int i = first_of(l, cs, ce);
       if (i < end && !loaded) load_man(R, i, m);
-      // the lane's next manifold of the colour loads while this one solves
+      // the lane's next manifold of the colour loads while this one
+      // solves, into the other of two records (m, n) in turn, so no record
+      // is copied
       while (i < end) {
-        const int inext = i + di;
-        const bool more = inext < end;
+        int inext = i + di;
+        bool more = inext < end;
         if (more) load_man(R, inext, n);
         solve_man<VLDS>(R, i, m, warm, V);
         if (!more) break;
-        m = n;
+        i = inext;
+        inext = i + di;
+        more = inext < end;
+        if (more) load_man(R, inext, m);
+        solve_man<VLDS>(R, i, n, warm, V);
+        if (!more) break;
         i = inext;
       }
       loaded = false;
