@@ -373,114 +373,73 @@ __device__ void assign_slots(int p0, int np, const CT* colour, int* scur, int ta
   }
 }
 
-// Warm-start pass and sweeps, colour by colour.  Phase h = pass * ncol + l;
-// thread t takes manifolds s_off[l] + t, + T, ... of colour l (the last
-// colour serially by thread 0 when colouring stopped after JP_ROUNDS).
+// Warm-start pass and sweeps, colour by colour, with prefetching.  Phase
+// h = pass * ncol + l; lane t of STEP takes manifolds off[l] + t, + STEP,
+// ... of colour l (the last colour serially by lane 0 when colouring
+// stopped after JP_ROUNDS), and `sync` ends each phase.  Two records per
+// lane: the lane's next manifold of a colour loads into the other record
+// while the current one solves, and its first manifold of the next colour
+// loads before the phase ends (not with one colour: that is the manifold
+// just solved, whose lagrangians the pass changes).  The colour bounds of
+// this phase (cs, ce) and the next (ns, ne) stay in registers; those of
+// the phase after next are read early in the phase.
+template <bool VLDS, int STEP, class Sync>
+__device__ __forceinline__ void sweep_prefetched(const Ctx& c, const RecW& R, int ncol,
+                                                 bool serial_tail, const int* off, int lane,
+                                                 const VelF<VLDS>& V, Sync sync) {
+  const int phases = (c.iterations + 1) * ncol;
+  auto first_of = [&](int l, int b0_, int b1_) {
+    const bool tail = serial_tail && l == ncol - 1;
+    return tail ? (lane == 0 ? b0_ : b1_) : b0_ + lane;
+  };
+  int cs = off[0], ce = off[1];
+  int ns = off[1 % ncol], ne = off[1 % ncol + 1];
+  ManF m, n;
+  bool loaded = false;
+  for (int h = 0; h < phases; ++h) {
+    const int l = h % ncol, l1 = (h + 1) % ncol, l2 = (h + 2) % ncol;
+    const int s2 = off[l2], e2 = off[l2 + 1];
+    const bool warm = h < ncol;
+    const int di = serial_tail && l == ncol - 1 ? 1 : STEP;
+    const int end = ce;
+    int i = first_of(l, cs, ce);
+    if (i < end && !loaded) load_man(R, i, m);
+    while (i < end) {
+      int inext = i + di;
+      bool more = inext < end;
+      if (more) load_man(R, inext, n);
+      solve_man<VLDS>(R, i, m, warm, V);
+      if (!more) break;
+      i = inext;
+      inext = i + di;
+      more = inext < end;
+      if (more) load_man(R, inext, m);
+      solve_man<VLDS>(R, i, n, warm, V);
+      if (!more) break;
+      i = inext;
+    }
+    loaded = false;
+    if (ncol > 1 && h + 1 < phases) {
+      const int j = first_of(l1, ns, ne);
+      if (j < ne) {
+        load_man(R, j, m);
+        loaded = true;
+      }
+    }
+    sync();
+    cs = ns;
+    ce = ne;
+    ns = s2;
+    ne = e2;
+  }
+}
+
+// Warm-start pass and sweeps of one world by one workgroup.
 template <bool VLDS, int T>
 __device__ void sweep_colours(const Ctx& c, const RecW& R, int ncol, bool serial_tail,
                               const int* s_off, const VelF<VLDS>& V) {
-  const int tid = threadIdx.x;
-  const int phases = (c.iterations + 1) * ncol;
-  if constexpr (T > 512) {
-    // 16 waves leave room for two records per lane: the lane's next
-    // manifold of a colour loads while the current one solves, and its
-    // first manifold of the next colour loads before the barrier (not with
-    // one colour: that is the manifold just solved, whose lagrangians the
-    // pass changes)
-    // the colour bounds of this phase (cs, ce) and the next (ns, ne) stay
-    // in registers; those of the phase after next are read from LDS early
-    // in the phase, so no LDS read sits between a barrier and the work
-    auto first_of = [&](int l, int b0_, int b1_) {
-      const bool tail = serial_tail && l == ncol - 1;
-      return tail ? (tid == 0 ? b0_ : b1_) : b0_ + tid;
-    };
-    int cs = s_off[0], ce = s_off[1];
-    int ns = s_off[1 % ncol], ne = s_off[1 % ncol + 1];
-    ManF m, n;
-    bool loaded = false;
-    for (int h = 0; h < phases; ++h) {
-      const int l = h % ncol, l1 = (h + 1) % ncol, l2 = (h + 2) % ncol;
-      const int s2 = s_off[l2], e2 = s_off[l2 + 1];
-      const bool warm = h < ncol;
-      const int di = serial_tail && l == ncol - 1 ? 1 : T;
-      const int end = ce;
-      int i = first_of(l, cs, ce);
-      if (i < end && !loaded) load_man(R, i, m);
-      // the lane's next manifold of the colour loads while this one
-      // solves, into the other of two records (m, n) in turn, so no record
-      // is copied
-      while (i < end) {
-        int inext = i + di;
-        bool more = inext < end;
-        if (more) load_man(R, inext, n);
-        solve_man<VLDS>(R, i, m, warm, V);
-        if (!more) break;
-        i = inext;
-        inext = i + di;
-        more = inext < end;
-        if (more) load_man(R, inext, m);
-        solve_man<VLDS>(R, i, n, warm, V);
-        if (!more) break;
-        i = inext;
-      }
-      loaded = false;
-      if (ncol > 1 && h + 1 < phases) {
-        const int j = first_of(l1, ns, ne);
-        if (j < ne) {
-          load_man(R, j, m);
-          loaded = true;
-        }
-      }
-      __syncthreads();
-      cs = ns;
-      ce = ne;
-      ns = s2;
-      ne = e2;
-    }
-    return;
-  }
-  auto first = [&](int h, int& M) {
-    const int l = h % ncol;
-    if (serial_tail && l == ncol - 1) return false;
-    M = s_off[l] + tid;
-    return M < s_off[l + 1];
-  };
-  // with one colour the next phase is the same manifold, whose lagrangians
-  // this phase changes: load it when it is used instead
-  const bool prefetch = ncol > 1;
-  ManF cur, nxt;
-  int Mc = 0, Mn = 0;
-  bool has = phases > 0 && first(0, Mc);
-  if (has) load_man(R, Mc, cur);
-  for (int h = 0; h < phases; ++h) {
-    const int l = h % ncol;
-    const bool warm = h < ncol;
-    const bool hasn = h + 1 < phases && first(h + 1, Mn);
-    if (hasn && prefetch) load_man(R, Mn, nxt);
-    if (serial_tail && l == ncol - 1) {
-      if (tid == 0) {
-        for (int i = s_off[l]; i < s_off[l + 1]; ++i) {
-          ManF m;
-          load_man(R, i, m);
-          solve_man<VLDS>(R, i, m, warm, V);
-        }
-      }
-    } else {
-      if (has) solve_man<VLDS>(R, Mc, cur, warm, V);
-      for (int i = Mc + T; i < s_off[l + 1]; i += T) {
-        ManF m;
-        load_man(R, i, m);
-        solve_man<VLDS>(R, i, m, warm, V);
-      }
-    }
-    __syncthreads();
-    Mc = Mn;
-    has = hasn;
-    if (prefetch)
-      cur = nxt;
-    else if (has)
-      load_man(R, Mc, cur);
-  }
+  sweep_prefetched<VLDS, T>(c, R, ncol, serial_tail, s_off, (int)threadIdx.x, V,
+                            [] { __syncthreads(); });
 }
 
 // Colour-order bookkeeping shared by both kernels: per-colour counts of the
@@ -959,28 +918,8 @@ __global__ __launch_bounds__(64 * SWEEP_WAVES) void k_sweep_waves(Ctx c, int fir
   }
   wave_sync();
   const PackF P = packf(c);
-  const RecW R = recw(c, P, p0);
-  const int phases = (c.iterations + 1) * ncol;
-  for (int h = 0; h < phases; ++h) {
-    const int l = h % ncol;
-    const bool warm = h < ncol;
-    const int s = off[l], e = off[l + 1];
-    if (serial_tail && l == ncol - 1) {  // the serial tail: one lane, in order
-      if (lane == 0)
-        for (int i = s; i < e; ++i) {
-          ManF m;
-          load_man(R, i, m);
-          solve_man<true>(R, i, m, warm, V);
-        }
-    } else {
-      for (int i = s + lane; i < e; i += 64) {
-        ManF m;
-        load_man(R, i, m);
-        solve_man<true>(R, i, m, warm, V);
-      }
-    }
-    wave_sync();
-  }
+  sweep_prefetched<true, 64>(c, recw(c, P, p0), ncol, serial_tail, off, lane, V,
+                             [] { wave_sync(); });
   for (int i = lane; i < nb; i += 64) {
     float v[3];
     V.load(b0 + i, v);
