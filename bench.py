@@ -176,12 +176,13 @@ def kernel_bytes(phase, nb, nv, st, word):
     P, C = st["pairs"], st["contacts"]
     M = min(P, C)  # manifolds: pairs with contacts
     if phase == "sweeps":
-        # per manifold its FP32 record (normal, materials, normal x inverse
-        # masses: 32 B; bodies 8 B; flags 1 B; two contact slots of 11 FP32
-        # fields and two FP64 lagrangians: 120 B) read and the lagrangians
-        # written (32 B); per body the FP64 velocity read (24 B) and the FP32
+        # per manifold its record (normal, materials, normal x inverse
+        # masses, bodies, flags, first contact's 11 FP32 fields and two FP64
+        # lagrangians: 112 B) read and the lagrangians written (16 B); per
+        # second contact its record (64 B) read and lagrangians written
+        # (16 B); per body the FP64 velocity read (24 B) and the FP32
         # velocity written (12 B)
-        return M * (41 + 120 + 32) + nb * (24 + 12)
+        return M * (112 + 16) + (C - M) * (64 + 16) + nb * (24 + 12)
     if phase == "colouring":
         # per pair its body word and contact count read, colour and slot
         # written (16 B); per body its colour mask and priority (12 B)
@@ -201,7 +202,7 @@ def kernel_bytes(phase, nb, nv, st, word):
     if phase == "contacts":
         return C * (43 + 2 * 40 + 24) + C * 226
     if phase == "solve_setup":  # manifold packing: contact rows in, records out
-        return C * (8 + 8 * 8 + 6 * 8 + 1) + M * (41 + 120)
+        return C * (8 + 8 * 8 + 6 * 8 + 1) + M * 112 + (C - M) * 64
     if phase == "solve_writeback":  # lagrangians back, FP32 -> FP64 velocities
         return C * (16 + 16) + nb * (12 + 48)
     if phase == "broadphase_fill":
