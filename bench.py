@@ -521,6 +521,7 @@ def main():
             q = r_plain["quality"]
         extra["headline"] = {
             "value_is": value_is,
+            "model": os.path.relpath(args.model, ROOT),
             "quality_matched": chosen is not None,
             "quality_tolerance": QUALITY_TOL,
             "plain_gpu": plain,

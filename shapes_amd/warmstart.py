@@ -7,7 +7,9 @@ the MLP on the GPU, evaluate it against the zero start.
 
 Training data: (features, targets) of every contact that missed the cache,
 logged after the plain solver (zero start, coloured FP32, 8 sweeps) on
-polygon-pile worlds whose seeds differ from the evaluation seed.  Targets
+polygon-pile worlds (dt 1/60; the committed warmstart_mlp.npy) or on
+batches of piles (`--scene pile_batch`, dt 1/120, logged after `--skip`
+settling steps) whose seeds differ from the evaluation seeds.  Targets
 are the solver's final impulses times the rows' inverse effective masses
 (velocity units).  The model is trained with Adam on mini-batches by the
 library's own BF16 matrix-core forward/backward kernels.
@@ -47,13 +49,22 @@ def init_params(seed: int = 0) -> np.ndarray:
 
 
 def collect(samples: int, count: int = 8192, steps: int = 600, iterations: int = 8,
-            solver: int = engine.SOLVER_COLORED_F32, log=print):
+            solver: int = engine.SOLVER_COLORED_F32, scene: str = "polygon_pile",
+            worlds: int = 64, skip: int = 0, log=print):
     feats, tgts = [], []
     total = 0
     seed = TRAIN_SEEDS_FROM
     while total < samples:
-        sc = scenes.polygon_pile(seed=seed, count=count, steps=steps, iterations=iterations)
-        b = engine.make_batch(sc.worlds, sc.config, solver=solver)
+        if scene == "pile_batch":
+            sc = scenes.pile_batch(seed=seed, n_worlds=worlds, iterations=iterations)
+            b = engine.make_batch(sc.worlds, sc.config, solver=solver,
+                                  broadphase=engine.BROADPHASE_GRID_1, max_pairs_per_body=4)
+        else:
+            sc = scenes.polygon_pile(seed=seed, count=count, steps=steps, iterations=iterations)
+            b = engine.make_batch(sc.worlds, sc.config, solver=solver)
+        if skip:
+            b.step_async(skip, use_graph=True)
+            b.sync()
         b.set_logging(samples - total)
         t0 = time.perf_counter()
         b.step(steps)
@@ -141,6 +152,9 @@ def main(argv=None):
     c.add_argument("--samples", type=int, default=5_000_000)
     c.add_argument("--count", type=int, default=8192)
     c.add_argument("--steps", type=int, default=600)
+    c.add_argument("--scene", default="polygon_pile", choices=["polygon_pile", "pile_batch"])
+    c.add_argument("--worlds", type=int, default=64)
+    c.add_argument("--skip", type=int, default=0)
     c.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--data", required=True)
@@ -156,7 +170,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)  # noqa: E731
     if a.cmd == "collect":
-        f, tg = collect(a.samples, count=a.count, steps=a.steps, log=log)
+        f, tg = collect(a.samples, count=a.count, steps=a.steps, scene=a.scene, worlds=a.worlds,
+                        skip=a.skip, log=log)
         np.savez(a.out, feats=f, targets=tg)
         log(f"collect: wrote {len(f)} samples to {a.out}")
     elif a.cmd == "train":
