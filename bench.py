@@ -60,16 +60,20 @@ BROADPHASES = {"auto": engine.BROADPHASE_AUTO, "scan": engine.BROADPHASE_SCAN,
                "grid16": engine.BROADPHASE_GRID_16}
 
 # workload -> (scene factory, settling pre-roll, pair capacity per body,
-# autotune candidates, description)
+# autotune candidates, description).  The pre-rolls end where the contact
+# count has settled: the 1024 piles collapse and rebound until about step
+# 700 (contacts 15-28 M, 22-32 ms per step) and then hold 22.6-23.6 M
+# contacts; the 8192-polygon pile lands around step 330 and settles by 500;
+# the 4096-world batch settles by 170.
 WORKLOADS = {
-    "pile_batch": (lambda w: scenes.pile_batch(n_worlds=w or 1024), 500, 4,
+    "pile_batch": (lambda w: scenes.pile_batch(n_worlds=w or 1024), 800, 4,
                    [("grid1", 512), ("grid1", 1024)],
                    "worlds of 2k-32k random 3-8-gons stacked in piles up to 60 layers"),
     "world_batch": (lambda w: scenes.world_batch_fast(n_worlds=w or 4096), 150, 4,
                     [(b, t) for b in ("grid1", "grid16", "scan") for t in (64, 256)] +
                     [("grid1", 512), ("grid1", 1024)],
                     "worlds of 8-1024 bodies (squares and 3-8-gons), each on its own floor"),
-    "polygon_pile": (lambda w: scenes.polygon_pile(), 300, 0,
+    "polygon_pile": (lambda w: scenes.polygon_pile(), 500, 0,
                      [(b, t) for b in ("grid", "scan") for t in (256, 512, 1024)],
                      "one world of 8192 random 3-8-gons dropped onto a 200-wide floor"),
     "pyramid": (lambda w: scenes.pyramid(), 150, 0, [("scan", 256)],
