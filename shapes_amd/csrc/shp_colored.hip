@@ -221,7 +221,7 @@ __device__ __forceinline__ void applyw(float l, const float* w, float* v) {
   for (int q = 0; q < 6; ++q) v[q] = fmaf(l, w[q], v[q]);
 }
 
-// FP32 velocities: LDS (3 floats per local body) or fvx/fvy/fw.
+// FP32 velocities: LDS (3 floats per local body) or fv (global memory).
 template <bool VLDS>
 struct VelF {
   float* lds;
@@ -234,9 +234,10 @@ struct VelF {
       v[1] = p[1];
       v[2] = p[2];
     } else {
-      v[0] = c->fvx[g];
-      v[1] = c->fvy[g];
-      v[2] = c->fw[g];
+      const float4 x = c->fv[g];
+      v[0] = x.x;
+      v[1] = x.y;
+      v[2] = x.z;
     }
   }
   __device__ __forceinline__ void store(int g, const float* v) const {
@@ -246,9 +247,7 @@ struct VelF {
       p[1] = v[1];
       p[2] = v[2];
     } else {
-      c->fvx[g] = v[0];
-      c->fvy[g] = v[1];
-      c->fw[g] = v[2];
+      c->fv[g] = make_float4(v[0], v[1], v[2], 0.0f);
     }
   }
 };
@@ -923,9 +922,7 @@ __global__ __launch_bounds__(64 * SWEEP_WAVES) void k_sweep_waves(Ctx c, int fir
   for (int i = lane; i < nb; i += 64) {
     float v[3];
     V.load(b0 + i, v);
-    c.fvx[b0 + i] = v[0];
-    c.fvy[b0 + i] = v[1];
-    c.fw[b0 + i] = v[2];
+    c.fv[b0 + i] = make_float4(v[0], v[1], v[2], 0.0f);
   }
 }
 
@@ -1039,9 +1036,7 @@ __device__ void sweep_world(const Ctx& c, int w, int ncol, bool serial_tail, flo
     for (int i = tid; i < nb; i += T) {
       float v[3];
       V.load(b0 + i, v);
-      c.fvx[b0 + i] = v[0];
-      c.fvy[b0 + i] = v[1];
-      c.fw[b0 + i] = v[2];
+      c.fv[b0 + i] = make_float4(v[0], v[1], v[2], 0.0f);
     }
   }
   tmark(c, 5);
@@ -1090,9 +1085,10 @@ __global__ void k_colour_unpack(Ctx c) {
     if (t < c.NB) {
       const int g = (int)t;
       if (!c.is_static[g] && (c.col_info[c.body_world[g]] & 0xFFFF) != 0) {
-        c.vx[g] = c.vx[g] + (double)(c.fvx[g] - (float)c.vx[g]);
-        c.vy[g] = c.vy[g] + (double)(c.fvy[g] - (float)c.vy[g]);
-        c.w[g] = c.w[g] + (double)(c.fw[g] - (float)c.w[g]);
+        const float4 f = c.fv[g];
+        c.vx[g] = c.vx[g] + (double)(f.x - (float)c.vx[g]);
+        c.vy[g] = c.vy[g] + (double)(f.y - (float)c.vy[g]);
+        c.w[g] = c.w[g] + (double)(f.z - (float)c.w[g]);
       }
     }
   }

@@ -144,8 +144,9 @@ struct Ctx {
   // solver sub-phase timestamps of world 0 (wall_clock64), timing mode only
   unsigned long long* tstamp;
   unsigned long long* tstamp_store;
-  // coloured solver state (FP32 velocities, FP64 accumulators)
-  float *fvx, *fvy, *fw;
+  // coloured solver state: FP32 velocities (vx, vy, w, unused) per body,
+  // one 16-byte word so that a body's gather reads one cache line
+  float4* fv;
   // learned warm start: padded FP32 master / BF16 weights, the step's
   // cache misses (contact index + 14 features) and the training-data log
   const float* mlp_w;

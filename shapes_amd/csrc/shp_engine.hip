@@ -1104,9 +1104,7 @@ static void batch_buffers(const StepPlan& P, Ctx& c, int32_t** scan_part, std::v
     add(&c.pk, COLOR_PACK_FLOATS(PC) * 4);
     add(&c.sched_level, PC * 4);
     add(&c.sched_order, PC * 4);
-    add(&c.fvx, NB * 4);
-    add(&c.fvy, NB * 4);
-    add(&c.fw, NB * 4);
+    add(&c.fv, NB * 16);
     add(&c.col_best, NB * 8);
     add(&c.col_used, NB * 4);
     add(&c.col_list, 2 * PC * 4);
