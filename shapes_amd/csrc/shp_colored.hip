@@ -764,10 +764,11 @@ __device__ void colour_world_lds(const Ctx& c, int w, int p0, int np, int b0, in
 }
 
 // The same colouring for worlds whose colouring does not fit in LDS:
-// 64-bit priorities and colour masks per body (col_best, col_used) and
-// colours per pair (sched_level) in global memory (L2-resident).  The
-// uncoloured manifolds of a round are kept as a list (col_list, two
-// buffers of the world's pair range): a round visits only them, clears
+// 64-bit priorities and colour masks per body (col_bu) and colours per
+// pair (sched_level) in global memory (L2-resident).  The uncoloured
+// manifolds of a round are kept as a list of (pair index, body word)
+// entries (col_list, two buffers of the world's pair range), so a round
+// reads no pair or body arrays beside it: a round visits only them, clears
 // only their bodies' priorities, and passes the manifolds it does not
 // colour on to the next round's list.  The selection of a round depends
 // only on which manifolds are uncoloured and on their priorities, so the
@@ -777,27 +778,30 @@ __device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int b0,
                                     int* s_off, int* s_cur, int* sm) {
   int &s_left = sm[0], &s_taken = sm[1], &s_top = sm[2], &s_fill = sm[3];
   const int tid = threadIdx.x;
-  unsigned long long* best = c.col_best + b0;
-  uint32_t* used_at = c.col_used + b0;
+  // per body: the round's largest priority (x) and the colours taken (y),
+  // one 16-byte word, so a body's test reads one cache line
+  ulonglong2* bu = c.col_bu + b0;
   int32_t* colour = c.sched_level + p0;  // per pair: colour, -1 uncoloured, -2 none
-  int32_t* list[2] = {c.col_list + p0, c.col_list + c.pair_cap + p0};
+  // list entries: the pair index and its pair_ab word (world-local bodies)
+  uint2* list[2] = {c.col_list + p0, c.col_list + c.pair_cap + p0};
   const unsigned long long below = (1ull << (tid & 63)) - 1ull;
   if (tid == 0) {
     s_top = 0;
     s_fill = 0;
   }
-  for (int i = tid; i < nb; i += T) used_at[i] = 0u;
+  for (int i = tid; i < nb; i += T) bu[i].y = 0ull;
   __syncthreads();
   // the first list: every pair with contacts
   for (int q0 = 0; q0 < np; q0 += T) {
     const int q = q0 + tid;
-    const bool has = q < np && c.ccount[p0 + q] > 0;
+    const uint32_t ab = q < np ? c.pair_ab[p0 + q] : NO_PAIR;
+    const bool has = ab != NO_PAIR;
     if (q < np) colour[q] = has ? -1 : -2;
     const unsigned long long bal = __ballot(has);
     int base = 0;
     if ((tid & 63) == 0 && bal) base = atomicAdd(&s_fill, (int)__popcll(bal));
     base = __shfl(base, 0);
-    if (has) list[0][base + (int)__popcll(bal & below)] = q;
+    if (has) list[0][base + (int)__popcll(bal & below)] = make_uint2((uint32_t)q, ab);
   }
   __syncthreads();
   if (tid == 0) s_left = s_fill;
@@ -805,13 +809,12 @@ __device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int b0,
   int rounds = 0, cur = 0;
   while (s_left > 0 && rounds < JP_ROUNDS) {
     const int n = s_left;
-    const int32_t* in = list[cur];
-    int32_t* out = list[cur ^ 1];
+    const uint2* in = list[cur];
+    uint2* out = list[cur ^ 1];
     for (int k = tid; k < n; k += T) {
-      const int p = p0 + in[k];
-      const int ga = c.pa[p], gb = c.pb[p];
-      if (!c.is_static[ga]) best[ga - b0] = 0ull;
-      if (!c.is_static[gb]) best[gb - b0] = 0ull;
+      const uint32_t ab = in[k].y, a = ab & 0xFFFFu, b = ab >> 16;
+      if (a != NO_BODY) bu[a].x = 0ull;
+      if (b != NO_BODY) bu[b].x = 0ull;
     }
     if (tid == 0) {
       s_taken = 0;
@@ -819,31 +822,32 @@ __device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int b0,
     }
     __syncthreads();
     for (int k = tid; k < n; k += T) {
-      const int q = in[k], p = p0 + q;
-      const unsigned long long prio = prio64(q, w);
-      const int ga = c.pa[p], gb = c.pb[p];
-      if (!c.is_static[ga]) atomicMax(&best[ga - b0], prio);
-      if (!c.is_static[gb]) atomicMax(&best[gb - b0], prio);
+      const uint2 e = in[k];
+      const unsigned long long prio = prio64((int)e.x, w);
+      const uint32_t a = e.y & 0xFFFFu, b = e.y >> 16;
+      if (a != NO_BODY) atomicMax(&bu[a].x, prio);
+      if (b != NO_BODY) atomicMax(&bu[b].x, prio);
     }
     __syncthreads();
     int taken = 0, top = 0;
     for (int k0 = 0; k0 < n; k0 += T) {
       const int k = k0 + tid;
       bool keep = false;
-      int q = 0;
+      uint2 e = make_uint2(0u, 0u);
       if (k < n) {
-        q = in[k];
-        const int p = p0 + q;
-        const unsigned long long prio = prio64(q, w);
-        const int ga = c.pa[p], gb = c.pb[p];
-        const bool sa = c.is_static[ga] != 0, sb = c.is_static[gb] != 0;
-        if ((sa || best[ga - b0] == prio) && (sb || best[gb - b0] == prio)) {
-          const uint32_t used = (sa ? 0u : used_at[ga - b0]) | (sb ? 0u : used_at[gb - b0]);
+        e = in[k];
+        const unsigned long long prio = prio64((int)e.x, w);
+        const uint32_t a = e.y & 0xFFFFu, b = e.y >> 16;
+        const bool sa = a == NO_BODY, sb = b == NO_BODY;
+        const ulonglong2 ra = sa ? make_ulonglong2(0ull, 0ull) : bu[a];
+        const ulonglong2 rb = sb ? make_ulonglong2(0ull, 0ull) : bu[b];
+        if ((sa || ra.x == prio) && (sb || rb.x == prio)) {
+          const uint32_t used = (uint32_t)ra.y | (uint32_t)rb.y;
           const int l = greedy_colour(used, rounds);
-          colour[q] = l;
+          colour[e.x] = l;
           if (l < GREEDY_COLOURS) {
-            if (!sa) used_at[ga - b0] = used_at[ga - b0] | (1u << l);
-            if (!sb) used_at[gb - b0] = used_at[gb - b0] | (1u << l);
+            if (!sa) bu[a].y = ra.y | (1ull << l);
+            if (!sb) bu[b].y = rb.y | (1ull << l);
           }
           top = max(top, l + 1);
           ++taken;
@@ -855,7 +859,7 @@ __device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int b0,
       int base = 0;
       if ((tid & 63) == 0 && bal) base = atomicAdd(&s_fill, (int)__popcll(bal));
       base = __shfl(base, 0);
-      if (keep) out[base + (int)__popcll(bal & below)] = q;
+      if (keep) out[base + (int)__popcll(bal & below)] = e;
     }
     taken = wave_sum(taken);
     top = wave_max(top);
@@ -872,7 +876,7 @@ __device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int b0,
   const bool serial_tail = s_left > 0;
   const int top = s_top;
   // the manifolds the rounds did not reach form the serial tail colour
-  for (int k = tid; k < s_left; k += T) colour[list[cur][k]] = top;
+  for (int k = tid; k < s_left; k += T) colour[list[cur][k].x] = top;
   __syncthreads();
   colour_finish<T>(c, w, p0, np, colour, top, serial_tail, s_off, s_cur);
 }

@@ -135,9 +135,8 @@ struct Ctx {
   double* pk;
   int32_t* pk_g;  // bodies a, b: 2 x contact_cap (ordered solver)
   uint8_t* pk_f;  // flags per contact (ordered solver)
-  unsigned long long* col_best;  // colouring priorities per body (large worlds)
-  uint32_t* col_used;            // colours taken at each body (large worlds)
-  int32_t* col_list;             // 2 x pair_cap: uncoloured manifolds per round (large worlds)
+  ulonglong2* col_bu;  // per body: round priority, colours taken (large worlds)
+  uint2* col_list;     // 2 x pair_cap: uncoloured manifolds, pair index and body word
   uint32_t* pair_ab;             // per pair: world-local bodies for the colouring (k_sat)
   int32_t* col_off;   // per world: COLOR_CAP + 3 colour offsets (manifolds)
   int32_t* col_info;  // per world: colours | serial tail << 16 (0: nothing to solve)

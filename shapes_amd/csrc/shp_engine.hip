@@ -1105,9 +1105,8 @@ static void batch_buffers(const StepPlan& P, Ctx& c, int32_t** scan_part, std::v
     add(&c.sched_level, PC * 4);
     add(&c.sched_order, PC * 4);
     add(&c.fv, NB * 16);
-    add(&c.col_best, NB * 8);
-    add(&c.col_used, NB * 4);
-    add(&c.col_list, 2 * PC * 4);
+    add(&c.col_bu, NB * 16);
+    add(&c.col_list, 2 * PC * 8);
     add(&c.col_off, W * (COLOR_CAP + 3) * 4);
     add(&c.col_info, W * 4);
     add(&c.pair_ab, PC * 4);
