@@ -866,10 +866,10 @@ uint32_t mix32(uint32_t x) {
   return x;
 }
 
-// Priority of manifold q (pair index in the world) of world w: 17 hash bits,
+// Priority of manifold q (pair index in the world) of world w: 15 hash bits,
 // then the index (distinct within a world).
 uint64_t colour_prio(int q, int w) {
-  return ((uint64_t)(mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x1FFFFu) << 32) |
+  return ((uint64_t)(mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x7FFFu) << 32) |
          (uint64_t)((uint32_t)q + 1u);
 }
 

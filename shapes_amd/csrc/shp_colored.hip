@@ -474,15 +474,23 @@ __device__ __forceinline__ bool colour_fits_lds(int nb, int np, int nm, int pool
 // FP32 velocities in LDS: 12 B per body.
 __device__ __forceinline__ bool vel_fits_lds(int nb, int pool_bytes) { return 12L * nb <= pool_bytes; }
 
-// Hashed priority of manifold q of world w: 17 hash bits, then the index
-// (distinct within a world).  prio32 packs it for q < 32768 (LDS colouring),
-// prio64 for any q (global colouring); both order manifolds alike, so the
-// colours do not depend on which kernel colours a world.
+// Pair indices below this fit prio32 (17 bits, prio32 - 1 below 2^32).
+#define PRIO32_PAIRS 131071
+// Colouring with the per-body priorities and colour masks in LDS (6 B per
+// body) and the manifold lists in global memory.
+__device__ __forceinline__ bool colour_fits_hybrid(int nb, int np, int pool_bytes) {
+  return nb < (int)NO_BODY && np <= PRIO32_PAIRS && 4L * nb + 2L * ((nb + 1) & ~1) <= pool_bytes;
+}
+
+// Hashed priority of manifold q of world w: 15 hash bits, then the index
+// (distinct within a world).  prio32 packs it for q < PRIO32_PAIRS (LDS
+// colouring), prio64 for any q (global colouring); both order manifolds
+// alike, so the colours do not depend on which path colours a world.
 __device__ __forceinline__ uint32_t prio32(int q, int w) {
-  return (((mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x1FFFFu) << 15) | (uint32_t)q) + 1u;
+  return (((mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x7FFFu) << 17) | (uint32_t)q) + 1u;
 }
 __device__ __forceinline__ unsigned long long prio64(int q, int w) {
-  return ((unsigned long long)(mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x1FFFFu) << 32) |
+  return ((unsigned long long)(mix32((uint32_t)q * 0x9E3779B9u + (uint32_t)w) & 0x7FFFu) << 32) |
          (unsigned long long)((uint32_t)q + 1u);
 }
 
@@ -531,7 +539,7 @@ __device__ bool colour_regs(const Ctx& c, int w, int p0, int nm, int nb, uint32_
   int* s_tk = sm;  // [2]: taken this round, next round
   int& s_top = sm[2];
   const int tid = threadIdx.x;
-  // pair index q = (prio32 - 1) & 0x7FFF
+  // pair index q = (prio32 - 1) & 0x1FFFF
   uint32_t pr[COLOUR_REG_ITEMS], ab[COLOUR_REG_ITEMS];
   uint32_t col[COLOUR_REG_ITEMS / 4];  // colour of manifold k in byte k % 4
   uint32_t alive = 0;
@@ -605,7 +613,7 @@ __device__ bool colour_regs(const Ctx& c, int w, int p0, int nm, int nb, uint32_
     // serial tail: colours per pair, the tail (colour ncol) for the rest
 #pragma unroll
     for (int k = 0; k < COLOUR_REG_ITEMS; ++k)
-      if (tid + k * T < nm) colour[(pr[k] - 1u) & 0x7FFFu] = (alive >> k) & 1u ? ncol : colour_at(k);
+      if (tid + k * T < nm) colour[(pr[k] - 1u) & 0x1FFFFu] = (alive >> k) & 1u ? ncol : colour_at(k);
     __syncthreads();
     return false;
   }
@@ -635,7 +643,7 @@ __device__ bool colour_regs(const Ctx& c, int w, int p0, int nm, int nb, uint32_
   for (int k = 0; k < COLOUR_REG_ITEMS; ++k)
     if (tid + k * T < nm) {
       const int m = atomicAdd(&s_cur[colour_at(k)], 1);
-      c.sched_order[p0 + ((pr[k] - 1u) & 0x7FFFu)] = p0 + m;
+      c.sched_order[p0 + ((pr[k] - 1u) & 0x1FFFFu)] = p0 + m;
     }
   int* off = c.col_off + (long)w * (COLOR_CAP + 3);
   for (int l = tid; l <= ncol; l += T) off[l] = s_off[l];
@@ -881,6 +889,112 @@ __device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int b0,
   colour_finish<T>(c, w, p0, np, colour, top, serial_tail, s_off, s_cur);
 }
 
+// The same colouring with the per-body priorities (prio32) and colour
+// masks in LDS and the uncoloured manifolds' lists in global memory (as in
+// colour_world_global): for worlds whose bodies fit LDS but whose
+// manifolds do not.
+template <int T>
+__device__ void colour_world_hybrid(const Ctx& c, int w, int p0, int np, int nb,
+                                    unsigned char* pool, int* s_off, int* s_cur, int* sm) {
+  int &s_left = sm[0], &s_taken = sm[1], &s_top = sm[2], &s_fill = sm[3];
+  const int tid = threadIdx.x;
+  uint32_t* s_best = (uint32_t*)pool;
+  uint16_t* s_used = (uint16_t*)(s_best + nb);
+  int32_t* colour = c.sched_level + p0;  // per pair: colour, -1 uncoloured, -2 none
+  uint2* list[2] = {c.col_list + p0, c.col_list + c.pair_cap + p0};
+  const unsigned long long below = (1ull << (tid & 63)) - 1ull;
+  if (tid == 0) {
+    s_top = 0;
+    s_fill = 0;
+  }
+  for (int i = tid; i < nb; i += T) s_used[i] = 0;
+  __syncthreads();
+  for (int q0 = 0; q0 < np; q0 += T) {
+    const int q = q0 + tid;
+    const uint32_t ab = q < np ? c.pair_ab[p0 + q] : NO_PAIR;
+    const bool has = ab != NO_PAIR;
+    if (q < np) colour[q] = has ? -1 : -2;
+    const unsigned long long bal = __ballot(has);
+    int base = 0;
+    if ((tid & 63) == 0 && bal) base = atomicAdd(&s_fill, (int)__popcll(bal));
+    base = __shfl(base, 0);
+    if (has) list[0][base + (int)__popcll(bal & below)] = make_uint2((uint32_t)q, ab);
+  }
+  __syncthreads();
+  if (tid == 0) s_left = s_fill;
+  __syncthreads();
+  int rounds = 0, cur = 0;
+  while (s_left > 0 && rounds < JP_ROUNDS) {
+    const int n = s_left;
+    const uint2* in = list[cur];
+    uint2* out = list[cur ^ 1];
+    for (int k = tid; k < n; k += T) {
+      const uint32_t ab = in[k].y, a = ab & 0xFFFFu, b = ab >> 16;
+      if (a != NO_BODY) s_best[a] = 0u;
+      if (b != NO_BODY) s_best[b] = 0u;
+    }
+    if (tid == 0) {
+      s_taken = 0;
+      s_fill = 0;
+    }
+    __syncthreads();
+    for (int k = tid; k < n; k += T) {
+      const uint2 e = in[k];
+      const uint32_t prio = prio32((int)e.x, w);
+      const uint32_t a = e.y & 0xFFFFu, b = e.y >> 16;
+      if (a != NO_BODY) atomicMax(&s_best[a], prio);
+      if (b != NO_BODY) atomicMax(&s_best[b], prio);
+    }
+    __syncthreads();
+    int taken = 0, top = 0;
+    for (int k0 = 0; k0 < n; k0 += T) {
+      const int k = k0 + tid;
+      bool keep = false;
+      uint2 e = make_uint2(0u, 0u);
+      if (k < n) {
+        e = in[k];
+        const uint32_t prio = prio32((int)e.x, w);
+        const uint32_t a = e.y & 0xFFFFu, b = e.y >> 16;
+        const bool sa = a == NO_BODY, sb = b == NO_BODY;
+        if ((sa || s_best[a] == prio) && (sb || s_best[b] == prio)) {
+          const uint32_t used = (sa ? 0u : s_used[a]) | (sb ? 0u : s_used[b]);
+          const int l = greedy_colour(used, rounds);
+          colour[e.x] = l;
+          if (l < GREEDY_COLOURS) {
+            if (!sa) s_used[a] = (uint16_t)(s_used[a] | (1u << l));
+            if (!sb) s_used[b] = (uint16_t)(s_used[b] | (1u << l));
+          }
+          top = max(top, l + 1);
+          ++taken;
+        } else {
+          keep = true;
+        }
+      }
+      const unsigned long long bal = __ballot(keep);
+      int base = 0;
+      if ((tid & 63) == 0 && bal) base = atomicAdd(&s_fill, (int)__popcll(bal));
+      base = __shfl(base, 0);
+      if (keep) out[base + (int)__popcll(bal & below)] = e;
+    }
+    taken = wave_sum(taken);
+    top = wave_max(top);
+    if ((tid & 63) == 0) {
+      atomicAdd(&s_taken, taken);
+      atomicMax(&s_top, top);
+    }
+    __syncthreads();
+    if (tid == 0) s_left = s_fill;
+    cur ^= 1;
+    ++rounds;
+    __syncthreads();
+  }
+  const bool serial_tail = s_left > 0;
+  const int top = s_top;
+  for (int k = tid; k < s_left; k += T) colour[list[cur][k].x] = top;
+  __syncthreads();
+  colour_finish<T>(c, w, p0, np, colour, top, serial_tail, s_off, s_cur);
+}
+
 // The LDS accesses of a wavefront before this point are done before any of
 // its accesses after it (the wavefront's lanes run together).
 __device__ __forceinline__ void wave_sync() {
@@ -932,7 +1046,8 @@ __global__ __launch_bounds__(64 * SWEEP_WAVES) void k_sweep_waves(Ctx c, int fir
 
 // Colouring, one workgroup per world, the largest worlds first: in LDS when
 // the world's colouring fits (by its pair count, else by its manifold
-// count), else in global memory.
+// count), else with the bodies in LDS and the manifold lists in global
+// memory when the bodies fit, else in global memory.
 template <int T>
 __global__ __launch_bounds__(T) void k_colour(Ctx c, int pool_bytes) {
   extern __shared__ int lds[];
@@ -960,6 +1075,8 @@ __global__ __launch_bounds__(T) void k_colour(Ctx c, int pool_bytes) {
   }
   if (in_lds)
     colour_world_lds<T>(c, w, p0, np, b0, nb, cap, pool, s_off, s_cur, sm);
+  else if (colour_fits_hybrid(nb, np, pool_bytes))
+    colour_world_hybrid<T>(c, w, p0, np, nb, pool, s_off, s_cur, sm);
   else
     colour_world_global<T>(c, w, p0, np, b0, nb, s_off, s_cur, sm);
   tmark(c, 1);
