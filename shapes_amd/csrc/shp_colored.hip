@@ -476,10 +476,13 @@ __device__ __forceinline__ bool vel_fits_lds(int nb, int pool_bytes) { return 12
 
 // Pair indices below this fit prio32 (17 bits, prio32 - 1 below 2^32).
 #define PRIO32_PAIRS 131071
-// Colouring with the per-body priorities and colour masks in LDS (6 B per
-// body) and the manifold lists in global memory.
+// Colouring with the per-body priorities (4 B per body) and, when they fit
+// too, colour masks (2 B) in LDS and the manifold lists in global memory.
 __device__ __forceinline__ bool colour_fits_hybrid(int nb, int np, int pool_bytes) {
-  return nb < (int)NO_BODY && np <= PRIO32_PAIRS && 4L * nb + 2L * ((nb + 1) & ~1) <= pool_bytes;
+  return nb < (int)NO_BODY && np <= PRIO32_PAIRS && 4L * nb <= pool_bytes;
+}
+__device__ __forceinline__ bool colour_masks_fit(int nb, int pool_bytes) {
+  return 4L * nb + 2L * ((nb + 1) & ~1) <= pool_bytes;
 }
 
 // Hashed priority of manifold q of world w: 15 hash bits, then the index
@@ -889,17 +892,28 @@ __device__ void colour_world_global(const Ctx& c, int w, int p0, int np, int b0,
   colour_finish<T>(c, w, p0, np, colour, top, serial_tail, s_off, s_cur);
 }
 
-// The same colouring with the per-body priorities (prio32) and colour
-// masks in LDS and the uncoloured manifolds' lists in global memory (as in
-// colour_world_global): for worlds whose bodies fit LDS but whose
-// manifolds do not.
-template <int T>
-__device__ void colour_world_hybrid(const Ctx& c, int w, int p0, int np, int nb,
+// The same colouring with the per-body priorities (prio32) in LDS, the
+// colour masks in LDS (MASKS_LDS) or in col_bu, and the uncoloured
+// manifolds' lists in global memory (as in colour_world_global): for
+// worlds whose bodies fit LDS but whose manifolds do not.  A mask is read
+// and written only by the manifold a round selects on the body.
+template <int T, bool MASKS_LDS>
+__device__ void colour_world_hybrid(const Ctx& c, int w, int p0, int np, int b0, int nb,
                                     unsigned char* pool, int* s_off, int* s_cur, int* sm) {
   int &s_left = sm[0], &s_taken = sm[1], &s_top = sm[2], &s_fill = sm[3];
   const int tid = threadIdx.x;
   uint32_t* s_best = (uint32_t*)pool;
-  uint16_t* s_used = (uint16_t*)(s_best + nb);
+  uint16_t* s_used = (uint16_t*)(s_best + nb);  // MASKS_LDS
+  unsigned long long* g_used = &c.col_bu[b0].y;  // else: body i's mask at g_used[2 i]
+  auto used_of = [&](uint32_t i) -> uint32_t {
+    return MASKS_LDS ? (uint32_t)s_used[i] : (uint32_t)g_used[2 * i];
+  };
+  auto set_used = [&](uint32_t i, uint32_t m) {
+    if (MASKS_LDS)
+      s_used[i] = (uint16_t)m;
+    else
+      g_used[2 * i] = m;
+  };
   int32_t* colour = c.sched_level + p0;  // per pair: colour, -1 uncoloured, -2 none
   uint2* list[2] = {c.col_list + p0, c.col_list + c.pair_cap + p0};
   const unsigned long long below = (1ull << (tid & 63)) - 1ull;
@@ -907,7 +921,7 @@ __device__ void colour_world_hybrid(const Ctx& c, int w, int p0, int np, int nb,
     s_top = 0;
     s_fill = 0;
   }
-  for (int i = tid; i < nb; i += T) s_used[i] = 0;
+  for (int i = tid; i < nb; i += T) set_used(i, 0u);
   __syncthreads();
   for (int q0 = 0; q0 < np; q0 += T) {
     const int q = q0 + tid;
@@ -957,12 +971,12 @@ __device__ void colour_world_hybrid(const Ctx& c, int w, int p0, int np, int nb,
         const uint32_t a = e.y & 0xFFFFu, b = e.y >> 16;
         const bool sa = a == NO_BODY, sb = b == NO_BODY;
         if ((sa || s_best[a] == prio) && (sb || s_best[b] == prio)) {
-          const uint32_t used = (sa ? 0u : s_used[a]) | (sb ? 0u : s_used[b]);
-          const int l = greedy_colour(used, rounds);
+          const uint32_t ua = sa ? 0u : used_of(a), ub = sb ? 0u : used_of(b);
+          const int l = greedy_colour(ua | ub, rounds);
           colour[e.x] = l;
           if (l < GREEDY_COLOURS) {
-            if (!sa) s_used[a] = (uint16_t)(s_used[a] | (1u << l));
-            if (!sb) s_used[b] = (uint16_t)(s_used[b] | (1u << l));
+            if (!sa) set_used(a, ua | (1u << l));
+            if (!sb) set_used(b, ub | (1u << l));
           }
           top = max(top, l + 1);
           ++taken;
@@ -1075,8 +1089,10 @@ __global__ __launch_bounds__(T) void k_colour(Ctx c, int pool_bytes) {
   }
   if (in_lds)
     colour_world_lds<T>(c, w, p0, np, b0, nb, cap, pool, s_off, s_cur, sm);
+  else if (colour_fits_hybrid(nb, np, pool_bytes) && colour_masks_fit(nb, pool_bytes))
+    colour_world_hybrid<T, true>(c, w, p0, np, b0, nb, pool, s_off, s_cur, sm);
   else if (colour_fits_hybrid(nb, np, pool_bytes))
-    colour_world_hybrid<T>(c, w, p0, np, nb, pool, s_off, s_cur, sm);
+    colour_world_hybrid<T, false>(c, w, p0, np, b0, nb, pool, s_off, s_cur, sm);
   else
     colour_world_global<T>(c, w, p0, np, b0, nb, s_off, s_cur, sm);
   tmark(c, 1);

@@ -121,10 +121,14 @@ def test_colored_matches_restatement_single_colour(orc):
     _colored_vs_restatement(orc, [b.build()], scenes.SceneConfig(iterations=4), steps=60, every=5)
 
 
-def test_colored_matches_restatement_global_path(orc):
-    """A world too large for LDS: colouring and velocities in global memory."""
+@pytest.mark.parametrize("n", [24000, 30000, 45000])
+def test_colored_matches_restatement_global_path(orc, n):
+    """Worlds too large for LDS, velocities in global memory: coloured with
+    the bodies' priorities and colour masks in LDS (24000), the priorities
+    only (30000), or entirely in global memory (45000)."""
     rng = np.random.Generator(np.random.Philox(78))
-    w = scenes.grid_world(rng, 24000, cols=200, spacing=1.05, y0=1.0, square_frac=0.5, vel=0.3)
+    w = scenes.grid_world(rng, n, cols=200, spacing=1.05, y0=1.0, square_frac=0.5, vel=0.3)
+    print(f"global-path colouring: {n} bodies", flush=True)
     _colored_vs_restatement(orc, [w], scenes.SceneConfig(iterations=3), steps=4, every=2)
 
 
