@@ -50,7 +50,7 @@ constexpr size_t lds16(size_t bytes) { return (bytes + 15) & ~(size_t)15; }
 enum { CNT_FLAGS = 0, CNT_WARM = 1, CNT_LEARNED = 2, CNT_LEVELS = 3, CNT_MISS = 4 };
 // grid broadphase: partners per body kept by the count pass for the emit
 // pass (bodies with more are visited again)
-enum { GRID_SLOTS = 4 };
+enum { GRID_SLOTS = 8 };
 // solve-quality maxima (doubles stored as their bit patterns): last step,
 // then since creation; then the sums of the steps' maxima and the number
 // of steps summed
