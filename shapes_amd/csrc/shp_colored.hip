@@ -1017,13 +1017,13 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Worlds small enough for one wavefront each (world_order[first, W), at
-// most SWEEP_WAVE_MAX_BODIES bodies, FP32 velocities in the wavefront's
-// slice of LDS): the warm pass and the sweeps of sweep_colours, colour by
-// colour, lane t taking manifolds s + t, s + t + 64, ... of a colour, with
-// the wavefront's own ordering of its LDS accesses between colours instead
-// of workgroup barriers; SWEEP_WAVES worlds per workgroup.  The same
-// updates in the same colour order as sweep_world, so the same results.
+// Worlds small enough for one wavefront each (world_order[first_world, W),
+// at most SWEEP_WAVE_MAX_BODIES bodies, FP32 velocities in the wavefront's
+// slice of LDS): the warm pass and the sweeps of sweep_prefetched with 64
+// lanes, colour by colour, with the wavefront's own ordering of its LDS
+// accesses between colours instead of workgroup barriers; SWEEP_WAVES
+// worlds per workgroup.  The same updates in the same colour order as
+// sweep_world, so the same results.
 __global__ __launch_bounds__(64 * SWEEP_WAVES) void k_sweep_waves(Ctx c, int first_world,
                                                                   int slice_floats) {
   extern __shared__ int lds[];
