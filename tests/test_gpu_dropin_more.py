@@ -63,8 +63,11 @@ def test_pair_overflow_reported_and_state_kept():
     rng = np.random.Generator(np.random.Philox(5))
     w = scenes.grid_world(rng, 3000, cols=60, spacing=0.55, y0=1.0, square_frac=0.5, vel=0.0)
     cfg = scenes.SceneConfig(iterations=2)
-    for bp in (engine.BROADPHASE_SCAN, engine.BROADPHASE_GRID):
-        b = engine.make_batch([w], cfg, max_pairs_per_body=1, broadphase=bp)
+    for bp, solver in ((engine.BROADPHASE_SCAN, engine.SOLVER_ORDERED_F64),
+                       (engine.BROADPHASE_GRID, engine.SOLVER_ORDERED_F64),
+                       (engine.BROADPHASE_GRID, engine.SOLVER_COLORED_F32)):
+        print(f"overflow: broadphase {bp}, solver {solver}", flush=True)
+        b = engine.make_batch([w], cfg, max_pairs_per_body=1, broadphase=bp, solver=solver)
         b.step_async(5, use_graph=True)
         with pytest.raises(engine.ShapesError) as ei:
             b.sync()
@@ -75,7 +78,7 @@ def test_pair_overflow_reported_and_state_kept():
         p = b.pairs(0)
         assert len(p) == w.n + 1024
         assert (p[:, 0] > p[:, 1]).all() and (p[:, 1] >= 0).all() and (p[:, 0] < w.n).all()
-        b.bodies(0)
+        assert np.isfinite(b.bodies(0)).all()
         b.close()
 
 
