@@ -565,7 +565,12 @@ __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
       c.lt[dst] = 0.0;
       c.warm[dst] = 0;
       if (c.collect_misses) {
-        int slot = atomicAdd(&c.counters[CNT_MISS], 1);
+        // one atomic per wavefront for the lanes that miss together
+        const unsigned long long act = __ballot(1);
+        const int lane = threadIdx.x & 63, leader = __ffsll((long long)act) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&c.counters[CNT_MISS], (int)__popcll(act));
+        const int slot = __shfl(base, leader) + (int)__popcll(act & ((1ull << lane) - 1ull));
         if (slot < c.miss_cap) {
           c.miss_idx[slot] = dst;
           double v[6] = {c.vx[ga], c.vy[ga], c.w[ga], c.vx[gb], c.vy[gb], c.w[gb]};
@@ -931,8 +936,13 @@ static shp_status enqueue_front(shp_batch* b, bool fork) {
     k_broadphase<<<P.g_broadphase, 256, 0, s>>>(c, 1);
   mark(b, 4);
   k_sat<<<P.g_sat, 256, 0, s>>>(c);
+  mark(b, 5);
+  // contacts per pair: this step's pairs only (the pair count is on the
+  // device).  Before the colouring starts: a colouring workgroup takes a
+  // whole CU, and the scan's short passes would wait behind them.
+  HIPCHK(exclusive_sum(b, P.scan_pairs, c.ccount, c.coff, c.off_rank + c.NB, c.pair_cap, s));
   // the colouring needs only the pairs and which of them have contacts: it
-  // runs beside the contact scan, contact generation and learned warm start
+  // runs beside the contact generation
   if (fork) {
     HIPCHK(hipEventRecord(b->fork_event, s));
     HIPCHK(hipStreamWaitEvent(b->side_stream, b->fork_event, 0));
@@ -941,9 +951,6 @@ static shp_status enqueue_front(shp_batch* b, bool fork) {
     if (b->timing) HIPCHK(hipEventRecord(b->side_events[1], b->side_stream));
     HIPCHK(hipEventRecord(b->join_event, b->side_stream));
   }
-  mark(b, 5);
-  // contacts per pair: this step's pairs only (the pair count is on the device)
-  HIPCHK(exclusive_sum(b, P.scan_pairs, c.ccount, c.coff, c.off_rank + c.NB, c.pair_cap, s));
   mark(b, 6);
   k_world_ranges<<<P.g_ranges, 256, 0, s>>>(c);
   mark(b, 7);
@@ -959,10 +966,13 @@ static shp_status enqueue_step(shp_batch* b) {
   hipStream_t s = b->stream;
   const bool fork = c.solver == SHP_SOLVER_COLORED_F32;
   TRY(enqueue_front(b, fork));
+  // the learned warm start; with the colouring, after it (phase "infer"
+  // then holds the wait for the colouring too), so that its workgroups do
+  // not wait for CUs behind colouring workgroups
+  if (fork) HIPCHK(hipStreamWaitEvent(s, b->join_event, 0));
   if (c.warm_mode == SHP_WARM_LEARNED && c.mlp_ready) mlp_launch_infer(b);
   mark(b, 9);
   if (fork) {
-    HIPCHK(hipStreamWaitEvent(s, b->join_event, 0));
     launch_colored_pack(b);
     mark(b, 10);
     launch_colored_sweep(b);

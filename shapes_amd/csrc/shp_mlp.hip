@@ -446,21 +446,27 @@ __global__ __launch_bounds__(256) void k_mlp_infer(Ctx c, MlpDev m) {
     }
     __syncthreads();
     forward_tile(m, sx, s1, s2, sy, nullptr, row0);
-    if (threadIdx.x < TILE && row0 + (int)threadIdx.x < n) {
-      int r = threadIdx.x;
-      int k = c.miss_idx[row0 + r];
-      double ln = (double)sy[r * 16];
-      ln = ln > 0.0 ? ln / c.mcn[k] : 0.0;
-      double lt = (double)sy[r * 16 + 1] / c.mct[k];
-      double mu = (c.mu[c.ga[k]] + c.mu[c.gb[k]]) / 2.0;
-      double thr = ln * mu;
-      lt = lt > thr ? thr : (lt < -thr ? -thr : lt);
-      if (ln - ln == 0.0 && lt - lt == 0.0) {
-        c.ln[k] = ln;
-        c.lt[k] = lt;
-        c.warm[k] = 2;
-        atomicAdd(&c.counters[CNT_LEARNED], 1);
+    static_assert(TILE == 64, "one row per lane of the first wavefront");
+    if (threadIdx.x < TILE) {  // the first wavefront: one row per lane
+      bool set = false;
+      if (row0 + (int)threadIdx.x < n) {
+        int r = threadIdx.x;
+        int k = c.miss_idx[row0 + r];
+        double ln = (double)sy[r * 16];
+        ln = ln > 0.0 ? ln / c.mcn[k] : 0.0;
+        double lt = (double)sy[r * 16 + 1] / c.mct[k];
+        double mu = (c.mu[c.ga[k]] + c.mu[c.gb[k]]) / 2.0;
+        double thr = ln * mu;
+        lt = lt > thr ? thr : (lt < -thr ? -thr : lt);
+        if (ln - ln == 0.0 && lt - lt == 0.0) {
+          c.ln[k] = ln;
+          c.lt[k] = lt;
+          c.warm[k] = 2;
+          set = true;
+        }
       }
+      const unsigned long long setm = __ballot(set);  // one atomic per tile
+      if (threadIdx.x == 0 && setm) atomicAdd(&c.counters[CNT_LEARNED], (int)__popcll(setm));
     }
     __syncthreads();
   }
