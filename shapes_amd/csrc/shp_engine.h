@@ -214,7 +214,8 @@ struct StepPlan {
   ScanPlan scan_buckets, scan_bodies, scan_pairs;
   long scan_parts_max = 0;  // scratch entries of the multi-workgroup scans
   // grids (workgroups) of the step's launches
-  long g_reset = 0, g_bodies = 0, g_pairs = 0, g_broadphase = 0, g_sat = 0, g_ranges = 0;
+  long g_reset = 0, g_bodies = 0, g_pairs = 0, g_pairs_count = 0, g_broadphase = 0, g_sat = 0,
+       g_ranges = 0;
   long g_contacts = 0, g_infer = 0, g_pack = 0, g_unpack = 0, g_log = 0, g_finish_ni = 0,
        g_finish_nq = 0, g_cache = 0;
   // device memory allocated at creation (bytes)

@@ -1332,6 +1332,7 @@ shp_status plan_staged(const shp_config& cfg, HostBatch* hb) {
   P.g_reset = blocks(std::max<long>(64, P.use_grid ? P.grid_buckets + 1 : 0), 256);
   P.g_bodies = blocks(NB, 256);
   P.g_pairs = P.grid_wide ? blocks(NB * 16, 256) : blocks(NB, 256);
+  P.g_pairs_count = P.grid_wide ? P.g_pairs : blocks(NB * 4, 256);
   P.g_broadphase = blocks(NB * 64, 256);
   P.g_sat = gs_blocks(PC);
   P.g_ranges = blocks(n_worlds + 1, 256);
@@ -1361,7 +1362,7 @@ void plan_launches(const StepPlan& P, bool infer, bool logging, std::vector<Laun
     add("k_grid_assign", P.g_bodies, 256, 0, 0);
     scan_launches(P.scan_buckets, "scan_buckets", out);
     add("k_grid_fill", P.g_bodies, 256, 0, 0);
-    add("k_grid_pairs", P.g_pairs, 256, 0, 0);
+    add("k_grid_pairs", P.g_pairs_count, 256, 0, 0);
   } else {
     add("k_broadphase", P.g_broadphase, 256, 0, 0);
   }

@@ -137,15 +137,18 @@ __device__ __forceinline__ void grid_visit(const Ctx& c, int g, int sl, F&& hit)
         hit(gj);
     return;
   }
-  if (LANES == 1 || sl >= CELL_LANES) {
-    const int l0 = LANES == 1 ? 0 : sl - CELL_LANES, ls = LANES == 1 ? 1 : LANES - CELL_LANES;
+  // the world's large bodies: lanes 9.. with 16 lanes, else the last lane
+  // (which also takes its share of the cells)
+  constexpr bool LARGE_LANES = LANES > CELL_LANES;
+  if (LARGE_LANES ? sl >= CELL_LANES : sl == LANES - 1) {
+    const int l0 = LARGE_LANES ? sl - CELL_LANES : 0, ls = LARGE_LANES ? LANES - CELL_LANES : 1;
     for (int q = c.large_off[w] + l0; q < c.large_off[w + 1]; q += ls) {
       const int gj = c.large_ids[q];
       if (gj >= g) break;  // ascending
       if (!(si && c.is_static[gj]) && !c.dead[gj] && aabb_hit(c, aminx, amaxx, aminy, amaxy, gj))
         hit(gj);
     }
-    if (LANES != 1) return;
+    if (LARGE_LANES) return;
   }
   const int t0 = c.tab_off[w];
   const uint32_t mask = (uint32_t)(c.tab_off[w + 1] - t0 - 1);
@@ -180,6 +183,28 @@ __global__ void k_grid_pairs(Ctx c, int fill) {
       ++count;
     });
     c.cnt_rank[rank] = count;
+    return;
+  }
+  if (LANES == 4) {  // count pass only: the cells split over 4 lanes
+    int count = 0, kept[GRID_SLOTS];
+    grid_visit<LANES>(c, g, sl, [&](int gj) {
+#pragma unroll
+      for (int k = 0; k < GRID_SLOTS; ++k)
+        if (k == count) kept[k] = gj;
+      ++count;
+    });
+    int incl = count;
+    for (int o = 1; o < LANES; o <<= 1) {
+      const int y = __shfl_up(incl, o, LANES);
+      if (sl >= o) incl += y;
+    }
+    const int total = __shfl(incl, LANES - 1, LANES);
+    // the first GRID_SLOTS partners of the body, lane by lane
+    const int first = incl - count;
+#pragma unroll
+    for (int k = 0; k < GRID_SLOTS; ++k)
+      if (k < count && first + k < GRID_SLOTS) slots[first + k] = kept[k];
+    if (sl == 0) c.cnt_rank[rank] = total;
     return;
   }
   // Past the capacity the step reports an overflow; the slots below it are
@@ -268,15 +293,22 @@ __global__ void k_grid_pairs(Ctx c, int fill) {
 }
 
 // 16 lanes per body while that still fits about one wave of work per SIMD
-// slot of the GPU several times over (batches up to 131072 bodies); beyond,
-// one lane per body (the plan's grid_wide).
+// slot of the GPU several times over (batches up to 131072 bodies); beyond
+// (the plan's grid_wide false), 4 lanes per body count and keep the first
+// partners, one lane per body emits.
+// The wide schedule gives each body 16 lanes in both passes.  Otherwise the
+// count pass splits a body's cells over 4 lanes (their gathers overlap, and
+// the pass is bound by their latency) and the emit pass takes 1 lane, which
+// for most bodies only copies the partners the count pass kept.
 static void launch_grid_pairs(shp_batch* b, int fill) {
   const Ctx& c = b->ctx;
   const StepPlan& P = b->plan;
   if (P.grid_wide)
     k_grid_pairs<16><<<P.g_pairs, 256, 0, b->stream>>>(c, fill);
+  else if (!fill)
+    k_grid_pairs<4><<<P.g_pairs_count, 256, 0, b->stream>>>(c, 0);
   else
-    k_grid_pairs<1><<<P.g_pairs, 256, 0, b->stream>>>(c, fill);
+    k_grid_pairs<1><<<P.g_pairs, 256, 0, b->stream>>>(c, 1);
 }
 
 // Grid broadphase, first half: binning and per-body pair counts into

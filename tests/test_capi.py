@@ -128,17 +128,20 @@ def test_plan_combined_at_scale(pile_batch, ppb):
 
 
 def test_plan_grid_lanes_at_scale(pile_batch):
-    """Either grid schedule (1 or 16 lanes per body) of the 1024-pile batch
-    passes the audit; 16 lanes launch 16 work-items per body."""
+    """Either grid schedule of the 1024-pile batch passes the audit.  The
+    16-lane schedule launches 16 work-items per body for the count and the
+    fill; the other counts with 4 work-items per body and fills with 1."""
     sc = pile_batch
-    for bp, per_body in ((engine.BROADPHASE_GRID_1, 1), (engine.BROADPHASE_GRID_16, 16)):
+    for bp, per_body in ((engine.BROADPHASE_GRID_1, (4, 1)), (engine.BROADPHASE_GRID_16, (16, 16))):
         st, msg, p, rows = engine.plan(sc.worlds, sc.config, solver=engine.SOLVER_COLORED_F32,
                                        max_pairs_per_body=4, broadphase=bp)
         assert st == 0, msg
         _check_launches(rows)
         g = [r for r in rows if r["name"].startswith("k_grid_pairs")]
-        assert g and all(r["workgroups"] * r["workgroup_size"] >= per_body * p["bodies"] for r in g)
-        assert all(r["workgroups"] * r["workgroup_size"] < per_body * p["bodies"] + 256 for r in g)
+        assert len(g) == 2
+        for r, lanes in zip(g, per_body):
+            assert lanes * p["bodies"] <= r["workgroups"] * r["workgroup_size"]
+            assert r["workgroups"] * r["workgroup_size"] < lanes * p["bodies"] + 256
 
 
 def test_plan_learned_combined_at_scale(pile_batch):
