@@ -94,6 +94,7 @@ struct Ctx {
   double *lx, *ly;
   uint16_t* ext;  // per edge: extent vertex indices, min | max << 8
   double4* wv;    // per vertex: world x, y and the unit normal of its edge
+  double2* wext;  // per edge: the hull's own extent (min, max) along its normal
   double *bminx, *bmaxx, *bminy, *bmaxy;
   // broadphase
   int32_t *cnt_rank, *off_rank;  // NB + 1, indexed by descending-body rank

@@ -61,7 +61,7 @@ __device__ __forceinline__ HullRef hull_of(const Ctx& c, int g) {
   int o = c.vert_off[g];
   HullRef h;
   h.vn = c.wv + o;
-  h.ext = c.ext + o;
+  h.sx = c.wext + o;
   h.n = c.vert_off[g + 1] - o;
   return h;
 }
@@ -112,9 +112,10 @@ __global__ void k_init_hulls(Ctx c) {
 // slot is placed from its state, which for an empty slot reproduces the
 // shape it was left with.
 // A workgroup takes 256 consecutive bodies: one lane per body builds the
-// transform, one lane per vertex transforms it and its successor and writes
-// the vertex and edge normal (coalesced), and one lane per body folds its
-// AABB in vertex order from LDS.
+// transform, one lane per vertex transforms it into LDS, then one lane per
+// vertex writes the vertex, its edge's normal and the hull's extent along
+// that normal (coalesced), and one lane per body folds its AABB in vertex
+// order from LDS.
 constexpr int INTEGRATE_THREADS = 256;
 constexpr int INTEGRATE_VERTS = INTEGRATE_THREADS * SHP_MAXV;
 
@@ -164,23 +165,27 @@ __device__ __forceinline__ void integrate_tile(const Ctx& c, int advance, int vb
   for (int i = t; i < nv; i += INTEGRATE_THREADS) {
     const int b = s_body[i];
     if (s_dead[b]) continue;
-    const int last = s_voff[b + 1] - v0 - 1;
-    const int i1 = i < last ? i + 1 : s_voff[b] - v0;
-    const double m00 = s_m[0][b], m01 = s_m[1][b], m02 = s_m[2][b];
-    const double m10 = s_m[3][b], m11 = s_m[4][b], m12 = s_m[5][b];
     const double lx = c.lx[v0 + i], ly = c.ly[v0 + i];
-    const double lx1 = c.lx[v0 + i1], ly1 = c.ly[v0 + i1];
-    const double x = (m00 * lx + m01 * ly) + m02 * one;
-    const double y = (m10 * lx + m11 * ly) + m12 * one;
-    const double x1 = (m00 * lx1 + m01 * ly1) + m02 * one;
-    const double y1 = (m10 * lx1 + m11 * ly1) + m12 * one;
-    s_wx[i] = x;
-    s_wy[i] = y;
-    d2 nn = normalize(clockwise(sub(mk(x1, y1), mk(x, y))));
-    // vertex and edge normal in one 32-byte record (coalesced)
-    c.wv[v0 + i] = make_double4(x, y, nn.x, nn.y);
+    s_wx[i] = (s_m[0][b] * lx + s_m[1][b] * ly) + s_m[2][b] * one;
+    s_wy[i] = (s_m[3][b] * lx + s_m[4][b] * ly) + s_m[5][b] * one;
   }
   __syncthreads();
+  // From the world vertices in LDS: each edge's unit normal and the hull's
+  // own extent along it (extentAlong, ConvexHull.hs:81-92, at the cached
+  // extent vertices), the projections the SAT reads, once per step here.
+  for (int i = t; i < nv; i += INTEGRATE_THREADS) {
+    const int b = s_body[i];
+    if (s_dead[b]) continue;
+    const int ex = c.ext[v0 + i];
+    const int o = s_voff[b] - v0, last = s_voff[b + 1] - v0 - 1;
+    const int i1 = i < last ? i + 1 : o;
+    const double x = s_wx[i], y = s_wy[i];
+    const d2 nn = normalize(clockwise(sub(mk(s_wx[i1], s_wy[i1]), mk(x, y))));
+    const int lo = o + (ex & 255), hi = o + (ex >> 8);
+    // vertex and edge normal in one 32-byte record, the extent in 16 (coalesced)
+    c.wv[v0 + i] = make_double4(x, y, nn.x, nn.y);
+    c.wext[v0 + i] = make_double2(dot(mk(s_wx[lo], s_wy[lo]), nn), dot(mk(s_wx[hi], s_wy[hi]), nn));
+  }
   if (t < nbk) {
     if (s_dead[t]) return;
     const double r = c.rad[g];
@@ -428,8 +433,9 @@ __device__ __forceinline__ void sat_pair(const Ctx& c, int p) {
   }
 }
 
-// grid-stride over this step's pairs (the launch is sized for the capacity)
-__global__ __launch_bounds__(256) void k_sat(Ctx c) {
+// grid-stride over this step's pairs (the launch is sized for the capacity);
+// at most 128 VGPRs, so that four waves per SIMD overlap their hull loads
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sat(Ctx c) {
   int tp = total_pairs(c);
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < tp; p += gridDim.x * blockDim.x) sat_pair(c, p);
 }
@@ -1052,6 +1058,7 @@ static void batch_buffers(const StepPlan& P, Ctx& c, int32_t** scan_part, std::v
   add(&c.ly, NV * 8);
   add(&c.ext, NV * 2);
   add(&c.wv, NV * 32);
+  add(&c.wext, NV * 16);
   for (double** q : {&c.bminx, &c.bmaxx, &c.bminy, &c.bmaxy}) add(q, NB * 8);
   add(&c.cnt_rank, (NB + 1) * 4);
   add(&c.off_rank, (NB + 1) * 4);

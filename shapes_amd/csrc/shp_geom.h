@@ -76,7 +76,7 @@ __device__ __forceinline__ int clip_segment(d2 bp, d2 bn, d2 ip, d2 inn, d2 a, d
 // indices (min | max << 8), so a hull of n vertices spans 32n + 2n bytes.
 struct HullRef {
   const double4* vn;
-  const uint16_t* ext;
+  const double2* sx;  // per edge: the hull's own extent along the edge normal
   int n;
   __device__ __forceinline__ d2 v(int k) const {
     const double2 p = *(const double2*)(vn + k);
@@ -86,8 +86,6 @@ struct HullRef {
     const double2 p = *((const double2*)(vn + k) + 1);
     return mk(p.x, p.y);
   }
-  __device__ __forceinline__ int emin(int e) const { return ext[e] & 255; }
-  __device__ __forceinline__ int emax(int e) const { return ext[e] >> 8; }
 };
 
 struct Overlap {
@@ -114,16 +112,30 @@ __device__ __forceinline__ void load_hull_v(const HullRef& h, HullV& r) {
     }
 }
 
-// overlap + minOverlap fold (Contact/SAT.hs:103-138; extentAlong
-// ConvexHull.hs:81-92): first separating edge wins; otherwise the first
-// strictly smallest depth.  Returns false if separated.
+// overlap + minOverlap fold (Contact/SAT.hs:103-138): first separating
+// edge wins; otherwise the first strictly smallest depth.  Returns false if
+// separated.  se's own extent along each edge normal (extentAlong,
+// ConvexHull.hs:81-92) comes precomputed with the step's transform; every
+// edge's axis and extent are loaded before the first projection, so that a
+// pair waits on one round of loads per hull rather than on two per edge.
 __device__ __forceinline__ bool min_overlap(const HullRef& se, const HullV& sp, Overlap* out) {
+  double ax[SHP_MAXV], ay[SHP_MAXV], lo[SHP_MAXV], hi[SHP_MAXV];
+#pragma unroll
+  for (int e = 0; e < SHP_MAXV; ++e)
+    if (e < se.n) {
+      const d2 d = se.nrm(e);
+      const double2 s = se.sx[e];
+      ax[e] = d.x;
+      ay[e] = d.y;
+      lo[e] = s.x;
+      hi[e] = s.y;
+    }
   Overlap best{0, 0, 0.0};
-  for (int e = 0; e < se.n; ++e) {
-    d2 dir = se.nrm(e);
-    const int ex = se.ext[e];
-    double smin = dot(se.v(ex & 255), dir);
-    double smax = dot(se.v(ex >> 8), dir);
+#pragma unroll
+  for (int e = 0; e < SHP_MAXV; ++e) {
+    if (e >= se.n) break;
+    const d2 dir = mk(ax[e], ay[e]);
+    const double smin = lo[e], smax = hi[e];
     double pmin = dot(mk(sp.x[0], sp.y[0]), dir), pmax = pmin;
     int imin = 0;
 #pragma unroll
