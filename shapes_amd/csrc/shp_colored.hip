@@ -174,9 +174,9 @@ __device__ __forceinline__ void load_man(const RecW& R, int m_loc, ManF& m) {
 // Pack the <= 2 contacts [k0, k1) of a pair as manifold M.  Every value is
 // loaded before the first store.
 __device__ __forceinline__ void pack_man(const Ctx& c, const PackF& P, int k0, int k1, int M) {
-  const long CC = c.contact_cap;
   const int ga = c.ga[k0], gb = c.gb[k0];
-  const double ux = c.jn[3 * CC + k0], uy = c.jn[4 * CC + k0];  // the row normal (flip applied)
+  const d2 u = row_normal(c, k0);
+  const double ux = u.x, uy = u.y;
   const double ima = c.iml[ga], ira = c.imr[ga], imb = c.iml[gb], irb = c.imr[gb];
   const float4 w0 = make_float4((float)ux, (float)uy, (float)((c.mu[ga] + c.mu[gb]) / 2.0),
                                 (float)hs_min(c.e[ga], c.e[gb]));
@@ -191,8 +191,8 @@ __device__ __forceinline__ void pack_man(const Ctx& c, const PackF& P, int k0, i
   for (int j = 0; j < 2; ++j) {
     if (j == 1 && !two) break;
     const int k = k0 + j;
-    const double jn2 = c.jn[2 * CC + k], jn5 = c.jn[5 * CC + k];
-    const double jt2 = c.jt[2 * CC + k], jt5 = c.jt[5 * CC + k];
+    const double4 x = c.jx[k];
+    const double jn2 = x.x, jn5 = x.y, jt2 = x.z, jt5 = x.w;
     cw[j][0] = make_float4((float)jn2, (float)jn5, (float)jt2, (float)jt5);
     cw[j][1] = make_float4((float)c.bn[k], (float)(1.0 / c.mcn[k]), (float)(1.0 / c.mct[k]),
                            (float)(ira * jn2));

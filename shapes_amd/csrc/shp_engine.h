@@ -110,7 +110,7 @@ struct Ctx {
   int32_t *ga, *gb;
   uint8_t* flip;
   double *cnx, *cny, *cpx, *cpy, *depth;
-  double *jn, *jt;  // 6 * contact_cap, row-major by component
+  double4* jx;      // per contact: the rows' cross terms (jn[2], jn[5], jt[2], jt[5])
   double *bn, *rax, *ray, *rbx, *rby, *mcn, *mct;
   double *ln, *lt;
   uint8_t* warm;

@@ -503,10 +503,9 @@ __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
         jt[q + 3] = t[q];
       }
     }
-    for (int q = 0; q < 6; ++q) {
-      c.jn[(size_t)q * c.contact_cap + dst] = jn[q];
-      c.jt[(size_t)q * c.contact_cap + dst] = jt[q];
-    }
+    // the other eight entries of the rows are +-n and +-cw(n), rebuilt
+    // exactly from the normal and flip (row_normal)
+    c.jx[dst] = make_double4(jn[2], jn[5], jt[2], jt[5]);
     // baumgarte (NonPenetration.hs:48-56)
     c.bn[dst] = depth > c.slop ? (c.beta / c.dt) * (c.slop - depth) : 0.0;
     // Restitution.constraintGen (Restitution.hs:21-31)
@@ -1076,8 +1075,7 @@ static void batch_buffers(const StepPlan& P, Ctx& c, int32_t** scan_part, std::v
   add(&c.gb, CC * 4);
   add(&c.flip, CC);
   for (double** q : {&c.cnx, &c.cny, &c.cpx, &c.cpy, &c.depth}) add(q, CC * 8);
-  add(&c.jn, 6 * CC * 8);
-  add(&c.jt, 6 * CC * 8);
+  add(&c.jx, CC * 32);
   for (double** q : {&c.bn, &c.rax, &c.ray, &c.rbx, &c.rby, &c.mcn, &c.mct, &c.ln, &c.lt})
     add(q, CC * 8);
   add(&c.warm, CC);

@@ -36,12 +36,31 @@ __device__ __forceinline__ void apply_lag(double l, const double* j, const doubl
   for (int k = 0; k < 6; ++k) v[k] = v[k] + ((j[k] * l) * im[k]);
 }
 
+// The row normal u of contact k: jn[3..4] as constraint generation computes
+// them, n for a pair in order and -n for a flipped one (whose halves are
+// swapped back).  Then jn = (-u, jn2, u, jn5) and jt = (-cw(u), jt2, cw(u),
+// jt5) in both cases, and negation is exact, so the rows are rebuilt
+// bit for bit from u and the stored cross terms.
+__device__ __forceinline__ d2 row_normal(const Ctx& c, int k) {
+  const double nx = c.cnx[k], ny = c.cny[k];
+  return c.flip[k] ? mk(-nx, -ny) : mk(nx, ny);
+}
+
 __device__ __forceinline__ void load_rows(const Ctx& c, int k, double* jn, double* jt) {
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    jn[q] = c.jn[(size_t)q * c.contact_cap + k];
-    jt[q] = c.jt[(size_t)q * c.contact_cap + k];
-  }
+  const d2 u = row_normal(c, k), t = clockwise(u);
+  const double4 x = c.jx[k];
+  jn[0] = -u.x;
+  jn[1] = -u.y;
+  jn[2] = x.x;
+  jn[3] = u.x;
+  jn[4] = u.y;
+  jn[5] = x.y;
+  jt[0] = -t.x;
+  jt[1] = -t.y;
+  jt[2] = x.z;
+  jt[3] = t.x;
+  jt[4] = t.y;
+  jt[5] = x.w;
 }
 
 }  // namespace shp

@@ -137,12 +137,14 @@ __device__ __forceinline__ void pack_flow(const Ctx& c, int k, int dst) {
   // every value is loaded before the first store, so the loads are not
   // serialised behind stores the compiler cannot prove disjoint
   double f[FL_FIELDS];
-  f[FL_UX] = c.jn[3 * CC + k];
-  f[FL_UY] = c.jn[4 * CC + k];
-  f[FL_JN2] = c.jn[2 * CC + k];
-  f[FL_JN5] = c.jn[5 * CC + k];
-  f[FL_JT2] = c.jt[2 * CC + k];
-  f[FL_JT5] = c.jt[5 * CC + k];
+  const d2 u = row_normal(c, k);
+  const double4 x = c.jx[k];
+  f[FL_UX] = u.x;
+  f[FL_UY] = u.y;
+  f[FL_JN2] = x.x;
+  f[FL_JN5] = x.y;
+  f[FL_JT2] = x.z;
+  f[FL_JT5] = x.w;
   f[FL_BN] = c.bn[k];
   f[FL_RAX] = c.rax[k];
   f[FL_RAY] = c.ray[k];
