@@ -102,10 +102,11 @@ __global__ void k_grid_fill(Ctx c) {
   *(int4*)(p + 2) = make_int4(cx, cy, g, c.is_static[g] != 0);
 }
 
-// LANES lanes per body i (1 or 16): count (fill = 0) or emit (fill = 1)
-// its pairs.  With one lane, the count pass also keeps the first
-// GRID_SLOTS partners, and the emit pass writes those without a second
-// visit when they are all.  With 16, a small body's lanes 0..8 take one cell each of the
+// LANES lanes per body i (1, 4 or 16): count (fill = 0) or emit (fill = 1)
+// its pairs.  With one lane (or four, which count only, splitting the
+// cells), the count pass also keeps the first GRID_SLOTS partners, and the
+// one-lane emit pass writes those without a second visit when they are
+// all.  With 16, a small body's lanes 0..8 take one cell each of the
 // <= 3x3 cells that can hold a partner's min corner and lanes 9..15 the
 // world's large bodies, and a large body's lanes stride over every j < i:
 // for batches too small to fill the GPU with one lane per body (the
