@@ -192,21 +192,28 @@ def kernel_bytes(phase, nb, nv, st, word):
         # written (16 B); per body its colour mask and priority (12 B)
         return P * 16 + nb * 12
     if phase == "finish":
-        # advance + transform: state read (6W) and pose written (3W), local
-        # vertices read (2W), world vertices and normals written (4W), AABB
-        # written (4W); quality: per contact bodies, velocities, rows,
-        # radii, normal, bias, lagrangians, masses, materials (~273 B)
-        return nb * (13 * word + 4) + nv * 6 * word + C * 273
+        # advance + transform (FP64 in both modes): state read (6 doubles)
+        # and pose written (3), AABB written (4); per vertex the local vertex
+        # and extent indices read (18 B), world vertex, normal and own
+        # extent written (48 B); quality: per contact bodies, velocities,
+        # cross terms, normal, flip, radii, bias, lagrangians, mass,
+        # depth, materials (209 B)
+        return nb * (13 * 8 + 4) + nv * (18 + 48) + C * 209
     if phase == "sat":
         # per pair its two body indices read, its contact count and (coloured)
         # its colouring word written; every hull's world vertices, normals
-        # and extent indices read once (FP64 in both modes); per contact
-        # slot its normal, point, depth, feature and flip written
-        return P * (8 + 4 + (4 if word == 4 else 0)) + nv * (4 * 8 + 2) + C * 43
+        # and own extents read once (FP64 in both modes); per contact slot
+        # its normal, point, depth, feature and flip written
+        return P * (8 + 4 + (4 if word == 4 else 0)) + nv * 48 + C * 43
     if phase == "contacts":
-        return C * (43 + 2 * 40 + 24) + C * 226
-    if phase == "solve_setup":  # manifold packing: contact rows in, records out
-        return C * (8 + 8 * 8 + 6 * 8 + 1) + M * 112 + (C - M) * 64
+        # per contact its slot (43 B), both bodies' positions and inverse
+        # masses (64 B) and the cache entry it joins (24 B) read; key, bodies,
+        # flip, normal, point, depth, the rows' cross terms, bias, radii,
+        # effective masses, lagrangians and warm flag written (162 B); for
+        # the coloured solver each manifold's record written (112 B, 64 B
+        # per second contact) with both bodies' materials read (34 B)
+        rec = M * (112 + 34) + (C - M) * 64 if word == 4 else 0
+        return C * (43 + 64 + 24 + 162) + rec
     if phase == "solve_writeback":  # lagrangians back, FP32 -> FP64 velocities
         return C * (16 + 16) + nb * (12 + 48)
     if phase == "broadphase_fill":
@@ -223,7 +230,7 @@ def roofline(instr, scene, word, broadphase="grid", threads=0):
     largest average HIP-event time among those holding a single kernel):
     algorithmic bytes per launch / its average launch duration."""
     ph = instr["phases_ms"]
-    single = ["sat", "contacts", "infer", "solve_setup", "sweeps", "solve_writeback", "finish",
+    single = ["sat", "contacts", "infer", "sweeps", "solve_writeback", "finish",
               "cache_store", "colouring", "broadphase_fill"]
     dom = max(single, key=lambda p: ph[p])
     ms = ph[dom]

@@ -36,6 +36,7 @@
 
 #include "shp_engine.h"
 #include "shp_geom.h"
+#include "shp_records.h"
 #include "shp_solve_common.h"
 
 using namespace shp;
@@ -47,25 +48,6 @@ using namespace shp;
 #define GREEDY_COLOURS 16
 #define JP_ROUNDS (COLOR_CAP - GREEDY_COLOURS - 3)
 
-// Manifold M's record: 7 words of 16 bytes, word q at
-// rec[(M / 64) * 7 * 64 + q * 64 + M % 64]
-//   0  ux, uy, mu, e       the row normal (flip applied), materials
-//   1  ax, ay, bx, by      the normal times the inverse masses of a / b
-//   2  ga, gb, flags, 0    bodies (global ids), flags
-//   3  jn2, jn5, jt2, jt5  the first contact: angular entries of the
-//                          normal / friction rows
-//   4  bn, icn, ict, wn2   bias, reciprocal effective masses, angular
-//   5  wn5, wt2, wt5, 0      entries times the inverse inertias
-//   6  ln, lt              its lagrangians (FP64)
-// and the second contact's 4 words (3-6 above) at
-// rec2[(M / 64) * 4 * 64 + q * 64 + M % 64].  Products and reciprocals are
-// formed in FP64 and rounded once.
-enum { REC_WORDS = 7, REC2_WORDS = 4 };
-static_assert(16 * 64 * (REC_WORDS + REC2_WORDS) == 4 * COLOR_PACK_FLOATS(64),
-              "record words differ from the buffer size");
-// manifold flags: 1 a static, 2 b static, 4 first contact warm, 8 second
-// contact present, 16 second contact warm
-enum { MFL_A_STATIC = 1, MFL_B_STATIC = 2, MFL_WARM0 = 4, MFL_TWO = 8, MFL_WARM1 = 16 };
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
   x ^= x >> 16;
@@ -82,27 +64,6 @@ __device__ __forceinline__ float dot6f(const float* a, const float* b) {
 #pragma unroll
   for (int q = 1; q < 6; ++q) d = fmaf(a[q], b[q], d);
   return d;
-}
-
-struct PackF {
-  float4* rec;   // REC_WORDS x PC (PC rounded up to blocks)
-  float4* rec2;  // REC2_WORDS x PC
-};
-
-__device__ __forceinline__ PackF packf(const Ctx& c) {
-  PackF p;
-  p.rec = (float4*)c.pk;
-  p.rec2 = p.rec + (long)REC_WORDS * ((c.pair_cap + 63) / 64 * 64);
-  return p;
-}
-
-// Word 0 of manifold M's record (words q at + 64 q) and of its second
-// contact's.
-__device__ __forceinline__ float4* rec_of(float4* rec, int M) {
-  return rec + (long)(M >> 6) * (REC_WORDS * 64) + (M & 63);
-}
-__device__ __forceinline__ float4* rec2_of(float4* rec2, int M) {
-  return rec2 + (long)(M >> 6) * (REC2_WORDS * 64) + (M & 63);
 }
 
 struct ConF {
@@ -169,50 +130,6 @@ __device__ __forceinline__ void load_man(const RecW& R, int m_loc, ManF& m) {
   m.flags = g.z;
   load_contact(r + 192, m.k[0]);
   if (m.flags & MFL_TWO) load_contact(rec2_of(R.rec2, R.p0 + m_loc), m.k[1]);
-}
-
-// Pack the <= 2 contacts [k0, k1) of a pair as manifold M.  Every value is
-// loaded before the first store.
-__device__ __forceinline__ void pack_man(const Ctx& c, const PackF& P, int k0, int k1, int M) {
-  const int ga = c.ga[k0], gb = c.gb[k0];
-  const d2 u = row_normal(c, k0);
-  const double ux = u.x, uy = u.y;
-  const double ima = c.iml[ga], ira = c.imr[ga], imb = c.iml[gb], irb = c.imr[gb];
-  const float4 w0 = make_float4((float)ux, (float)uy, (float)((c.mu[ga] + c.mu[gb]) / 2.0),
-                                (float)hs_min(c.e[ga], c.e[gb]));
-  const float4 w1 = make_float4((float)(ima * ux), (float)(ima * uy), (float)(imb * ux),
-                                (float)(imb * uy));
-  const bool two = k1 - k0 > 1;
-  int flags = (c.is_static[ga] ? MFL_A_STATIC : 0) | (c.is_static[gb] ? MFL_B_STATIC : 0) |
-              (c.warm[k0] ? MFL_WARM0 : 0) | (two ? MFL_TWO : 0);
-  float4 cw[2][3];
-  double2 lam[2];
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    if (j == 1 && !two) break;
-    const int k = k0 + j;
-    const double4 x = c.jx[k];
-    const double jn2 = x.x, jn5 = x.y, jt2 = x.z, jt5 = x.w;
-    cw[j][0] = make_float4((float)jn2, (float)jn5, (float)jt2, (float)jt5);
-    cw[j][1] = make_float4((float)c.bn[k], (float)(1.0 / c.mcn[k]), (float)(1.0 / c.mct[k]),
-                           (float)(ira * jn2));
-    cw[j][2] = make_float4((float)(irb * jn5), (float)(ira * jt2), (float)(irb * jt5), 0.0f);
-    lam[j] = make_double2(c.ln[k], c.lt[k]);
-    if (j == 1 && c.warm[k]) flags |= MFL_WARM1;
-  }
-  float4* r = rec_of(P.rec, M);
-  r[0] = w0;
-  r[64] = w1;
-  *(int4*)(r + 128) = make_int4(ga, gb, flags, 0);
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    if (j == 1 && !two) break;
-    float4* w = j ? rec2_of(P.rec2, M) : r + 192;
-    w[0] = cw[j][0];
-    w[64] = cw[j][1];
-    w[128] = cw[j][2];
-    *(double2*)(w + 192) = lam[j];
-  }
 }
 
 // v += l * w, fused per entry (w: a row times the inverse masses)
@@ -1098,22 +1015,9 @@ __global__ __launch_bounds__(T) void k_colour(Ctx c, int pool_bytes) {
   tmark(c, 1);
 }
 
-// A manifold whose contacts lie beyond the contact capacity (the step then
-// reports the overflow): a record that touches no velocity and no contact.
-__device__ __forceinline__ void pack_null(const Ctx& c, const PackF& P, int p, int M) {
-  float4* r = rec_of(P.rec, M);
-  const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  r[0] = z;
-  r[64] = z;
-  *(int4*)(r + 128) = make_int4(c.pa[p], c.pb[p], MFL_A_STATIC | MFL_B_STATIC, 0);
-  r[192] = z;
-  r[256] = z;
-  r[320] = z;
-  r[384] = z;
-}
-
 // Every coloured pair's <= 2 contacts become its manifold record, over the
-// whole batch at once.
+// whole batch at once, when the contact kernel ran beside the colouring.
+// Every value is loaded before the first store.
 __global__ void k_pack_manifolds(Ctx c) {
   const int tp = c.wpair_off[c.W];
   const PackF P = packf(c);
@@ -1122,12 +1026,35 @@ __global__ void k_pack_manifolds(Ctx c) {
     if (M < 0) continue;
     const int k0 = c.coff[p];
     if (k0 >= c.contact_cap) {
-      pack_null(c, P, p, M);
+      write_null_record(P, M, c.pa[p], c.pb[p]);
       continue;
     }
-    int k1 = k0 + c.ccount[p];
-    if (k1 > c.contact_cap) k1 = c.contact_cap;  // contacts beyond the capacity were not stored
-    pack_man(c, P, k0, k1, M);
+    const int cap = c.contact_cap - k0, nk = c.ccount[p] < cap ? c.ccount[p] : cap;
+    const int ga = c.ga[k0], gb = c.gb[k0];
+    const d2 u = row_normal(c, k0);
+    const double ima = c.iml[ga], ira = c.imr[ga], imb = c.iml[gb], irb = c.imr[gb];
+    double4 x[2];
+    double bn[2], mcn[2], mct[2], ln[2], lt[2];
+    int warm = 0;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (j >= nk) break;
+      const int k = k0 + j;
+      x[j] = c.jx[k];
+      bn[j] = c.bn[k];
+      mcn[j] = c.mcn[k];
+      mct[j] = c.mct[k];
+      ln[j] = c.ln[k];
+      lt[j] = c.lt[k];
+      warm |= (c.warm[k] ? 1 : 0) << j;
+    }
+    write_record_head(c, P, M, ga, gb, u, ima, imb, nk, warm);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (j >= nk) break;
+      write_record_rows(P, M, j, ira, irb, x[j].x, x[j].y, x[j].z, x[j].w, bn[j], mcn[j], mct[j]);
+      write_record_lambda(P, M, j, ln[j], lt[j]);
+    }
   }
 }
 
@@ -1250,6 +1177,13 @@ void plan_colored_solver(StepPlan* P, int solver_threads) {
   P->color_threads = solver_threads == 64 ? 256
                      : solver_threads     ? solver_threads
                                           : (nb > 2048 ? 512 : 256);
+  // A colouring workgroup of 1024 lanes takes a whole CU; with at least as
+  // many worlds as CUs the colouring fills the GPU and gains little beside
+  // the contact kernel (1024 piles: 6.60 ms for both plus the packing pass,
+  // 6.04 ms in turn with the records from the contact kernel), while a few
+  // worlds' colouring costs nothing there (one world: the contact kernel's
+  // time) and smaller workgroups share CUs with it.
+  P->records_in_contacts = P->color_threads >= 512 && P->W >= RECORDS_IN_CONTACTS_MIN_WORLDS;
 }
 
 // k_colour: sm[4]; k_sweep_colours: s_off[COLOR_CAP + 3]
@@ -1286,6 +1220,10 @@ shp_status colored_alloc(shp_batch* b) {
   return SHP_OK;
 }
 
+void launch_colored_pack(shp_batch* b) {
+  k_pack_manifolds<<<b->plan.g_pack, 256, 0, b->stream>>>(b->ctx);
+}
+
 void launch_colouring(shp_batch* b, hipStream_t s) {
   const Ctx& c = b->ctx;
   const StepPlan& P = b->plan;
@@ -1296,9 +1234,6 @@ void launch_colouring(shp_batch* b, hipStream_t s) {
     k_colour<256><<<c.W, 256, P.color_lds, s>>>(c, pool);
 }
 
-void launch_colored_pack(shp_batch* b) {
-  k_pack_manifolds<<<b->plan.g_pack, 256, 0, b->stream>>>(b->ctx);
-}
 
 void launch_colored_sweep(shp_batch* b) {
   const Ctx& c = b->ctx;

@@ -51,6 +51,10 @@ enum { CNT_FLAGS = 0, CNT_WARM = 1, CNT_LEARNED = 2, CNT_LEVELS = 3, CNT_MISS = 
 // grid broadphase: partners per body kept by the count pass for the emit
 // pass (bodies with more are visited again)
 enum { GRID_SLOTS = 8 };
+// worlds from which the coloured solver's records come from the contact
+// kernel, after the colouring (with 1024-lane colouring workgroups): the
+// CUs of an MI355X
+enum { RECORDS_IN_CONTACTS_MIN_WORLDS = 256 };
 // solve-quality maxima (doubles stored as their bit patterns): last step,
 // then since creation; then the sums of the steps' maxima and the number
 // of steps summed
@@ -155,6 +159,7 @@ struct Ctx {
   int collect_misses;
   int miss_cap;
   int32_t* miss_idx;
+  int32_t* miss_rec;  // per miss: its manifold record (slot * 2 + contact), or -1
   float* miss_feat;
   long log_cap;
   float *log_feat, *log_tgt;
@@ -162,6 +167,7 @@ struct Ctx {
   // uniform-grid broadphase (use_grid): per-world cell size and hash-table
   // offsets, per-body cell, large static bodies listed per world
   int use_grid;
+  int records_in_contacts;  // the coloured solver's records from k_contacts
   int grid_buckets;
   int probe;  // 1: contact generation of the current world only (no acceleration)
   double* cell_h;
@@ -207,6 +213,9 @@ struct StepPlan {
   // coloured solver: lanes per world, dynamic LDS of the colouring, LDS pool
   int color_threads = 256;
   size_t color_lds = 0, color_pool = 0;
+  // the contact kernel writes the manifold records itself, after the
+  // colouring, instead of beside it with a packing pass after both
+  bool records_in_contacts = false;
   // with solver_threads 64: the worlds of at most SWEEP_WAVE_MAX_BODIES
   // bodies (world_order[sweep_big, W)) sweep one wavefront each, several
   // per workgroup, velocities in the wavefront's LDS slice (floats)
@@ -299,6 +308,7 @@ shp_status colored_alloc(shp_batch* b);
 // plan values of the solvers (lanes, LDS) from the largest world
 void plan_ordered_solver(StepPlan* P, int solver_threads);
 void plan_colored_solver(StepPlan* P, int solver_threads);
+void launch_colored_pack(shp_batch* b);
 // static LDS of the solver kernels at `threads` lanes
 size_t ordered_static_lds(int threads);
 size_t colored_static_lds(bool sweep);
@@ -306,7 +316,6 @@ void launch_ordered_solver(shp_batch* b);
 // coloured solver: colouring (needs pairs and contact counts only), then,
 // after the contacts, manifold packing and the sweeps
 void launch_colouring(shp_batch* b, hipStream_t s);
-void launch_colored_pack(shp_batch* b);
 void launch_colored_sweep(shp_batch* b);
 void launch_colored_unpack(shp_batch* b);
 hipError_t launch_grid_count(shp_batch* b);

@@ -14,11 +14,11 @@
 //   k_sat            one lane per pair: SAT + clipping -> <= 2 contacts
 //   scan             exclusive sum of contacts per pair
 //   k_world_ranges   per-world pair and contact ranges
-//   k_contacts       compaction, constraintGen, warm-start join (beside it,
-//                    on a second stream, the coloured solver's k_colour)
+//   k_contacts       compaction, constraintGen, warm-start join; for the
+//                    coloured solver, after its k_colour (on a second
+//                    stream), also each manifold's record in colour order
 //   solve            warm-start application + Gauss-Seidel sweeps
-//                    (k_solve_flow, or k_pack_manifolds + k_sweep_colours +
-//                    k_colour_unpack)
+//                    (k_solve_flow, or k_sweep_colours + k_colour_unpack)
 //   k_finish         advance + moveShapes + AABBs (k_integrate's tiles) and
 //                    the solve quality maxima, in one launch
 //   k_cache_store    this frame's (key, lambda) become next frame's cache,
@@ -43,6 +43,7 @@
 #include "shp_geom.h"
 #include "shp_trig.h"
 #include "shp_engine.h"
+#include "shp_records.h"
 #include "shp_solve_common.h"
 
 using namespace shp;
@@ -458,7 +459,8 @@ __global__ void k_world_ranges(Ctx c) {
 // start join of applyCachedSlns (Solvers/Contact.hs:74-121): a binary search
 // of the key in last frame's cache of the same world, which is strictly
 // descending, finds exactly the element descZipVector (Utils/Descending.hs:
-// 47-71) would pair it with.
+// 47-71) would pair it with.  For the coloured solver, the pair's manifold
+// record too, at the slot the colouring gave it (none: no slot).
 __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
   int ga = c.pa[p], gb = c.pb[p];
   int w = c.body_world[ga];
@@ -467,8 +469,11 @@ __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
   double im[6] = {c.iml[ga], c.iml[ga], c.imr[ga], c.iml[gb], c.iml[gb], c.imr[gb]};
   int cache_lo = c.cache_woff[w], cache_hi = c.cache_woff[w + 1];
   int c_lo = c.wcont_off[w], c_hi = c.wcont_off[w + 1];
+  const int M = c.records_in_contacts ? c.sched_order[p] : -1;
+  const int k0 = c.coff[p];
+  const PackF P = packf(c);
   for (int k = 0; k < n; ++k) {
-    int dst = c.coff[p] + k;
+    int dst = k0 + k;
     if (dst >= c.contact_cap) continue;
     int s = 2 * p + k;
     int fa = c.s_feat[s] >> 8, fb = c.s_feat[s] & 255;
@@ -507,15 +512,18 @@ __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
     // exactly from the normal and flip (row_normal)
     c.jx[dst] = make_double4(jn[2], jn[5], jt[2], jt[5]);
     // baumgarte (NonPenetration.hs:48-56)
-    c.bn[dst] = depth > c.slop ? (c.beta / c.dt) * (c.slop - depth) : 0.0;
+    const double bn = depth > c.slop ? (c.beta / c.dt) * (c.slop - depth) : 0.0;
+    c.bn[dst] = bn;
     // Restitution.constraintGen (Restitution.hs:21-31)
     d2 ra = sub(pp, xa), rb = sub(pp, xb);
     c.rax[dst] = ra.x;
     c.ray[dst] = ra.y;
     c.rbx[dst] = rb.x;
     c.rby[dst] = rb.y;
-    c.mcn[dst] = eff_mass(jn, im);
-    c.mct[dst] = eff_mass(jt, im);
+    const double mcn = eff_mass(jn, im), mct = eff_mass(jt, im);
+    c.mcn[dst] = mcn;
+    c.mct[dst] = mct;
+    if (M >= 0) write_record_rows(P, M, k, im[2], im[5], jn[2], jn[5], jt[2], jt[5], bn, mcn, mct);
     // warm start lookup in the descending cache segment [cache_lo,
     // cache_hi): the first entry <= key, then equality.  This frame's
     // contacts come in the same order and mostly persist, so the search
@@ -558,14 +566,20 @@ __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
       }
       if (R < cache_hi && c.cache_key[R] == key) found = R;
     }
+    double ln = 0.0, lt = 0.0;
     if (found >= 0) {
-      c.ln[dst] = c.cache_ln[found];
-      c.lt[dst] = c.cache_lt[found];
+      ln = c.cache_ln[found];
+      lt = c.cache_lt[found];
+    }
+    if (M >= 0) write_record_lambda(P, M, k, ln, lt);
+    if (found >= 0) {
+      c.ln[dst] = ln;
+      c.lt[dst] = lt;
       c.warm[dst] = 1;
       ++hits;
     } else {
       // newCache (Solvers/Contact.hs:91-99): zero start; the learned warm
-      // start may replace it before the solve
+      // start may replace it before the solve (and patch the record)
       c.ln[dst] = 0.0;
       c.lt[dst] = 0.0;
       c.warm[dst] = 0;
@@ -578,12 +592,22 @@ __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
         const int slot = __shfl(base, leader) + (int)__popcll(act & ((1ull << lane) - 1ull));
         if (slot < c.miss_cap) {
           c.miss_idx[slot] = dst;
+          c.miss_rec[slot] = M >= 0 ? (M << 1) | k : -1;
           double v[6] = {c.vx[ga], c.vy[ga], c.w[ga], c.vx[gb], c.vy[gb], c.w[gb]};
           double sgn = flip ? -1.0 : 1.0;
           mlp_features(sgn * nn.x, sgn * nn.y, depth, v, ra.x, ra.y, rb.x, rb.y, c.iml[ga], c.iml[gb],
                        c.imr[ga], c.imr[gb], (c.mu[ga] + c.mu[gb]) / 2.0, c.miss_feat + (size_t)slot * MLP_IN);
         }
       }
+    }
+  }
+  if (M >= 0) {  // the manifold's words, from its first contact as stored above
+    if (k0 >= c.contact_cap) {
+      write_null_record(P, M, ga, gb);
+    } else {
+      const int nk = n < c.contact_cap - k0 ? n : c.contact_cap - k0;
+      const int warm = (c.warm[k0] ? 1 : 0) | (nk > 1 && c.warm[k0 + 1] ? 2 : 0);
+      write_record_head(c, P, M, ga, gb, row_normal(c, k0), im[0], im[3], nk, warm);
     }
   }
 }
@@ -946,8 +970,11 @@ static shp_status enqueue_front(shp_batch* b, bool fork) {
   // device).  Before the colouring starts: a colouring workgroup takes a
   // whole CU, and the scan's short passes would wait behind them.
   HIPCHK(exclusive_sum(b, P.scan_pairs, c.ccount, c.coff, c.off_rank + c.NB, c.pair_cap, s));
-  // the colouring needs only the pairs and which of them have contacts: it
-  // runs beside the contact generation
+  // the colouring needs only the pairs and which of them have contacts.  It
+  // runs beside the contact kernel, or, when its workgroups would take every
+  // CU anyway, before it, so that the contact kernel writes each manifold's
+  // record at its colour slot (phase "world_ranges" then holds the wait)
+  // and no packing pass reads the contacts back.
   if (fork) {
     HIPCHK(hipEventRecord(b->fork_event, s));
     HIPCHK(hipStreamWaitEvent(b->side_stream, b->fork_event, 0));
@@ -958,6 +985,7 @@ static shp_status enqueue_front(shp_batch* b, bool fork) {
   }
   mark(b, 6);
   k_world_ranges<<<P.g_ranges, 256, 0, s>>>(c);
+  if (fork && P.records_in_contacts) HIPCHK(hipStreamWaitEvent(s, b->join_event, 0));
   mark(b, 7);
   k_contacts<<<P.g_contacts, 256, 0, s>>>(c);
   mark(b, 8);
@@ -971,14 +999,16 @@ static shp_status enqueue_step(shp_batch* b) {
   hipStream_t s = b->stream;
   const bool fork = c.solver == SHP_SOLVER_COLORED_F32;
   TRY(enqueue_front(b, fork));
-  // the learned warm start; with the colouring, after it (phase "infer"
-  // then holds the wait for the colouring too), so that its workgroups do
-  // not wait for CUs behind colouring workgroups
-  if (fork) HIPCHK(hipStreamWaitEvent(s, b->join_event, 0));
+  // the learned warm start of the cache misses (which also patches their
+  // manifold records when the contact kernel wrote them); with the colouring
+  // beside the contact kernel, after it (phase "infer" then holds the wait
+  // for the colouring), so that its workgroups do not queue behind
+  // colouring workgroups
+  if (fork && !P.records_in_contacts) HIPCHK(hipStreamWaitEvent(s, b->join_event, 0));
   if (c.warm_mode == SHP_WARM_LEARNED && c.mlp_ready) mlp_launch_infer(b);
   mark(b, 9);
   if (fork) {
-    launch_colored_pack(b);
+    if (!P.records_in_contacts) launch_colored_pack(b);
     mark(b, 10);
     launch_colored_sweep(b);
     mark(b, 11);
@@ -1085,6 +1115,7 @@ static void batch_buffers(const StepPlan& P, Ctx& c, int32_t** scan_part, std::v
   add(&c.cache_woff, (W + 1) * 4);
   if (P.miss_cap > 0) {
     add(&c.miss_idx, (size_t)P.miss_cap * 4);
+    add(&c.miss_rec, (size_t)P.miss_cap * 4);
     add(&c.miss_feat, (size_t)P.miss_cap * MLP_IN * 4);
   }
   add(&c.log_count, 8);
@@ -1342,8 +1373,8 @@ shp_status plan_staged(const shp_config& cfg, HostBatch* hb) {
   P.g_sat = gs_blocks(PC);
   P.g_ranges = blocks(n_worlds + 1, 256);
   P.g_contacts = gs_blocks(PC);
+  P.g_pack = gs_blocks(PC);
   P.g_infer = std::min<long>(std::max<long>(1, blocks(P.miss_cap, 64)), 512);
-  P.g_pack = std::max(1L, std::min(blocks(PC, 256), 4096L));
   P.g_unpack = std::max(1L, std::min(blocks(std::max(PC, NB), 256), 4096L));
   P.g_log = std::max<long>(1, blocks(P.miss_cap, 256));
   P.g_finish_ni = blocks(NB, INTEGRATE_THREADS);
@@ -1385,7 +1416,7 @@ void plan_launches(const StepPlan& P, bool infer, bool logging, std::vector<Laun
   add("k_contacts", P.g_contacts, 256, 0, lds16(4));
   if (infer) add("k_mlp_infer", P.g_infer, 256, 0, mlp_infer_static_lds());
   if (col) {
-    add("k_pack_manifolds", P.g_pack, 256, 0, 0);
+    if (!P.records_in_contacts) add("k_pack_manifolds", P.g_pack, 256, 0, 0);
     if (P.sweep_big > 0)
       add("k_sweep_colours", P.sweep_big, P.color_threads, P.color_pool, colored_static_lds(true));
     if (P.g_sweep_waves > 0)
@@ -1558,6 +1589,7 @@ shp_status batch_alloc(shp_batch* b, const HostBatch& hb) {
   c.solver = b->cfg.solver;
   c.warm_mode = b->cfg.warm_start;
   c.use_grid = P.use_grid ? 1 : 0;
+  c.records_in_contacts = P.records_in_contacts ? 1 : 0;
   c.grid_buckets = (int)P.grid_buckets;
   b->h_body_off = hb.body_off;
   b->h_vert_off = hb.vert_off;
@@ -1875,6 +1907,7 @@ shp_status shp_batch_set_logging(shp_batch* b, int64_t capacity) {
     P.miss_cap = std::min(2 * P.pair_cap, 4 * P.NB + 4096);
     P.g_log = std::max<long>(1, blocks(P.miss_cap, 256));
     TRY(dalloc(b, &c.miss_idx, P.miss_cap));
+    TRY(dalloc(b, &c.miss_rec, P.miss_cap));
     TRY(dalloc(b, &c.miss_feat, (size_t)P.miss_cap * MLP_IN));
     c.miss_cap = (int)P.miss_cap;
   }
@@ -2126,6 +2159,7 @@ static shp_status rebuild_with(shp_batch* b, long g, const StagedBody& s) {
   c.log_cap = keep.log_cap;
   if (b->plan.miss_cap == 0 && keep.miss_cap > 0) {  // miss buffers added for logging
     c.miss_idx = keep.miss_idx;
+    c.miss_rec = keep.miss_rec;
     c.miss_feat = keep.miss_feat;
     c.miss_cap = keep.miss_cap;
     b->plan.miss_cap = old_plan.miss_cap;
@@ -2227,9 +2261,13 @@ shp_status shp_batch_probe_contacts(shp_batch* b) {
   HIPCHK(hipStreamSynchronize(b->stream));
   const bool timing = b->timing;
   b->timing = false;
+  // no colouring runs: the contact kernel writes no manifold records
+  const int records = b->ctx.records_in_contacts;
   b->ctx.probe = 1;
+  b->ctx.records_in_contacts = 0;
   const shp_status st = enqueue_front(b, false);
   b->ctx.probe = 0;
+  b->ctx.records_in_contacts = records;
   b->timing = timing;
   TRY(st);
   return shp_batch_sync(b);

@@ -25,6 +25,7 @@
 #include "../../include/shapes_gpu.h"
 #include "shp_engine.h"
 #include "shp_mlp.h"
+#include "shp_records.h"
 
 namespace {
 
@@ -428,7 +429,8 @@ void mlp_launch_forward(const MlpParams& p, const uint16_t* X, int rows, float* 
 // features rounded to BF16, the forward pass on the matrix cores, then the
 // output head lambda_n = max(y0, 0) / mn, lambda_t = y1 / mt projected onto
 // |lambda_t| <= mu * lambda_n.  The contact is marked warm (2) so the
-// solver applies the predicted impulses like a cached solution; a
+// solver applies the predicted impulses like a cached solution (the
+// coloured solver's manifold record, already written, gets them too); a
 // non-finite prediction keeps the zero start.
 __global__ __launch_bounds__(256) void k_mlp_infer(Ctx c, MlpDev m) {
   __shared__ uint16_t sx[TILE * 16];
@@ -463,6 +465,8 @@ __global__ __launch_bounds__(256) void k_mlp_infer(Ctx c, MlpDev m) {
           c.lt[k] = lt;
           c.warm[k] = 2;
           set = true;
+          const int mr = c.solver == SHP_SOLVER_COLORED_F32 ? c.miss_rec[row0 + r] : -1;
+          if (mr >= 0) shp::patch_record_warm(shp::packf(c), mr >> 1, mr & 1, ln, lt);
         }
       }
       const unsigned long long setm = __ballot(set);  // one atomic per tile

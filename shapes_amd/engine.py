@@ -121,7 +121,7 @@ PHASE_KERNELS = {
     "broadphase_fill": ["k_grid_pairs", "k_broadphase"],
     "sat": ["k_sat"], "scan_contacts": ["k_scan_small", "k_scan_up", "k_scan_mid", "k_scan_down"],
     "world_ranges": ["k_world_ranges"], "contacts": ["k_contacts"], "infer": ["k_mlp_infer"],
-    "solve_setup": ["k_pack_manifolds"], "sweeps": ["k_sweep_colours", "k_solve_flow"],
+    "solve_setup": [], "sweeps": ["k_sweep_colours", "k_sweep_waves", "k_solve_flow"],
     "solve_writeback": ["k_colour_unpack"], "finish": ["k_finish"],
     "cache_store": ["k_cache_store"], "colouring": ["k_colour"],
 }

@@ -144,6 +144,23 @@ def test_plan_grid_lanes_at_scale(pile_batch):
             assert r["workgroups"] * r["workgroup_size"] < lanes * p["bodies"] + 256
 
 
+def test_plan_record_path_at_scale(pile_batch):
+    """With 1024-lane colouring workgroups on the 1024-pile batch the contact
+    kernel writes the manifold records after the colouring (no packing
+    pass); with 256 lanes, or with fewer worlds than CUs, the colouring runs
+    beside the contact kernel and a packing pass writes them."""
+    sc = pile_batch
+    for threads, worlds, packs in ((1024, sc.worlds, False), (512, sc.worlds, False),
+                                   (256, sc.worlds, True), (1024, sc.worlds[:255], True)):
+        st, msg, p, rows = engine.plan(worlds, sc.config, solver=engine.SOLVER_COLORED_F32,
+                                       max_pairs_per_body=4, solver_threads=threads)
+        assert st == 0, msg
+        _check_launches(rows)
+        names = [r["name"] for r in rows]
+        assert ("k_pack_manifolds" in names) == packs, (threads, len(worlds))
+        assert names.index("k_colour") < names.index("k_contacts")
+
+
 def test_plan_learned_combined_at_scale(pile_batch):
     """The learned warm start on the 1024-pile batch: the inference launch
     and the miss buffers are planned within the limits."""

@@ -159,3 +159,38 @@ def test_wavefront_sweeps_mixed_sizes_bitwise(orc):
             assert np.array_equal(gpu.contacts(k)["lam"].view(np.uint64),
                                   r.contacts()["lam"].view(np.uint64))
     gpu.close()
+
+
+@pytest.mark.parametrize("warm", ["zero", "learned"])
+def test_record_paths_agree(warm):
+    """The coloured solver's manifold records written by the contact kernel
+    after the colouring (1024-lane colouring workgroups, at least 256
+    worlds) and by the packing pass after the colouring ran beside the
+    contact kernel (256 lanes): the same bodies bit for bit over 40 steps,
+    with the zero and with the learned warm start (whose predictions the
+    first path patches into records already written)."""
+    from shapes_amd import warmstart
+    worlds = [scenes.random_world(900 + k, count=30 + 10 * (k % 7), spacing=1.25)
+              for k in range(300)]
+    cfg = scenes.SceneConfig(iterations=4)
+    ws = engine.WARM_LEARNED if warm == "learned" else engine.WARM_ZERO
+    out = []
+    for threads, packs in ((1024, False), (256, True)):
+        st, msg, _, rows = engine.plan(worlds, cfg, solver=engine.SOLVER_COLORED_F32,
+                                       warm_start=ws, solver_threads=threads)
+        assert st == 0, msg
+        assert ("k_pack_manifolds" in [r["name"] for r in rows]) == packs
+        print(f"record path test: {warm} start, {threads} lanes", flush=True)
+        b = engine.make_batch(worlds, cfg, solver=engine.SOLVER_COLORED_F32, warm_start=ws,
+                              solver_threads=threads)
+        if ws == engine.WARM_LEARNED:
+            b.set_mlp(np.load(warmstart.DEFAULT_MODEL))
+        learned = 0
+        for _ in range(40):
+            b.step(1)
+            learned += b.stats()["learned_starts"]
+        assert b.stats()["flags"] == 0
+        assert (learned > 0) == (ws == engine.WARM_LEARNED)
+        out.append(np.concatenate([b.bodies(k) for k in range(len(worlds))]))
+        b.close()
+    assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
