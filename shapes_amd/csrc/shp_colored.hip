@@ -21,10 +21,12 @@
 // bodies once, then per contact the two cross terms of each row, bias,
 // lever arms, effective masses and the FP64 lagrangians; a second contact
 // has a record of its own (a manifold with one contact reads nothing of
-// it).  Records are interleaved in blocks of 64 manifolds, word by word,
-// so a wavefront reading consecutive manifolds reads each word as one
-// contiguous kilobyte, a lane with 16-byte loads from one address.  The
-// rows are
+// it).  Records are interleaved word by word in blocks of four manifolds
+// (shp_records.h): a wavefront reading 64 consecutive manifolds reads each
+// word as 16 runs of 64 bytes, the rest of each 128-byte line being the
+// next word, which its next load reads, with 16-byte loads from one
+// address per lane; one manifold's record spans about four lines, which
+// is what its writer stores to (blocks of 64: seven).  The rows are
 // rebuilt from them exactly (jn = (-u, jn2, u, jn5), jt = (-cw(u), jt2,
 // cw(u), jt5) with u the row normal; negation commutes with rounding to
 // FP32).  A lane keeps its slot of a colour in every pass, so the record
@@ -94,8 +96,8 @@ __device__ __forceinline__ RecW recw(const Ctx& c, const PackF& P, int p0) {
 
 // Words 3-6 of a record: one contact.
 __device__ __forceinline__ void load_contact(const float4* w, ConF& k) {
-  const float4 a = w[0], b = w[64], d = w[128];
-  const double2 l = *(const double2*)(w + 192);
+  const float4 a = w[0], b = w[RB], d = w[2 * RB];
+  const double2 l = *(const double2*)(w + 3 * RB);
   k.jn2 = a.x;
   k.jn5 = a.y;
   k.jt2 = a.z;
@@ -115,8 +117,8 @@ __device__ __forceinline__ void load_contact(const float4* w, ConF& k) {
 // contact's record loads only when the manifold has one.
 __device__ __forceinline__ void load_man(const RecW& R, int m_loc, ManF& m) {
   const float4* r = rec_of(R.rec, R.p0 + m_loc);
-  const float4 a = r[0], b = r[64];
-  const int4 g = *(const int4*)(r + 128);
+  const float4 a = r[0], b = r[RB];
+  const int4 g = *(const int4*)(r + 2 * RB);
   m.ux = a.x;
   m.uy = a.y;
   m.mu = a.z;
@@ -128,7 +130,7 @@ __device__ __forceinline__ void load_man(const RecW& R, int m_loc, ManF& m) {
   m.ga = g.x;
   m.gb = g.y;
   m.flags = g.z;
-  load_contact(r + 192, m.k[0]);
+  load_contact(r + 3 * RB, m.k[0]);
   if (m.flags & MFL_TWO) load_contact(rec2_of(R.rec2, R.p0 + m_loc), m.k[1]);
 }
 
@@ -220,9 +222,9 @@ __device__ __forceinline__ void solve_man(const RecW& R, int m_loc, ManF& m, boo
   }
   if (!changed) return;
   if (!warm) {  // the solved contacts' lagrangians
-    *(double2*)(rec_of(R.rec, R.p0 + m_loc) + 6 * 64) = make_double2(m.k[0].ln, m.k[0].lt);
+    *(double2*)(rec_of(R.rec, R.p0 + m_loc) + 6 * RB) = make_double2(m.k[0].ln, m.k[0].lt);
     if (nk > 1)
-      *(double2*)(rec2_of(R.rec2, R.p0 + m_loc) + 3 * 64) = make_double2(m.k[1].ln, m.k[1].lt);
+      *(double2*)(rec2_of(R.rec2, R.p0 + m_loc) + 3 * RB) = make_double2(m.k[1].ln, m.k[1].lt);
   }
   if (!(m.flags & MFL_A_STATIC)) V.store(m.ga, v);
   if (!(m.flags & MFL_B_STATIC)) V.store(m.gb, v + 3);
@@ -1140,7 +1142,7 @@ __global__ void k_colour_unpack(Ctx c) {
         const int nk = min(c.ccount[t], c.contact_cap - k0);  // contacts that were stored
         for (int j = 0; j < nk; ++j) {
           const double2 l =
-              *(const double2*)(j ? rec2_of(P.rec2, M) + 3 * 64 : rec_of(P.rec, M) + 6 * 64);
+              *(const double2*)(j ? rec2_of(P.rec2, M) + 3 * RB : rec_of(P.rec, M) + 6 * RB);
           c.ln[k0 + j] = l.x;
           c.lt[k0 + j] = l.y;
         }
