@@ -945,6 +945,18 @@ hipError_t scan_grid_buckets(shp_batch* b) {
 
 // The step's front: broadphase, SAT, contact generation and the warm-start
 // join; with `fork` the colouring starts on the second stream after k_sat.
+// The coloured solver's colouring on the second stream, from this point of
+// the batch's stream; join_event marks its end.
+static shp_status fork_colouring(shp_batch* b) {
+  HIPCHK(hipEventRecord(b->fork_event, b->stream));
+  HIPCHK(hipStreamWaitEvent(b->side_stream, b->fork_event, 0));
+  if (b->timing) HIPCHK(hipEventRecord(b->side_events[0], b->side_stream));
+  launch_colouring(b, b->side_stream);
+  if (b->timing) HIPCHK(hipEventRecord(b->side_events[1], b->side_stream));
+  HIPCHK(hipEventRecord(b->join_event, b->side_stream));
+  return SHP_OK;
+}
+
 static shp_status enqueue_front(shp_batch* b, bool fork) {
   Ctx& c = b->ctx;
   const StepPlan& P = b->plan;
@@ -966,23 +978,19 @@ static shp_status enqueue_front(shp_batch* b, bool fork) {
   mark(b, 4);
   k_sat<<<P.g_sat, 256, 0, s>>>(c);
   mark(b, 5);
+  // The colouring needs only the pairs and which of them have contacts.
+  // When its workgroups would take every CU anyway (records_in_contacts),
+  // it starts after the contact scan, whose short passes would otherwise
+  // wait for CUs behind it, and ends before the contact kernel, which then
+  // writes each manifold's record at its colour slot (phase "world_ranges"
+  // holds the wait) so that no packing pass reads the contacts back.
+  // Otherwise it runs beside the scan, the contact kernel and the learned
+  // inference, and joins before the packing pass.
+  if (fork && !P.records_in_contacts) TRY(fork_colouring(b));
   // contacts per pair: this step's pairs only (the pair count is on the
-  // device).  Before the colouring starts: a colouring workgroup takes a
-  // whole CU, and the scan's short passes would wait behind them.
+  // device)
   HIPCHK(exclusive_sum(b, P.scan_pairs, c.ccount, c.coff, c.off_rank + c.NB, c.pair_cap, s));
-  // the colouring needs only the pairs and which of them have contacts.  It
-  // runs beside the contact kernel, or, when its workgroups would take every
-  // CU anyway, before it, so that the contact kernel writes each manifold's
-  // record at its colour slot (phase "world_ranges" then holds the wait)
-  // and no packing pass reads the contacts back.
-  if (fork) {
-    HIPCHK(hipEventRecord(b->fork_event, s));
-    HIPCHK(hipStreamWaitEvent(b->side_stream, b->fork_event, 0));
-    if (b->timing) HIPCHK(hipEventRecord(b->side_events[0], b->side_stream));
-    launch_colouring(b, b->side_stream);
-    if (b->timing) HIPCHK(hipEventRecord(b->side_events[1], b->side_stream));
-    HIPCHK(hipEventRecord(b->join_event, b->side_stream));
-  }
+  if (fork && P.records_in_contacts) TRY(fork_colouring(b));
   mark(b, 6);
   k_world_ranges<<<P.g_ranges, 256, 0, s>>>(c);
   if (fork && P.records_in_contacts) HIPCHK(hipStreamWaitEvent(s, b->join_event, 0));
@@ -1000,15 +1008,14 @@ static shp_status enqueue_step(shp_batch* b) {
   const bool fork = c.solver == SHP_SOLVER_COLORED_F32;
   TRY(enqueue_front(b, fork));
   // the learned warm start of the cache misses (which also patches their
-  // manifold records when the contact kernel wrote them); with the colouring
-  // beside the contact kernel, after it (phase "infer" then holds the wait
-  // for the colouring), so that its workgroups do not queue behind
-  // colouring workgroups
-  if (fork && !P.records_in_contacts) HIPCHK(hipStreamWaitEvent(s, b->join_event, 0));
+  // manifold records when the contact kernel wrote them)
   if (c.warm_mode == SHP_WARM_LEARNED && c.mlp_ready) mlp_launch_infer(b);
   mark(b, 9);
   if (fork) {
-    if (!P.records_in_contacts) launch_colored_pack(b);
+    if (!P.records_in_contacts) {  // phase "solve_setup" holds the colouring's end
+      HIPCHK(hipStreamWaitEvent(s, b->join_event, 0));
+      launch_colored_pack(b);
+    }
     mark(b, 10);
     launch_colored_sweep(b);
     mark(b, 11);
