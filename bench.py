@@ -353,7 +353,7 @@ def headline(scene, args, model, sched, ppb, dyn, preroll):
         for it in range(base, 0, -1):
             try:
                 rl = run(scene, col, engine.WARM_LEARNED, it, preroll, args.steps, sched, ppb,
-                         model=model, positions=True)
+                         model=model, positions=True, instrument=args.profile_steps)
             except engine.ShapesError as e:
                 tried.append({"warm_start": "learned", "iterations": it, "error": str(e)})
                 note(f"learned at {it} sweeps: {e}")
@@ -541,7 +541,8 @@ def main():
             "sweeps_plain_over_learned": (round(plain["iterations"] / chosen["iterations"], 3)
                                           if chosen else None),
         }
-        instr = r_plain
+        # phases and roofline of the configuration `value` is quoted on
+        instr = runs[("learned", chosen["iterations"])] if chosen is not None else r_plain
         if not args.no_fp64:
             extra["fp64_reference"] = fp64_reference(scene, args, sched, ppb, preroll, runs)
             note("FP64 reference runs done")

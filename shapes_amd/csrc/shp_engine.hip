@@ -1381,7 +1381,7 @@ shp_status plan_staged(const shp_config& cfg, HostBatch* hb) {
   P.g_ranges = blocks(n_worlds + 1, 256);
   P.g_contacts = gs_blocks(PC);
   P.g_pack = gs_blocks(PC);
-  P.g_infer = std::min<long>(std::max<long>(1, blocks(P.miss_cap, 64)), 512);
+  P.g_infer = std::min<long>(std::max<long>(1, blocks(P.miss_cap, 64)), 256);
   P.g_unpack = std::max(1L, std::min(blocks(std::max(PC, NB), 256), 4096L));
   P.g_log = std::max<long>(1, blocks(P.miss_cap, 256));
   P.g_finish_ni = blocks(NB, INTEGRATE_THREADS);

@@ -113,6 +113,15 @@ struct TrainBufs {
 
 // One 64-row tile: X (64 x 16 BF16) in LDS -> Y (64 x 16 FP32).
 // Wave w owns rows 16w..16w+15.
+// The inference kernel's LDS copies of the weights: transposed ([out][in])
+// with rows of WT1 (first layer) or WT2 entries, padded so that its
+// transposing stores spread over the LDS banks while a fragment stays one
+// aligned 8-byte read.
+constexpr int WT1 = 20, WT2 = 132;
+
+// WT: the weights of m transposed and padded as above instead of
+// row-major [in][out].
+template <bool WT>
 __device__ void forward_tile(const MlpDev& m, uint16_t* sx, uint16_t* s1, uint16_t* s2,
                              float* yout /* 64 x 16 */, const TrainBufs* tb, int row0) {
   int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -120,7 +129,8 @@ __device__ void forward_tile(const MlpDev& m, uint16_t* sx, uint16_t* s1, uint16
   // layer 1: K = 16
   for (int ct = 0; ct < 8; ++ct) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = mfma16(frag_a(sx, 16, r0, 0), frag_b(m.w1, 128, 0, 16 * ct), acc);
+    acc = mfma16(frag_a(sx, 16, r0, 0),
+                 WT ? frag_bt(m.w1, WT1, 0, 16 * ct) : frag_b(m.w1, 128, 0, 16 * ct), acc);
     int col = 16 * ct + (l & 15);
     for (int i = 0; i < 4; ++i) {
       int row = r0 + 4 * (l >> 4) + i;
@@ -146,7 +156,8 @@ __device__ void forward_tile(const MlpDev& m, uint16_t* sx, uint16_t* s1, uint16
     for (int ct = 0; ct < 8; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < 128; k0 += 16) {
       bf16x4 a = frag_a(in, 128, r0, k0);
-      for (int ct = 0; ct < 8; ++ct) acc[ct] = mfma16(a, frag_b(W, 128, k0, 16 * ct), acc[ct]);
+      for (int ct = 0; ct < 8; ++ct)
+        acc[ct] = mfma16(a, WT ? frag_bt(W, WT2, k0, 16 * ct) : frag_b(W, 128, k0, 16 * ct), acc[ct]);
     }
     __syncthreads();  // every wave has read `in` before `out` (== in of layer 1) is overwritten
     for (int ct = 0; ct < 8; ++ct) {
@@ -166,7 +177,8 @@ __device__ void forward_tile(const MlpDev& m, uint16_t* sx, uint16_t* s1, uint16
   }
   // layer 4: 128 -> 16 (2 used), input in s1
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < 128; k0 += 16) acc = mfma16(frag_a(s1, 128, r0, k0), frag_b(m.w4, 16, k0, 0), acc);
+  for (int k0 = 0; k0 < 128; k0 += 16)
+    acc = mfma16(frag_a(s1, 128, r0, k0), WT ? frag_bt(m.w4, WT2, k0, 0) : frag_b(m.w4, 16, k0, 0), acc);
   int col = l & 15;
   for (int i = 0; i < 4; ++i) {
     int row = r0 + 4 * (l >> 4) + i;
@@ -186,7 +198,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(MlpDev m, const uint16_t* X
     int row0 = t * TILE;
     for (int i = threadIdx.x; i < TILE * 16; i += 256) sx[i] = X[(size_t)row0 * 16 + i];
     __syncthreads();
-    forward_tile(m, sx, s1, s2, sy, store ? &tb : nullptr, row0);
+    forward_tile<false>(m, sx, s1, s2, sy, store ? &tb : nullptr, row0);
     for (int i = threadIdx.x; i < TILE * 16; i += 256) Y[(size_t)row0 * 16 + i] = sy[i];
     __syncthreads();
   }
@@ -437,8 +449,32 @@ __global__ __launch_bounds__(256) void k_mlp_infer(Ctx c, MlpDev m) {
   __shared__ uint16_t s1[TILE * 128];
   __shared__ uint16_t s2[TILE * 128];
   __shared__ float sy[TILE * 16];
+  // the weights, transposed, and biases, once per workgroup for all its
+  // tiles: a B fragment is then one 8-byte LDS read
+  __shared__ __align__(16) uint16_t s_w1[128 * WT1], s_w2[128 * WT2], s_w3[128 * WT2], s_w4[16 * WT2];
+  __shared__ float s_b[3 * 128 + 16];
   int n = c.counters[CNT_MISS];
   if (n > c.miss_cap) n = c.miss_cap;
+  if ((int)blockIdx.x * TILE >= n) return;  // no tile: skip the staging (no barrier follows)
+  for (int i = threadIdx.x; i < 128 * 128; i += 256) {
+    const int k = i >> 7, o = i & 127;  // read row-major [k][o], coalesced
+    s_w2[o * WT2 + k] = m.w2[i];
+    s_w3[o * WT2 + k] = m.w3[i];
+    if (k < 16) s_w1[o * WT1 + k] = m.w1[i];
+    if (o < 16) s_w4[o * WT2 + k] = m.w4[k * 16 + o];
+  }
+  for (int i = threadIdx.x; i < 3 * 128 + 16; i += 256)
+    s_b[i] = i < 128 ? m.b1[i] : i < 256 ? m.b2[i - 128] : i < 384 ? m.b3[i - 256] : m.b4[i - 384];
+  __syncthreads();
+  MlpDev ml;
+  ml.w1 = s_w1;
+  ml.w2 = s_w2;
+  ml.w3 = s_w3;
+  ml.w4 = s_w4;
+  ml.b1 = s_b;
+  ml.b2 = s_b + 128;
+  ml.b3 = s_b + 256;
+  ml.b4 = s_b + 384;
   for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
     int row0 = t * TILE;
     for (int i = threadIdx.x; i < TILE * 16; i += 256) {
@@ -447,7 +483,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(Ctx c, MlpDev m) {
       sx[i] = f2bf(f);
     }
     __syncthreads();
-    forward_tile(m, sx, s1, s2, sy, nullptr, row0);
+    forward_tile<true>(ml, sx, s1, s2, sy, nullptr, row0);
     static_assert(TILE == 64, "one row per lane of the first wavefront");
     if (threadIdx.x < TILE) {  // the first wavefront: one row per lane
       bool set = false;
@@ -506,7 +542,9 @@ void mlp_launch_infer(shp_batch* b) {
 
 // sx, s1, s2 (BF16) and sy (FP32) of k_mlp_infer
 size_t mlp_infer_static_lds() {
-  return lds16(TILE * 16 * 2) + 2 * lds16(TILE * 128 * 2) + lds16(TILE * 16 * 4);
+  return lds16(TILE * 16 * 2) + 2 * lds16(TILE * 128 * 2) + lds16(TILE * 16 * 4) +
+         lds16(128 * WT1 * 2) + 2 * lds16(128 * WT2 * 2) + lds16(16 * WT2 * 2) +
+         lds16((3 * 128 + 16) * 4);
 }
 
 void mlp_launch_log(shp_batch* b) {
