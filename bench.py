@@ -206,14 +206,15 @@ def kernel_bytes(phase, nb, nv, st, word):
         # its normal, point, depth, feature and flip written
         return P * (8 + 4 + (4 if word == 4 else 0)) + nv * 48 + C * 43
     if phase == "contacts":
-        # per contact its slot (43 B), both bodies' positions and inverse
-        # masses (64 B) and the cache entry it joins (24 B) read; key, bodies,
-        # flip, normal, point, depth, the rows' cross terms, bias, radii,
-        # effective masses, lagrangians and warm flag written (162 B); for
-        # the coloured solver each manifold's record written (112 B, 64 B
-        # per second contact) with both bodies' materials read (34 B)
+        # per contact its slot (43 B) and the cache entry it joins (24 B)
+        # read, and key, bodies, flip, normal, point, depth, the rows' cross
+        # terms, bias, radii, effective masses, lagrangians and warm flag
+        # written (162 B); per pair with contacts both bodies' positions and
+        # inverse masses read (64 B); for the coloured solver each manifold's
+        # record written (112 B, 64 B per second contact) with both bodies'
+        # materials read (34 B)
         rec = M * (112 + 34) + (C - M) * 64 if word == 4 else 0
-        return C * (43 + 64 + 24 + 162) + rec
+        return C * (43 + 24 + 162) + M * 64 + rec
     if phase == "solve_writeback":  # lagrangians back, FP32 -> FP64 velocities
         return C * (16 + 16) + nb * (12 + 48)
     if phase == "broadphase_fill":
