@@ -132,12 +132,13 @@ def run(scene, solver, warm, iterations, preroll, steps, sched, ppb, model=None,
             out["positions"] = [b.bodies(k)[:, :2].copy() for k in range(len(scene.worlds))]
         if instrument:
             b.set_timing(True)
-            tot = {"pairs": 0, "contacts": 0}
+            tot = {"pairs": 0, "contacts": 0, "manifolds": 0}
             for _ in range(instrument):
                 b.step(1)
                 s = b.stats()
                 tot["pairs"] += s["pairs"]
                 tot["contacts"] += s["contacts"]
+                tot["manifolds"] += s["manifolds"]
             t = b.timing()
             out["phases_ms"] = {p: t[p] for p in engine.PHASES}
             out["solve_profile_us"] = b.solve_profile()
@@ -176,9 +177,8 @@ def divergence(pa, pb, worlds):
 def kernel_bytes(phase, nb, nv, st, word):
     """Algorithmic (compulsory) HBM bytes of one launch of a phase's kernel,
     from the step's counts: nb bodies, nv hull vertices, st['pairs'],
-    st['contacts'] (manifolds ~ contacts / 1.6 are bounded by pairs)."""
-    P, C = st["pairs"], st["contacts"]
-    M = min(P, C)  # manifolds: pairs with contacts
+    st['contacts'] and st['manifolds'] (pairs with stored contacts)."""
+    P, C, M = st["pairs"], st["contacts"], st["manifolds"]
     if phase == "sweeps":
         # per manifold its record (normal, materials, normal x inverse
         # masses, bodies, flags, first contact's 11 FP32 fields and two FP64

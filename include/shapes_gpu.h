@@ -152,6 +152,8 @@ typedef struct shp_step_stats {
   double mean_max_depth;
   double mean_normal_residual;
   double mean_friction_residual;
+  int64_t manifolds;    /* pairs with stored contacts (the coloured solver's
+                           manifolds) */
 } shp_step_stats;
 
 /* ---- engine lifecycle (initEngine / runEngine, Engine/Main.hs:43-98) ---- */

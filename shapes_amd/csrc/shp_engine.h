@@ -47,7 +47,7 @@ constexpr size_t lds16(size_t bytes) { return (bytes + 15) & ~(size_t)15; }
 // stream, is phase SHP_MAIN_PHASES of the timing ABI)
 #define SHP_MAIN_PHASES (SHP_NUM_PHASES - 1)
 
-enum { CNT_FLAGS = 0, CNT_WARM = 1, CNT_LEARNED = 2, CNT_LEVELS = 3, CNT_MISS = 4 };
+enum { CNT_FLAGS = 0, CNT_WARM = 1, CNT_LEARNED = 2, CNT_LEVELS = 3, CNT_MISS = 4, CNT_MANIFOLDS = 5 };
 // grid broadphase: partners per body kept by the count pass for the emit
 // pass (bodies with more are visited again)
 enum { GRID_SLOTS = 8 };

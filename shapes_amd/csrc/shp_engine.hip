@@ -615,19 +615,27 @@ __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
 
 __global__ void k_contacts(Ctx c) {
   int tp = total_pairs(c);
-  int hits = 0;
+  int hits = 0, mans = 0;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < tp; p += gridDim.x * blockDim.x) {
     int n = c.ccount[p];
-    if (n > 0) contacts_of_pair(c, p, n, hits);
+    if (n > 0) {
+      contacts_of_pair(c, p, n, hits);
+      mans += c.coff[p] < c.contact_cap;  // a manifold with stored contacts
+    }
   }
-  // one atomic per workgroup for the warm-start hit counter
-  __shared__ int s_hits;
-  if (threadIdx.x == 0) s_hits = 0;
+  // one atomic per workgroup for the warm-start hit and manifold counters
+  __shared__ int s_cnt[2];
+  if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
   __syncthreads();
-  for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off);
-  if ((threadIdx.x & 63) == 0 && hits) atomicAdd(&s_hits, hits);
+  for (int off = 32; off > 0; off >>= 1) {
+    hits += __shfl_xor(hits, off);
+    mans += __shfl_xor(mans, off);
+  }
+  if ((threadIdx.x & 63) == 0 && hits) atomicAdd(&s_cnt[0], hits);
+  if ((threadIdx.x & 63) == 0 && mans) atomicAdd(&s_cnt[1], mans);
   __syncthreads();
-  if (threadIdx.x == 0 && s_hits) atomicAdd(&c.counters[CNT_WARM], s_hits);
+  if (threadIdx.x == 0 && s_cnt[0]) atomicAdd(&c.counters[CNT_WARM], s_cnt[0]);
+  if (threadIdx.x == 0 && s_cnt[1]) atomicAdd(&c.counters[CNT_MANIFOLDS], s_cnt[1]);
 }
 
 // This frame's lagrangians become the next frame's EngineCache
@@ -1876,6 +1884,7 @@ shp_status shp_batch_last_stats(shp_batch* b, shp_step_stats* out) {
   out->learned_starts = cnt[CNT_LEARNED];
   out->misses = cnt[CNT_MISS];
   out->max_levels = cnt[CNT_LEVELS];
+  out->manifolds = cnt[CNT_MANIFOLDS];
   out->flags = cnt[CNT_FLAGS] | b->last_flags;
   unsigned long long q[Q_ENTRIES];
   HIPCHK(hipMemcpy(q, c.quality, sizeof(q), hipMemcpyDeviceToHost));

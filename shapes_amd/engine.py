@@ -78,7 +78,8 @@ class StepStatsC(C.Structure):
                 ("friction_residual", C.c_double), ("run_max_depth", C.c_double),
                 ("run_normal_residual", C.c_double),
                 ("run_friction_residual", C.c_double), ("mean_max_depth", C.c_double),
-                ("mean_normal_residual", C.c_double), ("mean_friction_residual", C.c_double)]
+                ("mean_normal_residual", C.c_double), ("mean_friction_residual", C.c_double),
+                ("manifolds", C.c_int64)]
 
 
 class PlanC(C.Structure):

@@ -93,3 +93,20 @@ def test_set_cache_rejects_out_of_range_keys():
     b.set_cache(0, np.array([[w.n - 1, 0, 0, 0]], dtype=np.int32), lam)
     b.step(1)
     b.close()
+
+
+@pytest.mark.parametrize("solver", ["ordered", "colored"])
+def test_manifold_count_is_pairs_with_contacts(solver):
+    """The step's manifold count (pairs with stored contacts) equals the
+    number of distinct body pairs among the step's contact keys, on a
+    batch of random worlds."""
+    worlds = [scenes.random_world(40 + k, count=30 + 15 * k, spacing=1.25) for k in range(4)]
+    sv = engine.SOLVER_ORDERED_F64 if solver == "ordered" else engine.SOLVER_COLORED_F32
+    b = engine.make_batch(worlds, scenes.SceneConfig(iterations=4), solver=sv)
+    for _ in range(3):
+        b.step(20)
+        s = b.stats()
+        pairs = sum(len({(int(i), int(j)) for i, j, _, _ in b.contacts(k)["keys"]})
+                    for k in range(len(worlds)))
+        assert s["manifolds"] == pairs > 0
+    b.close()
