@@ -110,3 +110,30 @@ def test_manifold_count_is_pairs_with_contacts(solver):
                     for k in range(len(worlds)))
         assert s["manifolds"] == pairs > 0
     b.close()
+
+
+def test_pair_overflow_with_records_from_the_contact_kernel():
+    """The coloured solver with 1024-lane colouring on 300 worlds (the
+    contact kernel writes the manifold records) and too little pair
+    capacity (one pair per body): the step reports SHP_E_OVERFLOW, the
+    pairs stay within the capacity and every body stays finite over
+    graph-replayed steps."""
+    rng = np.random.Generator(np.random.Philox(6))
+    worlds = [scenes.grid_world(rng, 40, cols=8, spacing=0.55, y0=1.0, square_frac=0.5, vel=0.0)
+              for _ in range(300)]
+    cfg = scenes.SceneConfig(iterations=2)
+    st, msg, _, rows = engine.plan(worlds, cfg, solver=engine.SOLVER_COLORED_F32,
+                                   max_pairs_per_body=1, solver_threads=1024)
+    assert st == 0, msg
+    assert "k_pack_manifolds" not in [r["name"] for r in rows]
+    b = engine.make_batch(worlds, cfg, solver=engine.SOLVER_COLORED_F32, max_pairs_per_body=1,
+                          solver_threads=1024)
+    b.step_async(5, use_graph=True)
+    with pytest.raises(engine.ShapesError) as ei:
+        b.sync()
+    assert ei.value.status == 5
+    s = b.stats()
+    n = sum(w.n for w in worlds)
+    assert s["flags"] & 1 and s["pairs"] == n + 1024
+    assert all(np.isfinite(b.bodies(k)).all() for k in range(len(worlds)))
+    b.close()
