@@ -157,6 +157,7 @@ struct Ctx {
   const uint16_t* mlp_wbf;
   int mlp_ready;
   int collect_misses;
+  int features_in_contacts;  // the misses' features from k_contacts (the log needs them)
   int miss_cap;
   int32_t* miss_idx;
   int32_t* miss_rec;  // per miss: its manifold record (slot * 2 + contact), or -1

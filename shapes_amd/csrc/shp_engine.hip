@@ -593,6 +593,11 @@ __device__ void contacts_of_pair(const Ctx& c, int p, int n, int& hits) {
         if (slot < c.miss_cap) {
           c.miss_idx[slot] = dst;
           c.miss_rec[slot] = M >= 0 ? (M << 1) | k : -1;
+        }
+        // the features here only for the training log (which reads them
+        // after the solve); else the inference computes them from the
+        // stored contact, off this kernel's divergent path
+        if (slot < c.miss_cap && c.features_in_contacts) {
           double v[6] = {c.vx[ga], c.vy[ga], c.w[ga], c.vx[gb], c.vy[gb], c.w[gb]};
           double sgn = flip ? -1.0 : 1.0;
           mlp_features(sgn * nn.x, sgn * nn.y, depth, v, ra.x, ra.y, rb.x, rb.y, c.iml[ga], c.iml[gb],
@@ -1910,6 +1915,7 @@ shp_status shp_batch_set_mlp(shp_batch* b, const float* params) {
   b->ctx.mlp_wbf = b->mlp.wbf;
   b->ctx.mlp_ready = 1;
   b->ctx.collect_misses = (b->cfg.warm_start == SHP_WARM_LEARNED) || b->logging;
+  b->ctx.features_in_contacts = b->logging;
   b->graph_ready = false;  // kernel arguments changed
   return SHP_OK;
 }
@@ -1936,6 +1942,7 @@ shp_status shp_batch_set_logging(shp_batch* b, int64_t capacity) {
   HIPCHK(hipMemcpy(c.log_count, &zero, 8, hipMemcpyHostToDevice));
   b->logging = capacity > 0;
   c.collect_misses = (b->cfg.warm_start == SHP_WARM_LEARNED && c.mlp_ready) || b->logging;
+  c.features_in_contacts = b->logging;
   b->graph_ready = false;
   return SHP_OK;
 }
@@ -2170,6 +2177,7 @@ static shp_status rebuild_with(shp_batch* b, long g, const StagedBody& s) {
   c.mlp_wbf = keep.mlp_wbf;
   c.mlp_ready = keep.mlp_ready;
   c.collect_misses = keep.collect_misses;
+  c.features_in_contacts = keep.features_in_contacts;
   c.log_feat = keep.log_feat;
   c.log_tgt = keep.log_tgt;
   c.log_cap = keep.log_cap;

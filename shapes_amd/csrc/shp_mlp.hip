@@ -477,10 +477,29 @@ __global__ __launch_bounds__(256) void k_mlp_infer(Ctx c, MlpDev m) {
   ml.b4 = s_b + 384;
   for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
     int row0 = t * TILE;
-    for (int i = threadIdx.x; i < TILE * 16; i += 256) {
-      int r = i >> 4, q = i & 15;
-      float f = (row0 + r < n && q < MLP_IN) ? c.miss_feat[(size_t)(row0 + r) * MLP_IN + q] : 0.0f;
-      sx[i] = f2bf(f);
+    if (c.features_in_contacts) {
+      for (int i = threadIdx.x; i < TILE * 16; i += 256) {
+        int r = i >> 4, q = i & 15;
+        float f = (row0 + r < n && q < MLP_IN) ? c.miss_feat[(size_t)(row0 + r) * MLP_IN + q] : 0.0f;
+        sx[i] = f2bf(f);
+      }
+    } else if (threadIdx.x < TILE) {
+      // one row per lane of the first wavefront: the features of the
+      // stored contact, as the contact kernel would compute them (the
+      // velocities are those it saw: nothing has changed them since)
+      const int r = threadIdx.x;
+      float f[MLP_IN];
+      for (int q = 0; q < MLP_IN; ++q) f[q] = 0.0f;
+      if (row0 + r < n) {
+        const int k = c.miss_idx[row0 + r];
+        const int ga = c.ga[k], gb = c.gb[k];
+        const double v[6] = {c.vx[ga], c.vy[ga], c.w[ga], c.vx[gb], c.vy[gb], c.w[gb]};
+        const double sgn = c.flip[k] ? -1.0 : 1.0;
+        mlp_features(sgn * c.cnx[k], sgn * c.cny[k], c.depth[k], v, c.rax[k], c.ray[k], c.rbx[k],
+                     c.rby[k], c.iml[ga], c.iml[gb], c.imr[ga], c.imr[gb],
+                     (c.mu[ga] + c.mu[gb]) / 2.0, f);
+      }
+      for (int q = 0; q < 16; ++q) sx[r * 16 + q] = f2bf(q < MLP_IN ? f[q] : 0.0f);
     }
     __syncthreads();
     forward_tile<true>(ml, sx, s1, s2, sy, nullptr, row0);
