@@ -477,6 +477,19 @@ __global__ __launch_bounds__(256) void k_mlp_infer(Ctx c, MlpDev m) {
   ml.b4 = s_b + 384;
   for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
     int row0 = t * TILE;
+    // the output head's inputs of this lane's row (first wavefront), loaded
+    // with the features so that the head waits on no load after the layers
+    static_assert(TILE == 64, "one row per lane of the first wavefront");
+    const bool row_ok = threadIdx.x < TILE && row0 + (int)threadIdx.x < n;
+    int hk = 0, hmr = -1;
+    double hmcn = 1.0, hmct = 1.0, hmu = 0.0;
+    if (row_ok) {
+      hk = c.miss_idx[row0 + threadIdx.x];
+      hmr = c.solver == SHP_SOLVER_COLORED_F32 ? c.miss_rec[row0 + threadIdx.x] : -1;
+      hmcn = c.mcn[hk];
+      hmct = c.mct[hk];
+      hmu = (c.mu[c.ga[hk]] + c.mu[c.gb[hk]]) / 2.0;
+    }
     if (c.features_in_contacts) {
       for (int i = threadIdx.x; i < TILE * 16; i += 256) {
         int r = i >> 4, q = i & 15;
@@ -490,8 +503,8 @@ __global__ __launch_bounds__(256) void k_mlp_infer(Ctx c, MlpDev m) {
       const int r = threadIdx.x;
       float f[MLP_IN];
       for (int q = 0; q < MLP_IN; ++q) f[q] = 0.0f;
-      if (row0 + r < n) {
-        const int k = c.miss_idx[row0 + r];
+      if (row_ok) {
+        const int k = hk;
         const int ga = c.ga[k], gb = c.gb[k];
         const double v[6] = {c.vx[ga], c.vy[ga], c.w[ga], c.vx[gb], c.vy[gb], c.w[gb]};
         const double sgn = c.flip[k] ? -1.0 : 1.0;
@@ -503,16 +516,15 @@ __global__ __launch_bounds__(256) void k_mlp_infer(Ctx c, MlpDev m) {
     }
     __syncthreads();
     forward_tile<true>(ml, sx, s1, s2, sy, nullptr, row0);
-    static_assert(TILE == 64, "one row per lane of the first wavefront");
     if (threadIdx.x < TILE) {  // the first wavefront: one row per lane
       bool set = false;
-      if (row0 + (int)threadIdx.x < n) {
+      if (row_ok) {
         int r = threadIdx.x;
-        int k = c.miss_idx[row0 + r];
+        int k = hk;
         double ln = (double)sy[r * 16];
-        ln = ln > 0.0 ? ln / c.mcn[k] : 0.0;
-        double lt = (double)sy[r * 16 + 1] / c.mct[k];
-        double mu = (c.mu[c.ga[k]] + c.mu[c.gb[k]]) / 2.0;
+        ln = ln > 0.0 ? ln / hmcn : 0.0;
+        double lt = (double)sy[r * 16 + 1] / hmct;
+        double mu = hmu;
         double thr = ln * mu;
         lt = lt > thr ? thr : (lt < -thr ? -thr : lt);
         if (ln - ln == 0.0 && lt - lt == 0.0) {
@@ -520,8 +532,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(Ctx c, MlpDev m) {
           c.lt[k] = lt;
           c.warm[k] = 2;
           set = true;
-          const int mr = c.solver == SHP_SOLVER_COLORED_F32 ? c.miss_rec[row0 + r] : -1;
-          if (mr >= 0) shp::patch_record_warm(shp::packf(c), mr >> 1, mr & 1, ln, lt);
+          if (hmr >= 0) shp::patch_record_warm(shp::packf(c), hmr >> 1, hmr & 1, ln, lt);
         }
       }
       const unsigned long long setm = __ballot(set);  // one atomic per tile
