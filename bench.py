@@ -408,10 +408,10 @@ def fp64_reference(scene, args, sched, ppb, preroll, runs):
 
 
 def c4_evaluation(model, seeds=(1002, 1003, 1004)):
-    """The learned warm start as the C4 configuration states it: the
-    8192-polygon world (dt 1/60, 600 steps) at 2 and 4 sweeps against the
-    plain solver at 8, quality averaged over three seeds of the generator
-    (not the model's training seeds), throughput of the first."""
+    """The learned warm start on the 8192-polygon world (dt 1/60, 600
+    steps) at 2 and 4 sweeps against the plain solver at 8, quality averaged
+    over three seeds of the generator (not the model's training seeds),
+    throughput of the first."""
     rows = []
     for warm, it in ((engine.WARM_ZERO, 8), (engine.WARM_LEARNED, 4), (engine.WARM_LEARNED, 2),
                      (engine.WARM_ZERO, 4), (engine.WARM_ZERO, 2)):
